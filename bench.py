@@ -1,0 +1,153 @@
+"""Benchmark of the plain single-pair grid search on one MI355X.
+
+A step is one full search of one image pair (every grid point evaluated, cost
+tensor written, fused argmin read back) with the inputs already resident in
+HBM.  Default workload: 256x256 FP32 pair, bilinear sampling, uncentred NCC,
+6^8 = 1,679,616 homography grid points (1.1008e11 pixel evaluations).
+
+Prints one JSON line.  value = Gpixel-evals/s; pairs_per_s and whether the
+argmin matches the oracle's full-grid argmin (committed fixture) ride along.
+roofline: algorithmic FLOP per launch / the search kernel's average duration
+measured with HIP events on its stream.  cpu_baseline: the C oracle on a
+bounded contiguous slice of the same grid, OpenMP over the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pairs/s at matched argmin; Gpixel-evals/s; % of VALU/LDS roofline"
+PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
+# algorithmic FLOP per pixel-eval (fixed convention): warp 17, bilinear 13,
+# NCC 6, MI 2, SAD 3
+FLOP_PER_PE = {("nearest", "sad"): 17 + 3, ("bilinear", "ncc"): 17 + 13 + 6, ("bilinear", "mi"): 17 + 13 + 2,
+               ("bilinear", "sad"): 17 + 13 + 3}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="bilinear_ncc_256")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
+    return ap.parse_args()
+
+
+def cpu_baseline(m, f, spec, start, end, count, pe_per_point, target_s):
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    c2, r = O.grid_update(start, end, count)
+    cost = {"sad": O.COST_SAD, "ncc": O.COST_NCC, "mi": O.COST_MI}[spec["cost"]]
+    samp = {"nearest": O.NEAREST, "bilinear": O.BILINEAR}[spec["sampler"]]
+    n = 256
+    t = time.perf_counter()
+    O.search(m, f, cost, samp, start, r, c2, bins=spec.get("bins", 32), i0=0, i1=n, nthreads=threads)
+    dt = time.perf_counter() - t
+    n = int(min(O.grid_num_elements(c2), max(256, n * target_s / max(dt, 1e-6))))
+    t = time.perf_counter()
+    O.search(m, f, cost, samp, start, r, c2, bins=spec.get("bins", 32), i0=0, i1=n, nthreads=threads)
+    dt = time.perf_counter() - t
+    return {"value": n * pe_per_point / dt / 1e9, "unit": "Gpixel-evals/s", "cores": threads, "kind": "port",
+            "sample": f"grid indices [0, {n}) of the same workload ({n * pe_per_point:.3e} pixel-evals, "
+                      f"{dt:.1f} s), C oracle -O3, OpenMP {threads} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the search shards by grid-index ranges across GPUs by design, but only the one-GPU path is built)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from cugridsearch_amd import _lib as L
+    from cugridsearch_amd import api as A
+    from cugridsearch_amd import synth
+
+    m, f, start, end, count, spec = synth.workload(args.workload)
+    fn = {"ncc": A.calcNCC, "sad": A.centredAbsoluteDifference, "mi": A.calcMIstream}[spec["cost"]]
+    fn = fn.with_(sampler=L.SAMPLER_NEAREST if spec["sampler"] == "nearest" else L.SAMPLER_BILINEAR,
+                  bins=spec.get("bins", 32))
+    dev = torch.device("cuda:0")
+    grid = A.CudaGrid(8)
+    grid.setStartPoint(start)
+    grid.setEndPoint(end)
+    grid.setNumSamples(count)
+    res = A.CudaTensor(grid.getAxisSampleCounts(), device=dev)
+    gm, gf = A.CudaImage.from_numpy(m, dev), A.CudaImage.from_numpy(f, dev)
+    n = grid.numElements()
+    pe_per_point = f.shape[0] * f.shape[1] * (f.shape[2] if f.ndim == 3 else 1)
+    pe = n * pe_per_point
+
+    stream = torch.cuda.Stream()
+    opts = A.make_opts(fn)
+    opts.flags = L.FLAG_TIME_KERNEL
+    import ctypes
+    out = L.Argmin()
+
+    def step():
+        L.check(L.lib().cgs_search(ctypes.byref(grid._g), ctypes.byref(gm._img), ctypes.byref(gf._img),
+                                   ctypes.byref(opts), ctypes.c_void_p(res.data.data_ptr()), ctypes.byref(out),
+                                   ctypes.c_void_p(stream.cuda_stream)))
+        return L.lib().cgs_last_kernel_ms()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kms.append(step())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1000.0 / args.steps
+    kernel_ms = float(np.mean(kms))
+
+    matched = None
+    gold_path = os.path.join(ROOT, "tests", "golden", "workloads.json")
+    if os.path.exists(gold_path):
+        gold = json.load(open(gold_path)).get(args.workload)
+        if gold:
+            rtol = 0.0 if spec["sampler"] == "nearest" else 1e-5
+            matched = bool(out.index == gold["argmin_index"] or
+                           abs(out.value - gold["argmin_value"]) <= rtol * abs(gold["argmin_value"]))
+
+    flop = pe * FLOP_PER_PE[(spec["sampler"], spec["cost"])]
+    achieved = flop / (kernel_ms / 1000.0) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"search_{args.workload}_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": pe / (ms_per_step / 1000.0) / 1e9,
+        "unit": "Gpixel-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "image": f"{f.shape[1]}x{f.shape[0]} {f.dtype}",
+                   "grid_points": n, "levels": spec["levels"], "cost": spec["cost"], "sampler": spec["sampler"],
+                   "pixel_evals_per_step": pe},
+        "pairs_per_s": 1000.0 / ms_per_step,
+        "argmin_matched": matched,
+        "argmin": {"index": out.index, "value": out.value},
+        "kernel_ms": kernel_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_VALU_TFLOPS, "traffic": traffic,
+                     "flop_per_pixel_eval": FLOP_PER_PE[(spec["sampler"], spec["cost"])]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(m, f, spec, start, end, count, pe_per_point, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,97 @@
+"""ctypes binding of libcgs.so (the C-ABI declared in include/cgs.h).
+
+The shared library is the product: nothing here computes a search result.
+If libcgs.so is missing the import of any compute entry point fails loudly.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcgs.so")
+
+CGS_OK = 0
+STATUS_NAMES = {0: "CGS_OK", 1: "CGS_ERR_INVALID_ARGUMENT", 2: "CGS_ERR_GRID_TOO_LARGE", 3: "CGS_ERR_UNSUPPORTED",
+                4: "CGS_ERR_HIP", 5: "CGS_ERR_NO_DEVICE", 6: "CGS_ERR_TOO_LARGE"}
+
+COST_SAD, COST_NCC, COST_MI, COST_SQD, COST_SAD_SUM, COST_MI_PLAIN = range(6)
+SAMPLER_NEAREST, SAMPLER_BILINEAR = 0, 1
+PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION = range(4)
+TIE_LOWEST, TIE_REFERENCE = 0, 1
+PIX_U8, PIX_F32 = 0, 1
+MAX_DIM = 8
+FLAG_TIME_KERNEL = 1
+
+
+class CgsError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("start", ctypes.c_float * 8), ("end", ctypes.c_float * 8),
+                ("count", ctypes.c_float * 8), ("resolution", ctypes.c_float * 8),
+                ("increments", ctypes.c_int32 * 8), ("num_elements", ctypes.c_int64)]
+
+
+class Image(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("channels", ctypes.c_int32), ("pixtype", ctypes.c_int32)]
+
+
+class SearchOpts(ctypes.Structure):
+    _fields_ = [("cost", ctypes.c_int32), ("sampler", ctypes.c_int32), ("param_mode", ctypes.c_int32),
+                ("tie_mode", ctypes.c_int32), ("mi_bins", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class Argmin(ctypes.Structure):
+    _fields_ = [("value", ctypes.c_float), ("index", ctypes.c_int32)]
+
+
+# name -> (restype, argtypes); every symbol include/cgs.h declares.
+_P = ctypes.POINTER
+SIGNATURES = {
+    "cgs_version": (ctypes.c_char_p, []),
+    "cgs_default_opts": (None, [_P(SearchOpts)]),
+    "cgs_grid_init": (ctypes.c_int, [_P(Grid), ctypes.c_int32, _P(ctypes.c_float), _P(ctypes.c_float),
+                                     _P(ctypes.c_float)]),
+    "cgs_grid_point": (ctypes.c_int, [_P(Grid), ctypes.c_int64, _P(ctypes.c_float)]),
+    "cgs_search": (ctypes.c_int, [_P(Grid), _P(Image), _P(Image), _P(SearchOpts), ctypes.c_void_p, _P(Argmin),
+                                  ctypes.c_void_p]),
+    "cgs_find_extrema": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _P(Argmin),
+                                        ctypes.c_void_p]),
+    "cgs_eval_point": (ctypes.c_int, [_P(Image), _P(Image), _P(SearchOpts), ctypes.c_int32, _P(ctypes.c_float),
+                                      _P(ctypes.c_float), _P(ctypes.c_uint32), ctypes.c_void_p]),
+    "cgs_last_kernel_ms": (ctypes.c_float, []),
+    "cgs_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libcgs.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc for gfx950) first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != CGS_OK:
+        raise CgsError(status, lib().cgs_last_error().decode())
+    return status
+
+
+def floats(vals, n=None):
+    vals = list(vals)
+    if n is not None:
+        vals = vals + [0.0] * (n - len(vals))
+    return (ctypes.c_float * len(vals))(*vals)

@@ -1,0 +1,255 @@
+"""Host-side mirror of the reference's search API, over the C-ABI.
+
+Names, argument meaning and call order follow the reference so that a driver
+reads the same (compare src/gpu/cudaGridSearch_main_mi.cu:300-345):
+
+    grid = CudaGrid(8)
+    grid.setStartPoint(start); grid.setEndPoint(end); grid.setNumSamples(counts)
+    func_values = CudaTensor(grid.getAxisSampleCounts())
+    searcher = CudaGridSearcher(grid, func_values)
+    searcher.search_by_value(calcNCC, image_mov, image_fix)
+    value, index = func_values.find_extrema()
+    point = grid.getGridPoint(index)
+
+Device memory is held in torch tensors (allocation and streams only); all
+compute runs in libcgs.so.  Functors are tags naming a cost of the C-ABI
+(the reference binds a __device__ function pointer instead,
+src/gpu/cudaGridSearch.cuh:284-285).
+"""
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+# ----------------------------------------------------------------- functors
+
+@dataclasses.dataclass(frozen=True)
+class Functor:
+    """A cost functor: which cost, how the grid point becomes a warp, and
+    which CudaImage sampler reads the moving image."""
+    name: str
+    cost: int
+    param_mode: int
+    sampler: int = L.SAMPLER_BILINEAR
+    bins: int = 64
+
+    def with_(self, **kw):
+        return dataclasses.replace(self, **kw)
+
+
+# src/gpu/cudaErrorFunctions.cuh:31-87 (non-centred H, valueAt = bilinear)
+averageAbsoluteDifference = Functor("averageAbsoluteDifference", L.COST_SAD, L.PARAM_NONCENTRED)
+# src/gpu/cudaErrorFunctions.cuh:194-224 (raw forward H)
+averageAbsoluteDifferenceH = Functor("averageAbsoluteDifferenceH", L.COST_SAD, L.PARAM_RAW_H)
+# src/gpu/cudaErrorFunctions.cuh:89-114 (2-D translation)
+sumOfAbsoluteDifferences = Functor("sumOfAbsoluteDifferences", L.COST_SAD_SUM, L.PARAM_TRANSLATION)
+# src/gpu/cudaErrorFunctions.cuh:378-446
+calcSQD = Functor("calcSQD", L.COST_SQD, L.PARAM_CENTRED)
+# src/gpu/cudaErrorFunctions.cuh:448-507
+calcNCC = Functor("calcNCC", L.COST_NCC, L.PARAM_CENTRED)
+# src/gpu/cudaErrorFunction_miStreams.cuh:212-368 (64 bins)
+calcMIstream = Functor("calcMIstream", L.COST_MI, L.PARAM_CENTRED, bins=64)
+# src/gpu/cudaErrorFunction_mi.cuh:8-115
+calcMI = Functor("calcMI", L.COST_MI_PLAIN, L.PARAM_CENTRED, bins=64)
+# the centred-homography SAD used by the search workloads
+centredAbsoluteDifference = Functor("centredAbsoluteDifference", L.COST_SAD, L.PARAM_CENTRED)
+
+
+# --------------------------------------------------------------- containers
+
+class CudaGrid:
+    """CudaGrid<float, D> (src/gpu/cudaGridSearch.cuh:74-231)."""
+
+    def __init__(self, dim=8):
+        if not 1 <= dim <= L.MAX_DIM:
+            raise ValueError("grid dimension must be 1..8")
+        self._dim = dim
+        self._start = [0.0] * dim
+        self._end = [0.0] * dim
+        self._count = [1.0] * dim
+        self._g = L.Grid()
+        self.update()
+
+    def update(self):
+        L.check(L.lib().cgs_grid_init(ctypes.byref(self._g), self._dim, L.floats(self._start),
+                                      L.floats(self._end), L.floats(self._count)))
+
+    def setStartPoint(self, v):
+        self._start = [float(x) for x in v]
+        self.update()
+
+    def setEndPoint(self, v):
+        self._end = [float(x) for x in v]
+        self.update()
+
+    def setNumSamples(self, v):
+        self._count = [float(x) for x in v]
+        self.update()
+
+    def getDimension(self):
+        return self._dim
+
+    def numElements(self):
+        return int(self._g.num_elements)
+
+    def getAxisSampleCounts(self):
+        return np.array(self._g.count[:self._dim], np.float32)
+
+    def start_point(self):
+        return np.array(self._g.start[:self._dim], np.float32)
+
+    def end_point(self):
+        return np.array(self._g.end[:self._dim], np.float32)
+
+    def resolution(self):
+        return np.array(self._g.resolution[:self._dim], np.float32)
+
+    def getGridPoint(self, index):
+        out = (ctypes.c_float * self._dim)()
+        L.check(L.lib().cgs_grid_point(ctypes.byref(self._g), int(index), out))
+        return np.array(out[:], np.float32)
+
+    def display(self, name=""):
+        """CudaMatrix::display format (include/cudaTensor.cuh:395-415): the four
+        rows start / end / resolution / count."""
+        rows = [self.start_point(), self.end_point(), self.resolution(), self.getAxisSampleCounts()]
+        lines = [f"Matrix {name} 4 x {self._dim} elements of f", ""]
+        for r in rows:
+            lines.append("{ " + ", ".join(f"{v:g}" for v in r) + " }")
+        text = "\n".join(lines) + "\n"
+        print(text)
+        return text
+
+
+class CudaTensor:
+    """CudaTensor<float, D> (include/cudaTensor.cuh:108-361) built from the
+    grid's axis counts: element order = grid index order (axis 0 fastest)."""
+
+    def __init__(self, dims, device="cuda", data=None):
+        torch = _torch()
+        self.dims = [int(d) for d in dims]
+        n = 1
+        for d in self.dims:
+            n *= d
+        self._total = n
+        self.data = data if data is not None else torch.empty(n, dtype=torch.float32, device=device)
+        self.last_argmin = None
+
+    def size(self, dim=-1):
+        return self._total if dim == -1 else self.dims[dim]
+
+    def bytesSize(self):
+        return self._total * 4
+
+    def find_extrema(self, tie_mode=L.TIE_LOWEST, stream=None):
+        """Returns (min value, int32 index) -- include/cudaTensor.cuh:215-265."""
+        out = L.Argmin()
+        L.check(L.lib().cgs_find_extrema(ctypes.c_void_p(self.data.data_ptr()), self._total, tie_mode,
+                                         ctypes.byref(out), _stream_ptr(stream)))
+        return out.value, out.index
+
+
+class CudaImage:
+    """CudaImage<pix, C>(H, W) (include/cudaTensor.cuh:507-596) over a device
+    tensor of shape (H, W) or (H, W, C), uint8 or float32."""
+
+    def __init__(self, tensor):
+        torch = _torch()
+        if tensor.dtype not in (torch.uint8, torch.float32):
+            raise TypeError("CudaImage holds uint8 or float32 pixels")
+        self.tensor = tensor.contiguous()
+        h, w = self.tensor.shape[:2]
+        c = self.tensor.shape[2] if self.tensor.dim() == 3 else 1
+        self._img = L.Image(self.tensor.data_ptr(), w, h, c,
+                            L.PIX_U8 if tensor.dtype == torch.uint8 else L.PIX_F32)
+
+    @classmethod
+    def from_numpy(cls, arr, device="cuda"):
+        return cls(_torch().from_numpy(np.ascontiguousarray(arr)).to(device))
+
+    def width(self):
+        return self._img.width
+
+    def height(self):
+        return self._img.height
+
+    def channels(self):
+        return self._img.channels
+
+
+def make_opts(functor, tie_mode=L.TIE_LOWEST):
+    o = L.SearchOpts()
+    L.lib().cgs_default_opts(ctypes.byref(o))
+    o.cost, o.sampler, o.param_mode, o.mi_bins = functor.cost, functor.sampler, functor.param_mode, functor.bins
+    o.tie_mode = tie_mode
+    return o
+
+
+class CudaGridSearcher:
+    """CudaGridSearcher<float, float, D> (src/gpu/cudaGridSearch.cuh:540-855)."""
+
+    def __init__(self, grid, result):
+        self._grid = grid
+        self._result = result
+
+    def search(self, functor, img_moved, img_fixed, **kw):
+        return self.search_by_value(functor, img_moved, img_fixed, **kw)
+
+    def search_by_value(self, functor, img_moved, img_fixed, tie_mode=L.TIE_LOWEST, stream=None):
+        """Fills the result tensor with the functor's cost at every grid point
+        and returns the fused argmin (value, index)."""
+        if self._result.size() != self._grid.numElements():
+            raise ValueError("result tensor size differs from the grid's point count")
+        o = make_opts(functor, tie_mode)
+        out = L.Argmin()
+        L.check(L.lib().cgs_search(ctypes.byref(self._grid._g), ctypes.byref(img_moved._img),
+                                   ctypes.byref(img_fixed._img), ctypes.byref(o),
+                                   ctypes.c_void_p(self._result.data.data_ptr()), ctypes.byref(out),
+                                   _stream_ptr(stream)))
+        self._result.last_argmin = (out.value, out.index)
+        return out.value, out.index
+
+    # The reference's launch-policy variants compute the same costs; here they
+    # all run the fused kernel.  STREAM_BLOCK_DIM / numCores / numThreads are
+    # accepted for signature compatibility
+    # (src/gpu/cudaGridSearch.cuh:562-649, 796-827).
+    def search_by_value_stream(self, functor, STREAM_BLOCK_DIM, numCores, img_moved, img_fixed, **kw):
+        return self.search_by_value(functor, img_moved, img_fixed, **kw)
+
+    def search_by_value_block(self, functor, numThreads, img_moved, img_fixed, **kw):
+        return self.search_by_value(functor, img_moved, img_fixed, **kw)
+
+    def search_by_reference(self, functor, img_moved, img_fixed, **kw):
+        return self.search_by_value(functor, img_moved, img_fixed, **kw)
+
+    def search_by_reference_stream(self, functor, STREAM_BLOCK_DIM, numCores, img_moved, img_fixed, **kw):
+        return self.search_by_value(functor, img_moved, img_fixed, **kw)
+
+
+def eval_point(functor, img_moved, img_fixed, params, stream=None):
+    """One functor evaluation at params; returns (cost, histogram or None)."""
+    o = make_opts(functor)
+    cost = ctypes.c_float()
+    hist = None
+    hp = None
+    if functor.cost in (L.COST_MI, L.COST_MI_PLAIN):
+        hist = np.zeros(functor.bins * functor.bins, np.uint32)
+        hp = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    p = [float(v) for v in params]
+    L.check(L.lib().cgs_eval_point(ctypes.byref(img_moved._img), ctypes.byref(img_fixed._img), ctypes.byref(o),
+                                   len(p), L.floats(p), ctypes.byref(cost), hp, _stream_ptr(stream)))
+    return cost.value, (hist.reshape(functor.bins, functor.bins) if hist is not None else None)

@@ -1,0 +1,61 @@
+// cgs_host.hpp -- host-side helpers shared by the C-ABI translation units:
+// argument validation, the per-device workspace, error reporting and the
+// host restatement of CudaGrid's bookkeeping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "cgs.h"
+#include "cgs_device.hpp"
+
+namespace cgs {
+
+cgs_status set_error(cgs_status st, const std::string &msg);
+void clear_error();
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return ::cgs::set_error(CGS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// Device memory the library owns for one GPU: the zero-padded moving image,
+// the (cos, sin) table, the argmin key and small pinned host mirrors.  Grows
+// on demand and lives until the process ends.  Calls on one device
+// serialise on mu.
+struct Workspace {
+    std::mutex mu;
+    void *mov = nullptr;
+    size_t mov_cap = 0;
+    float *sincos = nullptr;
+    float *sincos_host = nullptr;
+    size_t sincos_cap = 0;
+    uint32_t *hist = nullptr;
+    size_t hist_cap = 0;
+    unsigned long long *key = nullptr;
+    unsigned long long *key_host = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    cgs_status reserve(size_t mov_bytes, size_t sincos_levels, size_t hist_words);
+    // HIP events around one launch on the caller's stream; elapsed_ms()
+    // stores the duration for cgs_last_kernel_ms() (after a stream sync).
+    hipError_t record_start(hipStream_t s);
+    hipError_t record_stop(hipStream_t s);
+    hipError_t elapsed_ms();
+};
+
+void set_last_kernel_ms(float ms);
+
+cgs_status workspace_for_current_device(Workspace **out);
+
+cgs_status validate_image(const cgs_image *im, const char *name);
+cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
+                           const cgs_search_opts *o);
+void fill_grid_dev(const cgs_grid *grid, GridDev *g);
+cgs_argmin decode_key(unsigned long long key, int tie_mode);
+uint32_t reference_rank_inverse(uint32_t rank);
+
+}  // namespace cgs

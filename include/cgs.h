@@ -1,0 +1,167 @@
+/*
+ * cgs.h -- C-ABI of the MI355X (gfx950) brute-force homography grid search.
+ *
+ * Plain C: pointers, sizes, status codes.  Device buffers are caller-owned
+ * (as in the reference, where the caller cudaMalloc's every CudaTensor /
+ * CudaImage / CudaGrid); the library never frees a caller pointer and never
+ * calls exit().  Streams are passed as void* (a hipStream_t, or NULL for the
+ * null stream).
+ *
+ * Each entry point names the reference interface it replaces (file:line,
+ * relative to the uncc-visionlab/cuGridSearch source root).
+ */
+#ifndef CGS_H
+#define CGS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGS_MAX_DIM 8
+
+typedef enum {
+    CGS_OK = 0,
+    CGS_ERR_INVALID_ARGUMENT = 1,   /* null pointer, bad dimension, bad enum */
+    CGS_ERR_GRID_TOO_LARGE = 2,     /* point count leaves the int32 range */
+    CGS_ERR_UNSUPPORTED = 3,        /* combination not implemented */
+    CGS_ERR_HIP = 4,                /* a HIP runtime call failed */
+    CGS_ERR_NO_DEVICE = 5,
+    CGS_ERR_TOO_LARGE = 6           /* a size beyond a stated maximum */
+} cgs_status;
+
+/* Cost functors of the reference (by-value device functions bound through a
+ * __device__ function pointer, src/gpu/cudaGridSearch.cuh:284-285). */
+typedef enum {
+    CGS_COST_SAD = 0,       /* averageAbsoluteDifference  src/gpu/cudaErrorFunctions.cuh:31-87  */
+    CGS_COST_NCC = 1,       /* calcNCC (-uncentred NCC)   src/gpu/cudaErrorFunctions.cuh:448-507 */
+    CGS_COST_MI = 2,        /* calcMIstream (-2MI/(Hx+Hy)) src/gpu/cudaErrorFunction_miStreams.cuh:212-368 */
+    CGS_COST_SQD = 3,       /* calcSQD                    src/gpu/cudaErrorFunctions.cuh:378-446 */
+    CGS_COST_SAD_SUM = 4,   /* sumOfAbsoluteDifferences   src/gpu/cudaErrorFunctions.cuh:89-114  */
+    CGS_COST_MI_PLAIN = 5   /* calcMI (-MI)               src/gpu/cudaErrorFunction_mi.cuh:8-115  */
+} cgs_cost;
+
+/* CudaImage samplers (include/cudaTensor.cuh:557-575). */
+typedef enum {
+    CGS_SAMPLER_NEAREST = 0,   /* valueAt_nearest_neighbor, truncated float linear index */
+    CGS_SAMPLER_BILINEAR = 1   /* valueAt_bilinear (= valueAt) */
+} cgs_sampler;
+
+/* How the grid point becomes the warp. */
+typedef enum {
+    CGS_PARAM_CENTRED = 0,      /* parametersToHomography, cx=W_m/2, cy=H_m/2
+                                   src/gpu/cudaImageFunctions.cuh:216-252, inverse warp :277-285 */
+    CGS_PARAM_NONCENTRED = 1,   /* averageAbsoluteDifference's H  src/gpu/cudaErrorFunctions.cuh:50-64 */
+    CGS_PARAM_RAW_H = 2,        /* averageAbsoluteDifferenceH, forward  src/gpu/cudaErrorFunctions.cuh:194-224 */
+    CGS_PARAM_TRANSLATION = 3   /* 2-D (tx, ty)  src/gpu/cudaErrorFunctions.cuh:89-114 */
+} cgs_param_mode;
+
+/* Argmin tie order. */
+typedef enum {
+    CGS_TIE_LOWEST = 0,      /* lowest grid index among equal minima */
+    CGS_TIE_REFERENCE = 1    /* find_extrema's order: include/cudaTensor.cuh:215-260 */
+} cgs_tie_mode;
+
+typedef enum { CGS_PIX_U8 = 0, CGS_PIX_F32 = 1 } cgs_pixtype;
+
+/* CudaGrid<float, D> (src/gpu/cudaGridSearch.cuh:74-231): start / end /
+ * resolution / sample-count rows, held on the host.  Filled by
+ * cgs_grid_init; read-only afterwards. */
+typedef struct {
+    int32_t dim;
+    float start[CGS_MAX_DIM];
+    float end[CGS_MAX_DIM];
+    float count[CGS_MAX_DIM];       /* after the 1e-7 rule of updateProcess */
+    float resolution[CGS_MAX_DIM];
+    int32_t increments[CGS_MAX_DIM];
+    int64_t num_elements;           /* numElements(), int32 semantics */
+} cgs_grid;
+
+/* CudaImage<pix, C>(H, W) view (include/cudaTensor.cuh:507-550): row-major,
+ * channels interleaved, element (y, x, c) at c + C*x + C*W*y.  data is a
+ * device pointer. */
+typedef struct {
+    const void *data;
+    int32_t width;
+    int32_t height;
+    int32_t channels;
+    int32_t pixtype;   /* cgs_pixtype */
+} cgs_image;
+
+typedef struct {
+    int32_t cost;        /* cgs_cost */
+    int32_t sampler;     /* cgs_sampler */
+    int32_t param_mode;  /* cgs_param_mode */
+    int32_t tie_mode;    /* cgs_tie_mode */
+    int32_t mi_bins;     /* joint-histogram bins per axis for the MI costs: 8..64 (reference: 64) */
+    int32_t flags;       /* CGS_FLAG_* bits */
+} cgs_search_opts;
+
+/* opts.flags: time the search kernel with HIP events on the caller's
+ * stream; read the duration with cgs_last_kernel_ms(). */
+#define CGS_FLAG_TIME_KERNEL 1
+
+/* find_extrema's outputs (include/cudaTensor.cuh:215): min value, int32
+ * index (-1 when no value is below +inf). */
+typedef struct {
+    float value;
+    int32_t index;
+} cgs_argmin;
+
+/* Library version string. */
+const char *cgs_version(void);
+
+/* Fills the options with the defaults (SAD, bilinear, centred, lowest-index
+ * ties, 32 MI bins). */
+void cgs_default_opts(cgs_search_opts *opts);
+
+/* CudaGrid::setStartPoint / setEndPoint / setNumSamples + update()
+ * (src/gpu/cudaGridSearch.cuh:198-215, 100-105, 235-272) and numElements
+ * (:89-98).  Host-only. */
+cgs_status cgs_grid_init(cgs_grid *grid, int32_t dim, const float *start, const float *end,
+                         const float *count);
+
+/* CudaGrid::indexToGridPoint / getGridPoint (src/gpu/cudaGridSearch.cuh:160-196).
+ * Host-only; out has grid->dim floats. */
+cgs_status cgs_grid_point(const cgs_grid *grid, int64_t index, float *out);
+
+/* CudaGridSearcher::search / search_by_value (src/gpu/cudaGridSearch.cuh:553-556,
+ * 736-765) with the functor chosen by opts->cost, fused with
+ * CudaTensor::find_extrema (include/cudaTensor.cuh:215-265).
+ * result: device float[num_elements] in grid-index order (axis 0 fastest), or
+ * NULL to skip writing the cost tensor.  argmin_out: host pointer, or NULL.
+ * Synchronises the stream before returning (the reference synchronises after
+ * every search). */
+cgs_status cgs_search(const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
+                      const cgs_search_opts *opts, float *result, cgs_argmin *argmin_out,
+                      void *stream);
+
+/* CudaTensor::find_extrema (include/cudaTensor.cuh:215-265) over a device
+ * float array of n values. */
+cgs_status cgs_find_extrema(const float *values, int64_t n, int32_t tie_mode, cgs_argmin *out,
+                            void *stream);
+
+/* Evaluates a single grid point and returns its cost and, for the MI costs,
+ * the bins x bins joint histogram (row = fixed-image bin, column = moving
+ * bin) in hist_out (host, may be NULL).  Counts are exact integers.
+ * Diagnostic counterpart of calling the functor once, as
+ * src/gpu/cudaGridSearch_main_mi.cu:388 does on the host. */
+cgs_status cgs_eval_point(const cgs_image *moved, const cgs_image *fixed, const cgs_search_opts *opts,
+                          int32_t dim, const float *params, float *cost_out, uint32_t *hist_out,
+                          void *stream);
+
+/* Duration in ms of the calling thread's last timed search kernel (flags &
+ * CGS_FLAG_TIME_KERNEL), measured with HIP events around that launch; -1
+ * when none was timed.  No reference counterpart (the reference prints wall
+ * time, src/gpu/cudaGridSearch.cuh:755-764). */
+float cgs_last_kernel_ms(void);
+
+/* Last error message of the calling thread ("" when none). */
+const char *cgs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CGS_H */

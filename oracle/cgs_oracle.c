@@ -1,0 +1,473 @@
+/*
+ * cgs_oracle.c -- CPU oracle for the brute-force homography grid search.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * may load it.  The product path (cugridsearch_amd/csrc) never links it.
+ *
+ * It restates, in plain C, the arithmetic of the reference search path
+ * (uncc-visionlab/cuGridSearch) with every floating-point step declared:
+ *   - compile with -ffp-contract=off: every + - * / is one IEEE-754 binary32
+ *     (or binary64 where stated) operation; FMA appears only where fmaf() is
+ *     written explicitly;
+ *   - sin/cos of the rotation angle: (float)sin((double)theta), likewise cos;
+ *   - division and sqrt: IEEE correctly rounded;
+ *   - NCC / SQD / bilinear-SAD sums: binary64 accumulation of binary32 terms;
+ *   - MI entropies: binary64 with log2 from libm.
+ *
+ * Reference citations (file:line, relative to the reference root):
+ *   grid            src/gpu/cudaGridSearch.cuh:89-98, 160-187, 235-272
+ *   homography      src/gpu/cudaImageFunctions.cuh:216-252 (centred)
+ *                   src/gpu/cudaErrorFunctions.cuh:50-57 (non-centred)
+ *                   src/gpu/cudaErrorFunctions.cuh:205-207 (raw H, forward)
+ *                   src/gpu/cudaErrorFunctions.cuh:99-100 (2-D translation)
+ *   inverse warp    src/gpu/cudaImageFunctions.cuh:277-285
+ *   image layout    include/cudaTensor.cuh:507-550, 171-180
+ *   samplers        include/cudaTensor.cuh:557-575
+ *   SAD             src/gpu/cudaErrorFunctions.cuh:31-87, 89-114
+ *   SQD             src/gpu/cudaErrorFunctions.cuh:378-446
+ *   NCC             src/gpu/cudaErrorFunctions.cuh:448-507
+ *   MI              src/gpu/cudaErrorFunction_miStreams.cuh:212-368 (normalised)
+ *                   src/gpu/cudaErrorFunction_mi.cuh:8-115 (plain -MI)
+ *   argmin          include/cudaTensor.cuh:215-265,
+ *                   include/cudaTensorKernels.cuh:133-220
+ *
+ * Declared decisions where the reference is undefined (each is pinned by a
+ * test in tests/test_oracle_*.py):
+ *   - an axis with count 1 gets resolution 0 (the reference reads
+ *     end[-1]-start[-1], out of bounds);
+ *   - grid point = fmaf(i, res, start);
+ *   - the nearest-neighbour index keeps the reference's truncated float
+ *     linear index, with fmaf for each "index += incr * coord" step;
+ *   - reads past the last pixel (bilinear ceil on the last row, the
+ *     nearest-neighbour index overflow) return 0;
+ *   - bilinear weights are evaluated unfused, as written;
+ *   - MI bins are clamped to [0, bins-1] (the reference writes out of bounds
+ *     for values outside [0, 256)).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_MAXD 8
+
+enum { ORC_COST_SAD = 0, ORC_COST_NCC = 1, ORC_COST_MI = 2, ORC_COST_SQD = 3,
+       ORC_COST_SAD_SUM = 4, ORC_COST_MI_PLAIN = 5 };
+enum { ORC_NEAREST = 0, ORC_BILINEAR = 1 };
+enum { ORC_PARAM_CENTRED = 0, ORC_PARAM_NONCENTRED = 1, ORC_PARAM_RAW_H = 2,
+       ORC_PARAM_TRANSLATION = 3 };
+enum { ORC_TIE_LOWEST = 0, ORC_TIE_REFERENCE = 1 };
+enum { ORC_PIX_U8 = 0, ORC_PIX_F32 = 1 };
+
+typedef struct {
+    const void *data;
+    int width, height, channels, pixtype;
+} orc_image;
+
+/* ------------------------------------------------------------------ grid */
+
+/* CudaGrid::update() -> updateProcess (cudaGridSearch.cuh:256-270), the
+ * changed_axis = -1 branch, which every setter uses.  count is updated in
+ * place (the 1e-7 rule forces it to 1).  The comparison is in double, as
+ * "abs(float) < 1.0e-7 * float" promotes to double. */
+void orc_grid_update(int D, const float *start, const float *end,
+                     float *count, float *res) {
+    for (int a = 0; a < D; a++) {
+        float diff = end[a] - start[a];
+        if ((double) fabsf(diff) < 1.0e-7 * (double) count[a]) count[a] = 1.0f;
+        if (count[a] == 1.0f) res[a] = 0.0f;
+        else res[a] = diff / (count[a] - 1.0f);
+    }
+}
+
+/* CudaGrid::numElements (cudaGridSearch.cuh:89-98): int32 total, each step
+ * (float)total * count truncated to int.  Returns -1 when a step leaves the
+ * int32 range (the reference overflows silently). */
+int64_t orc_grid_num_elements(int D, const float *count) {
+    int32_t total = 1;
+    for (int a = 0; a < D; a++) {
+        float p = (float) total * count[a];
+        if (!(p < 2147483648.0f) || p < 0.0f) return -1;
+        total = (int32_t) p;
+    }
+    return total;
+}
+
+/* dimensional increments (cudaGridSearch.cuh:168-172): int32 from a float
+ * product of the previous count and increment. */
+void orc_grid_increments(int D, const float *count, int32_t *incr) {
+    incr[0] = 1;
+    for (int a = 1; a < D; a++) incr[a] = (int32_t) (count[a - 1] * (float) incr[a - 1]);
+}
+
+/* indexToGridPoint (cudaGridSearch.cuh:160-187); axis 0 varies fastest. */
+void orc_grid_indices(int D, const int32_t *incr, int64_t index, int32_t *idx) {
+    int32_t rem = (int32_t) index;
+    for (int a = D - 1; a >= 0; a--) {
+        idx[a] = rem / incr[a];
+        rem -= idx[a] * incr[a];
+    }
+}
+
+void orc_grid_point(int D, const float *start, const float *res, const float *count,
+                    int64_t index, float *out) {
+    int32_t incr[ORC_MAXD], idx[ORC_MAXD];
+    orc_grid_increments(D, count, incr);
+    orc_grid_indices(D, incr, index, idx);
+    for (int a = 0; a < D; a++) out[a] = fmaf((float) idx[a], res[a], start[a]);
+}
+
+/* ------------------------------------------------------------ homography */
+
+/* parametersToHomography (cudaImageFunctions.cuh:216-252), every expression
+ * in the reference's own association order. */
+static void homography_centred(const float *p, float cx, float cy, float *h) {
+    const float theta = p[0], scaleX = p[1], scaleY = p[2], shearXY = p[3];
+    const float translateX = p[4], translateY = p[5], keystoneX = p[6], keystoneY = p[7];
+    const float c = (float) cos((double) theta);
+    const float s = (float) sin((double) theta);
+    float h11, h12, h13, h21, h22, h23, h31, h32, h33;
+    h11 = scaleX*(c*(cx*keystoneX + 1) + cx*keystoneY*s);
+    h12 = scaleY*(shearXY*(c*(cx*keystoneX + 1) + cx*keystoneY*s) - s*(cx*keystoneX + 1) + cx*keystoneY*c);
+    h13 = cx + translateX*(cx*keystoneX + 1) - cx*scaleX*(c*(cx*keystoneX + 1) + cx*keystoneY*s) - cy*scaleY*(shearXY*(c*(cx*keystoneX + 1) + cx*keystoneY*s) - s*(cx*keystoneX + 1) + cx*keystoneY*c) + cx*keystoneY*translateY;
+    h21 = scaleX*(s*(cy*keystoneY + 1) + cy*keystoneX*c);
+    h22 = scaleY*(shearXY*(s*(cy*keystoneY + 1) + cy*keystoneX*c) + c*(cy*keystoneY + 1) - cy*keystoneX*s);
+    h23 = cy + translateY*(cy*keystoneY + 1) - cx*scaleX*(s*(cy*keystoneY + 1) + cy*keystoneX*c) - cy*scaleY*(shearXY*(s*(cy*keystoneY + 1) + cy*keystoneX*c) + c*(cy*keystoneY + 1) - cy*keystoneX*s) + cy*keystoneX*translateX;
+    h31 = scaleX*(keystoneX*c + keystoneY*s);
+    h32 = scaleY*(shearXY*(keystoneX*c + keystoneY*s) + keystoneY*c - keystoneX*s);
+    h33 = keystoneX*translateX + keystoneY*translateY - cy*scaleY*(shearXY*(keystoneX*c + keystoneY*s) + keystoneY*c - keystoneX*s) - cx*scaleX*(keystoneX*c + keystoneY*s) + 1;
+    h[0] = h11 / h33; h[1] = h12 / h33; h[2] = h13 / h33;
+    h[3] = h21 / h33; h[4] = h22 / h33; h[5] = h23 / h33;
+    h[6] = h31 / h33; h[7] = h32 / h33;
+}
+
+/* averageAbsoluteDifference's inline non-centred H (cudaErrorFunctions.cuh:50-57). */
+static void homography_noncentred(const float *p, float *h) {
+    const float theta = p[0], scaleX = p[1], scaleY = p[2], shearXY = p[3];
+    const float c = (float) cos((double) theta);
+    const float s = (float) sin((double) theta);
+    h[0] = scaleX * c;
+    h[1] = scaleY * shearXY * c - scaleY * s;
+    h[2] = scaleX * p[4];
+    h[3] = scaleX * s;
+    h[4] = scaleY * shearXY * s + scaleY * c;
+    h[5] = scaleY * p[5];
+    h[6] = p[6];
+    h[7] = p[7];
+}
+
+void orc_homography(int mode, const float *params, float cx, float cy, float *h) {
+    if (mode == ORC_PARAM_CENTRED) homography_centred(params, cx, cy, h);
+    else if (mode == ORC_PARAM_NONCENTRED) homography_noncentred(params, h);
+    else for (int i = 0; i < 8; i++) h[i] = params[i];
+}
+
+/* Per-grid-point coefficients of the three projective forms evaluated per
+ * pixel as  F(x, y) = ((A*x + B*y) + P) - Q  (each op rounded), stored as
+ * coef[4*k + {0,1,2,3}] = {A, B, P, Q} for k = 0 (X), 1 (Y), 2 (W).
+ * The moving-image sample point is (X/W, Y/W).
+ *   inverse warp, calcNewCoordH (cudaImageFunctions.cuh:282-284):
+ *     W = (h21h32 - h22h31)x + (h12h31 - h11h32)y + h11h22 - h12h21
+ *     X = (h22 - h23h32)x + (h13h32 - h12)y + h12h23 - h13h22
+ *     Y = (h23h31 - h21)x + (h11 - h13h31)y + h13h21 - h11h23
+ *   forward raw H (cudaErrorFunctions.cuh:205-207):
+ *     W = h7x + h8y + 1,  X = h1x + h2y + h3,  Y = h4x + h5y + h6
+ *   translation (cudaErrorFunctions.cuh:99-100): X = x + t0, Y = y + t1, W = 1
+ */
+void orc_warp_coeffs(int mode, const float *params, float cx, float cy, float *coef) {
+    if (mode == ORC_PARAM_TRANSLATION) {
+        const float c[12] = {1.0f, 0.0f, params[0], 0.0f,
+                             0.0f, 1.0f, params[1], 0.0f,
+                             0.0f, 0.0f, 1.0f, 0.0f};
+        memcpy(coef, c, sizeof(c));
+        return;
+    }
+    float h[8];
+    orc_homography(mode, params, cx, cy, h);
+    const float h11 = h[0], h12 = h[1], h13 = h[2], h21 = h[3], h22 = h[4], h23 = h[5];
+    const float h31 = h[6], h32 = h[7];
+    if (mode == ORC_PARAM_RAW_H) {
+        const float c[12] = {h11, h12, h13, 0.0f,
+                             h21, h22, h23, 0.0f,
+                             h31, h32, 1.0f, 0.0f};
+        memcpy(coef, c, sizeof(c));
+        return;
+    }
+    coef[0] = h22 - h23 * h32;   coef[1] = h13 * h32 - h12;  coef[2] = h12 * h23;  coef[3] = h13 * h22;
+    coef[4] = h23 * h31 - h21;   coef[5] = h11 - h13 * h31;  coef[6] = h13 * h21;  coef[7] = h11 * h23;
+    coef[8] = h21 * h32 - h22 * h31; coef[9] = h12 * h31 - h11 * h32; coef[10] = h11 * h22; coef[11] = h12 * h21;
+}
+
+static inline float form(const float *c, float x, float y) {
+    return ((c[0] * x + c[1] * y) + c[2]) - c[3];
+}
+
+void orc_warp_point(const float *coef, int x, int y, float *out_xy) {
+    const float fx = (float) x, fy = (float) y;
+    const float w = form(coef + 8, fx, fy);
+    out_xy[0] = form(coef, fx, fy) / w;
+    out_xy[1] = form(coef + 4, fx, fy) / w;
+}
+
+/* -------------------------------------------------------------- sampling */
+
+static inline float pix(const orc_image *im, int64_t lin) {
+    const int64_t n = (int64_t) im->width * im->height * im->channels;
+    if (lin < 0 || lin >= n) return 0.0f;   /* declared zero pad past the end */
+    if (im->pixtype == ORC_PIX_U8) return (float) ((const uint8_t *) im->data)[lin];
+    return ((const float *) im->data)[lin];
+}
+
+/* CudaImage::inImage (cudaTensor.cuh:548-550). */
+static inline int in_image(const orc_image *im, float y, float x) {
+    return x >= 0 && y >= 0 && x < (float) im->width && y < (float) im->height;
+}
+
+/* toIndex1d<float>({c, x, y}) (cudaTensor.cuh:171-180) with increments
+ * {1, C, C*W}: int index = (int)c; index = (int)fmaf(incr, coord, (float)index). */
+static inline int64_t lin_index_f(const orc_image *im, float y, float x, int c) {
+    int32_t i = c;
+    i = (int32_t) fmaf((float) im->channels, x, (float) i);
+    i = (int32_t) fmaf((float) (im->channels * im->width), y, (float) i);
+    return i;
+}
+
+/* valueAt_nearest_neighbor (cudaTensor.cuh:557-559). */
+float orc_sample_nearest(const orc_image *im, float y, float x, int c) {
+    if (!in_image(im, y, x)) return 0.0f;
+    return pix(im, lin_index_f(im, y, x, c));
+}
+
+/* valueAt_bilinear (cudaTensor.cuh:561-575); ceil(x) == W reads the next
+ * row's first pixel through the linear index, as the reference does. */
+float orc_sample_bilinear(const orc_image *im, float y, float x, int c) {
+    if (!in_image(im, y, x)) return 0.0f;
+    const float fy = floorf(y), fx = floorf(x), cy = ceilf(y), cx = ceilf(x);
+    const float tlc = pix(im, lin_index_f(im, fy, fx, c));
+    const float trc = pix(im, lin_index_f(im, fy, cx, c));
+    const float blc = pix(im, lin_index_f(im, cy, fx, c));
+    const float brc = pix(im, lin_index_f(im, cy, cx, c));
+    const float alpha_x = x - fx;
+    const float alpha_y = y - fy;
+    const float value_top = (1.0f - alpha_x) * tlc + alpha_x * trc;
+    const float value_bottom = (1.0f - alpha_x) * blc + alpha_x * brc;
+    return (1.0f - alpha_y) * value_top + alpha_y * value_bottom;
+}
+
+static inline float sample(const orc_image *im, float y, float x, int c, int sampler) {
+    return sampler == ORC_NEAREST ? orc_sample_nearest(im, y, x, c)
+                                  : orc_sample_bilinear(im, y, x, c);
+}
+
+/* ----------------------------------------------------------------- costs */
+
+typedef struct {
+    double sum, i1, i2, ic;     /* SAD sum | NCC/SQD sums (see orc_cost) */
+    int64_t count;              /* in-image (pixel, channel) samples */
+    int64_t isum;               /* exact integer SAD sum (nearest, u8) */
+} orc_acc;
+
+/* MI bin of an intensity: floor(bins * (v / 256.0f)), clamped. */
+static inline int mi_bin(float v, int bins) {
+    float t = floorf((float) bins * (v / 256.0f));
+    int b = (int) t;
+    if (!(t >= 0.0f)) b = 0;
+    if (b > bins - 1) b = bins - 1;
+    return b;
+}
+
+/* Evaluates one grid point.  hist (bins*bins uint32, row = fixed bin,
+ * column = moving bin) is filled for the MI costs when non-NULL or needed.
+ * Returns the cost as the product writes it (binary32). */
+float orc_cost_point(const orc_image *moved, const orc_image *fixed, int cost, int sampler,
+                     int mode, const float *params, int bins, uint32_t *hist_out,
+                     double *aux_out) {
+    float coef[12];
+    const float cx = (float) moved->width / 2, cy = (float) moved->height / 2;
+    orc_warp_coeffs(mode, params, cx, cy, coef);
+    const int C = fixed->channels;
+    const int is_mi = (cost == ORC_COST_MI || cost == ORC_COST_MI_PLAIN);
+    uint32_t *hist = NULL;
+    if (is_mi) {
+        hist = hist_out ? hist_out : (uint32_t *) malloc(sizeof(uint32_t) * bins * bins);
+        memset(hist, 0, sizeof(uint32_t) * bins * bins);
+    }
+    orc_acc a = {0, 0, 0, 0, 0, 0};
+    const int exact_int = (sampler == ORC_NEAREST && moved->pixtype == ORC_PIX_U8 &&
+                           fixed->pixtype == ORC_PIX_U8);
+    for (int y = 0; y < fixed->height; y++) {
+        for (int x = 0; x < fixed->width; x++) {
+            float xy[2];
+            orc_warp_point(coef, x, y, xy);
+            if (!in_image(moved, xy[1], xy[0])) continue;
+            for (int c = 0; c < C; c++) {
+                const float m = sample(moved, xy[1], xy[0], c, sampler);
+                const float f = pix(fixed, ((int64_t) y * fixed->width + x) * C + c);
+                a.count++;
+                switch (cost) {
+                case ORC_COST_SAD: case ORC_COST_SAD_SUM:
+                    if (exact_int) a.isum += (int64_t) fabsf(m - f);
+                    else a.sum += (double) fabsf(m - f);
+                    break;
+                case ORC_COST_NCC: case ORC_COST_SQD: {
+                    const double md = (double) m / 255.0, fd = (double) f / 255.0;
+                    a.i1 += fd * fd;
+                    a.i2 += md * md;
+                    if (cost == ORC_COST_NCC) a.ic += fd * md;
+                    else a.ic += (md - fd) * (md - fd);
+                    break;
+                }
+                default:
+                    hist[mi_bin(f, bins) * bins + mi_bin(m, bins)]++;
+                }
+            }
+        }
+    }
+    float out = 0.0f;
+    switch (cost) {
+    case ORC_COST_SAD:
+        if (exact_int) out = (float) a.isum / (float) a.count;
+        else out = (float) (a.sum / (double) a.count);
+        break;
+    case ORC_COST_SAD_SUM:
+        out = exact_int ? (float) a.isum : (float) a.sum;
+        break;
+    case ORC_COST_NCC:
+        out = a.count > 0 ? (float) (-a.ic / sqrt(a.i1 * a.i2)) : 0.0f;
+        break;
+    case ORC_COST_SQD:
+        out = a.count > 0 ? (float) (a.ic / sqrt(a.i1 * a.i2)) : 123123.0f;
+        break;
+    default: {
+        /* calcMIstream (cudaErrorFunction_miStreams.cuh:306-365) in binary64 */
+        const double n = (double) a.count;
+        double mi = 0, hx = 0, hy = 0;
+        double *px = (double *) calloc(bins, sizeof(double));
+        double *py = (double *) calloc(bins, sizeof(double));
+        for (int i = 0; i < bins; i++)
+            for (int j = 0; j < bins; j++) {
+                px[i] += hist[i * bins + j];
+                py[j] += hist[i * bins + j];
+            }
+        for (int i = 0; i < bins; i++)
+            for (int j = 0; j < bins; j++) {
+                const double pxy = hist[i * bins + j] / n, pxi = px[i] / n, pyj = py[j] / n;
+                if (pxy > 0 && pxi > 0 && pyj > 0) mi += pxy * (log2(pxy) - (log2(pxi) + log2(pyj)));
+            }
+        for (int i = 0; i < bins; i++) {
+            const double pxi = px[i] / n, pyi = py[i] / n;
+            if (pxi > 0) hx -= pxi * log2(pxi);
+            if (pyi > 0) hy -= pyi * log2(pyi);
+        }
+        free(px); free(py);
+        double r = (cost == ORC_COST_MI) ? -2.0 * mi / (hx + hy) : -mi;
+        if (a.count == 0) r = (cost == ORC_COST_MI) ? NAN : -0.0;
+        out = (float) r;
+        if (aux_out) { aux_out[5] = mi; aux_out[6] = hx; aux_out[7] = hy; aux_out[8] = r; }
+    }
+    }
+    if (aux_out) {
+        aux_out[0] = exact_int ? (double) a.isum : a.sum;
+        aux_out[1] = a.i1; aux_out[2] = a.i2; aux_out[3] = a.ic; aux_out[4] = (double) a.count;
+    }
+    if (is_mi && !hist_out) free(hist);
+    return out;
+}
+
+/* Evaluates grid indices [i0, i1) into result[0 .. i1-i0).  Threads: OpenMP
+ * over grid points (nthreads <= 0 keeps the OpenMP default). */
+int orc_search(const orc_image *moved, const orc_image *fixed, int cost, int sampler, int mode,
+               int D, const float *start, const float *res, const float *count, int bins,
+               int64_t i0, int64_t i1, float *result, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    int32_t incr[ORC_MAXD];
+    orc_grid_increments(D, count, incr);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = i0; i < i1; i++) {
+        int32_t idx[ORC_MAXD];
+        float p[ORC_MAXD];
+        orc_grid_indices(D, incr, i, idx);
+        for (int a = 0; a < D; a++) p[a] = fmaf((float) idx[a], res[a], start[a]);
+        result[i - i0] = orc_cost_point(moved, fixed, cost, sampler, mode, p, bins, NULL, NULL);
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- argmin */
+
+/* Rank of an index in find_extrema's tie order (cudaTensor.cuh:215-260,
+ * cudaTensorKernels.cuh:133-220): block = idx / 1024 first, then the 9-bit
+ * thread id bit-reversed (the LDS tree resolves bits 8..5 and the shuffles
+ * bits 4..0; ties keep the lower slot), then which half of the block's 1024
+ * elements the thread read first. */
+int64_t orc_reference_rank(int64_t idx) {
+    const int64_t blk = idx / 1024, t = idx % 512, half = (idx % 1024) / 512;
+    int64_t r = 0;
+    for (int b = 0; b < 9; b++) if (t & (1 << b)) r |= (int64_t) 1 << (8 - b);
+    return blk * 1024 + r * 2 + half;
+}
+
+/* Global minimum with strict '<' (NaN and +inf never win; nothing finite or
+ * -inf -> index -1, value +inf). */
+void orc_argmin(const float *r, int64_t n, int tie_mode, float *value, int64_t *index) {
+    float best = INFINITY;
+    int64_t bi = -1, brank = INT64_MAX;
+    for (int64_t i = 0; i < n; i++) {
+        const float v = r[i];
+        if (!(v < INFINITY)) continue;
+        const int64_t rank = tie_mode == ORC_TIE_REFERENCE ? orc_reference_rank(i) : i;
+        if (v < best || (v == best && rank < brank)) { best = v; bi = i; brank = rank; }
+    }
+    *value = best;
+    *index = bi;
+}
+
+/* Literal simulation of findExtremaProcess/reduceMin6<float,512> plus the
+ * host pass (cudaTensor.cuh:215-260), used to pin orc_reference_rank. */
+void orc_argmin_reference_sim(const float *g, int64_t n, float *value, int64_t *index) {
+    const int bs = 512;
+    const int64_t nblocks = n / bs + 1;
+    float *bv = (float *) malloc(sizeof(float) * nblocks);
+    int64_t *bix = (int64_t *) malloc(sizeof(int64_t) * nblocks);
+    float sv[512];
+    int64_t si[512];
+    for (int64_t b = 0; b < nblocks; b++) {
+        for (int t = 0; t < bs; t++) {
+            float mn = INFINITY;
+            int64_t mi = -1;
+            int64_t i = b * bs * 2 + t;
+            const int64_t grid = (int64_t) bs * 2 * nblocks;
+            while (i < n) {
+                if (g[i] < mn) { mi = i; mn = g[i]; }
+                if (i + bs < n && g[i + bs] < mn) { mi = i + bs; mn = g[i + bs]; }
+                i += grid;
+            }
+            sv[t] = mn; si[t] = mi;
+        }
+        for (int off = 256; off >= 1; off /= 2)
+            for (int t = 0; t < off; t++)
+                if (sv[t + off] < sv[t]) { sv[t] = sv[t + off]; si[t] = si[t + off]; }
+        bv[b] = sv[0]; bix[b] = si[0];
+    }
+    float best = bv[0];
+    int64_t bi = bix[0];
+    for (int64_t b = 1; b < nblocks; b++)
+        if (bv[b] < best) { best = bv[b]; bi = bix[b]; }
+    *value = best; *index = bi;
+    free(bv); free(bix);
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

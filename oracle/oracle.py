@@ -1,0 +1,195 @@
+"""ctypes binding of the CPU oracle (cgs_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (cugridsearch_amd)
+never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libcgs_oracle.so")
+
+COST_SAD, COST_NCC, COST_MI, COST_SQD, COST_SAD_SUM, COST_MI_PLAIN = range(6)
+NEAREST, BILINEAR = 0, 1
+PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION = range(4)
+TIE_LOWEST, TIE_REFERENCE = 0, 1
+PIX_U8, PIX_F32 = 0, 1
+
+
+class _Image(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("channels", ctypes.c_int), ("pixtype", ctypes.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = ctypes.CDLL(_SO)
+        fp = ctypes.POINTER(ctypes.c_float)
+        i64 = ctypes.c_int64
+        L.orc_grid_update.argtypes = [ctypes.c_int, fp, fp, fp, fp]
+        L.orc_grid_num_elements.argtypes = [ctypes.c_int, fp]
+        L.orc_grid_num_elements.restype = i64
+        L.orc_grid_point.argtypes = [ctypes.c_int, fp, fp, fp, i64, fp]
+        L.orc_homography.argtypes = [ctypes.c_int, fp, ctypes.c_float, ctypes.c_float, fp]
+        L.orc_warp_coeffs.argtypes = [ctypes.c_int, fp, ctypes.c_float, ctypes.c_float, fp]
+        L.orc_warp_point.argtypes = [fp, ctypes.c_int, ctypes.c_int, fp]
+        L.orc_sample_nearest.argtypes = [ctypes.POINTER(_Image), ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.orc_sample_nearest.restype = ctypes.c_float
+        L.orc_sample_bilinear.argtypes = [ctypes.POINTER(_Image), ctypes.c_float, ctypes.c_float, ctypes.c_int]
+        L.orc_sample_bilinear.restype = ctypes.c_float
+        L.orc_cost_point.argtypes = [ctypes.POINTER(_Image), ctypes.POINTER(_Image), ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, fp, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]
+        L.orc_cost_point.restype = ctypes.c_float
+        L.orc_search.argtypes = [ctypes.POINTER(_Image), ctypes.POINTER(_Image), ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, fp, fp, fp, ctypes.c_int, i64, i64, fp,
+                                 ctypes.c_int]
+        L.orc_reference_rank.argtypes = [i64]
+        L.orc_reference_rank.restype = i64
+        L.orc_argmin.argtypes = [fp, i64, ctypes.c_int, fp, ctypes.POINTER(i64)]
+        L.orc_argmin_reference_sim.argtypes = [fp, i64, fp, ctypes.POINTER(i64)]
+        L.orc_max_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _f32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def image(arr):
+    """Wraps an HxW or HxWxC uint8/float32 numpy array (row-major, channels
+    interleaved, as CudaImage stores it)."""
+    arr = np.ascontiguousarray(arr)
+    if arr.dtype == np.uint8:
+        pt = PIX_U8
+    elif arr.dtype == np.float32:
+        pt = PIX_F32
+    else:
+        raise TypeError("oracle images are uint8 or float32")
+    h, w = arr.shape[:2]
+    c = arr.shape[2] if arr.ndim == 3 else 1
+    im = _Image(arr.ctypes.data, w, h, c, pt)
+    im._keep = arr
+    return im
+
+
+def grid_update(start, end, count):
+    s = np.ascontiguousarray(start, np.float32)
+    e = np.ascontiguousarray(end, np.float32)
+    c = np.array(count, np.float32)
+    r = np.zeros_like(c)
+    lib().orc_grid_update(len(c), _f32p(s), _f32p(e), _f32p(c), _f32p(r))
+    return c, r
+
+
+def grid_num_elements(count):
+    c = np.ascontiguousarray(count, np.float32)
+    return int(lib().orc_grid_num_elements(len(c), _f32p(c)))
+
+
+def grid_point(start, res, count, index):
+    s = np.ascontiguousarray(start, np.float32)
+    r = np.ascontiguousarray(res, np.float32)
+    c = np.ascontiguousarray(count, np.float32)
+    out = np.zeros(len(c), np.float32)
+    lib().orc_grid_point(len(c), _f32p(s), _f32p(r), _f32p(c), int(index), _f32p(out))
+    return out
+
+
+def homography(params, cx, cy, mode=PARAM_CENTRED):
+    p = np.zeros(8, np.float32)
+    p[:len(params)] = params
+    h = np.zeros(8, np.float32)
+    lib().orc_homography(mode, _f32p(p), cx, cy, _f32p(h))
+    return h
+
+
+def warp_coeffs(params, cx, cy, mode=PARAM_CENTRED):
+    p = np.zeros(8, np.float32)
+    p[:len(params)] = params
+    c = np.zeros(12, np.float32)
+    lib().orc_warp_coeffs(mode, _f32p(p), cx, cy, _f32p(c))
+    return c
+
+
+def warp_point(coef, x, y):
+    c = np.ascontiguousarray(coef, np.float32)
+    out = np.zeros(2, np.float32)
+    lib().orc_warp_point(_f32p(c), int(x), int(y), _f32p(out))
+    return out
+
+
+def sample(img, y, x, sampler, c=0):
+    im = image(img)
+    f = lib().orc_sample_nearest if sampler == NEAREST else lib().orc_sample_bilinear
+    return f(ctypes.byref(im), float(y), float(x), c)
+
+
+def cost_point(moved, fixed, cost, sampler, params, mode=PARAM_CENTRED, bins=32, want_hist=False):
+    """Returns (cost, aux) with aux = [sum, i1, i2, ic, count, mi, hx, hy, mi_cost]
+    (and the bins x bins histogram when want_hist)."""
+    m, f = image(moved), image(fixed)
+    p = np.zeros(8, np.float32)
+    p[:len(params)] = params
+    aux = np.zeros(9, np.float64)
+    hist = np.zeros(bins * bins, np.uint32) if want_hist else None
+    hp = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if want_hist else None
+    v = lib().orc_cost_point(ctypes.byref(m), ctypes.byref(f), cost, sampler, mode, _f32p(p), bins, hp,
+                             aux.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if want_hist:
+        return v, aux, hist.reshape(bins, bins)
+    return v, aux
+
+
+def search(moved, fixed, cost, sampler, start, res, count, mode=PARAM_CENTRED, bins=32,
+           i0=0, i1=None, nthreads=0):
+    m, f = image(moved), image(fixed)
+    s = np.ascontiguousarray(start, np.float32)
+    r = np.ascontiguousarray(res, np.float32)
+    c = np.ascontiguousarray(count, np.float32)
+    n = grid_num_elements(c)
+    if i1 is None:
+        i1 = n
+    out = np.zeros(i1 - i0, np.float32)
+    lib().orc_search(ctypes.byref(m), ctypes.byref(f), cost, sampler, mode, len(c), _f32p(s), _f32p(r),
+                     _f32p(c), bins, i0, i1, _f32p(out), nthreads)
+    return out
+
+
+def reference_rank(idx):
+    return int(lib().orc_reference_rank(int(idx)))
+
+
+def argmin(r, tie_mode=TIE_LOWEST):
+    a = np.ascontiguousarray(r, np.float32)
+    v = ctypes.c_float()
+    i = ctypes.c_int64()
+    lib().orc_argmin(_f32p(a), len(a), tie_mode, ctypes.byref(v), ctypes.byref(i))
+    return v.value, i.value
+
+
+def argmin_reference_sim(r):
+    a = np.ascontiguousarray(r, np.float32)
+    v = ctypes.c_float()
+    i = ctypes.c_int64()
+    lib().orc_argmin_reference_sim(_f32p(a), len(a), ctypes.byref(v), ctypes.byref(i))
+    return v.value, i.value
+
+
+def max_threads():
+    return int(lib().orc_max_threads())

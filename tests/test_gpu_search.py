@@ -1,0 +1,209 @@
+"""GPU parity of the plain search against the CPU oracle, through the C-ABI.
+
+Bars: bit-exact costs and argmin for integer nearest-neighbour SAD; MI joint
+histogram counts bit-exact; float costs (bilinear SAD / NCC / SQD, MI
+entropy) within the relative tolerances written in each test."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cugridsearch_amd import synth
+from cugridsearch_amd import api as A
+from cugridsearch_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "workloads.json")
+NCC_RTOL = 1e-5       # bilinear FP32 costs vs the FP64-accumulating oracle
+MI_RTOL = 1e-6        # MI cost: FP64 entropy on both sides, rounded to FP32
+
+
+def gpu_search(functor, moved, fixed, start, end, count, tie=L.TIE_LOWEST):
+    grid = A.CudaGrid(len(count))
+    grid.setStartPoint(start)
+    grid.setEndPoint(end)
+    grid.setNumSamples(count)
+    res = A.CudaTensor(grid.getAxisSampleCounts())
+    searcher = A.CudaGridSearcher(grid, res)
+    v, i = searcher.search_by_value(functor, A.CudaImage.from_numpy(moved), A.CudaImage.from_numpy(fixed),
+                                    tie_mode=tie)
+    return res.data.cpu().numpy(), (v, i), grid
+
+
+def oracle_search(oracle, functor, moved, fixed, start, end, count):
+    c2, r = oracle.grid_update(start, end, count)
+    return oracle.search(moved, fixed, functor.cost, functor.sampler, start, r, c2, mode=functor.param_mode,
+                         bins=functor.bins)
+
+
+def close(a, b, rtol, atol=0.0):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    ok = both_nan | (np.abs(a - b) <= rtol * np.abs(b) + atol)
+    return bool(ok.all()), np.where(~ok)[0][:10]
+
+
+def test_nn_sad_64_bit_exact(gpu, oracle):
+    m, f, s, e, c, spec = synth.workload("nn_sad_64")
+    fn = A.centredAbsoluteDifference.with_(sampler=L.SAMPLER_NEAREST)
+    got, (v, i), _ = gpu_search(fn, m, f, s, e, c)
+    want = oracle_search(oracle, fn, m, f, s, e, c)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert (v, i) == oracle.argmin(want)
+    got_r, (vr, ir), _ = gpu_search(fn, m, f, s, e, c, tie=L.TIE_REFERENCE)
+    assert (vr, ir) == oracle.argmin(want, oracle.TIE_REFERENCE)
+
+
+def test_nn_sad_ties_reference_order(gpu, oracle):
+    """Identical flat images: every in-range point costs 0, so the argmin is
+    decided by the tie order alone."""
+    img = np.full((16, 16), 9, np.uint8)
+    s, e, c = [-2, -2], [2, 2], [40, 40]
+    fn = A.sumOfAbsoluteDifferences.with_(sampler=L.SAMPLER_NEAREST)
+    got, (v, i), _ = gpu_search(fn, img, img, s, e, c, tie=L.TIE_REFERENCE)
+    want = oracle_search(oracle, fn, img, img, s, e, c)
+    assert np.array_equal(got, want)
+    assert (v, i) == oracle.argmin(want, oracle.TIE_REFERENCE) == oracle.argmin_reference_sim(want)
+    _, (v2, i2), _ = gpu_search(fn, img, img, s, e, c, tie=L.TIE_LOWEST)
+    assert (v2, i2) == oracle.argmin(want)
+
+
+def test_bilinear_ncc_64_full(gpu, oracle):
+    m, f, _ = synth.make_pair(21, 64, sigma=3.0, perturb=3.0, dtype=np.float32)
+    s, e, c = synth.grid_ranges(4, 4.0)
+    got, (v, i), _ = gpu_search(A.calcNCC, m, f, s, e, c)
+    want = oracle_search(oracle, A.calcNCC, m, f, s, e, c)
+    ok, bad = close(got, want, NCC_RTOL)
+    assert ok, (bad, got[bad], want[bad])
+    vo, io = oracle.argmin(want)
+    assert abs(v - vo) <= NCC_RTOL * abs(vo)
+    assert abs(want[i] - vo) <= 2 * NCC_RTOL * abs(vo)   # same argmin up to near-ties
+
+
+@pytest.mark.parametrize("name", ["bilinear_ncc_256", "bilinear_mi32_512"])
+def test_full_workload_sample_and_golden_argmin(gpu, oracle, name):
+    m, f, s, e, c, spec = synth.workload(name)
+    fn = A.calcNCC if spec["cost"] == "ncc" else A.calcMIstream.with_(bins=spec["bins"])
+    got, (v, i), grid = gpu_search(fn, m, f, s, e, c)
+    n = grid.numElements()
+    assert got.shape == (n,)
+    rng = np.random.default_rng(5)
+    idx = rng.choice(n, 64, replace=False)
+    rtol = NCC_RTOL if spec["cost"] == "ncc" else MI_RTOL
+    for k in idx:
+        want, _ = oracle.cost_point(m, f, fn.cost, fn.sampler, grid.getGridPoint(k), bins=fn.bins)
+        ok, _ = close([got[k]], [want], rtol)
+        assert ok, (k, got[k], want)
+    if os.path.exists(GOLDEN):
+        gold = json.load(open(GOLDEN)).get(name)
+        if gold:
+            vo = gold["argmin_value"]
+            assert abs(v - vo) <= rtol * abs(vo), (v, vo)
+            assert abs(got[gold["argmin_index"]] - vo) <= rtol * abs(vo)
+            ov, _ = oracle.cost_point(m, f, fn.cost, fn.sampler, grid.getGridPoint(i), bins=fn.bins)
+            assert abs(ov - vo) <= 2 * rtol * abs(vo)
+
+
+def test_mi32_full_and_histograms(gpu, oracle):
+    m, f, _ = synth.make_pair(31, 80, sigma=3.0, perturb=4.0, dtype=np.uint8, gamma=True, noise=4.0)
+    s, e, c = synth.grid_ranges(3, 4.0)
+    fn = A.calcMIstream.with_(bins=32)
+    got, (v, i), grid = gpu_search(fn, m, f, s, e, c)
+    want = oracle_search(oracle, fn, m, f, s, e, c)
+    ok, bad = close(got, want, MI_RTOL)
+    assert ok, (bad, got[bad], want[bad])
+    vo, io = oracle.argmin(want)
+    assert abs(want[i] - vo) <= 2 * MI_RTOL * abs(vo)
+    gm, gf = A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f)
+    rng = np.random.default_rng(3)
+    for k in list(rng.choice(grid.numElements(), 12, replace=False)) + [i]:
+        p = grid.getGridPoint(k)
+        for bins in (32, 64):
+            fb = fn.with_(bins=bins)
+            cost, hist = A.eval_point(fb, gm, gf, p)
+            oc, _, oh = oracle.cost_point(m, f, fb.cost, fb.sampler, p, bins=bins, want_hist=True)
+            assert np.array_equal(hist, oh), (k, bins)
+            assert close([cost], [oc], MI_RTOL)[0]
+
+
+CASES = [
+    ("sad_noncentred_bilinear_u8", A.averageAbsoluteDifference, np.uint8, 1, 1e-6),
+    ("sad_rawH_bilinear_f32", A.averageAbsoluteDifferenceH, np.float32, 1, 1e-6),
+    ("sqd_centred_bilinear_u8", A.calcSQD, np.uint8, 1, NCC_RTOL),
+    ("ncc_centred_nearest_u8", A.calcNCC.with_(sampler=L.SAMPLER_NEAREST), np.uint8, 1, NCC_RTOL),
+    ("mi_plain_64_nearest", A.calcMI.with_(sampler=L.SAMPLER_NEAREST), np.uint8, 1, MI_RTOL),
+    ("sad_centred_nearest_rgb", A.centredAbsoluteDifference.with_(sampler=L.SAMPLER_NEAREST), np.uint8, 3, 0.0),
+    ("ncc_centred_bilinear_rgb_f32", A.calcNCC, np.float32, 3, NCC_RTOL),
+    ("mi_centred_bilinear_rgb", A.calcMIstream.with_(bins=16), np.uint8, 3, MI_RTOL),
+]
+
+
+@pytest.mark.parametrize("name,fn,dtype,channels,rtol", CASES, ids=[c[0] for c in CASES])
+def test_functor_modes(gpu, oracle, name, fn, dtype, channels, rtol):
+    rng = np.random.default_rng(hash(name) % 2 ** 32)
+    if channels == 1:
+        m, f, _ = synth.make_pair(int(rng.integers(1 << 30)), 40, 36, sigma=2.0, perturb=3.0, dtype=dtype)
+    else:
+        planes = [synth.make_pair(int(rng.integers(1 << 30)), 40, 36, sigma=2.0, perturb=3.0, dtype=dtype)
+                  for _ in range(channels)]
+        m = np.stack([p[0] for p in planes], -1)
+        f = np.stack([p[1] for p in planes], -1)
+    if fn.param_mode == L.PARAM_RAW_H:
+        s = [0.97, -0.03, -2.0, -0.03, 0.97, -2.0, -4e-4, -4e-4]
+        e = [1.03, 0.03, 2.0, 0.03, 1.03, 2.0, 4e-4, 4e-4]
+    elif fn.param_mode == L.PARAM_NONCENTRED:
+        s = [-0.05, 0.97, 0.97, -0.03, -2.0, -2.0, -2e-4, -2e-4]
+        e = [0.05, 1.03, 1.03, 0.03, 2.0, 2.0, 2e-4, 2e-4]
+    else:
+        s, e, _ = synth.grid_ranges(3, 3.0)
+    c = [3] * 8
+    got, (v, i), _ = gpu_search(fn, m, f, s, e, c)
+    want = oracle_search(oracle, fn, m, f, s, e, c)
+    if rtol == 0.0:
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        assert (v, i) == oracle.argmin(want)
+    else:
+        ok, bad = close(got, want, rtol, atol=1e-30)
+        assert ok, (bad, got[bad], want[bad])
+        vo, _ = oracle.argmin(want)
+        assert abs(want[i] - vo) <= 2 * rtol * abs(vo)
+
+
+def test_driver_images_translation(gpu, oracle):
+    d = json.load(open(os.path.join(HERE, "golden", "driver_images.json")))
+    Aimg = np.array(d["imageA"], np.uint8).reshape(6, 6) * 255
+    Cimg = np.array(d["imageC"], np.uint8).reshape(6, 6) * 255
+    for fn in (A.sumOfAbsoluteDifferences.with_(sampler=L.SAMPLER_NEAREST), A.sumOfAbsoluteDifferences):
+        got, (v, i), _ = gpu_search(fn, Aimg, Cimg, [-5, -5], [5, 5], [21, 21])
+        want = oracle_search(oracle, fn, Aimg, Cimg, [-5, -5], [5, 5], [21, 21])
+        assert np.allclose(got, want, rtol=1e-6, atol=1e-3)
+        assert oracle.argmin(want)[1] == i or abs(want[i] - oracle.argmin(want)[0]) < 1e-3
+
+
+def test_find_extrema_matches_oracle(gpu, oracle):
+    import torch
+    rng = np.random.default_rng(2)
+    for n in [1, 7, 512, 1025, 100000]:
+        v = rng.integers(0, 5, n).astype(np.float32)
+        v[rng.integers(0, n, max(1, n // 20))] = np.nan
+        t = A.CudaTensor([n], data=torch.from_numpy(v).cuda())
+        assert t.find_extrema(L.TIE_LOWEST) == oracle.argmin(v)
+        assert t.find_extrema(L.TIE_REFERENCE) == oracle.argmin(v, oracle.TIE_REFERENCE)
+    t = A.CudaTensor([3], data=torch.tensor([math.nan, math.inf, math.nan], device="cuda"))
+    assert t.find_extrema() == (math.inf, -1)
+
+
+def test_invalid_arguments_raise(gpu):
+    m = A.CudaImage.from_numpy(np.zeros((8, 8), np.uint8))
+    grid = A.CudaGrid(2)
+    grid.setNumSamples([2, 2])
+    res = A.CudaTensor(grid.getAxisSampleCounts())
+    with pytest.raises(L.CgsError):
+        A.CudaGridSearcher(grid, res).search_by_value(A.calcNCC, m, m)   # 2-D grid, 8-D functor
+    with pytest.raises(L.CgsError):
+        A.CudaGridSearcher(grid, res).search_by_value(A.calcMIstream.with_(bins=100), m, m)

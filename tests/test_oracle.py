@@ -1,0 +1,298 @@
+"""Pins the CPU oracle against closed forms, hand-derived known answers, brute
+force on tiny inputs and an independent float64 numpy restatement.  Runs on
+the CPU (no GPU, no libcgs.so needed)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cugridsearch_amd import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER_IMAGES = json.load(open(os.path.join(HERE, "golden", "driver_images.json")))
+
+
+# ----------------------------------------------------------------- grid
+
+def test_grid_resolution_closed_form(oracle):
+    s, e, c = synth.grid_ranges(6, 8.0)
+    c2, r = oracle.grid_update(s, e, c)
+    assert np.array_equal(c2, c)
+    expect = (e - s) / (c - np.float32(1))          # binary32, as updateProcess
+    assert np.array_equal(r, expect.astype(np.float32))
+    assert oracle.grid_num_elements(c2) == 6 ** 8
+
+
+def test_grid_decode_axis0_fastest(oracle):
+    counts = np.array([3, 4, 2, 5], np.float32)
+    start = np.array([0, 10, 20, 30], np.float32)
+    end = np.array([2, 13, 21, 34], np.float32)
+    c2, r = oracle.grid_update(start, end, counts)
+    n = oracle.grid_num_elements(c2)
+    assert n == 120
+    for idx in range(n):
+        ii = np.unravel_index(idx, [3, 4, 2, 5], order="F")   # axis 0 fastest
+        p = oracle.grid_point(start, r, c2, idx)
+        want = start + np.array(ii, np.float32) * r          # integer-step grid: exact
+        assert np.array_equal(p, want), (idx, p, want)
+
+
+def test_grid_degenerate_axes(oracle):
+    # count 1 -> resolution 0 (declared; the reference reads end[-1]-start[-1])
+    c2, r = oracle.grid_update([1.0, 0.0], [5.0, 0.0], [1.0, 7.0])
+    assert r[0] == 0.0 and c2[0] == 1.0
+    # |end-start| < 1e-7 * count collapses the axis to one sample
+    assert c2[1] == 1.0 and r[1] == 0.0
+    p = oracle.grid_point([1.0, 0.0], r, c2, 0)
+    assert p[0] == 1.0
+
+
+def test_grid_fractional_counts_truncate(oracle):
+    # numElements truncates (float)total * count at every step
+    assert oracle.grid_num_elements([2.5, 3.0]) == 6   # (int)2.5 = 2, (int)(2*3) = 6
+    assert oracle.grid_num_elements([3.0, 2.5]) == 7   # (int)(3*2.5) = 7
+    assert oracle.grid_num_elements([65536.0, 65536.0]) == -1   # beyond int32
+
+
+# ----------------------------------------------------------- homography
+
+def _centred_fp64(p, cx, cy):
+    th, sx, sy, sh, tx, ty, kx, ky = [float(v) for v in p]
+    C = np.array([[1, 0, cx], [0, 1, cy], [0, 0, 1]], float)
+    Ci = np.array([[1, 0, -cx], [0, 1, -cy], [0, 0, 1]], float)
+    K = np.array([[1, 0, 0], [0, 1, 0], [kx, ky, 1]], float)
+    T = np.array([[1, 0, tx], [0, 1, ty], [0, 0, 1]], float)
+    R = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]], float)
+    Sh = np.array([[1, sh, 0], [0, 1, 0], [0, 0, 1]], float)
+    Sc = np.diag([sx, sy, 1.0])
+    H = C @ K @ T @ R @ Sh @ Sc @ Ci
+    return H / H[2, 2]
+
+
+def test_homography_identity_exact(oracle):
+    h = oracle.homography([0, 1, 1, 0, 0, 0, 0, 0], 32.0, 32.0)
+    assert np.array_equal(h, np.array([1, 0, 0, 0, 1, 0, 0, 0], np.float32))
+
+
+def test_homography_matches_matrix_product(oracle):
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        p = np.array([rng.uniform(-0.3, 0.3), rng.uniform(0.9, 1.1), rng.uniform(0.9, 1.1), rng.uniform(-0.1, 0.1),
+                      rng.uniform(-8, 8), rng.uniform(-8, 8), rng.uniform(-1e-3, 1e-3), rng.uniform(-1e-3, 1e-3)],
+                     np.float32)
+        cx, cy = 128.0, 96.0
+        H = _centred_fp64(p, cx, cy)
+        h = oracle.homography(p, cx, cy)
+        want = H.reshape(-1)[:8]
+        assert np.allclose(h, want, rtol=2e-5, atol=2e-4 * np.abs(want).max()), (p, h, want)
+
+
+def test_inverse_warp_matches_fp64_inverse(oracle):
+    rng = np.random.default_rng(8)
+    for _ in range(100):
+        p = np.array([rng.uniform(-0.1, 0.1), rng.uniform(0.95, 1.05), rng.uniform(0.95, 1.05), rng.uniform(-.05, .05),
+                      rng.uniform(-4, 4), rng.uniform(-4, 4), rng.uniform(-1e-3, 1e-3), rng.uniform(-1e-3, 1e-3)],
+                     np.float32)
+        H = _centred_fp64(p, 32.0, 32.0)
+        Hi = np.linalg.inv(H)
+        coef = oracle.warp_coeffs(p, 32.0, 32.0)
+        for x, y in [(0, 0), (63, 0), (0, 63), (31, 17), (63, 63)]:
+            q = Hi @ np.array([x, y, 1.0])
+            got = oracle.warp_point(coef, x, y)
+            assert np.allclose(got, q[:2] / q[2], atol=2e-3), (p, x, y, got, q[:2] / q[2])
+
+
+def test_integer_translation_is_exact(oracle):
+    # centred H with only translation: the inverse warp samples (x - tx, y - ty) exactly
+    for tx, ty in [(3, -2), (-4, 4), (0, 1)]:
+        coef = oracle.warp_coeffs([0, 1, 1, 0, tx, ty, 0, 0], 32.0, 32.0)
+        for x, y in [(0, 0), (5, 9), (63, 63)]:
+            got = oracle.warp_point(coef, x, y)
+            assert got[0] == x - tx and got[1] == y - ty
+    coef = oracle.warp_coeffs([2.5, -1.0], 0, 0, mode=oracle.PARAM_TRANSLATION)
+    assert np.array_equal(oracle.warp_point(coef, 4, 7), np.array([6.5, 6.0], np.float32))
+
+
+# -------------------------------------------------------------- samplers
+
+def test_nearest_known_answers(oracle):
+    img = np.arange(60, dtype=np.uint8).reshape(6, 10)   # W=10, H=6
+    assert oracle.sample(img, 2.0, 3.0, oracle.NEAREST) == img[2, 3]
+    assert oracle.sample(img, 2.0, 3.9, oracle.NEAREST) == img[2, 3]        # truncation in x
+    # fractional y shifts the linear index by frac(y)*W columns: trunc(0 + 10*1.5) = 15 -> (y=1, x=5)
+    assert oracle.sample(img, 1.5, 0.0, oracle.NEAREST) == img[1, 5]
+    assert oracle.sample(img, 1.25, 8.0, oracle.NEAREST) == img[2, 0]       # trunc(8 + 12.5) = 20
+    # past the end of the buffer reads the declared zero pad
+    assert oracle.sample(img, 5.9, 9.5, oracle.NEAREST) == 0.0              # trunc(9 + 59) = 68 >= 60
+    assert oracle.sample(img, -0.1, 3.0, oracle.NEAREST) == 0.0
+    assert oracle.sample(img, 2.0, 10.0, oracle.NEAREST) == 0.0
+
+
+def test_bilinear_known_answers(oracle):
+    img = (np.arange(30, dtype=np.float32).reshape(5, 6) * 3.0)   # W=6, H=5
+    for y in range(5):
+        for x in range(6):
+            assert oracle.sample(img, y, x, oracle.BILINEAR) == img[y, x]
+    const = np.full((5, 6), 17.0, np.float32)
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        y, x = rng.uniform(0, 4), rng.uniform(0, 5)
+        assert oracle.sample(const, y, x, oracle.BILINEAR) == 17.0
+        # linear ramp f = 3 (6y + x): interior bilinear reproduces it
+        assert abs(oracle.sample(img, y, x, oracle.BILINEAR) - 3 * (6 * y + x)) < 1e-4
+    # x in (W-1, W): the right tap wraps to the next row's first pixel
+    v = oracle.sample(img, 1.0, 5.5, oracle.BILINEAR)
+    assert v == np.float32(0.5) * img[1, 5] + np.float32(0.5) * img[2, 0]
+    # y in (H-1, H): the bottom taps read past the buffer -> 0
+    v = oracle.sample(img, 4.5, 2.0, oracle.BILINEAR)
+    assert v == np.float32(0.5) * img[4, 2]
+
+
+# ------------------------------------------------------------------ costs
+
+def _shift_pair(seed=3, w=24, h=20):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 256, (h, w)).astype(np.uint8), rng.integers(0, 256, (h, w)).astype(np.uint8)
+
+
+def _numpy_translation_terms(moved, fixed, tx, ty):
+    """Independent float64 restatement for integer translations: fixed pixel
+    (x, y) meets moving pixel (x + tx, y + ty)."""
+    h, w = fixed.shape
+    ys, xs = np.mgrid[0:h, 0:w]
+    mx, my = xs + tx, ys + ty
+    ok = (mx >= 0) & (mx < moved.shape[1]) & (my >= 0) & (my < moved.shape[0])
+    f = fixed[ok].astype(np.float64)
+    m = moved[my[ok], mx[ok]].astype(np.float64)
+    return f, m
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_sad_ncc_sqd_translation_brute_force(oracle, sampler):
+    moved, fixed = _shift_pair()
+    for tx, ty in [(0, 0), (3, -2), (-5, 4), (10, 7)]:
+        f, m = _numpy_translation_terms(moved, fixed, tx, ty)
+        p = [tx, ty]
+        sad, aux = oracle.cost_point(moved, fixed, oracle.COST_SAD, sampler, p, mode=oracle.PARAM_TRANSLATION)
+        assert aux[4] == len(f)
+        assert aux[0] == np.abs(m - f).sum()
+        assert sad == np.float32(np.float32(np.abs(m - f).sum()) / np.float32(len(f)))
+        ncc, _ = oracle.cost_point(moved, fixed, oracle.COST_NCC, sampler, p, mode=oracle.PARAM_TRANSLATION)
+        want = -(f * m).sum() / math.sqrt((f * f).sum() * (m * m).sum())
+        assert abs(ncc - want) <= 1e-6 * abs(want)
+        sqd, _ = oracle.cost_point(moved, fixed, oracle.COST_SQD, sampler, p, mode=oracle.PARAM_TRANSLATION)
+        want = ((m - f) ** 2).sum() / math.sqrt((f * f).sum() * (m * m).sum())
+        assert abs(sqd - want) <= 1e-6 * abs(want)
+
+
+def test_ncc_invariants(oracle):
+    m, f, _ = synth.make_pair(5, 32, dtype=np.float32)
+    ident = [0, 1, 1, 0, 0, 0, 0, 0]
+    v, _ = oracle.cost_point(f, f, oracle.COST_NCC, oracle.BILINEAR, ident)
+    assert abs(v + 1.0) < 1e-6
+    p = [0.03, 1.02, 0.99, 0.01, 1.5, -0.7, 2e-4, -1e-4]
+    a, _ = oracle.cost_point(m, f, oracle.COST_NCC, oracle.BILINEAR, p)
+    b, _ = oracle.cost_point((m * np.float32(0.5)).astype(np.float32), f, oracle.COST_NCC, oracle.BILINEAR, p)
+    assert abs(a - b) <= 1e-6 * abs(a)   # uncentred NCC is scale invariant
+    # no overlap -> 0 (calcNCC's num_errors == 0 branch)
+    v, aux = oracle.cost_point(m, f, oracle.COST_NCC, oracle.BILINEAR, [0, 1, 1, 0, 500, 0, 0, 0])
+    assert v == 0.0 and aux[4] == 0
+
+
+def _mi_numpy(hist):
+    """Independent float64 restatement of the entropy step from a count matrix."""
+    n = hist.sum()
+    pxy = hist / n
+    px, py = pxy.sum(1), pxy.sum(0)
+    nz = pxy > 0
+    mi = (pxy[nz] * np.log2(pxy[nz] / np.outer(px, py)[nz])).sum()
+    hx = -(px[px > 0] * np.log2(px[px > 0])).sum()
+    hy = -(py[py > 0] * np.log2(py[py > 0])).sum()
+    return mi, hx, hy
+
+
+def test_mi_invariants_and_entropy(oracle):
+    m, f, _ = synth.make_pair(9, 48, dtype=np.uint8)
+    ident = [0, 1, 1, 0, 0, 0, 0, 0]
+    v, _ = oracle.cost_point(f, f, oracle.COST_MI, oracle.BILINEAR, ident, bins=32)
+    assert abs(v + 1.0) < 1e-6                    # identical images: -2H/(2H)
+    inv = (255 - f).astype(np.uint8)              # bijective bin relabelling
+    v, _ = oracle.cost_point(inv, f, oracle.COST_MI, oracle.BILINEAR, ident, bins=32)
+    assert abs(v + 1.0) < 1e-6
+    const = np.full_like(f, 77)
+    v, _ = oracle.cost_point(const, f, oracle.COST_MI, oracle.BILINEAR, ident, bins=32)
+    assert v == 0.0                               # independent: MI = 0
+    p = [0.02, 1.01, 0.98, -0.02, 2.0, -1.0, 3e-4, 2e-4]
+    for bins in (32, 64):
+        v, aux, hist = oracle.cost_point(m, f, oracle.COST_MI, oracle.BILINEAR, p, bins=bins, want_hist=True)
+        assert hist.sum() == aux[4]               # counts sum to the overlap
+        mi, hx, hy = _mi_numpy(hist.astype(np.float64))
+        assert abs(aux[5] - mi) < 1e-12 and abs(aux[6] - hx) < 1e-12 and abs(aux[7] - hy) < 1e-12
+        assert abs(v - (-2 * mi / (hx + hy))) <= 1e-6 * abs(v)
+        vp, _ = oracle.cost_point(m, f, oracle.COST_MI_PLAIN, oracle.BILINEAR, p, bins=bins)
+        assert abs(vp + mi) <= 1e-6 * abs(mi)
+
+
+def test_mi_histogram_brute_force_translation(oracle):
+    moved, fixed = _shift_pair(11)
+    for tx, ty in [(0, 0), (2, 3), (-6, 1)]:
+        f, m = _numpy_translation_terms(moved, fixed, tx, ty)
+        _, _, hist = oracle.cost_point(moved, fixed, oracle.COST_MI, oracle.NEAREST, [tx, ty],
+                                       mode=oracle.PARAM_TRANSLATION, bins=32, want_hist=True)
+        want = np.zeros((32, 32), np.int64)
+        np.add.at(want, ((f // 8).astype(int), (m // 8).astype(int)), 1)
+        assert np.array_equal(hist, want)
+
+
+def test_driver_images_translation_search(oracle):
+    """The reference drivers' 6x6 images (src/gpu/cudaGridSearch_example1.cu:51-71):
+    integer-translation SAD search, brute-forced in numpy."""
+    A = np.array(DRIVER_IMAGES["imageA"], np.uint8).reshape(6, 6) * 255
+    C = np.array(DRIVER_IMAGES["imageC"], np.uint8).reshape(6, 6) * 255
+    start, end, count = [-5, -5], [5, 5], [11, 11]
+    c2, r = oracle.grid_update(start, end, count)
+    res = oracle.search(A, C, oracle.COST_SAD_SUM, oracle.NEAREST, start, r, c2, mode=oracle.PARAM_TRANSLATION)
+    want = []
+    for ty in range(-5, 6):
+        for tx in range(-5, 6):
+            f, m = _numpy_translation_terms(A, C, tx, ty)
+            want.append(np.abs(m - f).sum())
+    want = np.array(want, np.float32)      # index order: tx fastest (axis 0)
+    assert np.array_equal(res, want)
+    assert oracle.argmin(res) == (0.0, int(np.argmin(want)))
+    aligned = (2 + 5) + 11 * (2 + 5)       # tx = ty = 2 puts imageA's block on imageC's
+    assert res[aligned] == 0.0 and list(oracle.grid_point(start, r, c2, aligned)) == [2.0, 2.0]
+
+
+# ----------------------------------------------------------------- argmin
+
+def test_argmin_reference_tie_order_matches_simulation(oracle):
+    rng = np.random.default_rng(0)
+    for n in [1, 5, 511, 512, 513, 1023, 1024, 1025, 3000, 70000]:
+        for _ in range(5):
+            v = rng.integers(0, 4, n).astype(np.float32)          # heavy ties
+            if rng.random() < 0.3:
+                v[rng.integers(0, n, max(1, n // 10))] = np.nan
+            if rng.random() < 0.3:
+                v[rng.integers(0, n, max(1, n // 10))] = np.inf
+            assert oracle.argmin(v, oracle.TIE_REFERENCE) == oracle.argmin_reference_sim(v)
+
+
+def test_argmin_lowest_and_specials(oracle):
+    assert oracle.argmin(np.array([3, 1, 2, 1], np.float32)) == (1.0, 1)
+    assert oracle.argmin(np.array([np.nan, np.inf, np.nan], np.float32)) == (math.inf, -1)
+    assert oracle.argmin(np.array([np.nan, 5, -np.inf, 5], np.float32)) == (-math.inf, 2)
+    # reference order: among equal minima at 1 and 256, idx 256 (bit-reversed 0b100000000 = 1)
+    # loses to idx 0b1 -> bit-reversed 256; lowest-index picks 1
+    v = np.full(600, 9, np.float32)
+    v[[1, 256]] = 0
+    assert oracle.argmin(v, oracle.TIE_REFERENCE) == (0.0, 256)
+    assert oracle.argmin(v, oracle.TIE_LOWEST) == (0.0, 1)
+    assert oracle.argmin_reference_sim(v) == (0.0, 256)
+
+
+def test_reference_rank_is_a_bijection(oracle):
+    ranks = sorted(oracle.reference_rank(i) for i in range(4096))
+    assert ranks == list(range(4096))
