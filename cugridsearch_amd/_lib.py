@@ -75,6 +75,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc for gfx950) first")
+        # torch ships its own libamdhip64; loading it first makes libcgs.so
+        # bind to that same HIP runtime (two runtimes in one process do not
+        # share devices, streams or allocations).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

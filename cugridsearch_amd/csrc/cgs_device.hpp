@@ -65,16 +65,17 @@ __device__ __forceinline__ void homography_centred(const float *p, float c, floa
     h[6] = h31 / h33; h[7] = h32 / h33;
 }
 
-// Coefficients {A, B, P, Q} of the three projective forms
-// F(x, y) = ((A*x + B*y) + P) - Q for X (k=0), Y (k=1), W (k=2); the sample
-// point is (X/W, Y/W).  Inverse warp: calcNewCoordH
-// (src/gpu/cudaImageFunctions.cuh:277-285).
+// Coefficients {A, B, C} of the three projective forms, evaluated per pixel
+// as F(x, y) = fmaf(B, y, fmaf(A, x, C)) for X (k=0), Y (k=1), W (k=2); the
+// sample point is (X/W, Y/W).  Inverse warp: calcNewCoordH
+// (src/gpu/cudaImageFunctions.cuh:277-285), whose (A*x + B*y + P) - Q is
+// declared here with C = P - Q rounded once and the two products fused.
 __device__ __forceinline__ void warp_coeffs(int mode, const float *p, float c, float s, float cx, float cy,
                                             float *coef) {
     if (mode == kTranslation) {
-        coef[0] = 1.0f; coef[1] = 0.0f; coef[2] = p[0]; coef[3] = 0.0f;
-        coef[4] = 0.0f; coef[5] = 1.0f; coef[6] = p[1]; coef[7] = 0.0f;
-        coef[8] = 0.0f; coef[9] = 0.0f; coef[10] = 1.0f; coef[11] = 0.0f;
+        coef[0] = 1.0f; coef[1] = 0.0f; coef[2] = p[0];
+        coef[3] = 0.0f; coef[4] = 1.0f; coef[5] = p[1];
+        coef[6] = 0.0f; coef[7] = 0.0f; coef[8] = 1.0f;
         return;
     }
     float h[8];
@@ -93,19 +94,28 @@ __device__ __forceinline__ void warp_coeffs(int mode, const float *p, float c, f
         h[7] = p[7];
     } else {
         // forward raw H: src/gpu/cudaErrorFunctions.cuh:205-207
-        coef[0] = p[0]; coef[1] = p[1]; coef[2] = p[2]; coef[3] = 0.0f;
-        coef[4] = p[3]; coef[5] = p[4]; coef[6] = p[5]; coef[7] = 0.0f;
-        coef[8] = p[6]; coef[9] = p[7]; coef[10] = 1.0f; coef[11] = 0.0f;
+        coef[0] = p[0]; coef[1] = p[1]; coef[2] = p[2];
+        coef[3] = p[3]; coef[4] = p[4]; coef[5] = p[5];
+        coef[6] = p[6]; coef[7] = p[7]; coef[8] = 1.0f;
         return;
     }
     const float h11 = h[0], h12 = h[1], h13 = h[2], h21 = h[3], h22 = h[4], h23 = h[5], h31 = h[6], h32 = h[7];
-    coef[0] = h22 - h23 * h32;       coef[1] = h13 * h32 - h12;       coef[2] = h12 * h23;  coef[3] = h13 * h22;
-    coef[4] = h23 * h31 - h21;       coef[5] = h11 - h13 * h31;       coef[6] = h13 * h21;  coef[7] = h11 * h23;
-    coef[8] = h21 * h32 - h22 * h31; coef[9] = h12 * h31 - h11 * h32; coef[10] = h11 * h22; coef[11] = h12 * h21;
+    coef[0] = h22 - h23 * h32;       coef[1] = h13 * h32 - h12;       coef[2] = h12 * h23 - h13 * h22;
+    coef[3] = h23 * h31 - h21;       coef[4] = h11 - h13 * h31;       coef[5] = h13 * h21 - h11 * h23;
+    coef[6] = h21 * h32 - h22 * h31; coef[7] = h12 * h31 - h11 * h32; coef[8] = h11 * h22 - h12 * h21;
 }
 
-__device__ __forceinline__ float form(float a, float b, float p, float q, float x, float y) {
-    return ((a * x + b * y) + p) - q;
+__device__ __forceinline__ float form(float a, float b, float c, float x, float y) {
+    return fmaf(b, y, fmaf(a, x, c));
+}
+
+// 0 <= n/w < lim decided on the exact quotient: fmaf(-lim, w, n) is the
+// exact n - lim*w rounded once, so its sign is exact.  CudaImage::inImage
+// (include/cudaTensor.cuh:548-550) applied to the exact quotient.
+__device__ __forceinline__ bool quotient_in(float n, float w, float lim) {
+    if (w > 0.0f) return n >= 0.0f && fmaf(-lim, w, n) < 0.0f;
+    if (w < 0.0f) return n <= 0.0f && fmaf(-lim, w, n) > 0.0f;
+    return false;
 }
 
 // Order-preserving map of a binary32 cost to 32 bits; NaN and +inf map to

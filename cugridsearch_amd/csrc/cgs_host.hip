@@ -42,41 +42,21 @@ hipError_t Workspace::elapsed_ms() {
     return e;
 }
 
-cgs_status Workspace::reserve(size_t mov_bytes, size_t levels, size_t hist_words) {
-    if (!key) {
-        if (hipMalloc(&key, 64) != hipSuccess) return set_error(CGS_ERR_HIP, "hipMalloc(key)");
-        if (hipHostMalloc(&key_host, 64, hipHostMallocDefault) != hipSuccess)
-            return set_error(CGS_ERR_HIP, "hipHostMalloc(key)");
+cgs_status DevBuf::ensure(size_t bytes, bool pinned_host) {
+    if (bytes <= cap && ptr) return CGS_OK;
+    if (ptr) {
+        if (pinned_host) hipHostFree(ptr);
+        else hipFree(ptr);
     }
-    if (mov_bytes > mov_cap) {
-        if (mov) hipFree(mov);
-        mov = nullptr;
-        size_t cap = mov_bytes + (mov_bytes >> 2) + 4096;
-        if (hipMalloc(&mov, cap) != hipSuccess) { mov_cap = 0; return set_error(CGS_ERR_HIP, "hipMalloc(moving)"); }
-        mov_cap = cap;
+    ptr = nullptr;
+    cap = 0;
+    const size_t want = bytes + (bytes >> 2) + 256;
+    const hipError_t e = pinned_host ? hipHostMalloc(&ptr, want, hipHostMallocDefault) : hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return set_error(CGS_ERR_HIP, std::string("workspace allocation: ") + hipGetErrorString(e));
     }
-    if (levels > sincos_cap) {
-        if (sincos) hipFree(sincos);
-        if (sincos_host) hipHostFree(sincos_host);
-        sincos = nullptr;
-        sincos_host = nullptr;
-        size_t cap = levels < 256 ? 256 : levels;
-        if (hipMalloc(&sincos, cap * 2 * sizeof(float)) != hipSuccess ||
-            hipHostMalloc(&sincos_host, cap * 2 * sizeof(float), hipHostMallocDefault) != hipSuccess) {
-            sincos_cap = 0;
-            return set_error(CGS_ERR_HIP, "hipMalloc(sincos)");
-        }
-        sincos_cap = cap;
-    }
-    if (hist_words > hist_cap) {
-        if (hist) hipFree(hist);
-        hist = nullptr;
-        if (hipMalloc(&hist, hist_words * sizeof(uint32_t)) != hipSuccess) {
-            hist_cap = 0;
-            return set_error(CGS_ERR_HIP, "hipMalloc(hist)");
-        }
-        hist_cap = hist_words;
-    }
+    cap = want;
     return CGS_OK;
 }
 

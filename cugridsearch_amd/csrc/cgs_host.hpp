@@ -24,22 +24,19 @@ void clear_error();
     } while (0)
 
 // Device memory the library owns for one GPU: the zero-padded moving image,
-// the (cos, sin) table, the argmin key and small pinned host mirrors.  Grows
-// on demand and lives until the process ends.  Calls on one device
-// serialise on mu.
+// its bilinear tap table, the (cos, sin) table, the argmin key and small
+// pinned host mirrors.  Buffers grow on demand and live until the process
+// ends.  Calls on one device serialise on mu.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    cgs_status ensure(size_t bytes, bool pinned_host = false);
+};
+
 struct Workspace {
     std::mutex mu;
-    void *mov = nullptr;
-    size_t mov_cap = 0;
-    float *sincos = nullptr;
-    float *sincos_host = nullptr;
-    size_t sincos_cap = 0;
-    uint32_t *hist = nullptr;
-    size_t hist_cap = 0;
-    unsigned long long *key = nullptr;
-    unsigned long long *key_host = nullptr;
+    DevBuf mov, quads, sincos, sincos_host, hist, key, key_host;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-    cgs_status reserve(size_t mov_bytes, size_t sincos_levels, size_t hist_words);
     // HIP events around one launch on the caller's stream; elapsed_ms()
     // stores the duration for cgs_last_kernel_ms() (after a stream sync).
     hipError_t record_start(hipStream_t s);

@@ -1,23 +1,32 @@
 // cgs_search.hip -- plain single-pair grid search for gfx950 and its C-ABI.
 //
-// One kernel launch evaluates every grid point: each 256-thread workgroup
-// owns GPT consecutive grid indices, computes their warp coefficients once,
-// sweeps the fixed image (every lane a pixel, every pixel against all GPT
-// points so the fixed-pixel load and its coordinates are shared), reduces the
+// One launch evaluates every grid point.  Each 256-thread workgroup owns GPT
+// consecutive grid indices, computes their warp coefficients once, sweeps the
+// fixed image (a lane per pixel, every pixel against all GPT points so the
+// fixed-pixel load and the column part of the warp are shared), reduces the
 // cost terms (wavefront shuffles -> LDS -> one lane), writes the cost tensor
 // and folds its minimum into a 64-bit (ordered cost, index) key with one
-// global atomicMin.  That replaces evaluationKernel_by_value + the separate
+// global atomicMin.  That replaces evaluationKernel_by_value and the separate
 // find_extrema pass of the reference (src/gpu/cudaGridSearch.cuh:287-314,
 // include/cudaTensor.cuh:215-265).
+//
+// Two kernels:
+//   search_exact_kernel -- every cost/sampler/channel count; sample points are
+//     correctly rounded quotients, so nearest-neighbour values and MI bins are
+//     bit-identical to the oracle.
+//   search_fast_kernel  -- bilinear SAD/NCC/SQD on one channel: the in-image
+//     decision is still exact (sign tests on the exact forms) but the sample
+//     point uses v_rcp_f32 and the taps come from a precomputed float4 table;
+//     costs agree with the oracle within FP32 rounding of the samples.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
-#include <map>
 #include <mutex>
 #include <string>
-#include <vector>
+#include <type_traits>
 
 #include "cgs.h"
 #include "cgs_device.hpp"
@@ -32,7 +41,8 @@ struct SearchParams {
     const float2 *sincos;   // (cos, sin) per level of axis 0, or null
     int param_mode;
     int cost;
-    const void *mov;        // moving image, zero padded by C*(W+1) elements
+    const void *mov;        // exact kernel: moving image + zero pad
+    const float4 *quads;    // fast kernel: tap table, indexable from -(W+1)
     int mw, mh, mc;
     const void *fix;
     int fw, fh;
@@ -46,6 +56,10 @@ struct SearchParams {
 
 constexpr int kThreads = 256;
 
+// Elements of zero padding after the moving image: covers every tap the
+// samplers can address for a sample point in [0, W] x [0, H].
+__host__ __device__ inline size_t mov_pad_elems(int W, int C) { return (size_t) C * (2 * (size_t) W + 4); }
+
 // valueAt_nearest_neighbor (include/cudaTensor.cuh:557-559): the truncated
 // float linear index of toIndex1d<float> (include/cudaTensor.cuh:171-180).
 template <typename MT>
@@ -58,9 +72,9 @@ __device__ __forceinline__ float sample_nearest(const MT *img, float X, float Y,
 
 // valueAt_bilinear (include/cudaTensor.cuh:561-575).  The right/bottom taps
 // are read at floor+1; where the coordinate is integral the reference reads
-// ceil == floor instead, and the tap's weight is exactly 0, so the value is
-// the same.  ceil(x) == W lands on the next row through the linear index
-// and ceil(y) == H on the zero pad, as in the reference's linear addressing.
+// ceil == floor instead, and that tap's weight is exactly 0, so the value is
+// the same.  ceil(x) == W lands on the next row through the linear index and
+// ceil(y) == H on the zero pad, as in the reference's linear addressing.
 template <typename MT>
 __device__ __forceinline__ float sample_bilinear(const MT *img, float X, float Y, int c, int C, int W) {
     const float fx = floorf(X), fy = floorf(Y);
@@ -97,8 +111,79 @@ __device__ __forceinline__ double block_sum(double v, double *scratch) {
     return scratch[0] + scratch[1] + scratch[2] + scratch[3];
 }
 
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Thread 0..GPT-1 of the workgroup: warp coefficients of grid index base+tid
+// (index 0 for the padding slots past the end of the grid).
+template <int GPT>
+__device__ __forceinline__ void block_coefficients(const SearchParams &P, int32_t base, float (*s_coef)[12]) {
+    const int tid = threadIdx.x;
+    if (tid < GPT) {
+        const int32_t gi = base + tid < P.grid.n ? base + tid : 0;
+        float p[kMaxDim];
+        int32_t i0;
+        grid_point(P.grid, gi, p, &i0);
+        float c = 1.0f, s = 0.0f;
+        if (P.sincos) {
+            const float2 cs = P.sincos[i0];
+            c = cs.x;
+            s = cs.y;
+        }
+        float co[9];
+        warp_coeffs(P.param_mode, p, c, s, P.cx, P.cy, co);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s_coef[tid][k] = co[k];
+        // W > 0 over the whole fixed image (W is affine: its minimum over the
+        // rectangle is at a corner; the margin covers the rounding of the fmas)
+        double wmin = 1e300;
+        for (int k = 0; k < 4; ++k) {
+            const double xx = (k & 1) ? P.fw - 1 : 0, yy = (k & 2) ? P.fh - 1 : 0;
+            const double w = (double) co[7] * yy + ((double) co[6] * xx + (double) co[8]);
+            wmin = w < wmin ? w : wmin;
+        }
+        const double scale = fabs((double) co[6]) * P.fw + fabs((double) co[7]) * P.fh + fabs((double) co[8]);
+        s_coef[tid][9] = (wmin > 1e-5 * scale && scale < 1e30) ? 1.0f : 0.0f;
+    }
+}
+
+// Final cost of one grid point from its reduced sums (t[0..2]) and sample
+// count t[3]; binary64 combine.
+template <int FAM>
+__device__ __forceinline__ float finish_cost(int cost, const double *t, bool exact_int) {
+    if constexpr (FAM == kFamSAD) {
+        if (cost == CGS_COST_SAD_SUM) return (float) t[0];
+        if (exact_int) return (float) t[0] / (float) t[3];
+        return (float) (t[0] / t[3]);
+    } else {
+        if (cost == CGS_COST_NCC) return t[3] > 0.0 ? (float) (-t[2] / sqrt(t[0] * t[1])) : 0.0f;
+        return t[3] > 0.0 ? (float) (t[2] / sqrt(t[0] * t[1])) : 123123.0f;
+    }
+}
+
+// Writes the costs of the workgroup's GPT points and folds their minimum
+// into the global argmin key.  Called by thread 0.
+template <int GPT>
+__device__ __forceinline__ void emit(const SearchParams &P, int32_t base, const float *s_cost) {
+    unsigned long long best = ~0ull;
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const int32_t gi = base + g;
+        if (gi < P.grid.n) {
+            if (P.result) P.result[gi] = s_cost[g];
+            const unsigned long long k = argmin_key(s_cost[g], (uint32_t) gi, P.tie_mode);
+            best = k < best ? k : best;
+        }
+    }
+    if (best != ~0ull && best < __hip_atomic_load(P.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(P.key, best);
+}
+
+// ------------------------------------------------------------ exact kernel
+
 template <int FAM, int SAMP, typename MT, typename FT, int GPT>
-__global__ __launch_bounds__(kThreads) void search_kernel(const SearchParams P) {
+__global__ __launch_bounds__(kThreads) void search_exact_kernel(const SearchParams P) {
     __shared__ float s_coef[GPT][12];
     __shared__ double s_red[4][GPT][4];
     __shared__ float s_cost[GPT];
@@ -112,32 +197,17 @@ __global__ __launch_bounds__(kThreads) void search_kernel(const SearchParams P) 
     const int32_t base = (int32_t) blockIdx.x * GPT;
     const int B2 = P.bins * P.bins;
 
-    if (tid < GPT) {
-        const int32_t gi = base + tid < P.grid.n ? base + tid : 0;
-        float p[kMaxDim];
-        int32_t i0;
-        grid_point(P.grid, gi, p, &i0);
-        float c = 1.0f, s = 0.0f;
-        if (P.sincos) {
-            const float2 cs = P.sincos[i0];
-            c = cs.x;
-            s = cs.y;
-        }
-        float co[12];
-        warp_coeffs(P.param_mode, p, c, s, P.cx, P.cy, co);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) s_coef[tid][k] = co[k];
-    }
+    block_coefficients<GPT>(P, base, s_coef);
     if constexpr (FAM == kFamMI) {
         for (int i = tid; i < GPT * B2; i += kThreads) s_hist[i] = 0u;
     }
     __syncthreads();
 
-    float cf[GPT][12];
+    float cf[GPT][9];
 #pragma unroll
     for (int g = 0; g < GPT; ++g)
 #pragma unroll
-        for (int k = 0; k < 12; ++k) cf[g][k] = s_coef[g][k];
+        for (int k = 0; k < 9; ++k) cf[g][k] = s_coef[g][k];
 
     const MT *mov = (const MT *) P.mov;
     const FT *fix = (const FT *) P.fix;
@@ -158,10 +228,11 @@ __global__ __launch_bounds__(kThreads) void search_kernel(const SearchParams P) 
         const float fxp = (float) x, fyp = (float) y;
 #pragma unroll
         for (int g = 0; g < GPT; ++g) {
-            const float W = form(cf[g][8], cf[g][9], cf[g][10], cf[g][11], fxp, fyp);
-            const float X = form(cf[g][0], cf[g][1], cf[g][2], cf[g][3], fxp, fyp) / W;
-            const float Y = form(cf[g][4], cf[g][5], cf[g][6], cf[g][7], fxp, fyp) / W;
-            if (X >= 0.0f && Y >= 0.0f && X < Wm && Y < Hm) {
+            const float Wn = form(cf[g][6], cf[g][7], cf[g][8], fxp, fyp);
+            const float Xn = form(cf[g][0], cf[g][1], cf[g][2], fxp, fyp);
+            const float Yn = form(cf[g][3], cf[g][4], cf[g][5], fxp, fyp);
+            if (quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm)) {
+                const float X = Xn / Wn, Y = Yn / Wn;
                 for (int c = 0; c < C; ++c) {
                     const float m = SAMP == CGS_SAMPLER_NEAREST ? sample_nearest(mov, X, Y, c, C, P.mw)
                                                                 : sample_bilinear(mov, X, Y, c, C, P.mw);
@@ -187,7 +258,6 @@ __global__ __launch_bounds__(kThreads) void search_kernel(const SearchParams P) 
         if (x >= P.fw) { x -= P.fw; ++y; }
     }
 
-    // ---- reduce per grid point: count, and the family's sums
 #pragma unroll
     for (int g = 0; g < GPT; ++g) {
         const double v0 = wave_sum(kExactInt ? (double) isum[g] : acc0[g]);
@@ -208,16 +278,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(const SearchParams P) 
             double t[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) t[k] = s_red[0][tid][k] + s_red[1][tid][k] + s_red[2][tid][k] + s_red[3][tid][k];
-            float cost;
-            if constexpr (FAM == kFamSAD) {
-                if (P.cost == CGS_COST_SAD_SUM) cost = (float) t[0];
-                else if (kExactInt) cost = (float) t[0] / (float) t[3];
-                else cost = (float) (t[0] / t[3]);
-            } else {
-                if (P.cost == CGS_COST_NCC) cost = t[3] > 0.0 ? (float) (-t[2] / sqrt(t[0] * t[1])) : 0.0f;
-                else cost = t[3] > 0.0 ? (float) (t[2] / sqrt(t[0] * t[1])) : 123123.0f;
-            }
-            s_cost[tid] = cost;
+            s_cost[tid] = finish_cost<FAM>(P.cost, t, kExactInt);
         }
     } else {
         // calcMIstream's entropy step (src/gpu/cudaErrorFunction_miStreams.cuh:306-365), binary64.
@@ -267,31 +328,189 @@ __global__ __launch_bounds__(kThreads) void search_kernel(const SearchParams P) 
             for (int j = tid; j < B2; j += kThreads) P.hist_dump[j] = s_hist[j];
     }
     __syncthreads();
+    if (tid == 0) emit<GPT>(P, base, s_cost);
+}
 
-    if (tid == 0) {
-        unsigned long long best = ~0ull;
+// ------------------------------------------------------------- fast kernel
+//
+// Work split: a thread owns one fixed-image column x for a whole sweep, so
+// fmaf(A, x, C) of every form is computed once per grid point and each pixel
+// costs one fmaf per form.  For W <= 256 the 256 lanes cover 256/W rows per
+// step; wider images are swept in 256-column groups.  The B coefficients are
+// wave-uniform and held in SGPRs.  Per-thread FP32 partial sums are folded
+// into FP64 every kFlush rows.
+
+constexpr int kFlush = 16;
+
+template <int COSTK, typename FT, int GPT, bool POSW>
+__device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*s_coef)[12], double (*acc)[3],
+                                           uint32_t *cnt) {
+    const int tid = threadIdx.x;
+    const int fw = P.fw, fh = P.fh;
+    const int cols = fw < kThreads ? fw : kThreads;
+    const int rpi = kThreads / cols;
+    const int r0 = tid / cols, xc = tid - r0 * cols;
+    const float Wm = (float) P.mw, Hm = (float) P.mh;
+    const FT *fix = (const FT *) P.fix;
+    const float4 *quads = P.quads;
+    const int mw = P.mw;
+
+    float B[GPT][3];
 #pragma unroll
-        for (int g = 0; g < GPT; ++g) {
-            const int32_t gi = base + g;
-            if (gi < P.grid.n) {
-                if (P.result) P.result[gi] = s_cost[g];
-                const unsigned long long k = argmin_key(s_cost[g], (uint32_t) gi, P.tie_mode);
-                best = k < best ? k : best;
+    for (int g = 0; g < GPT; ++g)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) B[g][k] = uniform(s_coef[g][3 * k + 1]);
+
+    for (int x0 = 0; x0 < fw; x0 += kThreads) {
+        const int x = x0 + xc;
+        const bool valid = (r0 < rpi) && (x < fw);
+        const float xf = (float) x;
+        float col[GPT][3];
+#pragma unroll
+        for (int g = 0; g < GPT; ++g)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                col[g][k] = fmaf(uniform(s_coef[g][3 * k]), xf, uniform(s_coef[g][3 * k + 2]));
+        float part[GPT][3];
+#pragma unroll
+        for (int g = 0; g < GPT; ++g) part[g][0] = part[g][1] = part[g][2] = 0.0f;
+        int since = 0;
+        // the same trip count on every lane keeps the ballot counts wave-uniform
+        const int steps = (fh + rpi - 1) / rpi;
+        for (int it = 0; it < steps; ++it) {
+            const int y = r0 + it * rpi;
+            const bool vy = valid && y < fh;
+            const float f = vy ? (float) fix[y * fw + x] : 0.0f;
+            const float f2 = f * f;
+            const float yf = (float) y;
+#pragma unroll
+            for (int g = 0; g < GPT; ++g) {
+                const float Xn = fmaf(B[g][0], yf, col[g][0]);
+                const float Yn = fmaf(B[g][1], yf, col[g][1]);
+                const float Wn = fmaf(B[g][2], yf, col[g][2]);
+                bool in;
+                if constexpr (POSW)
+                    in = Xn >= 0.0f && Yn >= 0.0f && fmaf(-Wm, Wn, Xn) < 0.0f && fmaf(-Hm, Wn, Yn) < 0.0f;
+                else
+                    in = quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm);
+                in = in && vy;
+                cnt[g] += (uint32_t) __popcll(__ballot(in));
+                if (in) {
+                    const float r = __builtin_amdgcn_rcpf(Wn);
+                    const float X = Xn * r, Y = Yn * r;
+                    const float fx = floorf(X), fy = floorf(Y);
+                    const float ax = X - fx, ay = Y - fy;
+                    int ix = (int) fx, iy = (int) fy;
+                    if constexpr (!POSW) {
+                        // W may be tiny or negative here: keep the taps inside the table
+                        ix = min(max(ix, -1), mw);
+                        iy = min(max(iy, -1), P.mh);
+                    }
+                    const float4 q = quads[iy * mw + ix];
+                    const float v = fmaf(ay, fmaf(ax, q.w, q.z), fmaf(ax, q.y, q.x));
+                    if constexpr (COSTK == 0) {
+                        part[g][0] += fabsf(v - f);
+                    } else {
+                        part[g][0] += f2;
+                        part[g][1] = fmaf(v, v, part[g][1]);
+                        if constexpr (COSTK == 1) part[g][2] = fmaf(f, v, part[g][2]);
+                        else part[g][2] = fmaf(v - f, v - f, part[g][2]);
+                    }
+                }
+            }
+            if (++since == kFlush) {
+                since = 0;
+#pragma unroll
+                for (int g = 0; g < GPT; ++g)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        acc[g][k] += (double) part[g][k];
+                        part[g][k] = 0.0f;
+                    }
             }
         }
-        if (best != ~0ull && best < __hip_atomic_load(P.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMin(P.key, best);
+#pragma unroll
+        for (int g = 0; g < GPT; ++g)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[g][k] += (double) part[g][k];
     }
 }
+
+template <int COSTK, typename FT, int GPT>
+__global__ __launch_bounds__(kThreads) void search_fast_kernel(const SearchParams P) {
+    __shared__ float s_coef[GPT][12];
+    __shared__ double s_red[4][GPT][4];
+    __shared__ float s_cost[GPT];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int32_t base = (int32_t) blockIdx.x * GPT;
+
+    block_coefficients<GPT>(P, base, s_coef);
+    __syncthreads();
+    bool posw = true;
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) posw = posw && s_coef[g][9] != 0.0f;
+
+    double acc[GPT][3];
+    uint32_t cnt[GPT];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) { acc[g][0] = acc[g][1] = acc[g][2] = 0.0; cnt[g] = 0u; }
+    if (posw) fast_sweep<COSTK, FT, GPT, true>(P, s_coef, acc, cnt);
+    else fast_sweep<COSTK, FT, GPT, false>(P, s_coef, acc, cnt);
+
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const double v0 = wave_sum(acc[g][0]);
+        const double v1 = wave_sum(acc[g][1]);
+        const double v2 = wave_sum(acc[g][2]);
+        if (lane == 0) {
+            s_red[wave][g][0] = v0;
+            s_red[wave][g][1] = v1;
+            s_red[wave][g][2] = v2;
+            s_red[wave][g][3] = (double) cnt[g];   // wave-uniform ballot count
+        }
+    }
+    __syncthreads();
+    if (tid < GPT) {
+        double t[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = s_red[0][tid][k] + s_red[1][tid][k] + s_red[2][tid][k] + s_red[3][tid][k];
+        if constexpr (COSTK == 0) s_cost[tid] = finish_cost<kFamSAD>(P.cost, t, false);
+        else s_cost[tid] = finish_cost<kFamNCC>(P.cost, t, false);
+    }
+    __syncthreads();
+    if (tid == 0) emit<GPT>(P, base, s_cost);
+}
+
+// ----------------------------------------------------------- prep kernels
 
 __global__ void fill_u64_kernel(unsigned long long *p, int n, unsigned long long v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
-__global__ void fill_u8_kernel(uint8_t *p, size_t n, uint8_t v) {
-    const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+// Copies the moving image into the workspace followed by its zero pad.
+template <typename T>
+__global__ void pad_copy_kernel(const T *src, T *dst, size_t n, size_t total) {
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t) gridDim.x * blockDim.x)
+        dst[i] = i < n ? src[i] : (T) 0;
+}
+
+// Bilinear tap table of a one-channel moving image: entry L (for L in
+// [-(W+1), W*H + W]) holds (p[L], p[L+1]-p[L], p[L+W]-p[L],
+// p[L+W+1]-p[L+W]-p[L+1]+p[L]) with p = 0 outside [0, W*H), so a sample at
+// linear tap L with weights (ax, ay) is q.x + ax q.y + ay (q.z + ax q.w).
+template <typename T>
+__global__ void quad_kernel(const T *src, float4 *dst, int W, int H) {
+    const long n = (long) W * H;
+    const long total = n + 2L * W + 2;
+    for (long j = (long) blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long) gridDim.x * blockDim.x) {
+        const long L = j - (W + 1);
+        auto P = [&](long i) { return (i >= 0 && i < n) ? (float) src[i] : 0.0f; };
+        const float tl = P(L), tr = P(L + 1), bl = P(L + W), br = P(L + W + 1);
+        dst[j] = make_float4(tl, tr - tl, bl - tl, (br - bl) - (tr - tl));
+    }
 }
 
 // find_extrema over an arbitrary device array: grid-stride keys, workgroup
@@ -320,45 +539,88 @@ __global__ __launch_bounds__(256) void extrema_kernel(const float *v, int64_t n,
 // ------------------------------------------------------------------ host
 
 template <int FAM, int SAMP, typename MT, typename FT, int GPT>
-static hipError_t launch_family(const SearchParams &P, hipStream_t stream) {
+static hipError_t launch_exact(const SearchParams &P, hipStream_t stream) {
     const int blocks = (P.grid.n + GPT - 1) / GPT;
     const size_t lds = FAM == kFamMI ? (size_t) GPT * P.bins * P.bins * sizeof(uint32_t) : 0;
-    hipLaunchKernelGGL((search_kernel<FAM, SAMP, MT, FT, GPT>), dim3(blocks), dim3(kThreads), lds, stream, P);
+    hipLaunchKernelGGL((search_exact_kernel<FAM, SAMP, MT, FT, GPT>), dim3(blocks), dim3(kThreads), lds, stream, P);
     return hipGetLastError();
 }
 
 template <int FAM, int SAMP, int G>
-static hipError_t dispatch_pix_g(const SearchParams &P, int mpix, int fpix, hipStream_t stream) {
-    if (mpix == CGS_PIX_U8 && fpix == CGS_PIX_U8) return launch_family<FAM, SAMP, uint8_t, uint8_t, G>(P, stream);
-    if (mpix == CGS_PIX_F32 && fpix == CGS_PIX_F32) return launch_family<FAM, SAMP, float, float, G>(P, stream);
-    if (mpix == CGS_PIX_U8 && fpix == CGS_PIX_F32) return launch_family<FAM, SAMP, uint8_t, float, G>(P, stream);
-    return launch_family<FAM, SAMP, float, uint8_t, G>(P, stream);
+static hipError_t exact_pix(const SearchParams &P, int mpix, int fpix, hipStream_t stream) {
+    if (mpix == CGS_PIX_U8 && fpix == CGS_PIX_U8) return launch_exact<FAM, SAMP, uint8_t, uint8_t, G>(P, stream);
+    if (mpix == CGS_PIX_F32 && fpix == CGS_PIX_F32) return launch_exact<FAM, SAMP, float, float, G>(P, stream);
+    if (mpix == CGS_PIX_U8 && fpix == CGS_PIX_F32) return launch_exact<FAM, SAMP, uint8_t, float, G>(P, stream);
+    return launch_exact<FAM, SAMP, float, uint8_t, G>(P, stream);
 }
 
 // Grid points per workgroup: 8 for the register-accumulating costs; for MI the
 // LDS histograms (GPT * bins^2 * 4 B) stay at <= 32 KiB: 8 up to 32 bins, 2 above.
 template <int FAM, int SAMP>
-static hipError_t dispatch_pix(const SearchParams &P, int mpix, int fpix, hipStream_t stream) {
-    if (FAM == kFamMI && P.bins > 32) return dispatch_pix_g<FAM, SAMP, 2>(P, mpix, fpix, stream);
-    return dispatch_pix_g<FAM, SAMP, 8>(P, mpix, fpix, stream);
+static hipError_t exact_gpt(const SearchParams &P, int mpix, int fpix, hipStream_t stream) {
+    if (FAM == kFamMI && P.bins > 32) return exact_pix<FAM, SAMP, 2>(P, mpix, fpix, stream);
+    return exact_pix<FAM, SAMP, 8>(P, mpix, fpix, stream);
 }
 
-static hipError_t dispatch(const SearchParams &P, int sampler, int mpix, int fpix, hipStream_t stream) {
-    int fam = kFamSAD;
-    if (P.cost == CGS_COST_NCC || P.cost == CGS_COST_SQD) fam = kFamNCC;
-    if (P.cost == CGS_COST_MI || P.cost == CGS_COST_MI_PLAIN) fam = kFamMI;
+static int family_of(int cost) {
+    if (cost == CGS_COST_NCC || cost == CGS_COST_SQD) return kFamNCC;
+    if (cost == CGS_COST_MI || cost == CGS_COST_MI_PLAIN) return kFamMI;
+    return kFamSAD;
+}
+
+static hipError_t dispatch_exact(const SearchParams &P, int sampler, int mpix, int fpix, hipStream_t stream) {
+    const int fam = family_of(P.cost);
     if (sampler == CGS_SAMPLER_NEAREST) {
-        if (fam == kFamSAD) return dispatch_pix<kFamSAD, CGS_SAMPLER_NEAREST>(P, mpix, fpix, stream);
-        if (fam == kFamNCC) return dispatch_pix<kFamNCC, CGS_SAMPLER_NEAREST>(P, mpix, fpix, stream);
-        return dispatch_pix<kFamMI, CGS_SAMPLER_NEAREST>(P, mpix, fpix, stream);
+        if (fam == kFamSAD) return exact_gpt<kFamSAD, CGS_SAMPLER_NEAREST>(P, mpix, fpix, stream);
+        if (fam == kFamNCC) return exact_gpt<kFamNCC, CGS_SAMPLER_NEAREST>(P, mpix, fpix, stream);
+        return exact_gpt<kFamMI, CGS_SAMPLER_NEAREST>(P, mpix, fpix, stream);
     }
-    if (fam == kFamSAD) return dispatch_pix<kFamSAD, CGS_SAMPLER_BILINEAR>(P, mpix, fpix, stream);
-    if (fam == kFamNCC) return dispatch_pix<kFamNCC, CGS_SAMPLER_BILINEAR>(P, mpix, fpix, stream);
-    return dispatch_pix<kFamMI, CGS_SAMPLER_BILINEAR>(P, mpix, fpix, stream);
+    if (fam == kFamSAD) return exact_gpt<kFamSAD, CGS_SAMPLER_BILINEAR>(P, mpix, fpix, stream);
+    if (fam == kFamNCC) return exact_gpt<kFamNCC, CGS_SAMPLER_BILINEAR>(P, mpix, fpix, stream);
+    return exact_gpt<kFamMI, CGS_SAMPLER_BILINEAR>(P, mpix, fpix, stream);
 }
 
-// Runs one plain search: pads the moving image into the workspace, uploads
-// the (cos, sin) table, clears the key, launches, and reads the key back.
+// Grid points per workgroup of the fast kernel: 4 or 8 (CGS_FAST_GPT in the
+// environment overrides the default 4 for tuning sweeps).
+static int fast_gpt() {
+    static int g = [] {
+        const char *e = std::getenv("CGS_FAST_GPT");
+        const int v = e ? std::atoi(e) : 4;
+        return v == 8 ? 8 : 4;
+    }();
+    return g;
+}
+
+template <int COSTK, int G>
+static hipError_t fast_launch(const SearchParams &P, int fpix, hipStream_t stream) {
+    const int blocks = (P.grid.n + G - 1) / G;
+    if (fpix == CGS_PIX_U8)
+        hipLaunchKernelGGL((search_fast_kernel<COSTK, uint8_t, G>), dim3(blocks), dim3(kThreads), 0, stream, P);
+    else
+        hipLaunchKernelGGL((search_fast_kernel<COSTK, float, G>), dim3(blocks), dim3(kThreads), 0, stream, P);
+    return hipGetLastError();
+}
+
+template <int COSTK>
+static hipError_t fast_pix(const SearchParams &P, int fpix, hipStream_t stream) {
+    return fast_gpt() == 8 ? fast_launch<COSTK, 8>(P, fpix, stream) : fast_launch<COSTK, 4>(P, fpix, stream);
+}
+
+static hipError_t dispatch_fast(const SearchParams &P, int fpix, hipStream_t stream) {
+    if (P.cost == CGS_COST_NCC) return fast_pix<1>(P, fpix, stream);
+    if (P.cost == CGS_COST_SQD) return fast_pix<2>(P, fpix, stream);
+    return fast_pix<0>(P, fpix, stream);
+}
+
+// The fast kernel covers bilinear SAD/NCC/SQD on one channel; everything
+// else (nearest neighbour, MI, several channels) runs the exact kernel.
+static bool use_fast(const cgs_image *moved, const cgs_search_opts *o) {
+    return o->sampler == CGS_SAMPLER_BILINEAR && moved->channels == 1 && family_of(o->cost) != kFamMI &&
+           moved->width <= 65536 && moved->height <= 65536;
+}
+
+// Runs one plain search: stages the moving image (padded copy or tap table),
+// uploads the (cos, sin) table, clears the key, launches, reads the key back.
 static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
                              const cgs_search_opts *o, float *result, cgs_argmin *out, hipStream_t stream,
                              uint32_t *hist_dump_host) {
@@ -369,37 +631,60 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     if (st != CGS_OK) return st;
     std::lock_guard<std::mutex> lock(ws->mu);
 
+    const bool fast = use_fast(moved, o);
     const size_t esz = moved->pixtype == CGS_PIX_U8 ? 1 : 4;
     const size_t nmov = (size_t) moved->width * moved->height * moved->channels;
-    const size_t npad = (size_t) moved->channels * (moved->width + 1);
+    const size_t ntot = nmov + mov_pad_elems(moved->width, moved->channels);
+    const size_t nquad = (size_t) moved->width * moved->height + 2 * (size_t) moved->width + 2;
     const int levels0 = (o->param_mode == CGS_PARAM_CENTRED || o->param_mode == CGS_PARAM_NONCENTRED)
                             ? (int) grid->count[0] : 0;
     const size_t B2 = (size_t) o->mi_bins * o->mi_bins;
-    if ((st = ws->reserve((nmov + npad) * esz, levels0, hist_dump_host ? B2 : 0)) != CGS_OK) return st;
-
-    HIP_TRY(hipMemcpyAsync(ws->mov, moved->data, nmov * esz, hipMemcpyDeviceToDevice, stream));
-    {
-        const size_t nb = npad * esz;
-        hipLaunchKernelGGL(fill_u8_kernel, dim3((unsigned) ((nb + 255) / 256)), dim3(256), 0, stream,
-                           (uint8_t *) ws->mov + nmov * esz, nb, (uint8_t) 0);
+    if ((st = ws->key.ensure(64)) != CGS_OK || (st = ws->key_host.ensure(64, true)) != CGS_OK) return st;
+    if (fast) {
+        if ((st = ws->quads.ensure(nquad * sizeof(float4))) != CGS_OK) return st;
+    } else if ((st = ws->mov.ensure(ntot * esz)) != CGS_OK) {
+        return st;
     }
+    if (levels0 > 0 && ((st = ws->sincos.ensure(8 * (size_t) levels0)) != CGS_OK ||
+                        (st = ws->sincos_host.ensure(8 * (size_t) levels0, true)) != CGS_OK))
+        return st;
+    if (hist_dump_host && (st = ws->hist.ensure(B2 * sizeof(uint32_t))) != CGS_OK) return st;
+
+    const unsigned prep_blocks = 1024;
+    if (fast) {
+        if (moved->pixtype == CGS_PIX_U8)
+            hipLaunchKernelGGL(quad_kernel<uint8_t>, dim3(prep_blocks), dim3(256), 0, stream,
+                               (const uint8_t *) moved->data, (float4 *) ws->quads.ptr, moved->width, moved->height);
+        else
+            hipLaunchKernelGGL(quad_kernel<float>, dim3(prep_blocks), dim3(256), 0, stream,
+                               (const float *) moved->data, (float4 *) ws->quads.ptr, moved->width, moved->height);
+    } else if (moved->pixtype == CGS_PIX_U8) {
+        hipLaunchKernelGGL(pad_copy_kernel<uint8_t>, dim3(prep_blocks), dim3(256), 0, stream,
+                           (const uint8_t *) moved->data, (uint8_t *) ws->mov.ptr, nmov, ntot);
+    } else {
+        hipLaunchKernelGGL(pad_copy_kernel<float>, dim3(prep_blocks), dim3(256), 0, stream,
+                           (const float *) moved->data, (float *) ws->mov.ptr, nmov, ntot);
+    }
+    HIP_TRY(hipGetLastError());
     if (levels0 > 0) {
+        float *h = (float *) ws->sincos_host.ptr;
         for (int i = 0; i < levels0; ++i) {
             const float theta = fmaf((float) i, grid->resolution[0], grid->start[0]);
-            ws->sincos_host[2 * i] = (float) std::cos((double) theta);
-            ws->sincos_host[2 * i + 1] = (float) std::sin((double) theta);
+            h[2 * i] = (float) std::cos((double) theta);
+            h[2 * i + 1] = (float) std::sin((double) theta);
         }
-        HIP_TRY(hipMemcpyAsync(ws->sincos, ws->sincos_host, sizeof(float) * 2 * levels0, hipMemcpyHostToDevice,
-                               stream));
+        HIP_TRY(hipMemcpyAsync(ws->sincos.ptr, h, sizeof(float) * 2 * levels0, hipMemcpyHostToDevice, stream));
     }
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, stream, ws->key, 1, ~0ull);
+    unsigned long long *key = (unsigned long long *) ws->key.ptr;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, stream, key, 1, ~0ull);
 
     SearchParams P{};
     fill_grid_dev(grid, &P.grid);
-    P.sincos = levels0 > 0 ? (const float2 *) ws->sincos : nullptr;
+    P.sincos = levels0 > 0 ? (const float2 *) ws->sincos.ptr : nullptr;
     P.param_mode = o->param_mode;
     P.cost = o->cost;
-    P.mov = ws->mov;
+    P.mov = ws->mov.ptr;
+    P.quads = fast ? (const float4 *) ws->quads.ptr + (moved->width + 1) : nullptr;
     P.mw = moved->width;
     P.mh = moved->height;
     P.mc = moved->channels;
@@ -409,20 +694,21 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     P.cx = (float) moved->width / 2;
     P.cy = (float) moved->height / 2;
     P.result = result;
-    P.key = ws->key;
+    P.key = key;
     P.tie_mode = o->tie_mode;
     P.bins = o->mi_bins;
-    P.hist_dump = hist_dump_host ? ws->hist : nullptr;
+    P.hist_dump = hist_dump_host ? (uint32_t *) ws->hist.ptr : nullptr;
     const bool timed = (o->flags & CGS_FLAG_TIME_KERNEL) != 0;
     if (timed) HIP_TRY(ws->record_start(stream));
-    HIP_TRY(dispatch(P, o->sampler, moved->pixtype, fixed->pixtype, stream));
+    HIP_TRY(fast ? dispatch_fast(P, fixed->pixtype, stream) : dispatch_exact(P, o->sampler, moved->pixtype,
+                                                                             fixed->pixtype, stream));
     if (timed) HIP_TRY(ws->record_stop(stream));
-    HIP_TRY(hipMemcpyAsync(ws->key_host, ws->key, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(ws->key_host.ptr, key, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     if (hist_dump_host)
-        HIP_TRY(hipMemcpyAsync(hist_dump_host, ws->hist, B2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(hist_dump_host, ws->hist.ptr, B2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     if (timed) HIP_TRY(ws->elapsed_ms());
-    if (out) *out = decode_key(*ws->key_host, o->tie_mode);
+    if (out) *out = decode_key(*(unsigned long long *) ws->key_host.ptr, o->tie_mode);
     return CGS_OK;
 }
 
@@ -432,7 +718,7 @@ using namespace cgs;
 
 extern "C" {
 
-const char *cgs_version(void) { return "cgs 0.1.0 gfx950"; }
+const char *cgs_version(void) { return "cgs 0.2.0 gfx950"; }
 
 void cgs_default_opts(cgs_search_opts *o) {
     if (!o) return;
@@ -460,13 +746,9 @@ cgs_status cgs_eval_point(const cgs_image *moved, const cgs_image *fixed, const 
     cgs_status st = cgs_grid_init(&g, dim, params, params, cnt);
     if (st != CGS_OK) return st;
     cgs_argmin am;
-    float *dres = nullptr;
-    if (hipMalloc(&dres, sizeof(float)) != hipSuccess) return set_error(CGS_ERR_HIP, "hipMalloc");
-    st = run_search(&g, moved, fixed, opts, dres, &am, (hipStream_t) stream,
+    st = run_search(&g, moved, fixed, opts, nullptr, &am, (hipStream_t) stream,
                     (opts && (opts->cost == CGS_COST_MI || opts->cost == CGS_COST_MI_PLAIN)) ? hist_out : nullptr);
-    if (st == CGS_OK && hipMemcpy(cost_out, dres, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
-        st = set_error(CGS_ERR_HIP, "hipMemcpy");
-    hipFree(dres);
+    if (st == CGS_OK) *cost_out = am.index < 0 ? NAN : am.value;
     return st;
 }
 
@@ -480,14 +762,16 @@ cgs_status cgs_find_extrema(const float *values, int64_t n, int32_t tie_mode, cg
     cgs_status st = workspace_for_current_device(&ws);
     if (st != CGS_OK) return st;
     std::lock_guard<std::mutex> lock(ws->mu);
+    if ((st = ws->key.ensure(64)) != CGS_OK || (st = ws->key_host.ensure(64, true)) != CGS_OK) return st;
     hipStream_t s = (hipStream_t) stream;
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, ws->key, 1, ~0ull);
-    int blocks = (int) std::min<int64_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(extrema_kernel, dim3(blocks), dim3(256), 0, s, values, n, tie_mode, ws->key);
+    unsigned long long *key = (unsigned long long *) ws->key.ptr;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, key, 1, ~0ull);
+    const int blocks = (int) std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(extrema_kernel, dim3(blocks), dim3(256), 0, s, values, n, tie_mode, key);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(ws->key_host, ws->key, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ws->key_host.ptr, key, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    *out = decode_key(*ws->key_host, tie_mode);
+    *out = decode_key(*(unsigned long long *) ws->key_host.ptr, tie_mode);
     return CGS_OK;
 }
 

@@ -10,6 +10,12 @@
  *   - compile with -ffp-contract=off: every + - * / is one IEEE-754 binary32
  *     (or binary64 where stated) operation; FMA appears only where fmaf() is
  *     written explicitly;
+ *   - the three projective forms of the warp are evaluated per pixel as
+ *     F(x, y) = fmaf(B, y, fmaf(A, x, C)) with C = P - Q rounded once (the
+ *     reference writes (A*x + B*y + P) - Q and leaves contraction to nvcc);
+ *   - inImage is decided on the exact rational quotient of those rounded
+ *     forms (0 <= X/W < width, 0 <= Y/W < height, by sign and fmaf tests);
+ *     the sample point itself is the correctly rounded quotient;
  *   - sin/cos of the rotation angle: (float)sin((double)theta), likewise cos;
  *   - division and sqrt: IEEE correctly rounded;
  *   - NCC / SQD / bilinear-SAD sums: binary64 accumulation of binary32 terms;
@@ -167,10 +173,10 @@ void orc_homography(int mode, const float *params, float cx, float cy, float *h)
     else for (int i = 0; i < 8; i++) h[i] = params[i];
 }
 
-/* Per-grid-point coefficients of the three projective forms evaluated per
- * pixel as  F(x, y) = ((A*x + B*y) + P) - Q  (each op rounded), stored as
- * coef[4*k + {0,1,2,3}] = {A, B, P, Q} for k = 0 (X), 1 (Y), 2 (W).
- * The moving-image sample point is (X/W, Y/W).
+/* Per-grid-point coefficients of the three projective forms, evaluated per
+ * pixel as F(x, y) = fmaf(B, y, fmaf(A, x, C)); coef[3*k + {0,1,2}] =
+ * {A, B, C} for k = 0 (X), 1 (Y), 2 (W).  The moving-image sample point is
+ * (X/W, Y/W).
  *   inverse warp, calcNewCoordH (cudaImageFunctions.cuh:282-284):
  *     W = (h21h32 - h22h31)x + (h12h31 - h11h32)y + h11h22 - h12h21
  *     X = (h22 - h23h32)x + (h13h32 - h12)y + h12h23 - h13h22
@@ -181,9 +187,9 @@ void orc_homography(int mode, const float *params, float cx, float cy, float *h)
  */
 void orc_warp_coeffs(int mode, const float *params, float cx, float cy, float *coef) {
     if (mode == ORC_PARAM_TRANSLATION) {
-        const float c[12] = {1.0f, 0.0f, params[0], 0.0f,
-                             0.0f, 1.0f, params[1], 0.0f,
-                             0.0f, 0.0f, 1.0f, 0.0f};
+        const float c[9] = {1.0f, 0.0f, params[0],
+                            0.0f, 1.0f, params[1],
+                            0.0f, 0.0f, 1.0f};
         memcpy(coef, c, sizeof(c));
         return;
     }
@@ -192,26 +198,37 @@ void orc_warp_coeffs(int mode, const float *params, float cx, float cy, float *c
     const float h11 = h[0], h12 = h[1], h13 = h[2], h21 = h[3], h22 = h[4], h23 = h[5];
     const float h31 = h[6], h32 = h[7];
     if (mode == ORC_PARAM_RAW_H) {
-        const float c[12] = {h11, h12, h13, 0.0f,
-                             h21, h22, h23, 0.0f,
-                             h31, h32, 1.0f, 0.0f};
+        const float c[9] = {h11, h12, h13,
+                            h21, h22, h23,
+                            h31, h32, 1.0f};
         memcpy(coef, c, sizeof(c));
         return;
     }
-    coef[0] = h22 - h23 * h32;   coef[1] = h13 * h32 - h12;  coef[2] = h12 * h23;  coef[3] = h13 * h22;
-    coef[4] = h23 * h31 - h21;   coef[5] = h11 - h13 * h31;  coef[6] = h13 * h21;  coef[7] = h11 * h23;
-    coef[8] = h21 * h32 - h22 * h31; coef[9] = h12 * h31 - h11 * h32; coef[10] = h11 * h22; coef[11] = h12 * h21;
+    coef[0] = h22 - h23 * h32;       coef[1] = h13 * h32 - h12;       coef[2] = h12 * h23 - h13 * h22;
+    coef[3] = h23 * h31 - h21;       coef[4] = h11 - h13 * h31;       coef[5] = h13 * h21 - h11 * h23;
+    coef[6] = h21 * h32 - h22 * h31; coef[7] = h12 * h31 - h11 * h32; coef[8] = h11 * h22 - h12 * h21;
 }
 
 static inline float form(const float *c, float x, float y) {
-    return ((c[0] * x + c[1] * y) + c[2]) - c[3];
+    return fmaf(c[1], y, fmaf(c[0], x, c[2]));
 }
 
-void orc_warp_point(const float *coef, int x, int y, float *out_xy) {
+/* 0 <= n/w < lim on the exact quotient: the sign of n - lim*w is exact in
+ * one fmaf (the exact value is rounded once and rounding keeps the sign). */
+static inline int quotient_in(float n, float w, float lim) {
+    if (w > 0) return n >= 0 && fmaf(-lim, w, n) < 0;
+    if (w < 0) return n <= 0 && fmaf(-lim, w, n) > 0;
+    return 0;
+}
+
+/* Sample point of pixel (x, y): returns inImage and the rounded quotients. */
+int orc_warp_point(const float *coef, int x, int y, float *out_xy, float mw, float mh) {
     const float fx = (float) x, fy = (float) y;
-    const float w = form(coef + 8, fx, fy);
-    out_xy[0] = form(coef, fx, fy) / w;
-    out_xy[1] = form(coef + 4, fx, fy) / w;
+    const float w = form(coef + 6, fx, fy);
+    const float xn = form(coef, fx, fy), yn = form(coef + 3, fx, fy);
+    out_xy[0] = xn / w;
+    out_xy[1] = yn / w;
+    return quotient_in(xn, w, mw) && quotient_in(yn, w, mh);
 }
 
 /* -------------------------------------------------------------- sampling */
@@ -237,16 +254,19 @@ static inline int64_t lin_index_f(const orc_image *im, float y, float x, int c) 
     return i;
 }
 
-/* valueAt_nearest_neighbor (cudaTensor.cuh:557-559). */
+/* valueAt_nearest_neighbor (cudaTensor.cuh:557-559) after the inImage test. */
+static inline float nearest_in(const orc_image *im, float y, float x, int c) {
+    return pix(im, lin_index_f(im, y, x, c));
+}
+
 float orc_sample_nearest(const orc_image *im, float y, float x, int c) {
     if (!in_image(im, y, x)) return 0.0f;
-    return pix(im, lin_index_f(im, y, x, c));
+    return nearest_in(im, y, x, c);
 }
 
 /* valueAt_bilinear (cudaTensor.cuh:561-575); ceil(x) == W reads the next
  * row's first pixel through the linear index, as the reference does. */
-float orc_sample_bilinear(const orc_image *im, float y, float x, int c) {
-    if (!in_image(im, y, x)) return 0.0f;
+static inline float bilinear_in(const orc_image *im, float y, float x, int c) {
     const float fy = floorf(y), fx = floorf(x), cy = ceilf(y), cx = ceilf(x);
     const float tlc = pix(im, lin_index_f(im, fy, fx, c));
     const float trc = pix(im, lin_index_f(im, fy, cx, c));
@@ -259,9 +279,13 @@ float orc_sample_bilinear(const orc_image *im, float y, float x, int c) {
     return (1.0f - alpha_y) * value_top + alpha_y * value_bottom;
 }
 
+float orc_sample_bilinear(const orc_image *im, float y, float x, int c) {
+    if (!in_image(im, y, x)) return 0.0f;
+    return bilinear_in(im, y, x, c);
+}
+
 static inline float sample(const orc_image *im, float y, float x, int c, int sampler) {
-    return sampler == ORC_NEAREST ? orc_sample_nearest(im, y, x, c)
-                                  : orc_sample_bilinear(im, y, x, c);
+    return sampler == ORC_NEAREST ? nearest_in(im, y, x, c) : bilinear_in(im, y, x, c);
 }
 
 /* ----------------------------------------------------------------- costs */
@@ -287,7 +311,7 @@ static inline int mi_bin(float v, int bins) {
 float orc_cost_point(const orc_image *moved, const orc_image *fixed, int cost, int sampler,
                      int mode, const float *params, int bins, uint32_t *hist_out,
                      double *aux_out) {
-    float coef[12];
+    float coef[9];
     const float cx = (float) moved->width / 2, cy = (float) moved->height / 2;
     orc_warp_coeffs(mode, params, cx, cy, coef);
     const int C = fixed->channels;
@@ -303,8 +327,7 @@ float orc_cost_point(const orc_image *moved, const orc_image *fixed, int cost, i
     for (int y = 0; y < fixed->height; y++) {
         for (int x = 0; x < fixed->width; x++) {
             float xy[2];
-            orc_warp_point(coef, x, y, xy);
-            if (!in_image(moved, xy[1], xy[0])) continue;
+            if (!orc_warp_point(coef, x, y, xy, (float) moved->width, (float) moved->height)) continue;
             for (int c = 0; c < C; c++) {
                 const float m = sample(moved, xy[1], xy[0], c, sampler);
                 const float f = pix(fixed, ((int64_t) y * fixed->width + x) * C + c);

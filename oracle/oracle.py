@@ -46,7 +46,8 @@ def lib():
         L.orc_grid_point.argtypes = [ctypes.c_int, fp, fp, fp, i64, fp]
         L.orc_homography.argtypes = [ctypes.c_int, fp, ctypes.c_float, ctypes.c_float, fp]
         L.orc_warp_coeffs.argtypes = [ctypes.c_int, fp, ctypes.c_float, ctypes.c_float, fp]
-        L.orc_warp_point.argtypes = [fp, ctypes.c_int, ctypes.c_int, fp]
+        L.orc_warp_point.argtypes = [fp, ctypes.c_int, ctypes.c_int, fp, ctypes.c_float, ctypes.c_float]
+        L.orc_warp_point.restype = ctypes.c_int
         L.orc_sample_nearest.argtypes = [ctypes.POINTER(_Image), ctypes.c_float, ctypes.c_float, ctypes.c_int]
         L.orc_sample_nearest.restype = ctypes.c_float
         L.orc_sample_bilinear.argtypes = [ctypes.POINTER(_Image), ctypes.c_float, ctypes.c_float, ctypes.c_int]
@@ -122,16 +123,17 @@ def homography(params, cx, cy, mode=PARAM_CENTRED):
 def warp_coeffs(params, cx, cy, mode=PARAM_CENTRED):
     p = np.zeros(8, np.float32)
     p[:len(params)] = params
-    c = np.zeros(12, np.float32)
+    c = np.zeros(9, np.float32)
     lib().orc_warp_coeffs(mode, _f32p(p), cx, cy, _f32p(c))
     return c
 
 
-def warp_point(coef, x, y):
+def warp_point(coef, x, y, mw=1e30, mh=1e30):
+    """Returns ((X, Y) rounded sample point, inImage of the exact quotient)."""
     c = np.ascontiguousarray(coef, np.float32)
     out = np.zeros(2, np.float32)
-    lib().orc_warp_point(_f32p(c), int(x), int(y), _f32p(out))
-    return out
+    inside = lib().orc_warp_point(_f32p(c), int(x), int(y), _f32p(out), float(mw), float(mh))
+    return out, bool(inside)
 
 
 def sample(img, y, x, sampler, c=0):

@@ -100,7 +100,7 @@ def test_inverse_warp_matches_fp64_inverse(oracle):
         coef = oracle.warp_coeffs(p, 32.0, 32.0)
         for x, y in [(0, 0), (63, 0), (0, 63), (31, 17), (63, 63)]:
             q = Hi @ np.array([x, y, 1.0])
-            got = oracle.warp_point(coef, x, y)
+            got, _ = oracle.warp_point(coef, x, y)
             assert np.allclose(got, q[:2] / q[2], atol=2e-3), (p, x, y, got, q[:2] / q[2])
 
 
@@ -109,10 +109,38 @@ def test_integer_translation_is_exact(oracle):
     for tx, ty in [(3, -2), (-4, 4), (0, 1)]:
         coef = oracle.warp_coeffs([0, 1, 1, 0, tx, ty, 0, 0], 32.0, 32.0)
         for x, y in [(0, 0), (5, 9), (63, 63)]:
-            got = oracle.warp_point(coef, x, y)
+            got, inside = oracle.warp_point(coef, x, y, 64, 64)
             assert got[0] == x - tx and got[1] == y - ty
+            assert inside == (0 <= x - tx < 64 and 0 <= y - ty < 64)
     coef = oracle.warp_coeffs([2.5, -1.0], 0, 0, mode=oracle.PARAM_TRANSLATION)
-    assert np.array_equal(oracle.warp_point(coef, 4, 7), np.array([6.5, 6.0], np.float32))
+    assert np.array_equal(oracle.warp_point(coef, 4, 7)[0], np.array([6.5, 6.0], np.float32))
+
+
+def test_in_image_on_exact_quotient(oracle):
+    """The declared inImage (exact quotient of the rounded forms) agrees with
+    the reference's inImage of the rounded quotient away from a 2^-24-wide
+    window at the upper edges, and is exact on a constructed window case."""
+    rng = np.random.default_rng(4)
+    disagree = 0
+    for _ in range(3000):
+        p = np.array([rng.uniform(-0.1, 0.1), rng.uniform(0.9, 1.1), rng.uniform(0.9, 1.1), rng.uniform(-.1, .1),
+                      rng.uniform(-20, 20), rng.uniform(-20, 20), rng.uniform(-2e-3, 2e-3), rng.uniform(-2e-3, 2e-3)],
+                     np.float32)
+        coef = oracle.warp_coeffs(p, 32.0, 24.0)
+        for x, y in rng.integers(-5, 70, (8, 2)):
+            (X, Y), inside = oracle.warp_point(coef, x, y, 64, 48)
+            disagree += inside != (0 <= X < 64 and 0 <= Y < 48)
+    assert disagree == 0
+    # Y/W with Y = 48 + 2^-18, W = 1 + 2^-23: the exact quotient is 48 - 2^-19 + 2^-42,
+    # below the height 48 but rounding to 48.0 -> inside by the declared rule, while
+    # inImage of the rounded quotient says outside
+    W = np.float32(1 + 2 ** -23)
+    Yn = np.float32(48 + 2 ** -18)
+    coef = np.array([0, 0, 10, 0, 0, Yn, 0, 0, W], np.float32)
+    (X, Y), inside = oracle.warp_point(coef, 0, 0, 64, 48)
+    assert Y == np.float32(48.0) and inside
+    coef[5] = np.float32(48 + 2 ** -17)       # exact quotient above 48: outside
+    assert not oracle.warp_point(coef, 0, 0, 64, 48)[1]
 
 
 # -------------------------------------------------------------- samplers
