@@ -42,7 +42,7 @@ struct SearchParams {
     int param_mode;
     int cost;
     const void *mov;        // exact kernel: moving image + zero pad
-    const float4 *quads;    // fast kernel: tap table, indexable from -(W+1)
+    const float4 *quads;    // fast kernel: tap table; entry j holds tap L = j - (W+1)
     int mw, mh, mc;
     const void *fix;
     int fw, fh;
@@ -339,8 +339,46 @@ __global__ __launch_bounds__(kThreads) void search_exact_kernel(const SearchPara
 // step; wider images are swept in 256-column groups.  The B coefficients are
 // wave-uniform and held in SGPRs.  Per-thread FP32 partial sums are folded
 // into FP64 every kFlush rows.
+//
+// Taps are read with raw buffer loads from the tap table: the offset of tap
+// L is (L + W + 1) * 16 bytes, and a lane outside the image gets an offset
+// past the table's end, which the hardware range check turns into zeros --
+// so the positive-W sweep has no per-lane branch and issues all GPT loads
+// before it consumes any.
 
 constexpr int kFlush = 16;
+constexpr uint32_t kOffsetOut = 0x80000000u;   // beyond any tap table
+
+__device__ __forceinline__ int floor_to_int(float v) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+// Byte offset of tap (ix, iy) in the tap table: 16 * ((iy + 1) * W + ix + 1)
+// = iy * 16W + (16 ix + 16 (W + 1)), one shift-add and one 24-bit mad.
+__device__ __forceinline__ uint32_t tap_offset(int ix, int iy, int w16, int base16) {
+    uint32_t r;
+    const uint32_t t = (uint32_t) ((ix << 4) + base16);
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(iy), "s"(w16), "v"(t));
+    return r;
+}
+
+// Lane select on a wave mask held in SGPRs: mask bit set -> b, else a.
+__device__ __forceinline__ uint32_t mask_select(unsigned long long m, uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(m));
+    return r;
+}
+
+// Wave masks of ordered float comparisons (llvm.amdgcn.fcmp predicates).
+__device__ __forceinline__ unsigned long long mask_ge0(float v) { return __builtin_amdgcn_fcmpf(v, 0.0f, 3); }
+__device__ __forceinline__ unsigned long long mask_lt0(float v) { return __builtin_amdgcn_fcmpf(v, 0.0f, 4); }
+
+__device__ __forceinline__ float4 load_tap(__amdgpu_buffer_rsrc_t rsrc, uint32_t offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, offset, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
 
 template <int COSTK, typename FT, int GPT, bool POSW>
 __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*s_coef)[12], double (*acc)[3],
@@ -352,8 +390,11 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
     const int r0 = tid / cols, xc = tid - r0 * cols;
     const float Wm = (float) P.mw, Hm = (float) P.mh;
     const FT *fix = (const FT *) P.fix;
-    const float4 *quads = P.quads;
     const int mw = P.mw;
+    const int w16 = mw * 16;
+    const int base16 = (mw + 1) * 16;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) P.quads, (short) 0, (int) (((size_t) mw * P.mh + 2 * (size_t) mw + 2) * 16), 0x00020000);
 
     float B[GPT][3];
 #pragma unroll
@@ -380,42 +421,52 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
         for (int it = 0; it < steps; ++it) {
             const int y = r0 + it * rpi;
             const bool vy = valid && y < fh;
+            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(vy);
             const float f = vy ? (float) fix[y * fw + x] : 0.0f;
             const float f2 = f * f;
             const float yf = (float) y;
+            unsigned long long in[GPT];   // wave mask of lanes whose sample is in the image
+            float ax[GPT], ay[GPT];
+            float4 q[GPT];
 #pragma unroll
             for (int g = 0; g < GPT; ++g) {
                 const float Xn = fmaf(B[g][0], yf, col[g][0]);
                 const float Yn = fmaf(B[g][1], yf, col[g][1]);
                 const float Wn = fmaf(B[g][2], yf, col[g][2]);
-                bool in;
-                if constexpr (POSW)
-                    in = Xn >= 0.0f && Yn >= 0.0f && fmaf(-Wm, Wn, Xn) < 0.0f && fmaf(-Hm, Wn, Yn) < 0.0f;
-                else
-                    in = quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm);
-                in = in && vy;
-                cnt[g] += (uint32_t) __popcll(__ballot(in));
-                if (in) {
-                    const float r = __builtin_amdgcn_rcpf(Wn);
-                    const float X = Xn * r, Y = Yn * r;
-                    const float fx = floorf(X), fy = floorf(Y);
-                    const float ax = X - fx, ay = Y - fy;
-                    int ix = (int) fx, iy = (int) fy;
-                    if constexpr (!POSW) {
-                        // W may be tiny or negative here: keep the taps inside the table
-                        ix = min(max(ix, -1), mw);
-                        iy = min(max(iy, -1), P.mh);
-                    }
-                    const float4 q = quads[iy * mw + ix];
-                    const float v = fmaf(ay, fmaf(ax, q.w, q.z), fmaf(ax, q.y, q.x));
-                    if constexpr (COSTK == 0) {
-                        part[g][0] += fabsf(v - f);
-                    } else {
-                        part[g][0] += f2;
-                        part[g][1] = fmaf(v, v, part[g][1]);
-                        if constexpr (COSTK == 1) part[g][2] = fmaf(f, v, part[g][2]);
-                        else part[g][2] = fmaf(v - f, v - f, part[g][2]);
-                    }
+                unsigned long long m;
+                if constexpr (POSW) {
+                    m = mask_ge0(Xn) & mask_ge0(Yn) & mask_lt0(fmaf(-Wm, Wn, Xn)) & mask_lt0(fmaf(-Hm, Wn, Yn));
+                } else {
+                    const bool inside = quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm);
+                    m = __builtin_amdgcn_ballot_w64(inside);
+                }
+                m &= vmask;
+                in[g] = m;
+                cnt[g] += (uint32_t) __builtin_popcountll(m);
+                const float r = __builtin_amdgcn_rcpf(Wn);
+                const float X = Xn * r, Y = Yn * r;
+                int ix = floor_to_int(X), iy = floor_to_int(Y);
+                if constexpr (!POSW) {
+                    // W may be tiny or negative here: keep the taps inside the table
+                    ix = min(max(ix, -1), mw);
+                    iy = min(max(iy, -1), P.mh);
+                }
+                ax[g] = __builtin_amdgcn_fractf(X);
+                ay[g] = __builtin_amdgcn_fractf(Y);
+                q[g] = load_tap(rsrc, mask_select(m, kOffsetOut, tap_offset(ix, iy, w16, base16)));
+            }
+#pragma unroll
+            for (int g = 0; g < GPT; ++g) {
+                // lanes outside read a zero tap: v == 0 exactly when the weights are finite
+                float v = fmaf(ay[g], fmaf(ax[g], q[g].w, q[g].z), fmaf(ax[g], q[g].y, q[g].x));
+                if constexpr (!POSW) v = __uint_as_float(mask_select(in[g], 0u, __float_as_uint(v)));
+                if constexpr (COSTK == 0) {
+                    part[g][0] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint(fabsf(v - f))));
+                } else {
+                    part[g][0] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint(f2)));
+                    part[g][1] = fmaf(v, v, part[g][1]);
+                    if constexpr (COSTK == 1) part[g][2] = fmaf(f, v, part[g][2]);
+                    else part[g][2] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint((v - f) * (v - f))));
                 }
             }
             if (++since == kFlush) {
@@ -684,7 +735,7 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     P.param_mode = o->param_mode;
     P.cost = o->cost;
     P.mov = ws->mov.ptr;
-    P.quads = fast ? (const float4 *) ws->quads.ptr + (moved->width + 1) : nullptr;
+    P.quads = fast ? (const float4 *) ws->quads.ptr : nullptr;   // entry 0 is tap L = -(W+1)
     P.mw = moved->width;
     P.mh = moved->height;
     P.mc = moved->channels;
