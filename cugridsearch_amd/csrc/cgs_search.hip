@@ -70,7 +70,8 @@ __device__ __forceinline__ float sample_nearest(const MT *img, float X, float Y,
     return (float) img[i];
 }
 
-// valueAt_bilinear (include/cudaTensor.cuh:561-575).  The right/bottom taps
+// valueAt_bilinear (include/cudaTensor.cuh:561-575), each lerp declared as
+// fmaf(a, q, (1-a)*p).  The right/bottom taps
 // are read at floor+1; where the coordinate is integral the reference reads
 // ceil == floor instead, and that tap's weight is exactly 0, so the value is
 // the same.  ceil(x) == W lands on the next row through the linear index and
@@ -82,9 +83,9 @@ __device__ __forceinline__ float sample_bilinear(const MT *img, float X, float Y
     const float tl = (float) img[b], tr = (float) img[b + C];
     const float bl = (float) img[b + C * W], br = (float) img[b + C * W + C];
     const float ax = X - fx, ay = Y - fy;
-    const float top = (1.0f - ax) * tl + ax * tr;
-    const float bottom = (1.0f - ax) * bl + ax * br;
-    return (1.0f - ay) * top + ay * bottom;
+    const float top = fmaf(ax, tr, (1.0f - ax) * tl);
+    const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
+    return fmaf(ay, bottom, (1.0f - ay) * top);
 }
 
 __device__ __forceinline__ int mi_bin(float v, int bins) {
@@ -143,8 +144,18 @@ __device__ __forceinline__ void block_coefficients(const SearchParams &P, int32_
             const double w = (double) co[7] * yy + ((double) co[6] * xx + (double) co[8]);
             wmin = w < wmin ? w : wmin;
         }
+        double wmax = -1e300;
+        for (int k = 0; k < 4; ++k) {
+            const double xx = (k & 1) ? P.fw - 1 : 0, yy = (k & 2) ? P.fh - 1 : 0;
+            const double w = (double) co[7] * yy + ((double) co[6] * xx + (double) co[8]);
+            wmax = w > wmax ? w : wmax;
+        }
         const double scale = fabs((double) co[6]) * P.fw + fabs((double) co[7]) * P.fh + fabs((double) co[8]);
-        s_coef[tid][9] = (wmin > 1e-5 * scale && scale < 1e30) ? 1.0f : 0.0f;
+        const bool posw = wmin > 1e-5 * scale && scale < 1e30;
+        s_coef[tid][9] = posw ? 1.0f : 0.0f;
+        // W within [1/64, 64] over the image: the refined-reciprocal quotient
+        // (div_by) then equals IEEE division (see div_by)
+        s_coef[tid][10] = (posw && wmin >= 1.0 / 64 && wmax <= 64.0) ? 1.0f : 0.0f;
     }
 }
 
@@ -534,6 +545,248 @@ __global__ __launch_bounds__(kThreads) void search_fast_kernel(const SearchParam
     if (tid == 0) emit<GPT>(P, base, s_cost);
 }
 
+// --------------------------------------------------------------- MI kernel
+//
+// calcMIstream on one channel with the column-per-thread sweep of the fast
+// kernel, but bit-exact: sample points are correctly rounded quotients and the
+// bilinear value follows the declared lerps, so every joint-histogram count
+// equals the oracle's.  The taps of a one-byte image are packed four to a
+// 32-bit word (tl, tr, bl, br), those of a float image four to a float4; both
+// tables are indexed like the fast kernel's.  Lanes outside the image add to
+// per-lane dummy counters past the histograms, so the atomics need no branch.
+
+struct Recip {
+    float d, r;
+};
+
+// Reciprocal of d refined by one Newton step (v_rcp_f32, then
+// r += r (1 - d r)), and the quotient n / d from it by two correction steps.
+// This is the arithmetic of the compiler's correctly rounded f32 division
+// (v_div_scale ... v_div_fmas, v_div_fixup) when v_div_scale leaves its
+// operands unscaled, which holds for d in [2^-6, 2^6] and finite n with
+// n == 0 or 2^-90 < |n| < 2^90; the kernel checks the W range per grid
+// point and uses '/' otherwise.
+__device__ __forceinline__ Recip refine_rcp(float d) {
+    float r = __builtin_amdgcn_rcpf(d);
+    const float e = fmaf(-d, r, 1.0f);
+    r = fmaf(e, r, r);
+    return {d, r};
+}
+
+__device__ __forceinline__ float div_by(float n, Recip R) {
+    float q = n * R.r;
+    float rem = fmaf(-R.d, q, n);
+    q = fmaf(rem, R.r, q);
+    rem = fmaf(-R.d, q, n);
+    return fmaf(rem, R.r, q);
+}
+
+template <typename MT> struct TapTable;
+template <> struct TapTable<uint8_t> {
+    static constexpr int kBytes = 4;
+    __device__ static float4 load(__amdgpu_buffer_rsrc_t rsrc, uint32_t off) {
+        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+        return make_float4((float) (w & 255u), (float) ((w >> 8) & 255u), (float) ((w >> 16) & 255u),
+                           (float) (w >> 24));
+    }
+};
+template <> struct TapTable<float> {
+    static constexpr int kBytes = 16;
+    __device__ static float4 load(__amdgpu_buffer_rsrc_t rsrc, uint32_t off) { return load_tap(rsrc, off); }
+};
+
+__device__ __forceinline__ int clamp_int(int v, int lo, int hi) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(lo), "v"(hi));
+    return r;
+}
+
+// floor(bins * (v / 256)) clamped to [0, bins-1].  POW2: bins is a power of
+// two, so both scalings are exact and one multiply by bins/256 gives the same
+// value.
+template <bool POW2>
+__device__ __forceinline__ int mi_bin_fast(float v, int bins, float fbins, float scale) {
+    const float t = POW2 ? v * scale : fbins * (v / 256.0f);
+    return clamp_int(floor_to_int(t), 0, bins - 1);
+}
+
+template <typename MT, typename FT, int GPT, bool FASTDIV, bool POW2>
+__device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_coef)[12], uint32_t *hist,
+                                         uint32_t *cnt) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int fw = P.fw, fh = P.fh;
+    const int cols = fw < kThreads ? fw : kThreads;
+    const int rpi = kThreads / cols;
+    const int r0 = tid / cols, xc = tid - r0 * cols;
+    const float Wm = (float) P.mw, Hm = (float) P.mh;
+    const FT *fix = (const FT *) P.fix;
+    const int mw = P.mw;
+    constexpr int EB = TapTable<MT>::kBytes;
+    const int wE = mw * EB, baseE = (mw + 1) * EB;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) P.quads, (short) 0, (int) (((size_t) mw * P.mh + 2 * (size_t) mw + 2) * EB), 0x00020000);
+    const int bins = P.bins, B2 = bins * bins;
+    const float fbins = (float) bins, bscale = fbins / 256.0f;
+    const uint32_t dummy = (uint32_t) (GPT * B2 + lane);
+
+    float B[GPT][3];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) B[g][k] = uniform(s_coef[g][3 * k + 1]);
+
+    for (int x0 = 0; x0 < fw; x0 += kThreads) {
+        const int x = x0 + xc;
+        const bool valid = (r0 < rpi) && (x < fw);
+        const float xf = (float) x;
+        float col[GPT][3];
+#pragma unroll
+        for (int g = 0; g < GPT; ++g)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                col[g][k] = fmaf(uniform(s_coef[g][3 * k]), xf, uniform(s_coef[g][3 * k + 2]));
+        const int steps = (fh + rpi - 1) / rpi;
+        for (int it = 0; it < steps; ++it) {
+            const int y = r0 + it * rpi;
+            const bool vy = valid && y < fh;
+            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(vy);
+            const float f = vy ? (float) fix[y * fw + x] : 0.0f;
+            const uint32_t rowbin = (uint32_t) (mi_bin_fast<POW2>(f, bins, fbins, bscale) * bins);
+            const float yf = (float) y;
+            unsigned long long in[GPT];
+            float ax[GPT], ay[GPT];
+            float4 q[GPT];
+#pragma unroll
+            for (int g = 0; g < GPT; ++g) {
+                const float Xn = fmaf(B[g][0], yf, col[g][0]);
+                const float Yn = fmaf(B[g][1], yf, col[g][1]);
+                const float Wn = fmaf(B[g][2], yf, col[g][2]);
+                unsigned long long m;
+                float X, Y;
+                if constexpr (FASTDIV) {
+                    m = mask_ge0(Xn) & mask_ge0(Yn) & mask_lt0(fmaf(-Wm, Wn, Xn)) & mask_lt0(fmaf(-Hm, Wn, Yn));
+                    const Recip R = refine_rcp(Wn);
+                    X = div_by(Xn, R);
+                    Y = div_by(Yn, R);
+                } else {
+                    m = __builtin_amdgcn_ballot_w64(quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm));
+                    X = Xn / Wn;
+                    Y = Yn / Wn;
+                }
+                m &= vmask;
+                in[g] = m;
+                cnt[g] += (uint32_t) __builtin_popcountll(m);
+                // in-image sample points lie in [0, W] x [0, H]; other lanes'
+                // offsets are replaced below, so only the general-W path clamps
+                // (its discarded quotients may be huge or NaN)
+                int ix = floor_to_int(X), iy = floor_to_int(Y);
+                if constexpr (!FASTDIV) {
+                    ix = clamp_int(ix, -1, mw);
+                    iy = clamp_int(iy, -1, P.mh);
+                }
+                ax[g] = __builtin_amdgcn_fractf(X);
+                ay[g] = __builtin_amdgcn_fractf(Y);
+                uint32_t off;
+                asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(off) : "v"(iy), "s"(wE), "v"(ix * EB + baseE));
+                q[g] = TapTable<MT>::load(rsrc, mask_select(m, kOffsetOut, off));
+            }
+#pragma unroll
+            for (int g = 0; g < GPT; ++g) {
+                const float top = fmaf(ax[g], q[g].y, (1.0f - ax[g]) * q[g].x);
+                const float bottom = fmaf(ax[g], q[g].w, (1.0f - ax[g]) * q[g].z);
+                const float v = fmaf(ay[g], bottom, (1.0f - ay[g]) * top);
+                const uint32_t h = (uint32_t) (g * B2) + rowbin + (uint32_t) mi_bin_fast<POW2>(v, bins, fbins, bscale);
+                atomicAdd(&hist[mask_select(in[g], dummy, h)], 1u);
+            }
+        }
+    }
+}
+
+// calcMIstream's entropy step (src/gpu/cudaErrorFunction_miStreams.cuh:306-365)
+// in binary64 for grid point g of the workgroup; n = in-image sample count.
+// Every thread calls it; thread 0 returns the cost.
+__device__ double mi_entropy_cost(const uint32_t *h, int bins, double n, int cost, uint32_t *s_px, uint32_t *s_py,
+                                  double *s_lpx, double *s_lpy, double *s_scr) {
+    const int tid = threadIdx.x;
+    const int B2 = bins * bins;
+    if (tid < bins) {
+        uint32_t r = 0;
+        for (int j = 0; j < bins; ++j) r += h[tid * bins + j];
+        s_px[tid] = r;
+    } else if (tid < 2 * bins) {
+        uint32_t r = 0;
+        const int col = tid - bins;
+        for (int i = 0; i < bins; ++i) r += h[i * bins + col];
+        s_py[col] = r;
+    }
+    __syncthreads();
+    double hxy = 0.0;   // -p log2 p of the marginals
+    if (tid < bins) {
+        const double pxi = s_px[tid] / n;
+        s_lpx[tid] = pxi > 0 ? log2(pxi) : 0.0;
+        if (pxi > 0) hxy -= pxi * s_lpx[tid];
+    } else if (tid < 2 * bins) {
+        const double pyi = s_py[tid - bins] / n;
+        s_lpy[tid - bins] = pyi > 0 ? log2(pyi) : 0.0;
+        if (pyi > 0) hxy -= pyi * s_lpy[tid - bins];
+    }
+    __syncthreads();
+    double mi = 0.0;
+    for (int j = tid; j < B2; j += kThreads) {
+        const int i = j / bins, k = j - i * bins;
+        const double pxy = h[j] / n, pxi = s_px[i] / n, pyj = s_py[k] / n;
+        if (pxy > 0 && pxi > 0 && pyj > 0) mi += pxy * (log2(pxy) - (s_lpx[i] + s_lpy[k]));
+    }
+    const double hx = block_sum(tid < bins ? hxy : 0.0, s_scr);
+    const double hy = block_sum(tid >= bins && tid < 2 * bins ? hxy : 0.0, s_scr + 4);
+    const double mis = block_sum(mi, s_scr);
+    double r = cost == CGS_COST_MI ? -2.0 * mis / (hx + hy) : -mis;
+    if (n == 0.0) r = cost == CGS_COST_MI ? __builtin_nan("") : -0.0;
+    __syncthreads();
+    return r;
+}
+
+template <typename MT, typename FT, int GPT>
+__global__ __launch_bounds__(kThreads) void search_mi_kernel(const SearchParams P) {
+    __shared__ float s_coef[GPT][12];
+    __shared__ double s_cnt[4][GPT];
+    __shared__ float s_cost[GPT];
+    __shared__ double s_scr[8];
+    __shared__ uint32_t s_px[64], s_py[64];
+    __shared__ double s_lpx[64], s_lpy[64];
+    extern __shared__ uint32_t s_hist[];   // GPT histograms + 64 dummy counters
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t base = (int32_t) blockIdx.x * GPT;
+    const int B2 = P.bins * P.bins;
+    block_coefficients<GPT>(P, base, s_coef);
+    for (int i = tid; i < GPT * B2 + 64; i += kThreads) s_hist[i] = 0u;
+    __syncthreads();
+    bool fastdiv = true;
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) fastdiv = fastdiv && s_coef[g][10] != 0.0f;
+    uint32_t cnt[GPT];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) cnt[g] = 0u;
+    const bool pow2 = (P.bins & (P.bins - 1)) == 0;
+    if (fastdiv && pow2) mi_sweep<MT, FT, GPT, true, true>(P, s_coef, s_hist, cnt);
+    else if (fastdiv) mi_sweep<MT, FT, GPT, true, false>(P, s_coef, s_hist, cnt);
+    else mi_sweep<MT, FT, GPT, false, false>(P, s_coef, s_hist, cnt);
+    if (lane == 0)
+#pragma unroll
+        for (int g = 0; g < GPT; ++g) s_cnt[wave][g] = (double) cnt[g];
+    __syncthreads();
+    for (int g = 0; g < GPT; ++g) {
+        const double n = s_cnt[0][g] + s_cnt[1][g] + s_cnt[2][g] + s_cnt[3][g];
+        const double r = mi_entropy_cost(s_hist + g * B2, P.bins, n, P.cost, s_px, s_py, s_lpx, s_lpy, s_scr);
+        if (tid == 0) s_cost[g] = (float) r;
+    }
+    if (P.hist_dump && blockIdx.x == 0)
+        for (int j = tid; j < B2; j += kThreads) P.hist_dump[j] = s_hist[j];
+    __syncthreads();
+    if (tid == 0) emit<GPT>(P, base, s_cost);
+}
+
 // ----------------------------------------------------------- prep kernels
 
 __global__ void fill_u64_kernel(unsigned long long *p, int n, unsigned long long v) {
@@ -561,6 +814,29 @@ __global__ void quad_kernel(const T *src, float4 *dst, int W, int H) {
         auto P = [&](long i) { return (i >= 0 && i < n) ? (float) src[i] : 0.0f; };
         const float tl = P(L), tr = P(L + 1), bl = P(L + W), br = P(L + W + 1);
         dst[j] = make_float4(tl, tr - tl, bl - tl, (br - bl) - (tr - tl));
+    }
+}
+
+// Exact-bilinear tap tables for the MI kernel: entry j (tap L = j - (W+1))
+// holds p[L], p[L+1], p[L+W], p[L+W+1] (0 outside the image) -- four bytes
+// packed in a word for uint8 images, a float4 for float images.
+__global__ void tap_u8_kernel(const uint8_t *src, uint32_t *dst, int W, int H) {
+    const long n = (long) W * H;
+    const long total = n + 2L * W + 2;
+    for (long j = (long) blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long) gridDim.x * blockDim.x) {
+        const long L = j - (W + 1);
+        auto P = [&](long i) { return (i >= 0 && i < n) ? (uint32_t) src[i] : 0u; };
+        dst[j] = P(L) | (P(L + 1) << 8) | (P(L + W) << 16) | (P(L + W + 1) << 24);
+    }
+}
+
+__global__ void tap_f32_kernel(const float *src, float4 *dst, int W, int H) {
+    const long n = (long) W * H;
+    const long total = n + 2L * W + 2;
+    for (long j = (long) blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long) gridDim.x * blockDim.x) {
+        const long L = j - (W + 1);
+        auto P = [&](long i) { return (i >= 0 && i < n) ? src[i] : 0.0f; };
+        dst[j] = make_float4(P(L), P(L + 1), P(L + W), P(L + W + 1));
     }
 }
 
@@ -663,11 +939,36 @@ static hipError_t dispatch_fast(const SearchParams &P, int fpix, hipStream_t str
     return fast_pix<0>(P, fpix, stream);
 }
 
-// The fast kernel covers bilinear SAD/NCC/SQD on one channel; everything
-// else (nearest neighbour, MI, several channels) runs the exact kernel.
-static bool use_fast(const cgs_image *moved, const cgs_search_opts *o) {
-    return o->sampler == CGS_SAMPLER_BILINEAR && moved->channels == 1 && family_of(o->cost) != kFamMI &&
-           moved->width <= 65536 && moved->height <= 65536;
+template <typename MT, typename FT, int G>
+static hipError_t mi_launch(const SearchParams &P, hipStream_t stream) {
+    const int blocks = (P.grid.n + G - 1) / G;
+    const size_t lds = ((size_t) G * P.bins * P.bins + 64) * sizeof(uint32_t);
+    hipLaunchKernelGGL((search_mi_kernel<MT, FT, G>), dim3(blocks), dim3(kThreads), lds, stream, P);
+    return hipGetLastError();
+}
+
+template <typename MT, typename FT>
+static hipError_t mi_gpt(const SearchParams &P, hipStream_t stream) {
+    return P.bins > 32 ? mi_launch<MT, FT, 2>(P, stream) : mi_launch<MT, FT, 4>(P, stream);
+}
+
+static hipError_t dispatch_mi(const SearchParams &P, int mpix, int fpix, hipStream_t stream) {
+    if (mpix == CGS_PIX_U8 && fpix == CGS_PIX_U8) return mi_gpt<uint8_t, uint8_t>(P, stream);
+    if (mpix == CGS_PIX_F32 && fpix == CGS_PIX_F32) return mi_gpt<float, float>(P, stream);
+    if (mpix == CGS_PIX_U8) return mi_gpt<uint8_t, float>(P, stream);
+    return mi_gpt<float, uint8_t>(P, stream);
+}
+
+// Which kernel runs a search: the fast kernel for bilinear SAD/NCC/SQD on one
+// channel, the MI kernel for bilinear MI on one channel, the exact kernel for
+// everything else (nearest neighbour, several channels).
+enum KernelKind { kKernelExact = 0, kKernelFast = 1, kKernelMI = 2 };
+
+static KernelKind kernel_kind(const cgs_image *moved, const cgs_search_opts *o) {
+    if (o->sampler != CGS_SAMPLER_BILINEAR || moved->channels != 1 || moved->width > 65536 ||
+        moved->height > 65536)
+        return kKernelExact;
+    return family_of(o->cost) == kFamMI ? kKernelMI : kKernelFast;
 }
 
 // Runs one plain search: stages the moving image (padded copy or tap table),
@@ -682,7 +983,9 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     if (st != CGS_OK) return st;
     std::lock_guard<std::mutex> lock(ws->mu);
 
-    const bool fast = use_fast(moved, o);
+    const KernelKind kind = kernel_kind(moved, o);
+    const bool fast = kind == kKernelFast;
+    const size_t tap_bytes = kind == kKernelFast ? sizeof(float4) : (moved->pixtype == CGS_PIX_U8 ? 4 : 16);
     const size_t esz = moved->pixtype == CGS_PIX_U8 ? 1 : 4;
     const size_t nmov = (size_t) moved->width * moved->height * moved->channels;
     const size_t ntot = nmov + mov_pad_elems(moved->width, moved->channels);
@@ -691,8 +994,8 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
                             ? (int) grid->count[0] : 0;
     const size_t B2 = (size_t) o->mi_bins * o->mi_bins;
     if ((st = ws->key.ensure(64)) != CGS_OK || (st = ws->key_host.ensure(64, true)) != CGS_OK) return st;
-    if (fast) {
-        if ((st = ws->quads.ensure(nquad * sizeof(float4))) != CGS_OK) return st;
+    if (kind != kKernelExact) {
+        if ((st = ws->quads.ensure(nquad * tap_bytes)) != CGS_OK) return st;
     } else if ((st = ws->mov.ensure(ntot * esz)) != CGS_OK) {
         return st;
     }
@@ -702,7 +1005,14 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     if (hist_dump_host && (st = ws->hist.ensure(B2 * sizeof(uint32_t))) != CGS_OK) return st;
 
     const unsigned prep_blocks = 1024;
-    if (fast) {
+    if (kind == kKernelMI) {
+        if (moved->pixtype == CGS_PIX_U8)
+            hipLaunchKernelGGL(tap_u8_kernel, dim3(prep_blocks), dim3(256), 0, stream, (const uint8_t *) moved->data,
+                               (uint32_t *) ws->quads.ptr, moved->width, moved->height);
+        else
+            hipLaunchKernelGGL(tap_f32_kernel, dim3(prep_blocks), dim3(256), 0, stream, (const float *) moved->data,
+                               (float4 *) ws->quads.ptr, moved->width, moved->height);
+    } else if (fast) {
         if (moved->pixtype == CGS_PIX_U8)
             hipLaunchKernelGGL(quad_kernel<uint8_t>, dim3(prep_blocks), dim3(256), 0, stream,
                                (const uint8_t *) moved->data, (float4 *) ws->quads.ptr, moved->width, moved->height);
@@ -735,7 +1045,7 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     P.param_mode = o->param_mode;
     P.cost = o->cost;
     P.mov = ws->mov.ptr;
-    P.quads = fast ? (const float4 *) ws->quads.ptr : nullptr;   // entry 0 is tap L = -(W+1)
+    P.quads = kind != kKernelExact ? (const float4 *) ws->quads.ptr : nullptr;   // entry 0: tap L = -(W+1)
     P.mw = moved->width;
     P.mh = moved->height;
     P.mc = moved->channels;
@@ -751,8 +1061,9 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     P.hist_dump = hist_dump_host ? (uint32_t *) ws->hist.ptr : nullptr;
     const bool timed = (o->flags & CGS_FLAG_TIME_KERNEL) != 0;
     if (timed) HIP_TRY(ws->record_start(stream));
-    HIP_TRY(fast ? dispatch_fast(P, fixed->pixtype, stream) : dispatch_exact(P, o->sampler, moved->pixtype,
-                                                                             fixed->pixtype, stream));
+    if (kind == kKernelFast) HIP_TRY(dispatch_fast(P, fixed->pixtype, stream));
+    else if (kind == kKernelMI) HIP_TRY(dispatch_mi(P, moved->pixtype, fixed->pixtype, stream));
+    else HIP_TRY(dispatch_exact(P, o->sampler, moved->pixtype, fixed->pixtype, stream));
     if (timed) HIP_TRY(ws->record_stop(stream));
     HIP_TRY(hipMemcpyAsync(ws->key_host.ptr, key, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     if (hist_dump_host)

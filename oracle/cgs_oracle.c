@@ -47,7 +47,9 @@
  *     linear index, with fmaf for each "index += incr * coord" step;
  *   - reads past the last pixel (bilinear ceil on the last row, the
  *     nearest-neighbour index overflow) return 0;
- *   - bilinear weights are evaluated unfused, as written;
+ *   - bilinear: each of the three lerps (1-a)*p + a*q is evaluated as
+ *     fmaf(a, q, (1-a)*p) (the reference writes the unfused form and leaves
+ *     contraction to nvcc);
  *   - MI bins are clamped to [0, bins-1] (the reference writes out of bounds
  *     for values outside [0, 256)).
  */
@@ -274,9 +276,9 @@ static inline float bilinear_in(const orc_image *im, float y, float x, int c) {
     const float brc = pix(im, lin_index_f(im, cy, cx, c));
     const float alpha_x = x - fx;
     const float alpha_y = y - fy;
-    const float value_top = (1.0f - alpha_x) * tlc + alpha_x * trc;
-    const float value_bottom = (1.0f - alpha_x) * blc + alpha_x * brc;
-    return (1.0f - alpha_y) * value_top + alpha_y * value_bottom;
+    const float value_top = fmaf(alpha_x, trc, (1.0f - alpha_x) * tlc);
+    const float value_bottom = fmaf(alpha_x, brc, (1.0f - alpha_x) * blc);
+    return fmaf(alpha_y, value_bottom, (1.0f - alpha_y) * value_top);
 }
 
 float orc_sample_bilinear(const orc_image *im, float y, float x, int c) {

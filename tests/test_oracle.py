@@ -167,12 +167,15 @@ def test_bilinear_known_answers(oracle):
     rng = np.random.default_rng(1)
     for _ in range(50):
         y, x = rng.uniform(0, 4), rng.uniform(0, 5)
-        assert oracle.sample(const, y, x, oracle.BILINEAR) == 17.0
+        assert abs(oracle.sample(const, y, x, oracle.BILINEAR) - 17.0) <= 4e-6   # a few ulp: (1-a) rounds
         # linear ramp f = 3 (6y + x): interior bilinear reproduces it
         assert abs(oracle.sample(img, y, x, oracle.BILINEAR) - 3 * (6 * y + x)) < 1e-4
     # x in (W-1, W): the right tap wraps to the next row's first pixel
     v = oracle.sample(img, 1.0, 5.5, oracle.BILINEAR)
-    assert v == np.float32(0.5) * img[1, 5] + np.float32(0.5) * img[2, 0]
+    assert v == np.float32(0.5) * img[1, 5] + np.float32(0.5) * img[2, 0]      # exact halves
+    # the lerp is fma(a, q, (1-a) p): a = 0.1 on (0, 1) gives RN(0.1f) exactly, not RN(RN(0.9f*0) + 0.1f)
+    ramp = np.array([[0.0, 1.0], [0.0, 1.0]], np.float32)
+    assert oracle.sample(ramp, 0.0, np.float32(0.1), oracle.BILINEAR) == np.float32(0.1)
     # y in (H-1, H): the bottom taps read past the buffer -> 0
     v = oracle.sample(img, 4.5, 2.0, oracle.BILINEAR)
     assert v == np.float32(0.5) * img[4, 2]
