@@ -41,11 +41,22 @@ class Image(ctypes.Structure):
 
 class SearchOpts(ctypes.Structure):
     _fields_ = [("cost", ctypes.c_int32), ("sampler", ctypes.c_int32), ("param_mode", ctypes.c_int32),
-                ("tie_mode", ctypes.c_int32), ("mi_bins", ctypes.c_int32), ("flags", ctypes.c_int32)]
+                ("tie_mode", ctypes.c_int32), ("mi_bins", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("tile_gpt", ctypes.c_int32)]
 
 
 class Argmin(ctypes.Structure):
     _fields_ = [("value", ctypes.c_float), ("index", ctypes.c_int32)]
+
+
+class BatchPair(ctypes.Structure):
+    _fields_ = [("moved", Image), ("fixed", Image), ("grid", Grid), ("opts", SearchOpts),
+                ("result", ctypes.c_void_p)]
+
+
+class TileConfig(ctypes.Structure):
+    _fields_ = [("fast_gpt", ctypes.c_int32), ("mi_gpt", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("items_per_claim", ctypes.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol include/cgs.h declares.
@@ -62,6 +73,13 @@ SIGNATURES = {
                                         ctypes.c_void_p]),
     "cgs_eval_point": (ctypes.c_int, [_P(Image), _P(Image), _P(SearchOpts), ctypes.c_int32, _P(ctypes.c_float),
                                       _P(ctypes.c_float), _P(ctypes.c_uint32), ctypes.c_void_p]),
+    "cgs_batch_create": (ctypes.c_int, [_P(ctypes.c_void_p)]),
+    "cgs_batch_plan": (ctypes.c_int, [ctypes.c_void_p, _P(BatchPair), ctypes.c_int32, _P(TileConfig)]),
+    "cgs_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "cgs_batch_results": (ctypes.c_int, [ctypes.c_void_p, _P(Argmin), ctypes.c_void_p]),
+    "cgs_batch_info": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int64), _P(ctypes.c_int64),
+                                      _P(ctypes.c_int64)]),
+    "cgs_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "cgs_last_kernel_ms": (ctypes.c_float, []),
     "cgs_last_error": (ctypes.c_char_p, []),
 }

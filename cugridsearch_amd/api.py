@@ -253,3 +253,59 @@ def eval_point(functor, img_moved, img_fixed, params, stream=None):
     L.check(L.lib().cgs_eval_point(ctypes.byref(img_moved._img), ctypes.byref(img_fixed._img), ctypes.byref(o),
                                    len(p), L.floats(p), ctypes.byref(cost), hp, _stream_ptr(stream)))
     return cost.value, (hist.reshape(functor.bins, functor.bins) if hist is not None else None)
+
+
+class BatchSearcher:
+    """Batched ragged search (one persistent kernel over (pair, grid-tile)
+    work items, optionally replayed from a hipGraph) -- no reference
+    counterpart; each pair's results equal CudaGridSearcher.search_by_value.
+
+        b = BatchSearcher()
+        b.plan([(functor, grid, img_mov, img_fix, result_tensor_or_None), ...], tiles=None)
+        b.run(use_graph=True, stream=s)
+        argmins = b.results(stream=s)          # [(value, index)] per pair
+    """
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        L.check(L.lib().cgs_batch_create(ctypes.byref(h)))
+        self._h = h
+        self._n = 0
+        self._keep = None
+
+    def plan(self, pairs, tiles=None, tie_mode=L.TIE_LOWEST):
+        arr = (L.BatchPair * len(pairs))()
+        keep = []
+        for k, (fn, grid, mov, fix, res) in enumerate(pairs):
+            arr[k].moved = mov._img
+            arr[k].fixed = fix._img
+            arr[k].grid = grid._g
+            arr[k].opts = make_opts(fn, tie_mode)
+            arr[k].result = res.data.data_ptr() if res is not None else None
+            keep.append((mov, fix, res))
+        t = None
+        if tiles is not None:
+            t = L.TileConfig(*[int(tiles.get(k, 0)) for k in ("fast_gpt", "mi_gpt", "wg_per_cu", "items_per_claim")])
+        L.check(L.lib().cgs_batch_plan(self._h, arr, len(pairs), ctypes.byref(t) if t is not None else None))
+        self._n = len(pairs)
+        self._keep = (arr, keep)
+
+    def run(self, use_graph=False, stream=None):
+        L.check(L.lib().cgs_batch_run(self._h, 1 if use_graph else 0, _stream_ptr(stream)))
+
+    def results(self, stream=None):
+        out = (L.Argmin * self._n)()
+        L.check(L.lib().cgs_batch_results(self._h, out, _stream_ptr(stream)))
+        return [(a.value, a.index) for a in out]
+
+    def info(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        L.check(L.lib().cgs_batch_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"items": a.value, "points": b.value, "pixel_evals": c.value}
+
+    def __del__(self):
+        try:
+            if self._h:
+                L.lib().cgs_batch_destroy(self._h)
+        except Exception:
+            pass

@@ -25,6 +25,28 @@ def hipcc_command(out=LIB, extra=()):
             os.path.join(HERE, "csrc"), *extra, "-o", out, *srcs]
 
 
+def driver_command(out):
+    """The C++ facade driver used by the GPU tests (tests/cpp/facade_driver.cpp)."""
+    return ["/opt/rocm/bin/hipcc", "--offload-arch=" + ARCH, "-O2", "-std=c++17", "-Wno-unused-result",
+            "-I", os.path.join(ROOT, "include"), "-o", out, os.path.join(ROOT, "tests", "cpp", "facade_driver.cpp"),
+            "-L", HERE, "-lcgs", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../../cugridsearch_amd"]
+
+
+DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "facade_driver")
+
+
+def build_driver(verbose=False):
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    cmd = driver_command(DRIVER)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building the facade driver")
+    return DRIVER
+
+
 def build(verbose=False):
     cmd = hipcc_command(out=LIB + ".tmp")
     if verbose:

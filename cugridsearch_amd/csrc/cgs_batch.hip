@@ -1,0 +1,358 @@
+// cgs_batch.hip -- batched ragged grid search: one persistent kernel over
+// (pair, grid-tile) work items of uneven size, optionally replayed from a
+// captured hipGraph.
+//
+// Every work item is one workgroup's tile of GPT grid points of one pair,
+// run by the same per-workgroup bodies as the single-pair kernels
+// (fast_block / mi_block in cgs_kernels.hpp) on per-pair descriptors kept in
+// device memory, so each pair's costs and argmin are identical to cgs_search
+// on that pair.  Workgroups claim items_per_claim items at a time from one
+// device counter and exit when the counter passes the last item, so the
+// ragged mix of image sizes, grid sizes and costs load-balances itself.
+// Float tap tables and float fixed images are staged per pair, so one code
+// path per cost serves uint8 and float pairs alike (uint8 -> float is exact).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cgs.h"
+#include "cgs_device.hpp"
+#include "cgs_host.hpp"
+#include "cgs_kernels.hpp"
+
+namespace cgs {
+
+enum BatchKind : int32_t { kBatchSAD = 0, kBatchNCC = 1, kBatchSQD = 2, kBatchMI = 3 };
+
+struct BatchDesc {
+    SearchParams P;         // as for a single-pair search, pointing at the batch's staging
+    int32_t kind;           // BatchKind
+    uint32_t item_begin;    // first work item of the pair
+};
+
+// Converts an image to float (exact for uint8).
+template <typename T>
+__global__ void to_float_kernel(const T *src, float *dst, long n) {
+    for (long i = (long) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long) gridDim.x * blockDim.x)
+        dst[i] = (float) src[i];
+}
+
+// Tap tables as in cgs_search.hip: DIFF selects the fast kernel's
+// (p, dx, dy, dxy) form, otherwise the MI kernel's raw (tl, tr, bl, br).
+template <typename T, bool DIFF>
+__global__ void batch_taps_kernel(const T *src, float4 *dst, int W, int H) {
+    const long n = (long) W * H;
+    const long total = n + 2L * W + 2;
+    for (long j = (long) blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long) gridDim.x * blockDim.x) {
+        const long L = j - (W + 1);
+        auto P = [&](long i) { return (i >= 0 && i < n) ? (float) src[i] : 0.0f; };
+        const float tl = P(L), tr = P(L + 1), bl = P(L + W), br = P(L + W + 1);
+        dst[j] = DIFF ? make_float4(tl, tr - tl, bl - tl, (br - bl) - (tr - tl)) : make_float4(tl, tr, bl, br);
+    }
+}
+
+__global__ void batch_clear_kernel(unsigned long long *keys, int n, uint32_t *counter) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) keys[i] = ~0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
+}
+
+template <int GF, int GM>
+__global__ __launch_bounds__(kThreads) void batch_kernel(const BatchDesc *descs, int npairs, uint32_t total,
+                                                         uint32_t *counter, int per_claim) {
+    __shared__ BlockShared<GF> SF;
+    __shared__ BlockShared<GM> SM;
+    __shared__ uint32_t s_claim;
+    extern __shared__ uint32_t s_hist[];   // GM MI histograms of the batch's largest bin count + 64 dummies
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_claim = atomicAdd(counter, (uint32_t) per_claim);
+        __syncthreads();
+        const uint32_t first = s_claim;
+        __syncthreads();   // every thread has read s_claim before the next claim
+        if (first >= total) break;
+        const uint32_t last = min(first + (uint32_t) per_claim, total);
+        for (uint32_t item = first; item < last; ++item) {
+            // the pair owning the item: last descriptor with item_begin <= item
+            int lo = 0, hi = npairs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (descs[mid].item_begin <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            const BatchDesc &D = descs[lo];
+            const int32_t tile = (int32_t) (item - D.item_begin);
+            switch (D.kind) {
+            case kBatchMI: mi_block<float, float, GM>(D.P, tile * GM, SM, s_hist); break;
+            case kBatchNCC: fast_block<1, float, GF>(D.P, tile * GF, SF); break;
+            case kBatchSQD: fast_block<2, float, GF>(D.P, tile * GF, SF); break;
+            default: fast_block<0, float, GF>(D.P, tile * GF, SF); break;
+            }
+        }
+    }
+}
+
+}  // namespace cgs
+
+using namespace cgs;
+
+struct cgs_batch_s {
+    std::vector<cgs_batch_pair> pairs;
+    cgs_tile_config tiles{};
+    int cus = 0;
+    int max_bins = 8;        // largest MI bin count of the batch (sizes the LDS histograms)
+    uint32_t total_items = 0;
+    int64_t total_points = 0, total_pe = 0;
+    // device staging
+    DevBuf descs, keys, keys_host, counter;
+    std::vector<DevBuf> taps, fixf, sincos;
+    std::vector<BatchDesc> host_descs;
+    DevBuf sincos_host;
+    // captured graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    ~cgs_batch_s() {
+        if (exec) (void) hipGraphExecDestroy(exec);
+        if (graph) (void) hipGraphDestroy(graph);
+        auto rel = [](DevBuf &b, bool host) {
+            if (b.ptr) (void) (host ? hipHostFree(b.ptr) : hipFree(b.ptr));
+            b.ptr = nullptr;
+        };
+        rel(descs, false); rel(keys, false); rel(counter, false); rel(keys_host, true); rel(sincos_host, true);
+        for (auto &b : taps) rel(b, false);
+        for (auto &b : fixf) rel(b, false);
+        for (auto &b : sincos) rel(b, false);
+    }
+};
+
+namespace {
+
+int batch_kind(int cost) {
+    switch (cost) {
+    case CGS_COST_NCC: return kBatchNCC;
+    case CGS_COST_SQD: return kBatchSQD;
+    case CGS_COST_MI: case CGS_COST_MI_PLAIN: return kBatchMI;
+    default: return kBatchSAD;
+    }
+}
+
+template <int GF, int GM>
+hipError_t launch_batch(cgs_batch *b, hipStream_t s) {
+    const size_t lds = ((size_t) GM * b->max_bins * b->max_bins + 64) * sizeof(uint32_t);
+    const int blocks = std::max(1, b->cus * b->tiles.wg_per_cu);
+    hipLaunchKernelGGL((batch_kernel<GF, GM>), dim3(blocks), dim3(kThreads), lds, s,
+                       (const BatchDesc *) b->descs.ptr, (int) b->pairs.size(), b->total_items,
+                       (uint32_t *) b->counter.ptr, b->tiles.items_per_claim);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_any(cgs_batch *b, hipStream_t s) {
+    const int gf = b->tiles.fast_gpt, gm = b->tiles.mi_gpt;
+    if (gf == 8 && gm == 4) return launch_batch<8, 4>(b, s);
+    if (gf == 8) return launch_batch<8, 2>(b, s);
+    if (gm == 4) return launch_batch<4, 4>(b, s);
+    return launch_batch<4, 2>(b, s);
+}
+
+// Everything one run enqueues: staging of every pair, clearing, the kernel.
+cgs_status enqueue_run(cgs_batch *b, hipStream_t s) {
+    const unsigned pb = 256;
+    for (size_t i = 0; i < b->pairs.size(); ++i) {
+        const cgs_batch_pair &p = b->pairs[i];
+        const bool mi = batch_kind(p.opts.cost) == kBatchMI;
+        const int W = p.moved.width, H = p.moved.height;
+        float4 *taps = (float4 *) b->taps[i].ptr;
+        if (p.moved.pixtype == CGS_PIX_U8) {
+            if (mi) hipLaunchKernelGGL((batch_taps_kernel<uint8_t, false>), dim3(pb), dim3(256), 0, s,
+                                       (const uint8_t *) p.moved.data, taps, W, H);
+            else hipLaunchKernelGGL((batch_taps_kernel<uint8_t, true>), dim3(pb), dim3(256), 0, s,
+                                    (const uint8_t *) p.moved.data, taps, W, H);
+        } else {
+            if (mi) hipLaunchKernelGGL((batch_taps_kernel<float, false>), dim3(pb), dim3(256), 0, s,
+                                       (const float *) p.moved.data, taps, W, H);
+            else hipLaunchKernelGGL((batch_taps_kernel<float, true>), dim3(pb), dim3(256), 0, s,
+                                    (const float *) p.moved.data, taps, W, H);
+        }
+        const long nf = (long) p.fixed.width * p.fixed.height;
+        if (p.fixed.pixtype == CGS_PIX_U8)
+            hipLaunchKernelGGL(to_float_kernel<uint8_t>, dim3(pb), dim3(256), 0, s, (const uint8_t *) p.fixed.data,
+                               (float *) b->fixf[i].ptr, nf);
+        else
+            hipLaunchKernelGGL(to_float_kernel<float>, dim3(pb), dim3(256), 0, s, (const float *) p.fixed.data,
+                               (float *) b->fixf[i].ptr, nf);
+    }
+    hipLaunchKernelGGL(batch_clear_kernel, dim3(16), dim3(256), 0, s, (unsigned long long *) b->keys.ptr,
+                       (int) b->pairs.size(), (uint32_t *) b->counter.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_batch_any(b, s));
+    return CGS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+cgs_status cgs_batch_create(cgs_batch **out) {
+    clear_error();
+    if (!out) return set_error(CGS_ERR_INVALID_ARGUMENT, "null out");
+    *out = new cgs_batch_s();
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_destroy(cgs_batch *b) {
+    delete b;
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_plan(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, const cgs_tile_config *tiles) {
+    clear_error();
+    if (!b || !pairs || n <= 0) return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_plan: bad arguments");
+    cgs_tile_config t{4, 4, 4, 8};
+    if (tiles) {
+        if (tiles->fast_gpt) t.fast_gpt = tiles->fast_gpt;
+        if (tiles->mi_gpt) t.mi_gpt = tiles->mi_gpt;
+        if (tiles->wg_per_cu) t.wg_per_cu = tiles->wg_per_cu;
+        if (tiles->items_per_claim) t.items_per_claim = tiles->items_per_claim;
+    }
+    if ((t.fast_gpt != 4 && t.fast_gpt != 8) || (t.mi_gpt != 2 && t.mi_gpt != 4) || t.wg_per_cu < 1 ||
+        t.wg_per_cu > 8 || t.items_per_claim < 1 || t.items_per_claim > 64)
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_plan: bad tile configuration");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return set_error(CGS_ERR_NO_DEVICE, "no HIP device");
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cgs_status st;
+    for (int32_t i = 0; i < n; ++i) {
+        const cgs_batch_pair &p = pairs[i];
+        if ((st = validate_search(&p.grid, &p.moved, &p.fixed, &p.opts)) != CGS_OK) return st;
+        if (p.opts.sampler != CGS_SAMPLER_BILINEAR || p.moved.channels != 1 || p.moved.width > 65536 ||
+            p.moved.height > 65536)
+            return set_error(CGS_ERR_UNSUPPORTED, "batch pairs must be one-channel bilinear searches");
+    }
+    if (b->exec) { (void) hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void) hipGraphDestroy(b->graph); b->graph = nullptr; }
+    b->graph_stream = nullptr;
+    b->pairs.assign(pairs, pairs + n);
+    b->tiles = t;
+    b->max_bins = 8;
+    b->cus = cus;
+    b->taps.resize(n);
+    b->fixf.resize(n);
+    b->sincos.resize(n);
+    b->host_descs.assign(n, BatchDesc{});
+    if ((st = b->keys.ensure(sizeof(unsigned long long) * n)) != CGS_OK ||
+        (st = b->keys_host.ensure(sizeof(unsigned long long) * n, true)) != CGS_OK ||
+        (st = b->counter.ensure(64)) != CGS_OK || (st = b->descs.ensure(sizeof(BatchDesc) * n)) != CGS_OK)
+        return st;
+    uint64_t items = 0;
+    int64_t points = 0, pe = 0;
+    std::vector<float> sc;
+    for (int32_t i = 0; i < n; ++i) {
+        const cgs_batch_pair &p = pairs[i];
+        const int kind = batch_kind(p.opts.cost);
+        const int gpt = kind == kBatchMI ? t.mi_gpt : t.fast_gpt;
+        if (kind == kBatchMI && p.opts.mi_bins > 32 && gpt > 2)
+            return set_error(CGS_ERR_UNSUPPORTED, "MI with more than 32 bins needs mi_gpt 2");
+        if (kind == kBatchMI) b->max_bins = std::max(b->max_bins, (int) p.opts.mi_bins);
+        const int W = p.moved.width, H = p.moved.height;
+        const size_t ntaps = (size_t) W * H + 2 * (size_t) W + 2;
+        if ((st = b->taps[i].ensure(ntaps * sizeof(float4))) != CGS_OK) return st;
+        if ((st = b->fixf[i].ensure((size_t) p.fixed.width * p.fixed.height * sizeof(float))) != CGS_OK) return st;
+        const bool has_theta = p.opts.param_mode == CGS_PARAM_CENTRED || p.opts.param_mode == CGS_PARAM_NONCENTRED;
+        const int levels0 = has_theta ? (int) p.grid.count[0] : 0;
+        if (levels0 > 0) {
+            if ((st = b->sincos[i].ensure(sizeof(float) * 2 * levels0)) != CGS_OK) return st;
+            sc.resize(2 * levels0);
+            for (int l = 0; l < levels0; ++l) {
+                const float theta = fmaf((float) l, p.grid.resolution[0], p.grid.start[0]);
+                sc[2 * l] = (float) std::cos((double) theta);
+                sc[2 * l + 1] = (float) std::sin((double) theta);
+            }
+            HIP_TRY(hipMemcpy(b->sincos[i].ptr, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice));
+        }
+        BatchDesc &D = b->host_descs[i];
+        std::memset(&D, 0, sizeof(D));
+        fill_grid_dev(&p.grid, &D.P.grid);
+        D.P.sincos = levels0 > 0 ? (const float2 *) b->sincos[i].ptr : nullptr;
+        D.P.param_mode = p.opts.param_mode;
+        D.P.cost = p.opts.cost;
+        D.P.quads = (const float4 *) b->taps[i].ptr;
+        D.P.mw = W;
+        D.P.mh = H;
+        D.P.mc = 1;
+        D.P.fix = b->fixf[i].ptr;
+        D.P.fw = p.fixed.width;
+        D.P.fh = p.fixed.height;
+        D.P.cx = (float) W / 2;
+        D.P.cy = (float) H / 2;
+        D.P.result = p.result;
+        D.P.key = (unsigned long long *) b->keys.ptr + i;
+        D.P.tie_mode = p.opts.tie_mode;
+        D.P.bins = p.opts.mi_bins;
+        D.kind = kind;
+        D.item_begin = (uint32_t) items;
+        items += (uint64_t) ((p.grid.num_elements + gpt - 1) / gpt);
+        points += p.grid.num_elements;
+        pe += p.grid.num_elements * (int64_t) p.fixed.width * p.fixed.height;
+        if (items >= 0xFFFFFFF0ull) return set_error(CGS_ERR_TOO_LARGE, "batch has 2^32 or more work items");
+    }
+    HIP_TRY(hipMemcpy(b->descs.ptr, b->host_descs.data(), sizeof(BatchDesc) * n, hipMemcpyHostToDevice));
+    b->total_items = (uint32_t) items;
+    b->total_points = points;
+    b->total_pe = pe;
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_run(cgs_batch *b, int32_t use_graph, void *stream) {
+    clear_error();
+    if (!b || b->pairs.empty()) return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_run: not planned");
+    hipStream_t s = (hipStream_t) stream;
+    if (!use_graph) return enqueue_run(b, s);
+    if (b->exec && b->graph_stream != s) {
+        (void) hipGraphExecDestroy(b->exec);
+        (void) hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    if (!b->exec) {
+        if (!s) return set_error(CGS_ERR_INVALID_ARGUMENT, "graph capture needs a non-null stream");
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const cgs_status st = enqueue_run(b, s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (st != CGS_OK) { if (g) (void) hipGraphDestroy(g); return st; }
+        if (e != hipSuccess) return set_error(CGS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+        b->graph_stream = s;
+    }
+    HIP_TRY(hipGraphLaunch(b->exec, s));
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_results(cgs_batch *b, cgs_argmin *out, void *stream) {
+    clear_error();
+    if (!b || !out || b->pairs.empty()) return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_results: bad arguments");
+    hipStream_t s = (hipStream_t) stream;
+    const size_t n = b->pairs.size();
+    HIP_TRY(hipMemcpyAsync(b->keys_host.ptr, b->keys.ptr, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const unsigned long long *k = (const unsigned long long *) b->keys_host.ptr;
+    for (size_t i = 0; i < n; ++i) out[i] = decode_key(k[i], b->pairs[i].opts.tie_mode);
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_info(const cgs_batch *b, int64_t *items, int64_t *points, int64_t *pixel_evals) {
+    clear_error();
+    if (!b) return set_error(CGS_ERR_INVALID_ARGUMENT, "null batch");
+    if (items) *items = b->total_items;
+    if (points) *points = b->total_points;
+    if (pixel_evals) *pixel_evals = b->total_pe;
+    return CGS_OK;
+}
+
+}  // extern "C"

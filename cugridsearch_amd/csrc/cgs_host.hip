@@ -103,6 +103,8 @@ cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const c
         return set_error(CGS_ERR_INVALID_ARGUMENT, "bad tie mode");
     if ((o->cost == CGS_COST_MI || o->cost == CGS_COST_MI_PLAIN) && (o->mi_bins < 8 || o->mi_bins > 64))
         return set_error(CGS_ERR_INVALID_ARGUMENT, "mi_bins must be in [8, 64]");
+    if (o->tile_gpt != 0 && o->tile_gpt != 2 && o->tile_gpt != 4 && o->tile_gpt != 8)
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "tile_gpt must be 0, 2, 4 or 8");
     const int need = o->param_mode == CGS_PARAM_TRANSLATION ? 2 : 8;
     if (grid->dim != need)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "grid dimension does not match the parameter mode");

@@ -1,0 +1,792 @@
+// cgs_kernels.hpp -- the per-workgroup search bodies shared by the plain
+// single-pair kernels (cgs_search.hip) and the persistent batch kernel
+// (cgs_batch.hip).  Compiled with -ffp-contract=off (see cgs_device.hpp).
+//
+// Two sweep bodies:
+//   fast_block -- bilinear SAD/NCC/SQD on one channel: the in-image decision
+//     is exact (sign tests on the exact forms) while the sample point uses
+//     v_rcp_f32 and the taps come from a precomputed float4 table; costs agree
+//     with the oracle within FP32 rounding of the samples.
+//   mi_block   -- bilinear MI on one channel, bit-exact sample values and
+//     histogram counts.
+// and the generic search_exact_kernel (every cost/sampler/channel count).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <type_traits>
+
+#include "cgs.h"
+#include "cgs_device.hpp"
+
+namespace cgs {
+
+enum Family { kFamSAD = 0, kFamNCC = 1, kFamMI = 2 };
+
+struct SearchParams {
+    GridDev grid;
+    const float2 *sincos;   // (cos, sin) per level of axis 0, or null
+    int param_mode;
+    int cost;
+    const void *mov;        // exact kernel: moving image + zero pad
+    const float4 *quads;    // fast kernel: tap table; entry j holds tap L = j - (W+1)
+    int mw, mh, mc;
+    const void *fix;
+    int fw, fh;
+    float cx, cy;
+    float *result;
+    unsigned long long *key;
+    int tie_mode;
+    int bins;
+    uint32_t *hist_dump;    // histogram of grid point 0 of block 0 (diagnostics)
+};
+
+constexpr int kThreads = 256;
+
+// Elements of zero padding after the moving image: covers every tap the
+// samplers can address for a sample point in [0, W] x [0, H].
+__host__ __device__ inline size_t mov_pad_elems(int W, int C) { return (size_t) C * (2 * (size_t) W + 4); }
+
+// valueAt_nearest_neighbor (include/cudaTensor.cuh:557-559): the truncated
+// float linear index of toIndex1d<float> (include/cudaTensor.cuh:171-180).
+template <typename MT>
+__device__ __forceinline__ float sample_nearest(const MT *img, float X, float Y, int c, int C, int W) {
+    int32_t i = c;
+    i = (int32_t) fmaf((float) C, X, (float) i);
+    i = (int32_t) fmaf((float) (C * W), Y, (float) i);
+    return (float) img[i];
+}
+
+// valueAt_bilinear (include/cudaTensor.cuh:561-575), each lerp declared as
+// fmaf(a, q, (1-a)*p).  The right/bottom taps
+// are read at floor+1; where the coordinate is integral the reference reads
+// ceil == floor instead, and that tap's weight is exactly 0, so the value is
+// the same.  ceil(x) == W lands on the next row through the linear index and
+// ceil(y) == H on the zero pad, as in the reference's linear addressing.
+template <typename MT>
+__device__ __forceinline__ float sample_bilinear(const MT *img, float X, float Y, int c, int C, int W) {
+    const float fx = floorf(X), fy = floorf(Y);
+    const int b = ((int) fy * W + (int) fx) * C + c;
+    const float tl = (float) img[b], tr = (float) img[b + C];
+    const float bl = (float) img[b + C * W], br = (float) img[b + C * W + C];
+    const float ax = X - fx, ay = Y - fy;
+    const float top = fmaf(ax, tr, (1.0f - ax) * tl);
+    const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
+    return fmaf(ay, bottom, (1.0f - ay) * top);
+}
+
+__device__ __forceinline__ int mi_bin(float v, int bins) {
+    const float t = floorf((float) bins * (v / 256.0f));
+    int b = (int) t;
+    if (!(t >= 0.0f)) b = 0;
+    if (b > bins - 1) b = bins - 1;
+    return b;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Sums v over the workgroup; every thread gets the total.  scratch: >= 4 doubles.
+__device__ __forceinline__ double block_sum(double v, double *scratch) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    return scratch[0] + scratch[1] + scratch[2] + scratch[3];
+}
+
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Thread 0..GPT-1 of the workgroup: warp coefficients of grid index base+tid
+// (index 0 for the padding slots past the end of the grid).
+template <int GPT>
+__device__ __forceinline__ void block_coefficients(const SearchParams &P, int32_t base, float (*s_coef)[12]) {
+    const int tid = threadIdx.x;
+    if (tid < GPT) {
+        const int32_t gi = base + tid < P.grid.n ? base + tid : 0;
+        float p[kMaxDim];
+        int32_t i0;
+        grid_point(P.grid, gi, p, &i0);
+        float c = 1.0f, s = 0.0f;
+        if (P.sincos) {
+            const float2 cs = P.sincos[i0];
+            c = cs.x;
+            s = cs.y;
+        }
+        float co[9];
+        warp_coeffs(P.param_mode, p, c, s, P.cx, P.cy, co);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s_coef[tid][k] = co[k];
+        // W > 0 over the whole fixed image (W is affine: its minimum over the
+        // rectangle is at a corner; the margin covers the rounding of the fmas)
+        double wmin = 1e300;
+        for (int k = 0; k < 4; ++k) {
+            const double xx = (k & 1) ? P.fw - 1 : 0, yy = (k & 2) ? P.fh - 1 : 0;
+            const double w = (double) co[7] * yy + ((double) co[6] * xx + (double) co[8]);
+            wmin = w < wmin ? w : wmin;
+        }
+        double wmax = -1e300;
+        for (int k = 0; k < 4; ++k) {
+            const double xx = (k & 1) ? P.fw - 1 : 0, yy = (k & 2) ? P.fh - 1 : 0;
+            const double w = (double) co[7] * yy + ((double) co[6] * xx + (double) co[8]);
+            wmax = w > wmax ? w : wmax;
+        }
+        const double scale = fabs((double) co[6]) * P.fw + fabs((double) co[7]) * P.fh + fabs((double) co[8]);
+        const bool posw = wmin > 1e-5 * scale && scale < 1e30;
+        s_coef[tid][9] = posw ? 1.0f : 0.0f;
+        // W within [1/64, 64] over the image: the refined-reciprocal quotient
+        // (div_by) then equals IEEE division (see div_by)
+        s_coef[tid][10] = (posw && wmin >= 1.0 / 64 && wmax <= 64.0) ? 1.0f : 0.0f;
+    }
+}
+
+// Final cost of one grid point from its reduced sums (t[0..2]) and sample
+// count t[3]; binary64 combine.
+template <int FAM>
+__device__ __forceinline__ float finish_cost(int cost, const double *t, bool exact_int) {
+    if constexpr (FAM == kFamSAD) {
+        if (cost == CGS_COST_SAD_SUM) return (float) t[0];
+        if (exact_int) return (float) t[0] / (float) t[3];
+        return (float) (t[0] / t[3]);
+    } else {
+        if (cost == CGS_COST_NCC) return t[3] > 0.0 ? (float) (-t[2] / sqrt(t[0] * t[1])) : 0.0f;
+        return t[3] > 0.0 ? (float) (t[2] / sqrt(t[0] * t[1])) : 123123.0f;
+    }
+}
+
+// Writes the costs of the workgroup's GPT points and folds their minimum
+// into the global argmin key.  Called by thread 0.
+template <int GPT>
+__device__ __forceinline__ void emit(const SearchParams &P, int32_t base, const float *s_cost) {
+    unsigned long long best = ~0ull;
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const int32_t gi = base + g;
+        if (gi < P.grid.n) {
+            if (P.result) P.result[gi] = s_cost[g];
+            const unsigned long long k = argmin_key(s_cost[g], (uint32_t) gi, P.tie_mode);
+            best = k < best ? k : best;
+        }
+    }
+    if (best != ~0ull && best < __hip_atomic_load(P.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(P.key, best);
+}
+
+// ------------------------------------------------------------ exact kernel
+
+template <int FAM, int SAMP, typename MT, typename FT, int GPT>
+__global__ __launch_bounds__(kThreads) void search_exact_kernel(const SearchParams P) {
+    __shared__ float s_coef[GPT][12];
+    __shared__ double s_red[4][GPT][4];
+    __shared__ float s_cost[GPT];
+    __shared__ double s_scr[8];
+    __shared__ uint32_t s_px[64], s_py[64];
+    __shared__ double s_lpx[64], s_lpy[64];
+    extern __shared__ uint32_t s_hist[];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int32_t base = (int32_t) blockIdx.x * GPT;
+    const int B2 = P.bins * P.bins;
+
+    block_coefficients<GPT>(P, base, s_coef);
+    if constexpr (FAM == kFamMI) {
+        for (int i = tid; i < GPT * B2; i += kThreads) s_hist[i] = 0u;
+    }
+    __syncthreads();
+
+    float cf[GPT][9];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) cf[g][k] = s_coef[g][k];
+
+    const MT *mov = (const MT *) P.mov;
+    const FT *fix = (const FT *) P.fix;
+    const int C = P.mc;
+    const float Wm = (float) P.mw, Hm = (float) P.mh;
+    const int npix = P.fw * P.fh;
+    constexpr bool kExactInt = (FAM == kFamSAD) && (SAMP == CGS_SAMPLER_NEAREST) &&
+                               std::is_same<MT, uint8_t>::value && std::is_same<FT, uint8_t>::value;
+
+    double acc0[GPT], acc1[GPT], acc2[GPT];
+    uint32_t isum[GPT], cnt[GPT];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) { acc0[g] = acc1[g] = acc2[g] = 0.0; isum[g] = cnt[g] = 0u; }
+
+    int x = tid % P.fw, y = tid / P.fw;
+    const int step_x = kThreads % P.fw, step_y = kThreads / P.fw;
+    for (int p = tid; p < npix; p += kThreads) {
+        const float fxp = (float) x, fyp = (float) y;
+#pragma unroll
+        for (int g = 0; g < GPT; ++g) {
+            const float Wn = form(cf[g][6], cf[g][7], cf[g][8], fxp, fyp);
+            const float Xn = form(cf[g][0], cf[g][1], cf[g][2], fxp, fyp);
+            const float Yn = form(cf[g][3], cf[g][4], cf[g][5], fxp, fyp);
+            if (quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm)) {
+                const float X = Xn / Wn, Y = Yn / Wn;
+                for (int c = 0; c < C; ++c) {
+                    const float m = SAMP == CGS_SAMPLER_NEAREST ? sample_nearest(mov, X, Y, c, C, P.mw)
+                                                                : sample_bilinear(mov, X, Y, c, C, P.mw);
+                    const float f = (float) fix[p * C + c];
+                    cnt[g] += 1u;
+                    if constexpr (FAM == kFamSAD) {
+                        if constexpr (kExactInt) isum[g] += (uint32_t) fabsf(m - f);
+                        else acc0[g] += (double) fabsf(m - f);
+                    } else if constexpr (FAM == kFamNCC) {
+                        const double md = (double) m, fd = (double) f;
+                        acc0[g] = fma(fd, fd, acc0[g]);
+                        acc1[g] = fma(md, md, acc1[g]);
+                        if (P.cost == CGS_COST_NCC) acc2[g] = fma(fd, md, acc2[g]);
+                        else acc2[g] = fma(md - fd, md - fd, acc2[g]);
+                    } else {
+                        atomicAdd(&s_hist[g * B2 + mi_bin(f, P.bins) * P.bins + mi_bin(m, P.bins)], 1u);
+                    }
+                }
+            }
+        }
+        x += step_x;
+        y += step_y;
+        if (x >= P.fw) { x -= P.fw; ++y; }
+    }
+
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const double v0 = wave_sum(kExactInt ? (double) isum[g] : acc0[g]);
+        const double v1 = wave_sum(acc1[g]);
+        const double v2 = wave_sum(acc2[g]);
+        const double v3 = wave_sum((double) cnt[g]);
+        if (lane == 0) {
+            s_red[wave][g][0] = v0;
+            s_red[wave][g][1] = v1;
+            s_red[wave][g][2] = v2;
+            s_red[wave][g][3] = v3;
+        }
+    }
+    __syncthreads();
+
+    if constexpr (FAM != kFamMI) {
+        if (tid < GPT) {
+            double t[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) t[k] = s_red[0][tid][k] + s_red[1][tid][k] + s_red[2][tid][k] + s_red[3][tid][k];
+            s_cost[tid] = finish_cost<FAM>(P.cost, t, kExactInt);
+        }
+    } else {
+        // calcMIstream's entropy step (src/gpu/cudaErrorFunction_miStreams.cuh:306-365), binary64.
+        const int bins = P.bins;
+        for (int g = 0; g < GPT; ++g) {
+            const uint32_t *h = s_hist + g * B2;
+            const double n = s_red[0][g][3] + s_red[1][g][3] + s_red[2][g][3] + s_red[3][g][3];
+            if (tid < bins) {
+                uint32_t r = 0;
+                for (int j = 0; j < bins; ++j) r += h[tid * bins + j];
+                s_px[tid] = r;
+            } else if (tid < 2 * bins) {
+                uint32_t r = 0;
+                const int col = tid - bins;
+                for (int i = 0; i < bins; ++i) r += h[i * bins + col];
+                s_py[col] = r;
+            }
+            __syncthreads();
+            double hxy = 0.0;   // -p log2 p of the marginals
+            if (tid < bins) {
+                const double pxi = s_px[tid] / n;
+                s_lpx[tid] = pxi > 0 ? log2(pxi) : 0.0;
+                if (pxi > 0) hxy -= pxi * s_lpx[tid];
+            } else if (tid < 2 * bins) {
+                const double pyi = s_py[tid - bins] / n;
+                s_lpy[tid - bins] = pyi > 0 ? log2(pyi) : 0.0;
+                if (pyi > 0) hxy -= pyi * s_lpy[tid - bins];
+            }
+            __syncthreads();
+            double mi = 0.0;
+            for (int j = tid; j < B2; j += kThreads) {
+                const int i = j / bins, k = j - i * bins;
+                const double pxy = h[j] / n, pxi = s_px[i] / n, pyj = s_py[k] / n;
+                if (pxy > 0 && pxi > 0 && pyj > 0) mi += pxy * (log2(pxy) - (s_lpx[i] + s_lpy[k]));
+            }
+            const double hx = block_sum(tid < bins ? hxy : 0.0, s_scr);
+            const double hy = block_sum(tid >= bins && tid < 2 * bins ? hxy : 0.0, s_scr + 4);
+            const double mis = block_sum(mi, s_scr);
+            if (tid == 0) {
+                double r = P.cost == CGS_COST_MI ? -2.0 * mis / (hx + hy) : -mis;
+                if (n == 0.0) r = P.cost == CGS_COST_MI ? __builtin_nan("") : -0.0;
+                s_cost[g] = (float) r;
+            }
+            __syncthreads();
+        }
+        if (P.hist_dump && blockIdx.x == 0)
+            for (int j = tid; j < B2; j += kThreads) P.hist_dump[j] = s_hist[j];
+    }
+    __syncthreads();
+    if (tid == 0) emit<GPT>(P, base, s_cost);
+}
+
+// ------------------------------------------------------------- fast kernel
+//
+// Work split: a thread owns one fixed-image column x for a whole sweep, so
+// fmaf(A, x, C) of every form is computed once per grid point and each pixel
+// costs one fmaf per form.  For W <= 256 the 256 lanes cover 256/W rows per
+// step; wider images are swept in 256-column groups.  The B coefficients are
+// wave-uniform and held in SGPRs.  Per-thread FP32 partial sums are folded
+// into FP64 every kFlush rows.
+//
+// Taps are read with raw buffer loads from the tap table: the offset of tap
+// L is (L + W + 1) * 16 bytes, and a lane outside the image gets an offset
+// past the table's end, which the hardware range check turns into zeros --
+// so the positive-W sweep has no per-lane branch and issues all GPT loads
+// before it consumes any.
+
+constexpr int kFlush = 16;
+constexpr uint32_t kOffsetOut = 0x80000000u;   // beyond any tap table
+
+__device__ __forceinline__ int floor_to_int(float v) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+// Byte offset of tap (ix, iy) in the tap table: 16 * ((iy + 1) * W + ix + 1)
+// = iy * 16W + (16 ix + 16 (W + 1)), one shift-add and one 24-bit mad.
+__device__ __forceinline__ uint32_t tap_offset(int ix, int iy, int w16, int base16) {
+    uint32_t r;
+    const uint32_t t = (uint32_t) ((ix << 4) + base16);
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(iy), "s"(w16), "v"(t));
+    return r;
+}
+
+// Lane select on a wave mask held in SGPRs: mask bit set -> b, else a.
+__device__ __forceinline__ uint32_t mask_select(unsigned long long m, uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(m));
+    return r;
+}
+
+// Wave masks of ordered float comparisons (llvm.amdgcn.fcmp predicates).
+__device__ __forceinline__ unsigned long long mask_ge0(float v) { return __builtin_amdgcn_fcmpf(v, 0.0f, 3); }
+__device__ __forceinline__ unsigned long long mask_lt0(float v) { return __builtin_amdgcn_fcmpf(v, 0.0f, 4); }
+
+__device__ __forceinline__ float4 load_tap(__amdgpu_buffer_rsrc_t rsrc, uint32_t offset) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, offset, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
+template <int COSTK, typename FT, int GPT, bool POSW>
+__device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*s_coef)[12], double (*acc)[3],
+                                           uint32_t *cnt) {
+    const int tid = threadIdx.x;
+    const int fw = P.fw, fh = P.fh;
+    const int cols = fw < kThreads ? fw : kThreads;
+    const int rpi = kThreads / cols;
+    const int r0 = tid / cols, xc = tid - r0 * cols;
+    const float Wm = (float) P.mw, Hm = (float) P.mh;
+    const FT *fix = (const FT *) P.fix;
+    const int mw = P.mw;
+    const int w16 = mw * 16;
+    const int base16 = (mw + 1) * 16;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) P.quads, (short) 0, (int) (((size_t) mw * P.mh + 2 * (size_t) mw + 2) * 16), 0x00020000);
+
+    float B[GPT][3];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) B[g][k] = uniform(s_coef[g][3 * k + 1]);
+
+    for (int x0 = 0; x0 < fw; x0 += kThreads) {
+        const int x = x0 + xc;
+        const bool valid = (r0 < rpi) && (x < fw);
+        const float xf = (float) x;
+        float col[GPT][3];
+#pragma unroll
+        for (int g = 0; g < GPT; ++g)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                col[g][k] = fmaf(uniform(s_coef[g][3 * k]), xf, uniform(s_coef[g][3 * k + 2]));
+        float part[GPT][3];
+#pragma unroll
+        for (int g = 0; g < GPT; ++g) part[g][0] = part[g][1] = part[g][2] = 0.0f;
+        int since = 0;
+        // the same trip count on every lane keeps the ballot counts wave-uniform
+        const int steps = (fh + rpi - 1) / rpi;
+        for (int it = 0; it < steps; ++it) {
+            const int y = r0 + it * rpi;
+            const bool vy = valid && y < fh;
+            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(vy);
+            const float f = vy ? (float) fix[y * fw + x] : 0.0f;
+            const float f2 = f * f;
+            const float yf = (float) y;
+            unsigned long long in[GPT];   // wave mask of lanes whose sample is in the image
+            float ax[GPT], ay[GPT];
+            float4 q[GPT];
+#pragma unroll
+            for (int g = 0; g < GPT; ++g) {
+                const float Xn = fmaf(B[g][0], yf, col[g][0]);
+                const float Yn = fmaf(B[g][1], yf, col[g][1]);
+                const float Wn = fmaf(B[g][2], yf, col[g][2]);
+                unsigned long long m;
+                if constexpr (POSW) {
+                    m = mask_ge0(Xn) & mask_ge0(Yn) & mask_lt0(fmaf(-Wm, Wn, Xn)) & mask_lt0(fmaf(-Hm, Wn, Yn));
+                } else {
+                    const bool inside = quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm);
+                    m = __builtin_amdgcn_ballot_w64(inside);
+                }
+                m &= vmask;
+                in[g] = m;
+                cnt[g] += (uint32_t) __builtin_popcountll(m);
+                const float r = __builtin_amdgcn_rcpf(Wn);
+                const float X = Xn * r, Y = Yn * r;
+                int ix = floor_to_int(X), iy = floor_to_int(Y);
+                if constexpr (!POSW) {
+                    // W may be tiny or negative here: keep the taps inside the table
+                    ix = min(max(ix, -1), mw);
+                    iy = min(max(iy, -1), P.mh);
+                }
+                ax[g] = __builtin_amdgcn_fractf(X);
+                ay[g] = __builtin_amdgcn_fractf(Y);
+                q[g] = load_tap(rsrc, mask_select(m, kOffsetOut, tap_offset(ix, iy, w16, base16)));
+            }
+#pragma unroll
+            for (int g = 0; g < GPT; ++g) {
+                // lanes outside read a zero tap: v == 0 exactly when the weights are finite
+                float v = fmaf(ay[g], fmaf(ax[g], q[g].w, q[g].z), fmaf(ax[g], q[g].y, q[g].x));
+                if constexpr (!POSW) v = __uint_as_float(mask_select(in[g], 0u, __float_as_uint(v)));
+                if constexpr (COSTK == 0) {
+                    part[g][0] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint(fabsf(v - f))));
+                } else {
+                    part[g][0] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint(f2)));
+                    part[g][1] = fmaf(v, v, part[g][1]);
+                    if constexpr (COSTK == 1) part[g][2] = fmaf(f, v, part[g][2]);
+                    else part[g][2] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint((v - f) * (v - f))));
+                }
+            }
+            if (++since == kFlush) {
+                since = 0;
+#pragma unroll
+                for (int g = 0; g < GPT; ++g)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        acc[g][k] += (double) part[g][k];
+                        part[g][k] = 0.0f;
+                    }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < GPT; ++g)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[g][k] += (double) part[g][k];
+    }
+}
+
+// Shared memory of one workgroup's GPT grid points (fast and MI paths).
+template <int GPT>
+struct BlockShared {
+    float coef[GPT][12];
+    double red[4][GPT][4];
+    float cost[GPT];
+    double scr[8];
+    uint32_t px[64], py[64];
+    double lpx[64], lpy[64];
+};
+
+// One workgroup's GPT grid points [base, base+GPT) with the fast sweep.
+template <int COSTK, typename FT, int GPT>
+__device__ __forceinline__ void fast_block(const SearchParams &P, int32_t base, BlockShared<GPT> &S) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    block_coefficients<GPT>(P, base, S.coef);
+    __syncthreads();
+    bool posw = true;
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) posw = posw && S.coef[g][9] != 0.0f;
+
+    double acc[GPT][3];
+    uint32_t cnt[GPT];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) { acc[g][0] = acc[g][1] = acc[g][2] = 0.0; cnt[g] = 0u; }
+    if (posw) fast_sweep<COSTK, FT, GPT, true>(P, S.coef, acc, cnt);
+    else fast_sweep<COSTK, FT, GPT, false>(P, S.coef, acc, cnt);
+
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) {
+        const double v0 = wave_sum(acc[g][0]);
+        const double v1 = wave_sum(acc[g][1]);
+        const double v2 = wave_sum(acc[g][2]);
+        if (lane == 0) {
+            S.red[wave][g][0] = v0;
+            S.red[wave][g][1] = v1;
+            S.red[wave][g][2] = v2;
+            S.red[wave][g][3] = (double) cnt[g];   // wave-uniform ballot count
+        }
+    }
+    __syncthreads();
+    if (tid < GPT) {
+        double t[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] = S.red[0][tid][k] + S.red[1][tid][k] + S.red[2][tid][k] + S.red[3][tid][k];
+        if constexpr (COSTK == 0) S.cost[tid] = finish_cost<kFamSAD>(P.cost, t, false);
+        else S.cost[tid] = finish_cost<kFamNCC>(P.cost, t, false);
+    }
+    __syncthreads();
+    if (tid == 0) emit<GPT>(P, base, S.cost);
+    __syncthreads();
+}
+
+template <int COSTK, typename FT, int GPT>
+__global__ __launch_bounds__(kThreads) void search_fast_kernel(const SearchParams P) {
+    __shared__ BlockShared<GPT> S;
+    fast_block<COSTK, FT, GPT>(P, (int32_t) blockIdx.x * GPT, S);
+}
+
+// --------------------------------------------------------------- MI kernel
+//
+// calcMIstream on one channel with the column-per-thread sweep of the fast
+// kernel, but bit-exact: sample points are correctly rounded quotients and the
+// bilinear value follows the declared lerps, so every joint-histogram count
+// equals the oracle's.  The taps of a one-byte image are packed four to a
+// 32-bit word (tl, tr, bl, br), those of a float image four to a float4; both
+// tables are indexed like the fast kernel's.  Lanes outside the image add to
+// per-lane dummy counters past the histograms, so the atomics need no branch.
+
+struct Recip {
+    float d, r;
+};
+
+// Reciprocal of d refined by one Newton step (v_rcp_f32, then
+// r += r (1 - d r)), and the quotient n / d from it by two correction steps.
+// This is the arithmetic of the compiler's correctly rounded f32 division
+// (v_div_scale ... v_div_fmas, v_div_fixup) when v_div_scale leaves its
+// operands unscaled, which holds for d in [2^-6, 2^6] and finite n with
+// n == 0 or 2^-90 < |n| < 2^90; the kernel checks the W range per grid
+// point and uses '/' otherwise.
+__device__ __forceinline__ Recip refine_rcp(float d) {
+    float r = __builtin_amdgcn_rcpf(d);
+    const float e = fmaf(-d, r, 1.0f);
+    r = fmaf(e, r, r);
+    return {d, r};
+}
+
+__device__ __forceinline__ float div_by(float n, Recip R) {
+    float q = n * R.r;
+    float rem = fmaf(-R.d, q, n);
+    q = fmaf(rem, R.r, q);
+    rem = fmaf(-R.d, q, n);
+    return fmaf(rem, R.r, q);
+}
+
+template <typename MT> struct TapTable;
+template <> struct TapTable<uint8_t> {
+    static constexpr int kBytes = 4;
+    __device__ static float4 load(__amdgpu_buffer_rsrc_t rsrc, uint32_t off) {
+        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0);
+        return make_float4((float) (w & 255u), (float) ((w >> 8) & 255u), (float) ((w >> 16) & 255u),
+                           (float) (w >> 24));
+    }
+};
+template <> struct TapTable<float> {
+    static constexpr int kBytes = 16;
+    __device__ static float4 load(__amdgpu_buffer_rsrc_t rsrc, uint32_t off) { return load_tap(rsrc, off); }
+};
+
+__device__ __forceinline__ int clamp_int(int v, int lo, int hi) {
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(lo), "v"(hi));
+    return r;
+}
+
+// floor(bins * (v / 256)) clamped to [0, bins-1].  POW2: bins is a power of
+// two, so both scalings are exact and one multiply by bins/256 gives the same
+// value.
+template <bool POW2>
+__device__ __forceinline__ int mi_bin_fast(float v, int bins, float fbins, float scale) {
+    const float t = POW2 ? v * scale : fbins * (v / 256.0f);
+    return clamp_int(floor_to_int(t), 0, bins - 1);
+}
+
+template <typename MT, typename FT, int GPT, bool FASTDIV, bool POW2>
+__device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_coef)[12], uint32_t *hist,
+                                         uint32_t *cnt) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int fw = P.fw, fh = P.fh;
+    const int cols = fw < kThreads ? fw : kThreads;
+    const int rpi = kThreads / cols;
+    const int r0 = tid / cols, xc = tid - r0 * cols;
+    const float Wm = (float) P.mw, Hm = (float) P.mh;
+    const FT *fix = (const FT *) P.fix;
+    const int mw = P.mw;
+    constexpr int EB = TapTable<MT>::kBytes;
+    const int wE = mw * EB, baseE = (mw + 1) * EB;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) P.quads, (short) 0, (int) (((size_t) mw * P.mh + 2 * (size_t) mw + 2) * EB), 0x00020000);
+    const int bins = P.bins, B2 = bins * bins;
+    const float fbins = (float) bins, bscale = fbins / 256.0f;
+    const uint32_t dummy = (uint32_t) (GPT * B2 + lane);
+
+    float B[GPT][3];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) B[g][k] = uniform(s_coef[g][3 * k + 1]);
+
+    for (int x0 = 0; x0 < fw; x0 += kThreads) {
+        const int x = x0 + xc;
+        const bool valid = (r0 < rpi) && (x < fw);
+        const float xf = (float) x;
+        float col[GPT][3];
+#pragma unroll
+        for (int g = 0; g < GPT; ++g)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                col[g][k] = fmaf(uniform(s_coef[g][3 * k]), xf, uniform(s_coef[g][3 * k + 2]));
+        const int steps = (fh + rpi - 1) / rpi;
+        for (int it = 0; it < steps; ++it) {
+            const int y = r0 + it * rpi;
+            const bool vy = valid && y < fh;
+            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(vy);
+            const float f = vy ? (float) fix[y * fw + x] : 0.0f;
+            const uint32_t rowbin = (uint32_t) (mi_bin_fast<POW2>(f, bins, fbins, bscale) * bins);
+            const float yf = (float) y;
+            unsigned long long in[GPT];
+            float ax[GPT], ay[GPT];
+            float4 q[GPT];
+#pragma unroll
+            for (int g = 0; g < GPT; ++g) {
+                const float Xn = fmaf(B[g][0], yf, col[g][0]);
+                const float Yn = fmaf(B[g][1], yf, col[g][1]);
+                const float Wn = fmaf(B[g][2], yf, col[g][2]);
+                unsigned long long m;
+                float X, Y;
+                if constexpr (FASTDIV) {
+                    m = mask_ge0(Xn) & mask_ge0(Yn) & mask_lt0(fmaf(-Wm, Wn, Xn)) & mask_lt0(fmaf(-Hm, Wn, Yn));
+                    const Recip R = refine_rcp(Wn);
+                    X = div_by(Xn, R);
+                    Y = div_by(Yn, R);
+                } else {
+                    m = __builtin_amdgcn_ballot_w64(quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm));
+                    X = Xn / Wn;
+                    Y = Yn / Wn;
+                }
+                m &= vmask;
+                in[g] = m;
+                cnt[g] += (uint32_t) __builtin_popcountll(m);
+                // in-image sample points lie in [0, W] x [0, H]; other lanes'
+                // offsets are replaced below, so only the general-W path clamps
+                // (its discarded quotients may be huge or NaN)
+                int ix = floor_to_int(X), iy = floor_to_int(Y);
+                if constexpr (!FASTDIV) {
+                    ix = clamp_int(ix, -1, mw);
+                    iy = clamp_int(iy, -1, P.mh);
+                }
+                ax[g] = __builtin_amdgcn_fractf(X);
+                ay[g] = __builtin_amdgcn_fractf(Y);
+                uint32_t off;
+                asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(off) : "v"(iy), "s"(wE), "v"(ix * EB + baseE));
+                q[g] = TapTable<MT>::load(rsrc, mask_select(m, kOffsetOut, off));
+            }
+#pragma unroll
+            for (int g = 0; g < GPT; ++g) {
+                const float top = fmaf(ax[g], q[g].y, (1.0f - ax[g]) * q[g].x);
+                const float bottom = fmaf(ax[g], q[g].w, (1.0f - ax[g]) * q[g].z);
+                const float v = fmaf(ay[g], bottom, (1.0f - ay[g]) * top);
+                const uint32_t h = (uint32_t) (g * B2) + rowbin + (uint32_t) mi_bin_fast<POW2>(v, bins, fbins, bscale);
+                atomicAdd(&hist[mask_select(in[g], dummy, h)], 1u);
+            }
+        }
+    }
+}
+
+// calcMIstream's entropy step (src/gpu/cudaErrorFunction_miStreams.cuh:306-365)
+// in binary64 for grid point g of the workgroup; n = in-image sample count.
+// Every thread calls it; thread 0 returns the cost.
+__device__ inline double mi_entropy_cost(const uint32_t *h, int bins, double n, int cost, uint32_t *s_px, uint32_t *s_py,
+                                  double *s_lpx, double *s_lpy, double *s_scr) {
+    const int tid = threadIdx.x;
+    const int B2 = bins * bins;
+    if (tid < bins) {
+        uint32_t r = 0;
+        for (int j = 0; j < bins; ++j) r += h[tid * bins + j];
+        s_px[tid] = r;
+    } else if (tid < 2 * bins) {
+        uint32_t r = 0;
+        const int col = tid - bins;
+        for (int i = 0; i < bins; ++i) r += h[i * bins + col];
+        s_py[col] = r;
+    }
+    __syncthreads();
+    double hxy = 0.0;   // -p log2 p of the marginals
+    if (tid < bins) {
+        const double pxi = s_px[tid] / n;
+        s_lpx[tid] = pxi > 0 ? log2(pxi) : 0.0;
+        if (pxi > 0) hxy -= pxi * s_lpx[tid];
+    } else if (tid < 2 * bins) {
+        const double pyi = s_py[tid - bins] / n;
+        s_lpy[tid - bins] = pyi > 0 ? log2(pyi) : 0.0;
+        if (pyi > 0) hxy -= pyi * s_lpy[tid - bins];
+    }
+    __syncthreads();
+    double mi = 0.0;
+    for (int j = tid; j < B2; j += kThreads) {
+        const int i = j / bins, k = j - i * bins;
+        const double pxy = h[j] / n, pxi = s_px[i] / n, pyj = s_py[k] / n;
+        if (pxy > 0 && pxi > 0 && pyj > 0) mi += pxy * (log2(pxy) - (s_lpx[i] + s_lpy[k]));
+    }
+    const double hx = block_sum(tid < bins ? hxy : 0.0, s_scr);
+    const double hy = block_sum(tid >= bins && tid < 2 * bins ? hxy : 0.0, s_scr + 4);
+    const double mis = block_sum(mi, s_scr);
+    double r = cost == CGS_COST_MI ? -2.0 * mis / (hx + hy) : -mis;
+    if (n == 0.0) r = cost == CGS_COST_MI ? __builtin_nan("") : -0.0;
+    __syncthreads();
+    return r;
+}
+
+// One workgroup's GPT grid points with the MI sweep; hist: GPT * bins^2 + 64
+// words of LDS.
+template <typename MT, typename FT, int GPT>
+__device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, BlockShared<GPT> &S, uint32_t *hist) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int B2 = P.bins * P.bins;
+    block_coefficients<GPT>(P, base, S.coef);
+    for (int i = tid; i < GPT * B2 + 64; i += kThreads) hist[i] = 0u;
+    __syncthreads();
+    bool fastdiv = true;
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) fastdiv = fastdiv && S.coef[g][10] != 0.0f;
+    uint32_t cnt[GPT];
+#pragma unroll
+    for (int g = 0; g < GPT; ++g) cnt[g] = 0u;
+    const bool pow2 = (P.bins & (P.bins - 1)) == 0;
+    if (fastdiv && pow2) mi_sweep<MT, FT, GPT, true, true>(P, S.coef, hist, cnt);
+    else if (fastdiv) mi_sweep<MT, FT, GPT, true, false>(P, S.coef, hist, cnt);
+    else mi_sweep<MT, FT, GPT, false, false>(P, S.coef, hist, cnt);
+    if (lane == 0)
+#pragma unroll
+        for (int g = 0; g < GPT; ++g) S.red[wave][g][3] = (double) cnt[g];
+    __syncthreads();
+    for (int g = 0; g < GPT; ++g) {
+        const double n = S.red[0][g][3] + S.red[1][g][3] + S.red[2][g][3] + S.red[3][g][3];
+        const double r = mi_entropy_cost(hist + g * B2, P.bins, n, P.cost, S.px, S.py, S.lpx, S.lpy, S.scr);
+        if (tid == 0) S.cost[g] = (float) r;
+    }
+    if (P.hist_dump && base == 0)
+        for (int j = tid; j < B2; j += kThreads) P.hist_dump[j] = hist[j];
+    __syncthreads();
+    if (tid == 0) emit<GPT>(P, base, S.cost);
+    __syncthreads();
+}
+
+template <typename MT, typename FT, int GPT>
+__global__ __launch_bounds__(kThreads) void search_mi_kernel(const SearchParams P) {
+    __shared__ BlockShared<GPT> S;
+    extern __shared__ uint32_t s_hist[];   // GPT histograms + 64 dummy counters
+    mi_block<MT, FT, GPT>(P, (int32_t) blockIdx.x * GPT, S, s_hist);
+}
+
+}  // namespace cgs

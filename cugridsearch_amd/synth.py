@@ -152,3 +152,26 @@ def workload(name, levels=None, width=None):
                         dtype=spec["dtype"], gamma=spec.get("gamma", False), noise=spec.get("noise", 0.0))
     start, end, count = grid_ranges(spec["levels"], spec["translate"])
     return m, f, start, end, count, spec
+
+
+def batch_workload(n=256, seed=0xC6504, sides=(64, 512), levels=(4, 5, 6)):
+    """Ragged batch: n pairs with width and height drawn independently from
+    U{sides}, uint8 and float32 alternating, grid levels from `levels`, costs
+    SAD / NCC / MI 1:1:1 in a seeded shuffle.  Returns a list of dicts with
+    moving, fixed, start, end, count, cost ("sad" | "ncc" | "mi")."""
+    rng = np.random.default_rng(seed)
+    costs = (["sad", "ncc", "mi"] * ((n + 2) // 3))[:n]
+    rng.shuffle(costs)
+    out = []
+    for k in range(n):
+        w = int(rng.integers(sides[0], sides[1] + 1))
+        h = int(rng.integers(sides[0], sides[1] + 1))
+        dtype = np.uint8 if k % 2 == 0 else np.float32
+        lv = int(rng.choice(levels))
+        sigma = 3.0 if max(w, h) <= 128 else 6.0
+        perturb = max(2.0, min(w, h) / 32.0)
+        m, f, _ = make_pair(int(rng.integers(1 << 31)), w, h, sigma=sigma, perturb=perturb, dtype=dtype,
+                            gamma=costs[k] == "mi", noise=4.0 if costs[k] == "mi" else 0.0)
+        start, end, count = grid_ranges(lv, max(2.0, min(w, h) / 32.0))
+        out.append(dict(moving=m, fixed=f, start=start, end=end, count=count, cost=costs[k], levels=lv))
+    return out

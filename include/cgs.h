@@ -96,6 +96,7 @@ typedef struct {
     int32_t tie_mode;    /* cgs_tie_mode */
     int32_t mi_bins;     /* joint-histogram bins per axis for the MI costs: 8..64 (reference: 64) */
     int32_t flags;       /* CGS_FLAG_* bits */
+    int32_t tile_gpt;    /* grid points per workgroup; 0 = library default (autotuner choice) */
 } cgs_search_opts;
 
 /* opts.flags: time the search kernel with HIP events on the caller's
@@ -150,6 +151,53 @@ cgs_status cgs_find_extrema(const float *values, int64_t n, int32_t tie_mode, cg
 cgs_status cgs_eval_point(const cgs_image *moved, const cgs_image *fixed, const cgs_search_opts *opts,
                           int32_t dim, const float *params, float *cost_out, uint32_t *hist_out,
                           void *stream);
+
+/* ------------------------------------------------------------------------
+ * Batched ragged search.  No reference counterpart: the reference searches
+ * one pair per CudaGridSearcher call (src/gpu/cudaGridSearch.cuh:736-765).
+ * One persistent kernel takes (pair, grid-tile) work items of uneven size
+ * from a device work counter; every pair's costs and argmin are identical to
+ * cgs_search on that pair.  Pairs: one channel, bilinear sampling, any cost
+ * (SAD, SAD_SUM, NCC, SQD, MI, MI_PLAIN), uint8 or float images of any size.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    cgs_image moved;
+    cgs_image fixed;
+    cgs_grid grid;           /* from cgs_grid_init */
+    cgs_search_opts opts;    /* sampler must be CGS_SAMPLER_BILINEAR */
+    float *result;           /* device cost tensor (grid order) or NULL */
+} cgs_batch_pair;
+
+/* Tile shapes of the persistent kernel; 0 fields take the defaults. */
+typedef struct {
+    int32_t fast_gpt;        /* grid points per workgroup for SAD/NCC/SQD: 4 or 8 */
+    int32_t mi_gpt;          /* grid points per workgroup for MI: 2 or 4 */
+    int32_t wg_per_cu;       /* resident workgroups per compute unit: 1..8 */
+    int32_t items_per_claim; /* work items taken per atomic claim: 1..64 */
+} cgs_tile_config;
+
+typedef struct cgs_batch_s cgs_batch;
+
+cgs_status cgs_batch_create(cgs_batch **out);
+
+/* Validates the pairs, records their views and allocates the batch's device
+ * workspace (tap tables, descriptors, keys).  The images may change between
+ * runs; their shapes, grids and options may not (plan again). */
+cgs_status cgs_batch_plan(cgs_batch *batch, const cgs_batch_pair *pairs, int32_t n, const cgs_tile_config *tiles);
+
+/* Enqueues one batched search on the stream: restages every pair's tap
+ * table, clears the keys and the work counter, launches the persistent
+ * kernel.  use_graph != 0: the first run on a stream captures this sequence
+ * into a hipGraph and later runs replay it.  Asynchronous. */
+cgs_status cgs_batch_run(cgs_batch *batch, int32_t use_graph, void *stream);
+
+/* Waits for the stream and returns every pair's argmin (n entries). */
+cgs_status cgs_batch_results(cgs_batch *batch, cgs_argmin *out, void *stream);
+
+/* Total work items and grid points of the planned batch (diagnostics). */
+cgs_status cgs_batch_info(const cgs_batch *batch, int64_t *items, int64_t *points, int64_t *pixel_evals);
+
+cgs_status cgs_batch_destroy(cgs_batch *batch);
 
 /* Duration in ms of the calling thread's last timed search kernel (flags &
  * CGS_FLAG_TIME_KERNEL), measured with HIP events around that launch; -1

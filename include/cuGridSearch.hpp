@@ -1,0 +1,259 @@
+// cuGridSearch.hpp -- C++ host-side drop-in for the reference's search API,
+// implemented over the C-ABI of cgs.h (libcgs.so, gfx950).
+//
+// Same class names, member names and argument meaning as the reference's
+// header templates, so a driver ports by swapping the include and the functor
+// binding:
+//   CudaGrid<P, D>          src/gpu/cudaGridSearch.cuh:74-231
+//   CudaTensor<P, D>        include/cudaTensor.cuh:108-361
+//   CudaImage<P, C>         include/cudaTensor.cuh:507-596
+//   CudaGridSearcher<F,G,D> src/gpu/cudaGridSearch.cuh:540-855
+//   umbrella include        include/cuGridSearch.cuh:8-11
+// Containers stay shallow views over caller-allocated device memory
+// (hipMalloc(&x.data(), x.bytesSize())), as in the reference.
+//
+// Differences, by design:
+//   * functors are tags naming a cost of the C-ABI (calcNCC, calcMIstream,
+//     averageAbsoluteDifference, ...) instead of __device__ function pointers
+//     read back with cudaMemcpyFromSymbol; the kernels are specialised per
+//     cost;
+//   * errors throw cgs_error (the reference prints and exits the process);
+//   * the grid's start / end / resolution / count rows are kept on the host
+//     and mirrored into the caller's 4 x D device matrix when one is set.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cgs.h"
+
+struct cgs_error : std::runtime_error {
+    cgs_status status;
+    cgs_error(cgs_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+inline void cgs_check(cgs_status s, const char *what) {
+    if (s != CGS_OK) throw cgs_error(s, std::string(what) + ": " + cgs_last_error());
+}
+
+inline void cgs_check_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw cgs_error(CGS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ functors
+
+// A cost functor: the C-ABI cost, how a grid point becomes the warp, which
+// sampler reads the moving image, and the MI bin count.
+struct CgsFunctor {
+    cgs_cost cost;
+    cgs_param_mode mode;
+    cgs_sampler sampler;
+    int bins;
+    constexpr CgsFunctor with_sampler(cgs_sampler s) const { return {cost, mode, s, bins}; }
+    constexpr CgsFunctor with_bins(int b) const { return {cost, mode, sampler, b}; }
+};
+
+// The reference functors (src/gpu/cudaErrorFunctions.cuh, cudaErrorFunction_mi*.cuh).
+constexpr CgsFunctor averageAbsoluteDifference{CGS_COST_SAD, CGS_PARAM_NONCENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor averageAbsoluteDifferenceH{CGS_COST_SAD, CGS_PARAM_RAW_H, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor sumOfAbsoluteDifferences{CGS_COST_SAD_SUM, CGS_PARAM_TRANSLATION, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcSQD{CGS_COST_SQD, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcNCC{CGS_COST_NCC, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcMIstream{CGS_COST_MI, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcMI{CGS_COST_MI_PLAIN, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+// centred-homography SAD (the search workloads' SAD)
+constexpr CgsFunctor centredAbsoluteDifference{CGS_COST_SAD, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+
+// ------------------------------------------------------------------ tensors
+
+template <typename T> struct CgsPixType;
+template <> struct CgsPixType<uint8_t> { static constexpr cgs_pixtype value = CGS_PIX_U8; };
+template <> struct CgsPixType<float> { static constexpr cgs_pixtype value = CGS_PIX_F32; };
+
+// CudaTensor<P, D> (include/cudaTensor.cuh:108-361): a shallow view over
+// caller-allocated device memory; axis 0 varies fastest.
+template <typename precision, unsigned int D>
+struct CudaTensor {
+    precision *_data = nullptr;
+    int32_t _dims[D] = {};
+    int32_t _total_size = 0;
+
+    CudaTensor() = default;
+
+    template <typename T>
+    explicit CudaTensor(const T (&dims)[D]) {
+        _total_size = 1;
+        for (unsigned d = 0; d < D; ++d) {
+            _dims[d] = (int32_t) dims[d];
+            _total_size *= _dims[d];
+        }
+    }
+
+    precision *&data() { return _data; }
+    uint32_t size(int dim = -1) const { return dim == -1 ? (uint32_t) _total_size : (uint32_t) _dims[dim]; }
+    uint32_t bytesSize() const { return size() * sizeof(precision); }
+
+    void setValuesFromVector(const std::vector<precision> &vals) const {
+        cgs_check_hip(hipMemcpy(_data, vals.data(), vals.size() * sizeof(precision), hipMemcpyHostToDevice),
+                      "setValuesFromVector");
+    }
+
+    std::vector<precision> values() const {
+        std::vector<precision> h(size());
+        cgs_check_hip(hipMemcpy(h.data(), _data, bytesSize(), hipMemcpyDeviceToHost), "values");
+        return h;
+    }
+
+    void fill(precision value) { setValuesFromVector(std::vector<precision>(size(), value)); }
+
+    // include/cudaTensor.cuh:215-265: global minimum and its int32 index (-1
+    // when no value is below +inf); prints the reference's line.
+    void find_extrema(precision &extrema_value, int32_t &extrema_index1d, cgs_tie_mode tie = CGS_TIE_LOWEST) {
+        cgs_argmin a;
+        cgs_check(cgs_find_extrema((const float *) _data, _total_size, tie, &a, nullptr), "find_extrema");
+        extrema_value = a.value;
+        extrema_index1d = a.index;
+        std::printf("\n Grid MIN value has idx: %d  value: %f\n", extrema_index1d, (double) extrema_value);
+    }
+};
+
+// CudaImage<P, C>(H, W) (include/cudaTensor.cuh:507-596): dims {C, W, H}.
+template <typename precision, uint32_t CHANNELS = 1>
+struct CudaImage : CudaTensor<precision, 3> {
+    CudaImage(uint32_t height, uint32_t width) : CudaTensor<precision, 3>({CHANNELS, width, height}) {}
+    int32_t channels() const { return this->_dims[0]; }
+    int32_t width() const { return this->_dims[1]; }
+    int32_t height() const { return this->_dims[2]; }
+    cgs_image view() const {
+        return cgs_image{this->_data, width(), height(), channels(), CgsPixType<precision>::value};
+    }
+};
+
+// --------------------------------------------------------------------- grid
+
+// CudaGrid<P, D> (src/gpu/cudaGridSearch.cuh:74-231).
+template <typename precision, unsigned int D>
+struct CudaGrid {
+    precision *_data = nullptr;   // optional 4 x D device mirror (start, end, resolution, count rows)
+    cgs_grid _g{};
+    float _start[D] = {}, _end[D] = {}, _count[D];
+
+    CudaGrid() {
+        for (unsigned d = 0; d < D; ++d) _count[d] = 1.0f;
+        update();
+    }
+
+    precision *&data() { return _data; }
+    uint32_t bytesSize() const { return 4 * D * sizeof(precision); }
+    int getDimension() const { return D; }
+
+    // update() -> updateProcess (src/gpu/cudaGridSearch.cuh:100-105, 256-270)
+    void update(int = -1) {
+        cgs_check(cgs_grid_init(&_g, D, _start, _end, _count), "CudaGrid::update");
+        if (_data) {
+            precision rows[4 * D];
+            for (unsigned d = 0; d < D; ++d) {
+                rows[d] = _g.start[d];
+                rows[D + d] = _g.end[d];
+                rows[2 * D + d] = _g.resolution[d];
+                rows[3 * D + d] = _g.count[d];
+            }
+            cgs_check_hip(hipMemcpy(_data, rows, sizeof(rows), hipMemcpyHostToDevice), "CudaGrid::update");
+        }
+    }
+
+    template <typename T> void setStartPoint(std::vector<T> v) { set_row(_start, v); }
+    template <typename T> void setEndPoint(std::vector<T> v) { set_row(_end, v); }
+    template <typename T> void setNumSamples(std::vector<T> v) { set_row(_count, v); }
+
+    uint32_t numElements() const { return (uint32_t) _g.num_elements; }
+
+    void getAxisSampleCounts(precision *axis_sample_counts) const {
+        for (unsigned d = 0; d < D; ++d) axis_sample_counts[d] = _g.count[d];
+    }
+
+    void getGridPoint(precision (&grid_point)[D], int index) const {
+        float p[CGS_MAX_DIM];
+        cgs_check(cgs_grid_point(&_g, index, p), "getGridPoint");
+        for (unsigned d = 0; d < D; ++d) grid_point[d] = p[d];
+    }
+
+    // CudaMatrix::display format (include/cudaTensor.cuh:395-415); the first
+    // four {..} groups are what the MATLAB harness parses.
+    void display(const std::string &name = "") const {
+        std::cout << "Matrix " << name << " 4 x " << D << " elements of f\n\n";
+        const float *rows[4] = {_g.start, _g.end, _g.resolution, _g.count};
+        for (auto r : rows) {
+            std::cout << "{ ";
+            for (unsigned d = 0; d + 1 < D; ++d) std::cout << r[d] << ", ";
+            std::cout << r[D - 1] << " }\n";
+        }
+        std::cout << std::endl;
+    }
+
+  private:
+    template <typename T> void set_row(float *row, const std::vector<T> &v) {
+        if (v.size() != D) throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "CudaGrid: vector size differs from D");
+        for (unsigned d = 0; d < D; ++d) row[d] = (float) v[d];
+        update();
+    }
+};
+
+// --------------------------------------------------------------- searcher
+
+// CudaGridSearcher<F, G, D> (src/gpu/cudaGridSearch.cuh:540-855).  Every
+// search variant fills the result tensor with the functor's cost per grid
+// point (grid-index order) and synchronises; the fused argmin of the last
+// search is kept in last_argmin.
+template <typename func_precision, typename grid_precision, unsigned int D>
+struct CudaGridSearcher {
+    CudaGrid<grid_precision, D> *_grid;
+    CudaTensor<func_precision, D> *_result;
+    cgs_argmin last_argmin{INFINITY, -1};
+    cgs_tie_mode tie_mode = CGS_TIE_LOWEST;
+    hipStream_t stream = nullptr;
+
+    CudaGridSearcher(CudaGrid<grid_precision, D> &grid, CudaTensor<func_precision, D> &result)
+        : _grid(&grid), _result(&result) {}
+
+    template <typename Img> void search(const CgsFunctor &f, Img &moved, Img &fixed) {
+        search_by_value(f, moved, fixed);
+    }
+
+    template <typename Img> void search_by_value(const CgsFunctor &f, Img &moved, Img &fixed) {
+        if (_result->size() != _grid->numElements())
+            throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "result tensor size differs from the grid's point count");
+        std::cout << "CudaGrid has " << _grid->numElements() << " search samples." << std::endl;
+        cgs_search_opts o;
+        cgs_default_opts(&o);
+        o.cost = f.cost;
+        o.param_mode = f.mode;
+        o.sampler = f.sampler;
+        o.mi_bins = f.bins;
+        o.tie_mode = tie_mode;
+        const cgs_image m = moved.view(), x = fixed.view();
+        cgs_check(cgs_search(&_grid->_g, &m, &x, &o, (float *) _result->data(), &last_argmin, stream),
+                  "search_by_value");
+    }
+
+    // The launch-policy variants compute the same costs with the one fused
+    // kernel; their tuning arguments are accepted and unused.
+    template <typename Img> void search_by_value_stream(const CgsFunctor &f, int, int, Img &m, Img &x) {
+        search_by_value(f, m, x);
+    }
+    template <typename Img> void search_by_value_block(const CgsFunctor &f, int, Img &m, Img &x) {
+        search_by_value(f, m, x);
+    }
+    template <typename Img> void search_by_reference(const CgsFunctor &f, Img *m, Img *x) {
+        search_by_value(f, *m, *x);
+    }
+    template <typename Img> void search_by_reference_stream(const CgsFunctor &f, int, int, Img *m, Img *x) {
+        search_by_value(f, *m, *x);
+    }
+};
