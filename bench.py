@@ -28,6 +28,7 @@ PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
 # NCC 6, MI 2, SAD 3
 FLOP_PER_PE = {("nearest", "sad"): 17 + 3, ("bilinear", "ncc"): 17 + 13 + 6, ("bilinear", "mi"): 17 + 13 + 2,
                ("bilinear", "sad"): 17 + 13 + 3}
+BATCH_WORKLOAD = "batch_ragged_256"
 
 
 def parse():
@@ -38,6 +39,8 @@ def parse():
     ap.add_argument("--workload", default="bilinear_ncc_256")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
+    ap.add_argument("--pairs", type=int, default=256, help="pairs of the ragged batch workload")
+    ap.add_argument("--no-graph", action="store_true", help="batch workload: eager launches instead of a hipGraph")
     return ap.parse_args()
 
 
@@ -60,6 +63,91 @@ def cpu_baseline(m, f, spec, start, end, count, pe_per_point, target_s):
                       f"{dt:.1f} s), C oracle -O3, OpenMP {threads} threads"}
 
 
+def run_batch(args):
+    """256 ragged pairs (sides U{64..512}, uint8/float alternating, 4^8..6^8
+    grids, SAD/NCC/MI 1:1:1) in one persistent hipGraph-replayed launch,
+    against the same pairs searched one plain launch at a time."""
+    import torch
+    from cugridsearch_amd import api as A
+    from cugridsearch_amd import synth, tuner
+
+    specs = synth.batch_workload(n=args.pairs)
+    fns = {"sad": A.centredAbsoluteDifference, "ncc": A.calcNCC, "mi": A.calcMIstream.with_(bins=32)}
+    pairs = []
+    for sp in specs:
+        g = A.CudaGrid(8)
+        g.setStartPoint(sp["start"])
+        g.setEndPoint(sp["end"])
+        g.setNumSamples(sp["count"])
+        pairs.append((fns[sp["cost"]], g, A.CudaImage.from_numpy(sp["moving"]), A.CudaImage.from_numpy(sp["fixed"]),
+                      None))
+    stream = torch.cuda.Stream()
+    tiles = tuner.tuned_batch(pairs)
+    tune_ms = None
+    if tiles is None:
+        times, tiles = tuner.tune_batch(pairs, stream=stream)
+        tune_ms = times
+    b = A.BatchSearcher()
+    b.plan(pairs, tiles=tiles)
+    info = b.info()
+    use_graph = not args.no_graph
+    for _ in range(args.warmup):
+        b.run(use_graph=use_graph, stream=stream)
+        b.results(stream=stream)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        b.run(use_graph=use_graph, stream=stream)
+    ev1.record(stream)
+    batch_am = b.results(stream=stream)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    # per-pair plain launch loop over the same pairs
+    loop_am = []
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for fn, g, m, f, _ in pairs:
+        res = A.CudaTensor(g.getAxisSampleCounts())
+        loop_am.append(A.CudaGridSearcher(g, res).search_by_value(fn, m, f, stream=stream))
+    torch.cuda.synchronize()
+    loop_dt = time.perf_counter() - t1
+    matched = sum(1 for a, c in zip(batch_am, loop_am) if a == c)
+    flop = sum(g.numElements() * f.width() * f.height() * FLOP_PER_PE[("bilinear", sp["cost"])]
+               for (fn, g, m, f, _), sp in zip(pairs, specs))
+    achieved = flop / (kernel_ms / 1000.0) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": info["pixel_evals"] / dt / 1e9,
+        "unit": "Gpixel-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1000.0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": BATCH_WORKLOAD, "pairs": len(pairs), "sides": "U{64..512} x U{64..512}",
+                   "grids": "4^8..6^8", "costs": "SAD:NCC:MI 1:1:1", "pixel_evals_per_step": info["pixel_evals"],
+                   "grid_points_per_step": info["points"], "work_items": info["items"], "hipgraph": use_graph,
+                   "tiles": tiles},
+        "pairs_per_s": len(pairs) / dt,
+        "plain_loop_pairs_per_s": len(pairs) / loop_dt,
+        "speedup_vs_plain_loop": loop_dt / dt,
+        "argmin_matched": matched / len(pairs),
+        "kernel_ms": kernel_ms,
+        "autotune_ms": tune_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_VALU_TFLOPS, "traffic": None,
+                     "flop_per_pixel_eval": "33 SAD / 36 NCC / 32 MI"},
+    }
+    if not args.no_cpu:
+        sp = specs[0]
+        m, f = sp["moving"], sp["fixed"]
+        line["cpu_baseline"] = cpu_baseline(m, f, dict(cost=sp["cost"], sampler="bilinear", bins=32), sp["start"],
+                                            sp["end"], sp["count"], f.shape[0] * f.shape[1], args.cpu_seconds)
+        line["cpu_baseline"]["sample"] = "first pair of the batch: " + line["cpu_baseline"]["sample"]
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -67,6 +155,9 @@ def main():
               "(the search shards by grid-index ranges across GPUs by design, but only the one-GPU path is built)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == BATCH_WORKLOAD:
+        print(json.dumps(run_batch(args)), flush=True)
+        return
     import torch
     from cugridsearch_amd import _lib as L
     from cugridsearch_amd import api as A

@@ -215,6 +215,8 @@ class CudaGridSearcher:
         if self._result.size() != self._grid.numElements():
             raise ValueError("result tensor size differs from the grid's point count")
         o = make_opts(functor, tie_mode)
+        from . import tuner
+        o.tile_gpt = tuner.tuned_gpt(functor, img_moved, img_fixed)
         out = L.Argmin()
         L.check(L.lib().cgs_search(ctypes.byref(self._grid._g), ctypes.byref(img_moved._img),
                                    ctypes.byref(img_fixed._img), ctypes.byref(o),

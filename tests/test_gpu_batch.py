@@ -93,3 +93,25 @@ def test_batch_rejects_unsupported(gpu):
     b = A.BatchSearcher()
     with pytest.raises(L.CgsError):
         b.plan([(fn.with_(sampler=L.SAMPLER_NEAREST), g, m, f, r)])
+
+
+def test_autotuner_tiles_bit_identical(gpu, tmp_path, monkeypatch):
+    from cugridsearch_amd import tuner
+    monkeypatch.setenv("CGS_TUNING_FILE", str(tmp_path / "tuning.json"))
+    specs, pairs = make_batch(n=3, seed=21, sides=(60, 90))
+    for fn, g, m, f, res in pairs:
+        want = A.CudaTensor(g.getAxisSampleCounts())
+        wam = A.CudaGridSearcher(g, want).search_by_value(fn, m, f)   # library default tiles
+        times, choice = tuner.tune_plain(fn, m, f, g, max_points=4096)
+        assert choice in times and tuner.tuned_gpt(fn, m, f) == choice
+        for gpt in times:                                               # every candidate, incl. the choice
+            o = A.make_opts(fn)
+            o.tile_gpt = gpt
+            import ctypes
+            out = L.Argmin()
+            got = A.CudaTensor(g.getAxisSampleCounts())
+            L.check(L.lib().cgs_search(ctypes.byref(g._g), ctypes.byref(m._img), ctypes.byref(f._img),
+                                       ctypes.byref(o), ctypes.c_void_p(got.data.data_ptr()), ctypes.byref(out),
+                                       None))
+            assert np.array_equal(got.data.cpu().numpy().view(np.uint32), want.data.cpu().numpy().view(np.uint32))
+            assert (out.value, out.index) == wam
