@@ -55,23 +55,29 @@ __global__ void batch_taps_kernel(const T *src, float4 *dst, int W, int H) {
     }
 }
 
-__global__ void batch_clear_kernel(unsigned long long *keys, int n, uint32_t *counter) {
+constexpr int kBatchKinds = 4;
+
+__global__ void batch_clear_kernel(unsigned long long *keys, int n, uint32_t *counters) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) keys[i] = ~0ull;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < kBatchKinds) counters[threadIdx.x * 16] = 0u;
 }
 
-template <int GF, int GM>
-__global__ __launch_bounds__(kThreads) void batch_kernel(const BatchDesc *descs, int npairs, uint32_t total,
+// One persistent kernel per cost kind (only that kind's sweep is compiled in,
+// so it keeps the single-pair kernel's register budget and occupancy).
+// descs: the kind's pairs, item_begin relative to the kind's first item.
+template <int KIND, int G>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void batch_kernel(const BatchDesc *descs, int npairs, uint32_t total,
                                                          uint32_t *counter, int per_claim) {
-    __shared__ BlockShared<GF> SF;
-    __shared__ BlockShared<GM> SM;
+    __shared__ BlockShared<G> S;
     __shared__ uint32_t s_claim;
-    extern __shared__ uint32_t s_hist[];   // GM MI histograms of the batch's largest bin count + 64 dummies
+    extern __shared__ uint32_t s_hist[];   // MI: G histograms of the batch's largest bin count + 64 dummies
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s_claim = atomicAdd(counter, (uint32_t) per_claim);
         __syncthreads();
-        const uint32_t first = s_claim;
+        // uniform by construction; readfirstlane tells the compiler so, which
+        // keeps the descriptor and the tap table's buffer resource in SGPRs
+        const uint32_t first = (uint32_t) __builtin_amdgcn_readfirstlane((int) s_claim);
         __syncthreads();   // every thread has read s_claim before the next claim
         if (first >= total) break;
         const uint32_t last = min(first + (uint32_t) per_claim, total);
@@ -83,14 +89,10 @@ __global__ __launch_bounds__(kThreads) void batch_kernel(const BatchDesc *descs,
                 if (descs[mid].item_begin <= item) lo = mid;
                 else hi = mid - 1;
             }
-            const BatchDesc &D = descs[lo];
-            const int32_t tile = (int32_t) (item - D.item_begin);
-            switch (D.kind) {
-            case kBatchMI: mi_block<float, float, GM>(D.P, tile * GM, SM, s_hist); break;
-            case kBatchNCC: fast_block<1, float, GF>(D.P, tile * GF, SF); break;
-            case kBatchSQD: fast_block<2, float, GF>(D.P, tile * GF, SF); break;
-            default: fast_block<0, float, GF>(D.P, tile * GF, SF); break;
-            }
+            const SearchParams &P = descs[lo].P;
+            const int32_t tile = (int32_t) (item - descs[lo].item_begin);
+            if constexpr (KIND == kBatchMI) mi_block<float, float, G>(P, tile * G, S, s_hist);
+            else fast_block<KIND, float, G>(P, tile * G, S);
         }
     }
 }
@@ -105,6 +107,8 @@ struct cgs_batch_s {
     int cus = 0;
     int max_bins = 8;        // largest MI bin count of the batch (sizes the LDS histograms)
     uint32_t total_items = 0;
+    int kind_begin[kBatchKinds] = {}, kind_count[kBatchKinds] = {};   // descriptor ranges per kind
+    uint32_t kind_items[kBatchKinds] = {};
     int64_t total_points = 0, total_pe = 0;
     // device staging
     DevBuf descs, keys, keys_host, counter;
@@ -140,22 +144,25 @@ int batch_kind(int cost) {
     }
 }
 
-template <int GF, int GM>
-hipError_t launch_batch(cgs_batch *b, hipStream_t s) {
-    const size_t lds = ((size_t) GM * b->max_bins * b->max_bins + 64) * sizeof(uint32_t);
+template <int KIND, int G>
+hipError_t launch_kind(cgs_batch *b, hipStream_t s) {
+    const size_t lds = KIND == kBatchMI ? ((size_t) G * b->max_bins * b->max_bins + 64) * sizeof(uint32_t) : 0;
     const int blocks = std::max(1, b->cus * b->tiles.wg_per_cu);
-    hipLaunchKernelGGL((batch_kernel<GF, GM>), dim3(blocks), dim3(kThreads), lds, s,
-                       (const BatchDesc *) b->descs.ptr, (int) b->pairs.size(), b->total_items,
-                       (uint32_t *) b->counter.ptr, b->tiles.items_per_claim);
+    hipLaunchKernelGGL((batch_kernel<KIND, G>), dim3(blocks), dim3(kThreads), lds, s,
+                       (const BatchDesc *) b->descs.ptr + b->kind_begin[KIND], b->kind_count[KIND],
+                       b->kind_items[KIND], (uint32_t *) b->counter.ptr + 16 * KIND, b->tiles.items_per_claim);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_any(cgs_batch *b, hipStream_t s) {
-    const int gf = b->tiles.fast_gpt, gm = b->tiles.mi_gpt;
-    if (gf == 8 && gm == 4) return launch_batch<8, 4>(b, s);
-    if (gf == 8) return launch_batch<8, 2>(b, s);
-    if (gm == 4) return launch_batch<4, 4>(b, s);
-    return launch_batch<4, 2>(b, s);
+    const bool m4 = b->tiles.mi_gpt == 4;
+    hipError_t e = hipSuccess;
+    if (b->kind_count[kBatchSAD] && e == hipSuccess) e = launch_kind<kBatchSAD, 4>(b, s);
+    if (b->kind_count[kBatchNCC] && e == hipSuccess) e = launch_kind<kBatchNCC, 4>(b, s);
+    if (b->kind_count[kBatchSQD] && e == hipSuccess) e = launch_kind<kBatchSQD, 4>(b, s);
+    if (b->kind_count[kBatchMI] && e == hipSuccess)
+        e = m4 ? launch_kind<kBatchMI, 4>(b, s) : launch_kind<kBatchMI, 2>(b, s);
+    return e;
 }
 
 // Everything one run enqueues: staging of every pair, clearing, the kernel.
@@ -218,7 +225,7 @@ cgs_status cgs_batch_plan(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, 
         if (tiles->wg_per_cu) t.wg_per_cu = tiles->wg_per_cu;
         if (tiles->items_per_claim) t.items_per_claim = tiles->items_per_claim;
     }
-    if ((t.fast_gpt != 4 && t.fast_gpt != 8) || (t.mi_gpt != 2 && t.mi_gpt != 4) || t.wg_per_cu < 1 ||
+    if (t.fast_gpt != 4 || (t.mi_gpt != 2 && t.mi_gpt != 4) || t.wg_per_cu < 1 ||
         t.wg_per_cu > 8 || t.items_per_claim < 1 || t.items_per_claim > 64)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_plan: bad tile configuration");
     int dev = 0;
@@ -243,14 +250,15 @@ cgs_status cgs_batch_plan(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, 
     b->taps.resize(n);
     b->fixf.resize(n);
     b->sincos.resize(n);
-    b->host_descs.assign(n, BatchDesc{});
     if ((st = b->keys.ensure(sizeof(unsigned long long) * n)) != CGS_OK ||
         (st = b->keys_host.ensure(sizeof(unsigned long long) * n, true)) != CGS_OK ||
-        (st = b->counter.ensure(64)) != CGS_OK || (st = b->descs.ensure(sizeof(BatchDesc) * n)) != CGS_OK)
+        (st = b->counter.ensure(64 * kBatchKinds)) != CGS_OK ||
+        (st = b->descs.ensure(sizeof(BatchDesc) * n)) != CGS_OK)
         return st;
-    uint64_t items = 0;
+    uint64_t kitems[kBatchKinds] = {};
     int64_t points = 0, pe = 0;
     std::vector<float> sc;
+    std::vector<BatchDesc> by_pair(n);
     for (int32_t i = 0; i < n; ++i) {
         const cgs_batch_pair &p = pairs[i];
         const int kind = batch_kind(p.opts.cost);
@@ -274,7 +282,7 @@ cgs_status cgs_batch_plan(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, 
             }
             HIP_TRY(hipMemcpy(b->sincos[i].ptr, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice));
         }
-        BatchDesc &D = b->host_descs[i];
+        BatchDesc &D = by_pair[i];
         std::memset(&D, 0, sizeof(D));
         fill_grid_dev(&p.grid, &D.P.grid);
         D.P.sincos = levels0 > 0 ? (const float2 *) b->sincos[i].ptr : nullptr;
@@ -290,15 +298,26 @@ cgs_status cgs_batch_plan(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, 
         D.P.cx = (float) W / 2;
         D.P.cy = (float) H / 2;
         D.P.result = p.result;
-        D.P.key = (unsigned long long *) b->keys.ptr + i;
+        D.P.key = (unsigned long long *) b->keys.ptr + i;   // keys stay in the caller's pair order
         D.P.tie_mode = p.opts.tie_mode;
         D.P.bins = p.opts.mi_bins;
         D.kind = kind;
-        D.item_begin = (uint32_t) items;
-        items += (uint64_t) ((p.grid.num_elements + gpt - 1) / gpt);
+        D.item_begin = (uint32_t) kitems[kind];
+        kitems[kind] += (uint64_t) ((p.grid.num_elements + gpt - 1) / gpt);
         points += p.grid.num_elements;
         pe += p.grid.num_elements * (int64_t) p.fixed.width * p.fixed.height;
-        if (items >= 0xFFFFFFF0ull) return set_error(CGS_ERR_TOO_LARGE, "batch has 2^32 or more work items");
+        if (kitems[kind] >= 0xFFFFFFF0ull) return set_error(CGS_ERR_TOO_LARGE, "batch has 2^32 or more work items");
+    }
+    // descriptors grouped by kind, pair order kept inside a kind
+    uint64_t items = 0;
+    b->host_descs.clear();
+    for (int k = 0; k < kBatchKinds; ++k) {
+        b->kind_begin[k] = (int) b->host_descs.size();
+        for (int32_t i = 0; i < n; ++i)
+            if (by_pair[i].kind == k) b->host_descs.push_back(by_pair[i]);
+        b->kind_count[k] = (int) b->host_descs.size() - b->kind_begin[k];
+        b->kind_items[k] = (uint32_t) kitems[k];
+        items += kitems[k];
     }
     HIP_TRY(hipMemcpy(b->descs.ptr, b->host_descs.data(), sizeof(BatchDesc) * n, hipMemcpyHostToDevice));
     b->total_items = (uint32_t) items;

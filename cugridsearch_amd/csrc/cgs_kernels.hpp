@@ -104,6 +104,20 @@ __device__ __forceinline__ float uniform(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// The sweeps read their parameters through uniform(): when the parameters
+// come from a descriptor in memory (the batch kernel) rather than kernel
+// arguments, this keeps pointers, sizes and the tap table's buffer resource
+// in SGPRs instead of VGPRs with a waterfall loop around every buffer load.
+template <typename T>
+__device__ __forceinline__ T *uniform(T *p) {
+    const uint64_t v = (uint64_t) p;
+    const uint32_t lo = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) v);
+    const uint32_t hi = (uint32_t) __builtin_amdgcn_readfirstlane((int) (uint32_t) (v >> 32));
+    return (T *) (((uint64_t) hi << 32) | lo);
+}
+
 // Thread 0..GPT-1 of the workgroup: warp coefficients of grid index base+tid
 // (index 0 for the padding slots past the end of the grid).
 template <int GPT>
@@ -383,17 +397,17 @@ template <int COSTK, typename FT, int GPT, bool POSW>
 __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*s_coef)[12], double (*acc)[3],
                                            uint32_t *cnt) {
     const int tid = threadIdx.x;
-    const int fw = P.fw, fh = P.fh;
+    const int fw = uniform(P.fw), fh = uniform(P.fh);
     const int cols = fw < kThreads ? fw : kThreads;
     const int rpi = kThreads / cols;
     const int r0 = tid / cols, xc = tid - r0 * cols;
-    const float Wm = (float) P.mw, Hm = (float) P.mh;
-    const FT *fix = (const FT *) P.fix;
-    const int mw = P.mw;
+    const int mw = uniform(P.mw), mh = uniform(P.mh);
+    const float Wm = (float) mw, Hm = (float) mh;
+    const FT *fix = uniform((const FT *) P.fix);
     const int w16 = mw * 16;
     const int base16 = (mw + 1) * 16;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void *) P.quads, (short) 0, (int) (((size_t) mw * P.mh + 2 * (size_t) mw + 2) * 16), 0x00020000);
+        (void *) uniform(P.quads), (short) 0, (int) (((size_t) mw * mh + 2 * (size_t) mw + 2) * 16), 0x00020000);
 
     float B[GPT][3];
 #pragma unroll
@@ -448,7 +462,7 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
                 if constexpr (!POSW) {
                     // W may be tiny or negative here: keep the taps inside the table
                     ix = min(max(ix, -1), mw);
-                    iy = min(max(iy, -1), P.mh);
+                    iy = min(max(iy, -1), mh);
                 }
                 ax[g] = __builtin_amdgcn_fractf(X);
                 ay[g] = __builtin_amdgcn_fractf(Y);
@@ -615,18 +629,18 @@ template <typename MT, typename FT, int GPT, bool FASTDIV, bool POW2>
 __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_coef)[12], uint32_t *hist,
                                          uint32_t *cnt) {
     const int tid = threadIdx.x, lane = tid & 63;
-    const int fw = P.fw, fh = P.fh;
+    const int fw = uniform(P.fw), fh = uniform(P.fh);
     const int cols = fw < kThreads ? fw : kThreads;
     const int rpi = kThreads / cols;
     const int r0 = tid / cols, xc = tid - r0 * cols;
-    const float Wm = (float) P.mw, Hm = (float) P.mh;
-    const FT *fix = (const FT *) P.fix;
-    const int mw = P.mw;
+    const int mw = uniform(P.mw), mh = uniform(P.mh);
+    const float Wm = (float) mw, Hm = (float) mh;
+    const FT *fix = uniform((const FT *) P.fix);
     constexpr int EB = TapTable<MT>::kBytes;
     const int wE = mw * EB, baseE = (mw + 1) * EB;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void *) P.quads, (short) 0, (int) (((size_t) mw * P.mh + 2 * (size_t) mw + 2) * EB), 0x00020000);
-    const int bins = P.bins, B2 = bins * bins;
+        (void *) uniform(P.quads), (short) 0, (int) (((size_t) mw * mh + 2 * (size_t) mw + 2) * EB), 0x00020000);
+    const int bins = uniform(P.bins), B2 = bins * bins;
     const float fbins = (float) bins, bscale = fbins / 256.0f;
     const uint32_t dummy = (uint32_t) (GPT * B2 + lane);
 
@@ -683,7 +697,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 int ix = floor_to_int(X), iy = floor_to_int(Y);
                 if constexpr (!FASTDIV) {
                     ix = clamp_int(ix, -1, mw);
-                    iy = clamp_int(iy, -1, P.mh);
+                    iy = clamp_int(iy, -1, mh);
                 }
                 ax[g] = __builtin_amdgcn_fractf(X);
                 ay[g] = __builtin_amdgcn_fractf(Y);
