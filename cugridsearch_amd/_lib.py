@@ -31,7 +31,7 @@ class CgsError(RuntimeError):
 class Grid(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("start", ctypes.c_float * 8), ("end", ctypes.c_float * 8),
                 ("count", ctypes.c_float * 8), ("resolution", ctypes.c_float * 8),
-                ("increments", ctypes.c_int32 * 8), ("num_elements", ctypes.c_int64)]
+                ("increments", ctypes.c_int32 * 8), ("num_elements", ctypes.c_int64), ("offset", ctypes.c_int32 * 8)]
 
 
 class Image(ctypes.Structure):
@@ -59,6 +59,20 @@ class TileConfig(ctypes.Structure):
                 ("items_per_claim", ctypes.c_int32)]
 
 
+class RefineOpts(ctypes.Structure):
+    _fields_ = [("radius", ctypes.c_int32), ("max_steps", ctypes.c_int32), ("threshold", ctypes.c_float),
+                ("fallback", ctypes.c_int32)]
+
+
+class RefineResult(ctypes.Structure):
+    _fields_ = [("index", ctypes.c_int32), ("value", ctypes.c_float), ("steps", ctypes.c_int32),
+                ("outcome", ctypes.c_int32), ("local_index", ctypes.c_int32), ("local_value", ctypes.c_float),
+                ("points", ctypes.c_int64)]
+
+
+REFINE_LOCAL, REFINE_FALLBACK_BOUNDARY, REFINE_FALLBACK_THRESHOLD = range(3)
+
+
 # name -> (restype, argtypes); every symbol include/cgs.h declares.
 _P = ctypes.POINTER
 SIGNATURES = {
@@ -67,6 +81,7 @@ SIGNATURES = {
     "cgs_grid_init": (ctypes.c_int, [_P(Grid), ctypes.c_int32, _P(ctypes.c_float), _P(ctypes.c_float),
                                      _P(ctypes.c_float)]),
     "cgs_grid_point": (ctypes.c_int, [_P(Grid), ctypes.c_int64, _P(ctypes.c_float)]),
+    "cgs_grid_subbox": (ctypes.c_int, [_P(Grid), _P(ctypes.c_int32), _P(ctypes.c_int32), _P(Grid)]),
     "cgs_search": (ctypes.c_int, [_P(Grid), _P(Image), _P(Image), _P(SearchOpts), ctypes.c_void_p, _P(Argmin),
                                   ctypes.c_void_p]),
     "cgs_find_extrema": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _P(Argmin),
@@ -80,6 +95,20 @@ SIGNATURES = {
     "cgs_batch_info": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int64), _P(ctypes.c_int64),
                                       _P(ctypes.c_int64)]),
     "cgs_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "cgs_batch_refine": (ctypes.c_int, [ctypes.c_void_p, _P(BatchPair), ctypes.c_int32, _P(ctypes.c_int32),
+                                        _P(RefineOpts), _P(RefineResult), ctypes.c_void_p]),
+    "cgs_cnn_param_count": (ctypes.c_int64, []),
+    "cgs_cnn_create": (ctypes.c_int, [ctypes.c_int32, _P(ctypes.c_void_p)]),
+    "cgs_cnn_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "cgs_cnn_set_params": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float)]),
+    "cgs_cnn_get_params": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float)]),
+    "cgs_cnn_prepare": (ctypes.c_int, [_P(Image), _P(Image), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "cgs_cnn_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                       ctypes.c_void_p]),
+    "cgs_cnn_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _P(ctypes.c_float), ctypes.c_void_p]),
+    "cgs_cnn_get_grads": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_float)]),
+    "cgs_cnn_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_void_p]),
     "cgs_last_kernel_ms": (ctypes.c_float, []),
     "cgs_last_error": (ctypes.c_char_p, []),
 }

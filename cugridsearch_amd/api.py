@@ -123,6 +123,21 @@ class CudaGrid:
         L.check(L.lib().cgs_grid_point(ctypes.byref(self._g), int(index), out))
         return np.array(out[:], np.float32)
 
+    def subbox(self, lo, count):
+        """The grid's samples lo[a] .. lo[a]+count[a]-1 on every axis as a grid
+        of its own (cgs_grid_subbox): same points, bit for bit."""
+        g = CudaGrid.__new__(CudaGrid)
+        g._dim = self._dim
+        g._start, g._end, g._count = self._start, self._end, self._count
+        g._g = L.Grid()
+        lo_a = (ctypes.c_int32 * self._dim)(*[int(v) for v in lo])
+        cnt_a = (ctypes.c_int32 * self._dim)(*[int(v) for v in count])
+        L.check(L.lib().cgs_grid_subbox(ctypes.byref(self._g), lo_a, cnt_a, ctypes.byref(g._g)))
+        return g
+
+    def offset(self):
+        return np.array(self._g.offset[:self._dim], np.int32)
+
     def display(self, name=""):
         """CudaMatrix::display format (include/cudaTensor.cuh:395-415): the four
         rows start / end / resolution / count."""
@@ -275,7 +290,8 @@ class BatchSearcher:
         self._n = 0
         self._keep = None
 
-    def plan(self, pairs, tiles=None, tie_mode=L.TIE_LOWEST):
+    @staticmethod
+    def _pair_array(pairs, tie_mode):
         arr = (L.BatchPair * len(pairs))()
         keep = []
         for k, (fn, grid, mov, fix, res) in enumerate(pairs):
@@ -285,6 +301,10 @@ class BatchSearcher:
             arr[k].opts = make_opts(fn, tie_mode)
             arr[k].result = res.data.data_ptr() if res is not None else None
             keep.append((mov, fix, res))
+        return arr, keep
+
+    def plan(self, pairs, tiles=None, tie_mode=L.TIE_LOWEST):
+        arr, keep = self._pair_array(pairs, tie_mode)
         t = None
         if tiles is not None:
             t = L.TileConfig(*[int(tiles.get(k, 0)) for k in ("fast_gpt", "mi_gpt", "wg_per_cu", "items_per_claim")])
@@ -299,6 +319,28 @@ class BatchSearcher:
         out = (L.Argmin * self._n)()
         L.check(L.lib().cgs_batch_results(self._h, out, _stream_ptr(stream)))
         return [(a.value, a.index) for a in out]
+
+    def refine(self, pairs, centres, radius=1, max_steps=8, threshold=float("inf"), fallback=True,
+               tie_mode=L.TIE_LOWEST, stream=None):
+        """Local refinement around per-pair centres (sample indices, n x dim)
+        with full-grid fallback (cgs_batch_refine).  pairs: (functor, grid,
+        moved, fixed, ignored).  Returns a numpy record array with fields
+        index, value, steps, outcome, local_index, local_value, points.
+        Drops any plan the batch held."""
+        arr, keep = self._pair_array(pairs, tie_mode)
+        c = np.ascontiguousarray(centres, np.int32)
+        if c.shape != (len(pairs), pairs[0][1].getDimension()):
+            raise ValueError("centres must be n x grid dimension")
+        o = L.RefineOpts(int(radius), int(max_steps), float(threshold), 1 if fallback else 0)
+        out = (L.RefineResult * len(pairs))()
+        L.check(L.lib().cgs_batch_refine(self._h, arr, len(pairs), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                         ctypes.byref(o), out, _stream_ptr(stream)))
+        self._n = 0
+        self._keep = None
+        del keep
+        return np.array([(r.index, r.value, r.steps, r.outcome, r.local_index, r.local_value, r.points) for r in out],
+                        dtype=[("index", np.int32), ("value", np.float32), ("steps", np.int32), ("outcome", np.int32),
+                               ("local_index", np.int32), ("local_value", np.float32), ("points", np.int64)])
 
     def info(self):
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

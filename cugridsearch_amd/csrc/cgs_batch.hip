@@ -109,24 +109,31 @@ struct cgs_batch_s {
     uint32_t total_items = 0;
     int kind_begin[kBatchKinds] = {}, kind_count[kBatchKinds] = {};   // descriptor ranges per kind
     uint32_t kind_items[kBatchKinds] = {};
+    int nslots = 0;          // descriptors (= keys) of the current launch
+    std::vector<int> slot_pair;                                       // pair of each key slot
     int64_t total_points = 0, total_pe = 0;
     // device staging
     DevBuf descs, keys, keys_host, counter;
     std::vector<DevBuf> taps, fixf, sincos;
     std::vector<BatchDesc> host_descs;
-    DevBuf sincos_host;
     // captured graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
-    ~cgs_batch_s() {
+    void drop_graph() {
         if (exec) (void) hipGraphExecDestroy(exec);
         if (graph) (void) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_stream = nullptr;
+    }
+    ~cgs_batch_s() {
+        drop_graph();
         auto rel = [](DevBuf &b, bool host) {
             if (b.ptr) (void) (host ? hipHostFree(b.ptr) : hipFree(b.ptr));
             b.ptr = nullptr;
         };
-        rel(descs, false); rel(keys, false); rel(counter, false); rel(keys_host, true); rel(sincos_host, true);
+        rel(descs, false); rel(keys, false); rel(counter, false); rel(keys_host, true);
         for (auto &b : taps) rel(b, false);
         for (auto &b : fixf) rel(b, false);
         for (auto &b : sincos) rel(b, false);
@@ -165,8 +172,78 @@ hipError_t launch_batch_any(cgs_batch *b, hipStream_t s) {
     return e;
 }
 
-// Everything one run enqueues: staging of every pair, clearing, the kernel.
-cgs_status enqueue_run(cgs_batch *b, hipStream_t s) {
+cgs_status parse_tiles(const cgs_tile_config *tiles, cgs_tile_config *out) {
+    cgs_tile_config t{4, 4, 4, 8};
+    if (tiles) {
+        if (tiles->fast_gpt) t.fast_gpt = tiles->fast_gpt;
+        if (tiles->mi_gpt) t.mi_gpt = tiles->mi_gpt;
+        if (tiles->wg_per_cu) t.wg_per_cu = tiles->wg_per_cu;
+        if (tiles->items_per_claim) t.items_per_claim = tiles->items_per_claim;
+    }
+    if (t.fast_gpt != 4 || (t.mi_gpt != 2 && t.mi_gpt != 4) || t.wg_per_cu < 1 ||
+        t.wg_per_cu > 8 || t.items_per_claim < 1 || t.items_per_claim > 64)
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "batch: bad tile configuration");
+    *out = t;
+    return CGS_OK;
+}
+
+// Validates the pairs, records them and allocates their staging: float tap
+// table and float fixed image per pair, and the (cos, sin) table of each
+// pair grid's axis-0 levels.
+cgs_status stage_alloc(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, const cgs_tile_config &t) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return set_error(CGS_ERR_NO_DEVICE, "no HIP device");
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cgs_status st;
+    for (int32_t i = 0; i < n; ++i) {
+        const cgs_batch_pair &p = pairs[i];
+        if ((st = validate_search(&p.grid, &p.moved, &p.fixed, &p.opts)) != CGS_OK) return st;
+        if (p.opts.sampler != CGS_SAMPLER_BILINEAR || p.moved.channels != 1 || p.moved.width > 65536 ||
+            p.moved.height > 65536)
+            return set_error(CGS_ERR_UNSUPPORTED, "batch pairs must be one-channel bilinear searches");
+        if (batch_kind(p.opts.cost) == kBatchMI && p.opts.mi_bins > 32 && t.mi_gpt > 2)
+            return set_error(CGS_ERR_UNSUPPORTED, "MI with more than 32 bins needs mi_gpt 2");
+    }
+    b->drop_graph();
+    b->pairs.assign(pairs, pairs + n);
+    b->tiles = t;
+    b->max_bins = 8;
+    b->cus = cus;
+    b->taps.resize(n);
+    b->fixf.resize(n);
+    b->sincos.resize(n);
+    if ((st = b->keys.ensure(sizeof(unsigned long long) * n)) != CGS_OK ||
+        (st = b->keys_host.ensure(sizeof(unsigned long long) * n, true)) != CGS_OK ||
+        (st = b->counter.ensure(64 * kBatchKinds)) != CGS_OK ||
+        (st = b->descs.ensure(sizeof(BatchDesc) * n)) != CGS_OK)
+        return st;
+    std::vector<float> sc;
+    for (int32_t i = 0; i < n; ++i) {
+        const cgs_batch_pair &p = pairs[i];
+        if (batch_kind(p.opts.cost) == kBatchMI) b->max_bins = std::max(b->max_bins, (int) p.opts.mi_bins);
+        const int W = p.moved.width, H = p.moved.height;
+        const size_t ntaps = (size_t) W * H + 2 * (size_t) W + 2;
+        if ((st = b->taps[i].ensure(ntaps * sizeof(float4))) != CGS_OK) return st;
+        if ((st = b->fixf[i].ensure((size_t) p.fixed.width * p.fixed.height * sizeof(float))) != CGS_OK) return st;
+        const bool has_theta = p.opts.param_mode == CGS_PARAM_CENTRED || p.opts.param_mode == CGS_PARAM_NONCENTRED;
+        const int levels0 = has_theta ? (int) p.grid.count[0] : 0;
+        if (levels0 > 0) {
+            if ((st = b->sincos[i].ensure(sizeof(float) * 2 * levels0)) != CGS_OK) return st;
+            sc.resize(2 * levels0);
+            for (int l = 0; l < levels0; ++l) {
+                const float theta = fmaf((float) (p.grid.offset[0] + l), p.grid.resolution[0], p.grid.start[0]);
+                sc[2 * l] = (float) std::cos((double) theta);
+                sc[2 * l + 1] = (float) std::sin((double) theta);
+            }
+            HIP_TRY(hipMemcpy(b->sincos[i].ptr, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice));
+        }
+    }
+    return CGS_OK;
+}
+
+// Per-pair staging kernels: tap table and float fixed image.
+cgs_status enqueue_staging(cgs_batch *b, hipStream_t s) {
     const unsigned pb = 256;
     for (size_t i = 0; i < b->pairs.size(); ++i) {
         const cgs_batch_pair &p = b->pairs[i];
@@ -192,11 +269,94 @@ cgs_status enqueue_run(cgs_batch *b, hipStream_t s) {
             hipLaunchKernelGGL(to_float_kernel<float>, dim3(pb), dim3(256), 0, s, (const float *) p.fixed.data,
                                (float *) b->fixf[i].ptr, nf);
     }
-    hipLaunchKernelGGL(batch_clear_kernel, dim3(16), dim3(256), 0, s, (unsigned long long *) b->keys.ptr,
-                       (int) b->pairs.size(), (uint32_t *) b->counter.ptr);
+    HIP_TRY(hipGetLastError());
+    return CGS_OK;
+}
+
+// Descriptor of pair i searching g (the pair's grid or a box of it) into key slot `slot`.
+BatchDesc describe(const cgs_batch *b, int i, const cgs_grid &g, int slot, float *result) {
+    const cgs_batch_pair &p = b->pairs[i];
+    BatchDesc D;
+    std::memset(&D, 0, sizeof(D));
+    fill_grid_dev(&g, &D.P.grid);
+    const bool has_theta = p.opts.param_mode == CGS_PARAM_CENTRED || p.opts.param_mode == CGS_PARAM_NONCENTRED;
+    D.P.sincos = has_theta ? (const float2 *) b->sincos[i].ptr + (g.offset[0] - p.grid.offset[0]) : nullptr;
+    D.P.param_mode = p.opts.param_mode;
+    D.P.cost = p.opts.cost;
+    D.P.quads = (const float4 *) b->taps[i].ptr;
+    D.P.mw = p.moved.width;
+    D.P.mh = p.moved.height;
+    D.P.mc = 1;
+    D.P.fix = b->fixf[i].ptr;
+    D.P.fw = p.fixed.width;
+    D.P.fh = p.fixed.height;
+    D.P.cx = (float) p.moved.width / 2;
+    D.P.cy = (float) p.moved.height / 2;
+    D.P.result = result;
+    D.P.key = (unsigned long long *) b->keys.ptr + slot;
+    D.P.tie_mode = p.opts.tie_mode;
+    D.P.bins = p.opts.mi_bins;
+    D.kind = batch_kind(p.opts.cost);
+    return D;
+}
+
+// Groups the slot-ordered descriptors by kind (slot order kept inside a
+// kind), numbers their work items and uploads them.
+cgs_status set_descs(cgs_batch *b, std::vector<BatchDesc> &by_slot, const std::vector<int> &slot_pair) {
+    uint64_t kitems[kBatchKinds] = {};
+    int64_t points = 0, pe = 0;
+    for (size_t k = 0; k < by_slot.size(); ++k) {
+        BatchDesc &D = by_slot[k];
+        const int gpt = D.kind == kBatchMI ? b->tiles.mi_gpt : b->tiles.fast_gpt;
+        D.item_begin = (uint32_t) kitems[D.kind];
+        kitems[D.kind] += (uint64_t) ((D.P.grid.n + gpt - 1) / gpt);
+        points += D.P.grid.n;
+        pe += (int64_t) D.P.grid.n * D.P.fw * D.P.fh;
+        if (kitems[D.kind] >= 0xFFFFFFF0ull) return set_error(CGS_ERR_TOO_LARGE, "batch has 2^32 or more work items");
+    }
+    uint64_t items = 0;
+    b->host_descs.clear();
+    for (int k = 0; k < kBatchKinds; ++k) {
+        b->kind_begin[k] = (int) b->host_descs.size();
+        for (const BatchDesc &D : by_slot)
+            if (D.kind == k) b->host_descs.push_back(D);
+        b->kind_count[k] = (int) b->host_descs.size() - b->kind_begin[k];
+        b->kind_items[k] = (uint32_t) kitems[k];
+        items += kitems[k];
+    }
+    if (!b->host_descs.empty())
+        HIP_TRY(hipMemcpy(b->descs.ptr, b->host_descs.data(), sizeof(BatchDesc) * b->host_descs.size(),
+                          hipMemcpyHostToDevice));
+    b->nslots = (int) by_slot.size();
+    b->slot_pair = slot_pair;
+    b->total_items = (uint32_t) items;
+    b->total_points = points;
+    b->total_pe = pe;
+    return CGS_OK;
+}
+
+// Clears the keys of the current slots and the work counters, launches the kernels.
+cgs_status enqueue_search(cgs_batch *b, hipStream_t s) {
+    hipLaunchKernelGGL(batch_clear_kernel, dim3(16), dim3(256), 0, s, (unsigned long long *) b->keys.ptr, b->nslots,
+                       (uint32_t *) b->counter.ptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(launch_batch_any(b, s));
     return CGS_OK;
+}
+
+cgs_status read_keys(cgs_batch *b, hipStream_t s, std::vector<cgs_argmin> &out) {
+    const size_t n = (size_t) b->nslots;
+    HIP_TRY(hipMemcpyAsync(b->keys_host.ptr, b->keys.ptr, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const unsigned long long *k = (const unsigned long long *) b->keys_host.ptr;
+    out.resize(n);
+    for (size_t i = 0; i < n; ++i) out[i] = decode_key(k[i], b->pairs[b->slot_pair[i]].opts.tie_mode);
+    return CGS_OK;
+}
+
+cgs_status enqueue_run(cgs_batch *b, hipStream_t s) {
+    cgs_status st = enqueue_staging(b, s);
+    return st != CGS_OK ? st : enqueue_search(b, s);
 }
 
 }  // namespace
@@ -218,125 +378,26 @@ cgs_status cgs_batch_destroy(cgs_batch *b) {
 cgs_status cgs_batch_plan(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, const cgs_tile_config *tiles) {
     clear_error();
     if (!b || !pairs || n <= 0) return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_plan: bad arguments");
-    cgs_tile_config t{4, 4, 4, 8};
-    if (tiles) {
-        if (tiles->fast_gpt) t.fast_gpt = tiles->fast_gpt;
-        if (tiles->mi_gpt) t.mi_gpt = tiles->mi_gpt;
-        if (tiles->wg_per_cu) t.wg_per_cu = tiles->wg_per_cu;
-        if (tiles->items_per_claim) t.items_per_claim = tiles->items_per_claim;
-    }
-    if (t.fast_gpt != 4 || (t.mi_gpt != 2 && t.mi_gpt != 4) || t.wg_per_cu < 1 ||
-        t.wg_per_cu > 8 || t.items_per_claim < 1 || t.items_per_claim > 64)
-        return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_plan: bad tile configuration");
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return set_error(CGS_ERR_NO_DEVICE, "no HIP device");
-    int cus = 0;
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    cgs_status st;
+    cgs_tile_config t;
+    cgs_status st = parse_tiles(tiles, &t);
+    if (st != CGS_OK) return st;
+    if ((st = stage_alloc(b, pairs, n, t)) != CGS_OK) return st;
+    std::vector<BatchDesc> by_slot(n);
+    std::vector<int> slot_pair(n);
     for (int32_t i = 0; i < n; ++i) {
-        const cgs_batch_pair &p = pairs[i];
-        if ((st = validate_search(&p.grid, &p.moved, &p.fixed, &p.opts)) != CGS_OK) return st;
-        if (p.opts.sampler != CGS_SAMPLER_BILINEAR || p.moved.channels != 1 || p.moved.width > 65536 ||
-            p.moved.height > 65536)
-            return set_error(CGS_ERR_UNSUPPORTED, "batch pairs must be one-channel bilinear searches");
+        by_slot[i] = describe(b, i, pairs[i].grid, i, pairs[i].result);   // keys in the caller's pair order
+        slot_pair[i] = i;
     }
-    if (b->exec) { (void) hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-    if (b->graph) { (void) hipGraphDestroy(b->graph); b->graph = nullptr; }
-    b->graph_stream = nullptr;
-    b->pairs.assign(pairs, pairs + n);
-    b->tiles = t;
-    b->max_bins = 8;
-    b->cus = cus;
-    b->taps.resize(n);
-    b->fixf.resize(n);
-    b->sincos.resize(n);
-    if ((st = b->keys.ensure(sizeof(unsigned long long) * n)) != CGS_OK ||
-        (st = b->keys_host.ensure(sizeof(unsigned long long) * n, true)) != CGS_OK ||
-        (st = b->counter.ensure(64 * kBatchKinds)) != CGS_OK ||
-        (st = b->descs.ensure(sizeof(BatchDesc) * n)) != CGS_OK)
-        return st;
-    uint64_t kitems[kBatchKinds] = {};
-    int64_t points = 0, pe = 0;
-    std::vector<float> sc;
-    std::vector<BatchDesc> by_pair(n);
-    for (int32_t i = 0; i < n; ++i) {
-        const cgs_batch_pair &p = pairs[i];
-        const int kind = batch_kind(p.opts.cost);
-        const int gpt = kind == kBatchMI ? t.mi_gpt : t.fast_gpt;
-        if (kind == kBatchMI && p.opts.mi_bins > 32 && gpt > 2)
-            return set_error(CGS_ERR_UNSUPPORTED, "MI with more than 32 bins needs mi_gpt 2");
-        if (kind == kBatchMI) b->max_bins = std::max(b->max_bins, (int) p.opts.mi_bins);
-        const int W = p.moved.width, H = p.moved.height;
-        const size_t ntaps = (size_t) W * H + 2 * (size_t) W + 2;
-        if ((st = b->taps[i].ensure(ntaps * sizeof(float4))) != CGS_OK) return st;
-        if ((st = b->fixf[i].ensure((size_t) p.fixed.width * p.fixed.height * sizeof(float))) != CGS_OK) return st;
-        const bool has_theta = p.opts.param_mode == CGS_PARAM_CENTRED || p.opts.param_mode == CGS_PARAM_NONCENTRED;
-        const int levels0 = has_theta ? (int) p.grid.count[0] : 0;
-        if (levels0 > 0) {
-            if ((st = b->sincos[i].ensure(sizeof(float) * 2 * levels0)) != CGS_OK) return st;
-            sc.resize(2 * levels0);
-            for (int l = 0; l < levels0; ++l) {
-                const float theta = fmaf((float) l, p.grid.resolution[0], p.grid.start[0]);
-                sc[2 * l] = (float) std::cos((double) theta);
-                sc[2 * l + 1] = (float) std::sin((double) theta);
-            }
-            HIP_TRY(hipMemcpy(b->sincos[i].ptr, sc.data(), sizeof(float) * sc.size(), hipMemcpyHostToDevice));
-        }
-        BatchDesc &D = by_pair[i];
-        std::memset(&D, 0, sizeof(D));
-        fill_grid_dev(&p.grid, &D.P.grid);
-        D.P.sincos = levels0 > 0 ? (const float2 *) b->sincos[i].ptr : nullptr;
-        D.P.param_mode = p.opts.param_mode;
-        D.P.cost = p.opts.cost;
-        D.P.quads = (const float4 *) b->taps[i].ptr;
-        D.P.mw = W;
-        D.P.mh = H;
-        D.P.mc = 1;
-        D.P.fix = b->fixf[i].ptr;
-        D.P.fw = p.fixed.width;
-        D.P.fh = p.fixed.height;
-        D.P.cx = (float) W / 2;
-        D.P.cy = (float) H / 2;
-        D.P.result = p.result;
-        D.P.key = (unsigned long long *) b->keys.ptr + i;   // keys stay in the caller's pair order
-        D.P.tie_mode = p.opts.tie_mode;
-        D.P.bins = p.opts.mi_bins;
-        D.kind = kind;
-        D.item_begin = (uint32_t) kitems[kind];
-        kitems[kind] += (uint64_t) ((p.grid.num_elements + gpt - 1) / gpt);
-        points += p.grid.num_elements;
-        pe += p.grid.num_elements * (int64_t) p.fixed.width * p.fixed.height;
-        if (kitems[kind] >= 0xFFFFFFF0ull) return set_error(CGS_ERR_TOO_LARGE, "batch has 2^32 or more work items");
-    }
-    // descriptors grouped by kind, pair order kept inside a kind
-    uint64_t items = 0;
-    b->host_descs.clear();
-    for (int k = 0; k < kBatchKinds; ++k) {
-        b->kind_begin[k] = (int) b->host_descs.size();
-        for (int32_t i = 0; i < n; ++i)
-            if (by_pair[i].kind == k) b->host_descs.push_back(by_pair[i]);
-        b->kind_count[k] = (int) b->host_descs.size() - b->kind_begin[k];
-        b->kind_items[k] = (uint32_t) kitems[k];
-        items += kitems[k];
-    }
-    HIP_TRY(hipMemcpy(b->descs.ptr, b->host_descs.data(), sizeof(BatchDesc) * n, hipMemcpyHostToDevice));
-    b->total_items = (uint32_t) items;
-    b->total_points = points;
-    b->total_pe = pe;
-    return CGS_OK;
+    return set_descs(b, by_slot, slot_pair);
 }
 
 cgs_status cgs_batch_run(cgs_batch *b, int32_t use_graph, void *stream) {
     clear_error();
-    if (!b || b->pairs.empty()) return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_run: not planned");
+    if (!b || b->pairs.empty() || b->nslots != (int) b->pairs.size())
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_run: not planned");
     hipStream_t s = (hipStream_t) stream;
     if (!use_graph) return enqueue_run(b, s);
-    if (b->exec && b->graph_stream != s) {
-        (void) hipGraphExecDestroy(b->exec);
-        (void) hipGraphDestroy(b->graph);
-        b->exec = nullptr;
-        b->graph = nullptr;
-    }
+    if (b->exec && b->graph_stream != s) b->drop_graph();
     if (!b->exec) {
         if (!s) return set_error(CGS_ERR_INVALID_ARGUMENT, "graph capture needs a non-null stream");
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -355,13 +416,12 @@ cgs_status cgs_batch_run(cgs_batch *b, int32_t use_graph, void *stream) {
 
 cgs_status cgs_batch_results(cgs_batch *b, cgs_argmin *out, void *stream) {
     clear_error();
-    if (!b || !out || b->pairs.empty()) return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_results: bad arguments");
-    hipStream_t s = (hipStream_t) stream;
-    const size_t n = b->pairs.size();
-    HIP_TRY(hipMemcpyAsync(b->keys_host.ptr, b->keys.ptr, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const unsigned long long *k = (const unsigned long long *) b->keys_host.ptr;
-    for (size_t i = 0; i < n; ++i) out[i] = decode_key(k[i], b->pairs[i].opts.tie_mode);
+    if (!b || !out || b->pairs.empty() || b->nslots != (int) b->pairs.size())
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_results: bad arguments");
+    std::vector<cgs_argmin> r;
+    cgs_status st = read_keys(b, (hipStream_t) stream, r);
+    if (st != CGS_OK) return st;
+    for (size_t i = 0; i < r.size(); ++i) out[i] = r[i];
     return CGS_OK;
 }
 
@@ -371,6 +431,119 @@ cgs_status cgs_batch_info(const cgs_batch *b, int64_t *items, int64_t *points, i
     if (items) *items = b->total_items;
     if (points) *points = b->total_points;
     if (pixel_evals) *pixel_evals = b->total_pe;
+    return CGS_OK;
+}
+
+// Local refinement: every pair searches a (2r+1)^D box of its grid around
+// its centre; a pair whose box minimum lies on a box face that is not a
+// grid face re-centres there and searches again, up to max_steps boxes.
+// All pairs still moving share one batched launch per step; the staging
+// (tap tables, float images) is done once.
+cgs_status cgs_batch_refine(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, const int32_t *centres,
+                            const cgs_refine_opts *ro, cgs_refine_result *out, void *stream) {
+    clear_error();
+    if (!b || !pairs || n <= 0 || !centres || !ro || !out)
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_refine: bad arguments");
+    if (ro->radius < 1 || ro->radius > 16 || ro->max_steps < 1 || ro->max_steps > 1024 || std::isnan(ro->threshold))
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_refine: radius 1..16, max_steps 1..1024, threshold not NaN");
+    hipStream_t s = (hipStream_t) stream;
+    cgs_tile_config t;
+    cgs_status st = parse_tiles(nullptr, &t);
+    if (st != CGS_OK) return st;
+    if ((st = stage_alloc(b, pairs, n, t)) != CGS_OK) return st;
+    if ((st = enqueue_staging(b, s)) != CGS_OK) return st;
+    struct State {
+        int32_t c[CGS_MAX_DIM];
+        bool moving;
+    };
+    std::vector<State> S(n);
+    for (int32_t i = 0; i < n; ++i) {
+        const cgs_grid &g = pairs[i].grid;
+        for (int a = 0; a < g.dim; ++a)
+            S[i].c[a] = std::min(std::max(centres[(size_t) i * g.dim + a], 0), (int32_t) g.count[a] - 1);
+        S[i].moving = true;
+        out[i] = cgs_refine_result{-1, INFINITY, 0, CGS_REFINE_LOCAL, -1, INFINITY, 0};
+    }
+    std::vector<BatchDesc> descs;
+    std::vector<int> slot_pair;
+    std::vector<cgs_grid> boxes(n);
+    std::vector<cgs_argmin> r;
+    for (int step = 1; step <= ro->max_steps; ++step) {
+        descs.clear();
+        slot_pair.clear();
+        for (int32_t i = 0; i < n; ++i) {
+            if (!S[i].moving) continue;
+            const cgs_grid &g = pairs[i].grid;
+            int32_t lo[CGS_MAX_DIM], cnt[CGS_MAX_DIM];
+            for (int a = 0; a < g.dim; ++a) {
+                lo[a] = std::max(0, S[i].c[a] - ro->radius);
+                const int32_t hi = std::min((int32_t) g.count[a] - 1, S[i].c[a] + ro->radius);
+                cnt[a] = hi - lo[a] + 1;
+            }
+            if ((st = cgs_grid_subbox(&g, lo, cnt, &boxes[i])) != CGS_OK) return st;
+            descs.push_back(describe(b, i, boxes[i], (int) descs.size(), nullptr));
+            slot_pair.push_back(i);
+        }
+        if (descs.empty()) break;
+        if ((st = set_descs(b, descs, slot_pair)) != CGS_OK) return st;
+        if ((st = enqueue_search(b, s)) != CGS_OK) return st;
+        if ((st = read_keys(b, s, r)) != CGS_OK) return st;
+        for (size_t k = 0; k < r.size(); ++k) {
+            const int32_t i = slot_pair[k];
+            const cgs_grid &g = pairs[i].grid, &box = boxes[i];
+            cgs_refine_result &o = out[i];
+            o.steps = step;
+            o.points += box.num_elements;
+            if (r[k].index < 0) {   // no finite cost in the box
+                S[i].moving = false;
+                o.outcome = CGS_REFINE_FALLBACK_BOUNDARY;
+                continue;
+            }
+            int32_t rem = r[k].index, full = 0;
+            bool face = false;
+            for (int a = g.dim - 1; a >= 0; --a) {
+                const int32_t j = rem / box.increments[a];
+                rem -= j * box.increments[a];
+                const int32_t lo = box.offset[a] - g.offset[a], c = lo + j;
+                const int32_t hi = lo + (int32_t) box.count[a] - 1;
+                if ((c == lo && lo > 0) || (c == hi && hi < (int32_t) g.count[a] - 1)) face = true;
+                S[i].c[a] = c;
+                full += c * g.increments[a];
+            }
+            o.local_index = full;
+            o.local_value = r[k].value;
+            if (!face) {
+                S[i].moving = false;
+                o.outcome = r[k].value <= ro->threshold ? CGS_REFINE_LOCAL : CGS_REFINE_FALLBACK_THRESHOLD;
+            } else if (step == ro->max_steps) {
+                S[i].moving = false;
+                o.outcome = CGS_REFINE_FALLBACK_BOUNDARY;
+            }
+        }
+    }
+    descs.clear();
+    slot_pair.clear();
+    for (int32_t i = 0; i < n; ++i) {
+        if (out[i].outcome == CGS_REFINE_LOCAL) {
+            out[i].index = out[i].local_index;
+            out[i].value = out[i].local_value;
+        } else if (ro->fallback) {
+            descs.push_back(describe(b, i, pairs[i].grid, (int) descs.size(), nullptr));
+            slot_pair.push_back(i);
+        }
+    }
+    if (!descs.empty()) {
+        if ((st = set_descs(b, descs, slot_pair)) != CGS_OK) return st;
+        if ((st = enqueue_search(b, s)) != CGS_OK) return st;
+        if ((st = read_keys(b, s, r)) != CGS_OK) return st;
+        for (size_t k = 0; k < r.size(); ++k) {
+            cgs_refine_result &o = out[slot_pair[k]];
+            o.index = r[k].index;
+            o.value = r[k].value;
+            o.points += pairs[slot_pair[k]].grid.num_elements;
+        }
+    }
+    b->nslots = 0;   // the batch holds no plan for cgs_batch_run after a refinement
     return CGS_OK;
 }
 

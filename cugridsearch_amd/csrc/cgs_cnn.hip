@@ -1,0 +1,820 @@
+// cgs_cnn.hip -- the proposal network: forward, backward and Adam on gfx950.
+//
+// Network (fixed): input 2 x 128 x 128 (moving, fixed; standardised), six
+// 3x3 convolutions with zero padding 1 -- channels 32-32-64-64-128-128,
+// stride 2 on layers 2, 4, 6, ReLU after each -- global average pooling
+// and a fully connected layer to 8 outputs (corner offsets).
+//
+// Arithmetic: weights are BF16 (rounded from FP32 master weights after
+// every optimiser step), activations and the gradients between layers are
+// stored as BF16, every convolution accumulates in FP32 on the matrix cores
+// (v_mfma_f32_16x16x32_bf16); biases, pooling, the FC layer, the loss and
+// the optimiser are FP32.
+//
+// Layout: activations NHWC (pixel-major, channels contiguous), conv weights
+// [Cout][ky][kx][Cin] padded with zeros to a multiple of 32 along the
+// 9*Cin reduction, plus a [ky][kx][Cin][Cout] copy for the data gradient.
+//
+// All three convolution passes are implicit GEMMs run by one skeleton
+// (igemm_kernel): a workgroup of 4 waves owns an MT x NT output tile and
+// walks the reduction in 32-wide chunks; each chunk's A (MT x 32) and B
+// (NT x 32) tiles are gathered into LDS with the reduction index
+// contiguous, and each wave multiplies its (MT/2) x (NT/2) quadrant with
+// 16x16x32 MFMAs.  The pass decides what the tiles hold:
+//   forward      M = output pixels, N = Cout, K = 9*Cin (taps x channels);
+//   data grad    M = input pixels,  N = Cin,  K = 9*Cout;
+//   weight grad  M = Cout, N = 9*Cin, K = output pixels, split over
+//                blockIdx.z into slices whose partial sums a second kernel
+//                adds in slice order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cgs.h"
+#include "cgs_host.hpp"
+
+namespace cgs {
+namespace cnn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLayers = 6;
+constexpr int kIn = 128;            // input side
+constexpr int kFeat = 128;          // channels of the last conv = FC inputs
+constexpr int kOut = 8;             // FC outputs
+constexpr int kCin[kLayers] = {2, 32, 32, 64, 64, 128};
+constexpr int kCout[kLayers] = {32, 32, 64, 64, 128, 128};
+constexpr int kStride[kLayers] = {1, 2, 1, 2, 1, 2};
+constexpr int kHin[kLayers] = {128, 128, 64, 64, 32, 32};
+constexpr int hout(int l) { return kHin[l] / kStride[l]; }
+constexpr int ktrue(int l) { return 9 * kCin[l]; }
+constexpr int kpad(int l) { return (ktrue(l) + 31) / 32 * 32; }
+constexpr int kLastSide = hout(kLayers - 1);          // 16
+constexpr int kLastPix = kLastSide * kLastSide;       // 256
+constexpr int kLDK = 40;            // LDS row pitch in bf16 (32 + 8 pad)
+
+// Offsets of each tensor in the flat FP32 parameter vector:
+// per layer weight [Cout][9*Cin] then bias [Cout]; FC weight [8][128], bias [8].
+struct Offsets {
+    int64_t w[kLayers], b[kLayers], fcw, fcb, total;
+};
+constexpr Offsets offsets() {
+    Offsets o{};
+    int64_t p = 0;
+    for (int l = 0; l < kLayers; ++l) {
+        o.w[l] = p;
+        p += (int64_t) kCout[l] * ktrue(l);
+        o.b[l] = p;
+        p += kCout[l];
+    }
+    o.fcw = p;
+    p += kOut * kFeat;
+    o.fcb = p;
+    p += kOut;
+    o.total = p;
+    return o;
+}
+constexpr Offsets kOff = offsets();
+
+// FP32 -> BF16, round to nearest even (NaN stays a quiet NaN).
+__device__ __host__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t) ((u >> 16) | 0x40);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t) (u >> 16);
+}
+__device__ __host__ inline float bf2f(uint16_t h) {
+    const uint32_t u = (uint32_t) h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+// ReLU outputs are >= 0; "active" means strictly positive (+0 and -0 are not).
+__device__ inline bool positive(uint16_t h) { return h != 0 && (h & 0x8000u) == 0; }
+
+// ------------------------------------------------------------ GEMM skeleton
+
+template <class Prob, int MT, int NT>
+__global__ __launch_bounds__(256) void igemm_kernel(Prob pb) {
+    static_assert(MT % 32 == 0 && NT % 32 == 0, "tiles are multiples of 32");
+    constexpr int WM = MT / 2, WN = NT / 2, FM = WM / 16, FN = WN / 16;
+    __shared__ __attribute__((aligned(16))) uint16_t As[MT * kLDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[NT * kLDK];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.x * MT, n0 = blockIdx.y * NT;
+    int kbeg, kend;
+    pb.k_range(blockIdx.z, kbeg, kend);
+    f32x4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int lrow = lane & 15, lk = (lane >> 4) * 8;
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
+        pb.template fillA<MT>(As, m0, k0, tid);
+        pb.template fillB<NT>(Bs, n0, k0, tid);
+        __syncthreads();
+        bf16x8 a[FM], b[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+            a[i] = __builtin_bit_cast(bf16x8, *(const uint4 *) &As[(wm * WM + i * 16 + lrow) * kLDK + lk]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+            b[j] = __builtin_bit_cast(bf16x8, *(const uint4 *) &Bs[(wn * WN + j * 16 + lrow) * kLDK + lk]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    // D[i][j] of a 16x16 block sits in lane j + 16*(i/4), element i%4
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                pb.store(m0 + wm * WM + i * 16 + (lane >> 4) * 4 + r, n0 + wn * WN + j * 16 + lrow, acc[i][j][r]);
+}
+
+__device__ inline void put8(uint16_t *dst, uint4 v) { *(uint4 *) dst = v; }
+
+// Forward: out[m = (b, oy, ox)][co] = relu(bias + sum_{tap, ci} x[b][oy*S+ky-1][ox*S+kx-1][ci] w[co][tap][ci]).
+template <int L>
+struct FwdProb {
+    static constexpr int Cin = kCin[L], Cout = kCout[L], S = kStride[L], Hi = kHin[L], Ho = hout(L);
+    static constexpr int Kt = ktrue(L), Kp = kpad(L);
+    const uint16_t *x;      // [n][Hi][Hi][Cin]
+    const uint16_t *w;      // [Cout][Kp]
+    const float *bias;      // [Cout]
+    uint16_t *y;            // [n][Ho][Ho][Cout]
+    int M;                  // n * Ho * Ho
+    __device__ void k_range(int, int &b, int &e) const { b = 0; e = Kp; }
+    __device__ uint16_t tap_value(int b, int oy, int ox, int k) const {
+        if (k >= Kt) return 0;
+        const int tap = k / Cin, ci = k - tap * Cin;
+        const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
+        if (iy < 0 || iy >= Hi || ix < 0 || ix >= Hi) return 0;
+        return x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
+    }
+    template <int MT>
+    __device__ void fillA(uint16_t *As, int m0, int k0, int tid) const {
+        for (int t = tid; t < MT * 4; t += 256) {
+            const int r = t >> 2, kb = (t & 3) * 8, m = m0 + r;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (m < M) {
+                const int b = m / (Ho * Ho), rem = m - b * Ho * Ho, oy = rem / Ho, ox = rem - oy * Ho;
+                const int k = k0 + kb;
+                if constexpr (Cin % 8 == 0) {
+                    const int tap = k / Cin, ci = k - tap * Cin;
+                    const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
+                    if (iy >= 0 && iy < Hi && ix >= 0 && ix < Hi)
+                        v = *(const uint4 *) &x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
+                } else {
+                    uint16_t e[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) e[q] = tap_value(b, oy, ox, k + q);
+                    v = make_uint4(e[0] | (uint32_t) e[1] << 16, e[2] | (uint32_t) e[3] << 16,
+                                   e[4] | (uint32_t) e[5] << 16, e[6] | (uint32_t) e[7] << 16);
+                }
+            }
+            put8(&As[r * kLDK + kb], v);
+        }
+    }
+    template <int NT>
+    __device__ void fillB(uint16_t *Bs, int n0, int k0, int tid) const {
+        for (int t = tid; t < NT * 4; t += 256) {
+            const int r = t >> 2, kb = (t & 3) * 8;
+            put8(&Bs[r * kLDK + kb], *(const uint4 *) &w[(size_t) (n0 + r) * Kp + k0 + kb]);
+        }
+    }
+    __device__ void store(int m, int n, float v) const {
+        if (m < M) y[(size_t) m * Cout + n] = f2bf(fmaxf(v + bias[n], 0.0f));
+    }
+};
+
+// Data gradient: dx[m = (b, iy, ix)][ci] = sum_{tap, co} dz[b][oy][ox][co] w[co][tap][ci]
+// over the output pixels with oy*S + ky - 1 = iy; stored as the previous
+// layer's pre-activation gradient dz_prev = dx * (x > 0).
+template <int L>
+struct DataProb {
+    static constexpr int Cin = kCin[L], Cout = kCout[L], S = kStride[L], Hi = kHin[L], Ho = hout(L);
+    const uint16_t *dz;     // [n][Ho][Ho][Cout]
+    const uint16_t *wt;     // [9][Cin][Cout]
+    const uint16_t *x;      // [n][Hi][Hi][Cin] (this layer's input = previous ReLU output)
+    uint16_t *dzp;          // [n][Hi][Hi][Cin]
+    int M;                  // n * Hi * Hi
+    __device__ void k_range(int, int &b, int &e) const { b = 0; e = 9 * Cout; }
+    template <int MT>
+    __device__ void fillA(uint16_t *As, int m0, int k0, int tid) const {
+        const int tap = k0 / Cout, co0 = k0 - tap * Cout;   // a chunk never straddles a tap (Cout % 32 == 0)
+        const int ky = tap / 3, kx = tap % 3;
+        for (int t = tid; t < MT * 4; t += 256) {
+            const int r = t >> 2, kb = (t & 3) * 8, m = m0 + r;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (m < M) {
+                const int b = m / (Hi * Hi), rem = m - b * Hi * Hi, iy = rem / Hi, ix = rem - iy * Hi;
+                const int ty = iy + 1 - ky, tx = ix + 1 - kx;
+                if (ty >= 0 && tx >= 0 && ty % S == 0 && tx % S == 0 && ty / S < Ho && tx / S < Ho)
+                    v = *(const uint4 *) &dz[(((size_t) b * Ho + ty / S) * Ho + tx / S) * Cout + co0 + kb];
+            }
+            put8(&As[r * kLDK + kb], v);
+        }
+    }
+    template <int NT>
+    __device__ void fillB(uint16_t *Bs, int n0, int k0, int tid) const {
+        const int tap = k0 / Cout, co0 = k0 - tap * Cout;
+        for (int t = tid; t < NT * 4; t += 256) {
+            const int r = t >> 2, kb = (t & 3) * 8;
+            put8(&Bs[r * kLDK + kb], *(const uint4 *) &wt[((size_t) tap * Cin + n0 + r) * Cout + co0 + kb]);
+        }
+    }
+    __device__ void store(int m, int n, float v) const {
+        if (m < M) {
+            const size_t i = (size_t) m * Cin + n;
+            dzp[i] = positive(x[i]) ? f2bf(v) : (uint16_t) 0;
+        }
+    }
+};
+
+// Weight gradient: dw[co][k = (tap, ci)] = sum_p dz[p][co] x[shift_tap(p)][ci],
+// the pixel sum split into slices of `slice` pixels (blockIdx.z); partial
+// sums go to part[z][co][Kp].
+template <int L>
+struct WeightProb {
+    static constexpr int Cin = kCin[L], Cout = kCout[L], S = kStride[L], Hi = kHin[L], Ho = hout(L);
+    static constexpr int Kt = ktrue(L), Kp = kpad(L);
+    const uint16_t *dz;     // [P][Cout]
+    const uint16_t *x;      // [n][Hi][Hi][Cin]
+    float *part;            // [slices][Cout][Kp]
+    int P;                  // n * Ho * Ho
+    int slice;              // pixels per slice, multiple of 32
+    __device__ void k_range(int z, int &b, int &e) const {
+        b = z * slice;
+        e = min(P, b + slice);
+    }
+    template <int MT>
+    __device__ void fillA(uint16_t *As, int m0, int k0, int tid) const {
+        // As[co][px] = dz[k0 + px][m0 + co]: 16-byte loads along co, transposed into LDS
+        for (int t = tid; t < 32 * (MT / 8); t += 256) {
+            const int px = t % 32, o = (t / 32) * 8, p = k0 + px;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (p < P) v = *(const uint4 *) &dz[(size_t) p * Cout + m0 + o];
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) As[(o + q) * kLDK + px] = (uint16_t) (w4[q >> 1] >> ((q & 1) * 16));
+        }
+    }
+    template <int NT>
+    __device__ void fillB(uint16_t *Bs, int n0, int k0, int tid) const {
+        // Bs[n][px] = x at pixel k0+px shifted by tap(n), channel ci(n)
+        for (int t = tid; t < 32 * (NT / 8); t += 256) {
+            const int px = t % 32, o = (t / 32) * 8, p = k0 + px;
+            uint16_t e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (p < P) {
+                const int b = p / (Ho * Ho), rem = p - b * Ho * Ho, oy = rem / Ho, ox = rem - oy * Ho;
+                const int n = n0 + o;
+                if constexpr (Cin % 8 == 0) {
+                    const int tap = n / Cin, ci = n - tap * Cin;
+                    const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
+                    if (n < Kt && iy >= 0 && iy < Hi && ix >= 0 && ix < Hi) {
+                        const uint4 v = *(const uint4 *) &x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
+                        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) e[q] = (uint16_t) (w4[q >> 1] >> ((q & 1) * 16));
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const int k = n + q;
+                        if (k < Kt) {
+                            const int tap = k / Cin, ci = k - tap * Cin;
+                            const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
+                            if (iy >= 0 && iy < Hi && ix >= 0 && ix < Hi)
+                                e[q] = x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Bs[(o + q) * kLDK + px] = e[q];
+        }
+    }
+    __device__ void store(int m, int n, float v) const {
+        part[((size_t) blockIdx.z * Cout + m) * Kp + n] = v;
+    }
+};
+
+// grad[co][k] (k < Kt) = sum over slices of part[z][co][k], in slice order.
+__global__ void reduce_slices_kernel(const float *part, int slices, int cout, int kp, int kt, float *grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cout * kt) return;
+    const int co = i / kt, k = i - co * kt;
+    float s = 0.0f;
+    for (int z = 0; z < slices; ++z) s += part[((size_t) z * cout + co) * kp + k];
+    grad[i] = s;
+}
+
+// Bias gradient: column sums of dz [P][C], first per block of rows, then in block order.
+__global__ void colsum_partial_kernel(const uint16_t *dz, int P, int C, int rows_per_block, float *part) {
+    const int c = threadIdx.x;
+    if (c >= C) return;
+    const int r0 = blockIdx.x * rows_per_block, r1 = min(P, r0 + rows_per_block);
+    float s = 0.0f;
+    for (int r = r0; r < r1; ++r) s += bf2f(dz[(size_t) r * C + c]);
+    part[(size_t) blockIdx.x * C + c] = s;
+}
+
+__global__ void colsum_final_kernel(const float *part, int blocks, int C, float *out) {
+    const int c = threadIdx.x;
+    if (c >= C) return;
+    float s = 0.0f;
+    for (int b = 0; b < blocks; ++b) s += part[(size_t) b * C + c];
+    out[c] = s;
+}
+
+// ------------------------------------------------------------- small passes
+
+// Input preparation: per pair, each image reduced to 128 x 128 (2x2 box
+// average for 256 x 256, as is for 128 x 128), standardised to zero mean and
+// unit variance (binary64 moments; a flat image maps to 0), rounded to bf16
+// and interleaved as channel 0 = moving, 1 = fixed.  One workgroup per
+// (pair, image).
+template <typename T>
+__device__ float prep_pixel(const T *img, int side, int y, int x) {
+    if (side == kIn) return (float) img[y * side + x];
+    const int r = 2 * y, c = 2 * x;
+    const float a = (float) img[r * side + c], b = (float) img[r * side + c + 1];
+    const float d = (float) img[(r + 1) * side + c], e = (float) img[(r + 1) * side + c + 1];
+    return ((a + b) + (d + e)) * 0.25f;
+}
+
+struct PrepImage {
+    const void *data;
+    int side;
+    int u8;
+};
+
+__global__ __launch_bounds__(256) void prepare_kernel(const PrepImage *imgs, uint16_t *x) {
+    const int pair = blockIdx.x >> 1, ch = blockIdx.x & 1;
+    const PrepImage im = imgs[blockIdx.x];
+    __shared__ double red[2][256];
+    double s = 0.0, ss = 0.0;
+    for (int i = threadIdx.x; i < kIn * kIn; i += 256) {
+        const int y = i / kIn, xx = i % kIn;
+        const float v = im.u8 ? prep_pixel((const uint8_t *) im.data, im.side, y, xx)
+                              : prep_pixel((const float *) im.data, im.side, y, xx);
+        s += v;
+        ss += (double) v * v;
+    }
+    red[0][threadIdx.x] = s;
+    red[1][threadIdx.x] = ss;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    const double n = (double) kIn * kIn;
+    const double mean = red[0][0] / n;
+    const double var = red[1][0] / n - mean * mean;
+    const double inv = var > 1e-12 ? 1.0 / sqrt(var) : 0.0;
+    for (int i = threadIdx.x; i < kIn * kIn; i += 256) {
+        const int y = i / kIn, xx = i % kIn;
+        const float v = im.u8 ? prep_pixel((const uint8_t *) im.data, im.side, y, xx)
+                              : prep_pixel((const float *) im.data, im.side, y, xx);
+        x[((size_t) pair * kIn * kIn + i) * 2 + ch] = f2bf((float) (((double) v - mean) * inv));
+    }
+}
+
+// Global average pooling of the last activation + FC: one workgroup per sample.
+__global__ __launch_bounds__(kFeat) void gap_fc_kernel(const uint16_t *y6, const float *fcw, const float *fcb,
+                                                      float *feat, float *out) {
+    const int b = blockIdx.x, c = threadIdx.x;
+    __shared__ float f[kFeat];
+    float s = 0.0f;
+    for (int p = 0; p < kLastPix; ++p) s += bf2f(y6[((size_t) b * kLastPix + p) * kFeat + c]);
+    s *= 1.0f / kLastPix;
+    f[c] = s;
+    feat[(size_t) b * kFeat + c] = s;
+    __syncthreads();
+    if (c < kOut) {
+        float o = fcb[c];
+        for (int q = 0; q < kFeat; ++q) o = fmaf(fcw[c * kFeat + q], f[q], o);
+        out[(size_t) b * kOut + c] = o;
+    }
+}
+
+// Mean squared error over n x 8 outputs and the FC gradients.  One workgroup.
+// loss = sum (out - target)^2 / (8n); dout = 2 (out - target) / (8n).
+__global__ __launch_bounds__(256) void loss_fc_kernel(const float *out, const float *target, const float *feat,
+                                                     const float *fcw, int n, float *dout, float *dfeat,
+                                                     float *grad_fcw, float *grad_fcb, float *loss) {
+    __shared__ float red[256];
+    const float scale = 1.0f / (kOut * (float) n);
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < n * kOut; i += 256) {
+        const float d = out[i] - target[i];
+        s += d * d;
+        dout[i] = 2.0f * d * scale;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = red[0] * scale;
+    for (int i = threadIdx.x; i < kOut * kFeat; i += 256) {
+        const int j = i / kFeat, c = i % kFeat;
+        float g = 0.0f;
+        for (int b = 0; b < n; ++b) g = fmaf(dout[b * kOut + j], feat[(size_t) b * kFeat + c], g);
+        grad_fcw[i] = g;
+    }
+    if (threadIdx.x < kOut) {
+        float g = 0.0f;
+        for (int b = 0; b < n; ++b) g += dout[b * kOut + threadIdx.x];
+        grad_fcb[threadIdx.x] = g;
+    }
+    for (int i = threadIdx.x; i < n * kFeat; i += 256) {
+        const int b = i / kFeat, c = i % kFeat;
+        float g = 0.0f;
+        for (int j = 0; j < kOut; ++j) g = fmaf(dout[b * kOut + j], fcw[j * kFeat + c], g);
+        dfeat[i] = g;
+    }
+}
+
+// Gradient at the last conv's pre-activation: dfeat / 256 where y6 > 0.
+__global__ void gap_backward_kernel(const uint16_t *y6, const float *dfeat, int n, uint16_t *dz6) {
+    const size_t total = (size_t) n * kLastPix * kFeat;
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t) gridDim.x * blockDim.x) {
+        const size_t b = i / ((size_t) kLastPix * kFeat);
+        const int c = (int) (i % kFeat);
+        dz6[i] = positive(y6[i]) ? f2bf(dfeat[b * kFeat + c] * (1.0f / kLastPix)) : (uint16_t) 0;
+    }
+}
+
+// Adam with bias correction on the flat FP32 parameter vector.
+__global__ void adam_kernel(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// Master FP32 weights -> bf16 working copies (padded [Cout][Kp] and the
+// transposed [9][Cin][Cout]), FC weights rounded to bf16 (kept as float).
+struct PackArgs {
+    uint16_t *w[kLayers];
+    uint16_t *wt[kLayers];
+    float *fcw;
+};
+
+__global__ void pack_kernel(const float *master, PackArgs a) {
+    const int l = blockIdx.y;
+    if (l == kLayers) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kOut * kFeat; i += gridDim.x * blockDim.x)
+            a.fcw[i] = bf2f(f2bf(master[kOff.fcw + i]));
+        return;
+    }
+    const int cin = kCin[l], cout = kCout[l], kt = 9 * cin, kp = (kt + 31) / 32 * 32;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cout * kp; i += gridDim.x * blockDim.x) {
+        const int co = i / kp, k = i % kp;
+        const uint16_t h = k < kt ? f2bf(master[kOff.w[l] + (int64_t) co * kt + k]) : (uint16_t) 0;
+        a.w[l][i] = h;
+        if (k < kt) {
+            const int tap = k / cin, ci = k % cin;
+            a.wt[l][((size_t) tap * cin + ci) * cout + co] = h;
+        }
+    }
+}
+
+}  // namespace cnn
+}  // namespace cgs
+
+using namespace cgs;
+using namespace cgs::cnn;
+
+struct cgs_cnn_s {
+    int max_batch = 0;
+    int cus = 0;
+    int adam_t = 0;
+    int last_n = 0;                 // batch of the last forward (0: none)
+    const uint16_t *last_x = nullptr;
+    DevBuf master, grad, adam_m, adam_v, fcw, w[kLayers], wt[kLayers];
+    DevBuf act[kLayers];            // outputs of the conv layers
+    DevBuf dz[2], part, colpart, feat, out, dout, dfeat, loss, loss_host;
+    std::vector<float> host_master;
+    ~cgs_cnn_s() {
+        auto rel = [](DevBuf &b, bool host) {
+            if (b.ptr) (void) (host ? hipHostFree(b.ptr) : hipFree(b.ptr));
+            b.ptr = nullptr;
+        };
+        rel(master, false); rel(grad, false); rel(adam_m, false); rel(adam_v, false); rel(fcw, false);
+        for (int l = 0; l < kLayers; ++l) { rel(w[l], false); rel(wt[l], false); rel(act[l], false); }
+        rel(dz[0], false); rel(dz[1], false); rel(part, false); rel(colpart, false); rel(feat, false);
+        rel(out, false); rel(dout, false); rel(dfeat, false); rel(loss, false); rel(loss_host, true);
+    }
+};
+
+namespace {
+
+constexpr int kColBlocks = 64;
+
+// split-K slices of the weight gradient: about 2 waves of workgroups per CU
+int wgrad_slices(int l, int n, int cus, int tiles) {
+    const int P = n * hout(l) * hout(l);
+    int z = std::max(1, (2 * cus + tiles - 1) / tiles);
+    z = std::min(z, std::max(1, P / 256));
+    return z;
+}
+
+int wgrad_slice_len(int l, int n, int z) {
+    const int P = n * hout(l) * hout(l);
+    return ((P + z - 1) / z + 31) / 32 * 32;
+}
+
+constexpr int fwd_mt(int l) { return kCout[l] >= 128 ? 64 : kCout[l] == 64 ? 64 : 128; }
+constexpr int fwd_nt(int l) { return kCout[l]; }
+constexpr int dat_mt(int l) { return kCin[l] >= 64 ? 64 : 128; }
+constexpr int dat_nt(int l) { return kCin[l]; }
+constexpr int wgt_mt(int l) { return kCout[l] >= 128 ? 128 : kCout[l]; }
+constexpr int wgt_nt(int l) { return kpad(l) % 64 == 0 ? 64 : 32; }
+
+template <int L>
+hipError_t launch_fwd(const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *y, int n, hipStream_t s) {
+    constexpr int MT = fwd_mt(L), NT = fwd_nt(L);
+    FwdProb<L> pb{x, w, bias, y, n * hout(L) * hout(L)};
+    hipLaunchKernelGGL((igemm_kernel<FwdProb<L>, MT, NT>), dim3((pb.M + MT - 1) / MT, kCout[L] / NT, 1), dim3(256),
+                       0, s, pb);
+    return hipGetLastError();
+}
+
+template <int L>
+hipError_t launch_data(const uint16_t *dz, const uint16_t *wt, const uint16_t *x, uint16_t *dzp, int n,
+                       hipStream_t s) {
+    constexpr int MT = dat_mt(L), NT = dat_nt(L);
+    DataProb<L> pb{dz, wt, x, dzp, n * kHin[L] * kHin[L]};
+    hipLaunchKernelGGL((igemm_kernel<DataProb<L>, MT, NT>), dim3((pb.M + MT - 1) / MT, kCin[L] / NT, 1), dim3(256),
+                       0, s, pb);
+    return hipGetLastError();
+}
+
+template <int L>
+hipError_t launch_weight(const uint16_t *dz, const uint16_t *x, float *part, float *grad, float *colpart,
+                         float *gbias, int n, int cus, hipStream_t s) {
+    constexpr int MT = wgt_mt(L), NT = wgt_nt(L), Kp = kpad(L);
+    const int tiles = (kCout[L] / MT) * (Kp / NT);
+    const int z = wgrad_slices(L, n, cus, tiles);
+    const int P = n * hout(L) * hout(L);
+    WeightProb<L> pb{dz, x, part, P, wgrad_slice_len(L, n, z)};
+    hipLaunchKernelGGL((igemm_kernel<WeightProb<L>, MT, NT>), dim3(kCout[L] / MT, Kp / NT, z), dim3(256), 0, s, pb);
+    const int cnt = kCout[L] * ktrue(L);
+    hipLaunchKernelGGL(reduce_slices_kernel, dim3((cnt + 255) / 256), dim3(256), 0, s, (const float *) part, z,
+                       kCout[L], Kp, ktrue(L), grad);
+    const int rows = (P + kColBlocks - 1) / kColBlocks;
+    hipLaunchKernelGGL(colsum_partial_kernel, dim3(kColBlocks), dim3(kCout[L]), 0, s, dz, P, kCout[L], rows, colpart);
+    hipLaunchKernelGGL(colsum_final_kernel, dim3(1), dim3(kCout[L]), 0, s, (const float *) colpart, kColBlocks,
+                       kCout[L], gbias);
+    return hipGetLastError();
+}
+
+size_t act_elems(int l, int n) { return (size_t) n * hout(l) * hout(l) * kCout[l]; }
+
+size_t part_floats(int n, int cus) {
+    size_t mx = 0;
+    auto one = [&](int l, int mt, int nt) {
+        const int tiles = (kCout[l] / mt) * (kpad(l) / nt);
+        mx = std::max(mx, (size_t) wgrad_slices(l, n, cus, tiles) * kCout[l] * kpad(l));
+    };
+    one(0, wgt_mt(0), wgt_nt(0)); one(1, wgt_mt(1), wgt_nt(1)); one(2, wgt_mt(2), wgt_nt(2));
+    one(3, wgt_mt(3), wgt_nt(3)); one(4, wgt_mt(4), wgt_nt(4)); one(5, wgt_mt(5), wgt_nt(5));
+    return mx;
+}
+
+cgs_status pack(cgs_cnn *c, hipStream_t s) {
+    PackArgs a;
+    for (int l = 0; l < kLayers; ++l) {
+        a.w[l] = (uint16_t *) c->w[l].ptr;
+        a.wt[l] = (uint16_t *) c->wt[l].ptr;
+    }
+    a.fcw = (float *) c->fcw.ptr;
+    hipLaunchKernelGGL(pack_kernel, dim3(64, kLayers + 1), dim3(256), 0, s, (const float *) c->master.ptr, a);
+    HIP_TRY(hipGetLastError());
+    return CGS_OK;
+}
+
+cgs_status check_batch(const cgs_cnn *c, int32_t n) {
+    if (!c) return set_error(CGS_ERR_INVALID_ARGUMENT, "null network");
+    if (n < 1 || n > c->max_batch) return set_error(CGS_ERR_TOO_LARGE, "batch outside 1..max_batch");
+    return CGS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t cgs_cnn_param_count(void) { return kOff.total; }
+
+cgs_status cgs_cnn_create(int32_t max_batch, cgs_cnn **out) {
+    clear_error();
+    if (!out || max_batch < 1 || max_batch > 4096)
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_create: max_batch must be 1..4096");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return set_error(CGS_ERR_NO_DEVICE, "no HIP device");
+    cgs_cnn *c = new cgs_cnn_s();
+    c->max_batch = max_batch;
+    HIP_TRY(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t np = (size_t) kOff.total;
+    cgs_status st = CGS_OK;
+    auto need = [&](DevBuf &b, size_t bytes, bool host = false) {
+        if (st == CGS_OK) st = b.ensure(bytes, host);
+    };
+    need(c->master, np * 4); need(c->grad, np * 4); need(c->adam_m, np * 4); need(c->adam_v, np * 4);
+    need(c->fcw, kOut * kFeat * 4);
+    size_t dzmax = 0;
+    for (int l = 0; l < kLayers; ++l) {
+        need(c->w[l], (size_t) kCout[l] * kpad(l) * 2);
+        need(c->wt[l], (size_t) 9 * kCin[l] * kCout[l] * 2);
+        need(c->act[l], act_elems(l, max_batch) * 2);
+        dzmax = std::max(dzmax, act_elems(l, max_batch));
+    }
+    need(c->dz[0], dzmax * 2); need(c->dz[1], dzmax * 2);
+    need(c->part, part_floats(max_batch, c->cus) * 4);
+    need(c->colpart, (size_t) kColBlocks * 128 * 4);
+    need(c->feat, (size_t) max_batch * kFeat * 4);
+    need(c->out, (size_t) max_batch * kOut * 4);
+    need(c->dout, (size_t) max_batch * kOut * 4);
+    need(c->dfeat, (size_t) max_batch * kFeat * 4);
+    need(c->loss, 16); need(c->loss_host, 16, true);
+    if (st != CGS_OK) { delete c; return st; }
+    if (hipMemset(c->master.ptr, 0, np * 4) != hipSuccess || hipMemset(c->adam_m.ptr, 0, np * 4) != hipSuccess ||
+        hipMemset(c->adam_v.ptr, 0, np * 4) != hipSuccess) {
+        delete c;
+        return set_error(CGS_ERR_HIP, "cnn_create: clearing parameters failed");
+    }
+    if ((st = pack(c, nullptr)) != CGS_OK || hipDeviceSynchronize() != hipSuccess) {
+        delete c;
+        return st != CGS_OK ? st : set_error(CGS_ERR_HIP, "cnn_create: packing failed");
+    }
+    *out = c;
+    return CGS_OK;
+}
+
+cgs_status cgs_cnn_destroy(cgs_cnn *c) {
+    delete c;
+    return CGS_OK;
+}
+
+cgs_status cgs_cnn_set_params(cgs_cnn *c, const float *params) {
+    clear_error();
+    if (!c || !params) return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_set_params: null argument");
+    HIP_TRY(hipMemcpy(c->master.ptr, params, (size_t) kOff.total * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(c->adam_m.ptr, 0, (size_t) kOff.total * 4));
+    HIP_TRY(hipMemset(c->adam_v.ptr, 0, (size_t) kOff.total * 4));
+    c->adam_t = 0;
+    c->last_n = 0;
+    cgs_status st = pack(c, nullptr);
+    if (st != CGS_OK) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    return CGS_OK;
+}
+
+cgs_status cgs_cnn_get_params(const cgs_cnn *c, float *params) {
+    clear_error();
+    if (!c || !params) return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_get_params: null argument");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(params, c->master.ptr, (size_t) kOff.total * 4, hipMemcpyDeviceToHost));
+    return CGS_OK;
+}
+
+cgs_status cgs_cnn_get_grads(const cgs_cnn *c, float *grads) {
+    clear_error();
+    if (!c || !grads) return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_get_grads: null argument");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(grads, c->grad.ptr, (size_t) kOff.total * 4, hipMemcpyDeviceToHost));
+    return CGS_OK;
+}
+
+cgs_status cgs_cnn_prepare(const cgs_image *moved, const cgs_image *fixed, int32_t n, uint16_t *input, void *stream) {
+    clear_error();
+    if (!moved || !fixed || !input || n < 1) return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_prepare: bad arguments");
+    std::vector<PrepImage> h(2 * (size_t) n);
+    for (int32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 2; ++k) {
+            const cgs_image &im = k == 0 ? moved[i] : fixed[i];
+            cgs_status st = validate_image(&im, k == 0 ? "moved" : "fixed");
+            if (st != CGS_OK) return st;
+            if (im.channels != 1 || im.width != im.height || (im.width != kIn && im.width != 2 * kIn))
+                return set_error(CGS_ERR_UNSUPPORTED, "cnn_prepare: images must be one-channel 128x128 or 256x256");
+            h[2 * i + k] = PrepImage{im.data, im.width, im.pixtype == CGS_PIX_U8 ? 1 : 0};
+        }
+    }
+    Workspace *ws = nullptr;
+    cgs_status st = workspace_for_current_device(&ws);
+    if (st != CGS_OK) return st;
+    std::lock_guard<std::mutex> lock(ws->mu);
+    hipStream_t s = (hipStream_t) stream;
+    if ((st = ws->prep.ensure(h.size() * sizeof(PrepImage))) != CGS_OK ||
+        (st = ws->prep_host.ensure(h.size() * sizeof(PrepImage), true)) != CGS_OK)
+        return st;
+    // the pinned staging copy is reused by the next call: wait for this one
+    std::memcpy(ws->prep_host.ptr, h.data(), h.size() * sizeof(PrepImage));
+    HIP_TRY(hipMemcpyAsync(ws->prep.ptr, ws->prep_host.ptr, h.size() * sizeof(PrepImage), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(prepare_kernel, dim3(2 * n), dim3(256), 0, s, (const PrepImage *) ws->prep.ptr, input);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return CGS_OK;
+}
+
+cgs_status cgs_cnn_forward(cgs_cnn *c, const uint16_t *input, int32_t n, float *offsets, void *stream) {
+    clear_error();
+    cgs_status st = check_batch(c, n);
+    if (st != CGS_OK) return st;
+    if (!input) return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_forward: null input");
+    hipStream_t s = (hipStream_t) stream;
+    auto A = [&](int l) { return (uint16_t *) c->act[l].ptr; };
+    auto W = [&](int l) { return (const uint16_t *) c->w[l].ptr; };
+    const float *mp = (const float *) c->master.ptr;
+    HIP_TRY(launch_fwd<0>(input, W(0), mp + kOff.b[0], A(0), n, s));
+    HIP_TRY(launch_fwd<1>(A(0), W(1), mp + kOff.b[1], A(1), n, s));
+    HIP_TRY(launch_fwd<2>(A(1), W(2), mp + kOff.b[2], A(2), n, s));
+    HIP_TRY(launch_fwd<3>(A(2), W(3), mp + kOff.b[3], A(3), n, s));
+    HIP_TRY(launch_fwd<4>(A(3), W(4), mp + kOff.b[4], A(4), n, s));
+    HIP_TRY(launch_fwd<5>(A(4), W(5), mp + kOff.b[5], A(5), n, s));
+    hipLaunchKernelGGL(gap_fc_kernel, dim3(n), dim3(kFeat), 0, s, (const uint16_t *) A(5), (const float *) c->fcw.ptr,
+                       mp + kOff.fcb, (float *) c->feat.ptr, (float *) c->out.ptr);
+    HIP_TRY(hipGetLastError());
+    if (offsets)
+        HIP_TRY(hipMemcpyAsync(offsets, c->out.ptr, (size_t) n * kOut * 4, hipMemcpyDeviceToDevice, s));
+    c->last_n = n;
+    c->last_x = input;
+    return CGS_OK;
+}
+
+cgs_status cgs_cnn_backward(cgs_cnn *c, const float *target, float *loss_out, void *stream) {
+    clear_error();
+    if (!c || !target) return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_backward: null argument");
+    if (c->last_n < 1) return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_backward: no forward pass to differentiate");
+    const int n = c->last_n;
+    hipStream_t s = (hipStream_t) stream;
+    float *g = (float *) c->grad.ptr;
+    auto A = [&](int l) { return (const uint16_t *) c->act[l].ptr; };
+    auto WT = [&](int l) { return (const uint16_t *) c->wt[l].ptr; };
+    uint16_t *dz[2] = {(uint16_t *) c->dz[0].ptr, (uint16_t *) c->dz[1].ptr};
+    float *part = (float *) c->part.ptr, *colp = (float *) c->colpart.ptr;
+    hipLaunchKernelGGL(loss_fc_kernel, dim3(1), dim3(256), 0, s, (const float *) c->out.ptr, target,
+                       (const float *) c->feat.ptr, (const float *) c->fcw.ptr, n, (float *) c->dout.ptr,
+                       (float *) c->dfeat.ptr, g + kOff.fcw, g + kOff.fcb, (float *) c->loss.ptr);
+    hipLaunchKernelGGL(gap_backward_kernel, dim3(512), dim3(256), 0, s, A(5), (const float *) c->dfeat.ptr, n, dz[1]);
+    HIP_TRY(hipGetLastError());
+    // dz of layer l lives in dz[l & 1]
+    HIP_TRY(launch_weight<5>(dz[1], A(4), part, g + kOff.w[5], colp, g + kOff.b[5], n, c->cus, s));
+    HIP_TRY(launch_data<5>(dz[1], WT(5), A(4), dz[0], n, s));
+    HIP_TRY(launch_weight<4>(dz[0], A(3), part, g + kOff.w[4], colp, g + kOff.b[4], n, c->cus, s));
+    HIP_TRY(launch_data<4>(dz[0], WT(4), A(3), dz[1], n, s));
+    HIP_TRY(launch_weight<3>(dz[1], A(2), part, g + kOff.w[3], colp, g + kOff.b[3], n, c->cus, s));
+    HIP_TRY(launch_data<3>(dz[1], WT(3), A(2), dz[0], n, s));
+    HIP_TRY(launch_weight<2>(dz[0], A(1), part, g + kOff.w[2], colp, g + kOff.b[2], n, c->cus, s));
+    HIP_TRY(launch_data<2>(dz[0], WT(2), A(1), dz[1], n, s));
+    HIP_TRY(launch_weight<1>(dz[1], A(0), part, g + kOff.w[1], colp, g + kOff.b[1], n, c->cus, s));
+    HIP_TRY(launch_data<1>(dz[1], WT(1), A(0), dz[0], n, s));
+    HIP_TRY(launch_weight<0>(dz[0], c->last_x, part, g + kOff.w[0], colp, g + kOff.b[0], n, c->cus, s));
+    if (loss_out) {
+        HIP_TRY(hipMemcpyAsync(c->loss_host.ptr, c->loss.ptr, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        *loss_out = *(const float *) c->loss_host.ptr;
+    }
+    return CGS_OK;
+}
+
+cgs_status cgs_cnn_adam_step(cgs_cnn *c, float lr, float beta1, float beta2, float eps, void *stream) {
+    clear_error();
+    if (!c) return set_error(CGS_ERR_INVALID_ARGUMENT, "null network");
+    if (!(lr >= 0.0f) || !(beta1 >= 0.0f && beta1 < 1.0f) || !(beta2 >= 0.0f && beta2 < 1.0f) || !(eps > 0.0f))
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "cnn_adam_step: bad hyper-parameters");
+    hipStream_t s = (hipStream_t) stream;
+    c->adam_t += 1;
+    const float c1 = (float) (1.0 - std::pow((double) beta1, c->adam_t));
+    const float c2 = (float) (1.0 - std::pow((double) beta2, c->adam_t));
+    hipLaunchKernelGGL(adam_kernel, dim3(512), dim3(256), 0, s, (float *) c->master.ptr, (const float *) c->grad.ptr,
+                       (float *) c->adam_m.ptr, (float *) c->adam_v.ptr, kOff.total, lr, beta1, beta2, eps, c1, c2);
+    HIP_TRY(hipGetLastError());
+    return pack(c, s);
+}
+
+}  // extern "C"

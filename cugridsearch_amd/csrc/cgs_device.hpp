@@ -19,12 +19,14 @@ struct GridDev {
     float start[kMaxDim];
     float res[kMaxDim];
     int32_t incr[kMaxDim];
+    int32_t off[kMaxDim];     // first sample index per axis (grid sub-boxes)
     int32_t dim;
     int32_t n;
 };
 
 // indexToGridPoint (src/gpu/cudaGridSearch.cuh:160-187): mixed-radix decode,
-// axis 0 fastest; the point is fmaf(i, res, start).
+// axis 0 fastest; the point is fmaf(off + i, res, start); *i0 is the local
+// axis-0 index.
 __device__ __forceinline__ void grid_point(const GridDev &g, int32_t index, float *p, int32_t *i0) {
     int32_t rem = index;
     int32_t idx[kMaxDim];
@@ -38,7 +40,8 @@ __device__ __forceinline__ void grid_point(const GridDev &g, int32_t index, floa
         }
     }
 #pragma unroll
-    for (int a = 0; a < kMaxDim; ++a) p[a] = (a < g.dim) ? fmaf((float) idx[a], g.res[a], g.start[a]) : 0.0f;
+    for (int a = 0; a < kMaxDim; ++a)
+        p[a] = (a < g.dim) ? fmaf((float) (idx[a] + g.off[a]), g.res[a], g.start[a]) : 0.0f;
     *i0 = idx[0];
 }
 

@@ -109,6 +109,9 @@ cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const c
     if (grid->dim != need)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "grid dimension does not match the parameter mode");
     if (grid->num_elements <= 0) return set_error(CGS_ERR_INVALID_ARGUMENT, "empty grid");
+    for (int a = 0; a < grid->dim; ++a)   // sample indices stay exact in binary32
+        if (grid->offset[a] < 0 || (int64_t) grid->offset[a] + (int64_t) grid->count[a] > (1 << 24))
+            return set_error(CGS_ERR_INVALID_ARGUMENT, "grid offset outside 0..2^24");
     return CGS_OK;
 }
 
@@ -120,6 +123,7 @@ void fill_grid_dev(const cgs_grid *grid, GridDev *g) {
         g->start[a] = a < grid->dim ? grid->start[a] : 0.0f;
         g->res[a] = a < grid->dim ? grid->resolution[a] : 0.0f;
         g->incr[a] = a < grid->dim ? grid->increments[a] : 1;
+        g->off[a] = a < grid->dim ? grid->offset[a] : 0;
     }
 }
 
@@ -203,7 +207,32 @@ cgs_status cgs_grid_point(const cgs_grid *g, int64_t index, float *out) {
         idx[a] = rem / g->increments[a];
         rem -= idx[a] * g->increments[a];
     }
-    for (int a = 0; a < g->dim; ++a) out[a] = std::fma((float) idx[a], g->resolution[a], g->start[a]);
+    for (int a = 0; a < g->dim; ++a)
+        out[a] = std::fma((float) (idx[a] + g->offset[a]), g->resolution[a], g->start[a]);
+    return CGS_OK;
+}
+
+cgs_status cgs_grid_subbox(const cgs_grid *g, const int32_t *lo, const int32_t *count, cgs_grid *out) {
+    clear_error();
+    if (!g || !lo || !count || !out) return set_error(CGS_ERR_INVALID_ARGUMENT, "grid_subbox: null pointer");
+    if (g->dim < 1 || g->dim > CGS_MAX_DIM) return set_error(CGS_ERR_INVALID_ARGUMENT, "grid_subbox: bad grid");
+    cgs_grid r = *g;
+    int64_t total = 1;
+    for (int a = 0; a < g->dim; ++a) {
+        const int32_t n = (int32_t) g->count[a];
+        if (lo[a] < 0 || count[a] < 1 || (int64_t) lo[a] + count[a] > n)
+            return set_error(CGS_ERR_INVALID_ARGUMENT, "grid_subbox: box outside the grid");
+        r.offset[a] = g->offset[a] + lo[a];
+        r.count[a] = (float) count[a];
+        r.start[a] = g->start[a];
+        r.resolution[a] = g->resolution[a];
+        r.end[a] = std::fma((float) (r.offset[a] + count[a] - 1), g->resolution[a], g->start[a]);
+        total *= count[a];
+    }
+    r.num_elements = total;
+    r.increments[0] = 1;
+    for (int a = 1; a < g->dim; ++a) r.increments[a] = r.increments[a - 1] * count[a - 1];
+    *out = r;
     return CGS_OK;
 }
 

@@ -276,7 +276,7 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     if (levels0 > 0) {
         float *h = (float *) ws->sincos_host.ptr;
         for (int i = 0; i < levels0; ++i) {
-            const float theta = fmaf((float) i, grid->resolution[0], grid->start[0]);
+            const float theta = fmaf((float) (grid->offset[0] + i), grid->resolution[0], grid->start[0]);
             h[2 * i] = (float) std::cos((double) theta);
             h[2 * i + 1] = (float) std::sin((double) theta);
         }

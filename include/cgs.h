@@ -76,6 +76,8 @@ typedef struct {
     float resolution[CGS_MAX_DIM];
     int32_t increments[CGS_MAX_DIM];
     int64_t num_elements;           /* numElements(), int32 semantics */
+    int32_t offset[CGS_MAX_DIM];    /* first sample index per axis: 0 for a grid from
+                                       cgs_grid_init, lo for a cgs_grid_subbox */
 } cgs_grid;
 
 /* CudaImage<pix, C>(H, W) view (include/cudaTensor.cuh:507-550): row-major,
@@ -124,8 +126,18 @@ cgs_status cgs_grid_init(cgs_grid *grid, int32_t dim, const float *start, const 
                          const float *count);
 
 /* CudaGrid::indexToGridPoint / getGridPoint (src/gpu/cudaGridSearch.cuh:160-196).
- * Host-only; out has grid->dim floats. */
+ * Host-only; out has grid->dim floats.  Point a = start + (offset + i) * res
+ * as fmaf((float) (offset + i), res, start). */
 cgs_status cgs_grid_point(const cgs_grid *grid, int64_t index, float *out);
+
+/* A box of a grid's samples: axis a keeps the grid's start and resolution
+ * and takes count[a] samples from sample index lo[a] on.  Its points are
+ * exactly the grid's points with those indices (the same binary32
+ * operations), in the same axis-0-fastest order, so a search of the box
+ * returns bit-identical costs.  No reference counterpart (the reference
+ * narrows a search by building a new grid, src/cpu/multispectral_ImageMatcher.cu:751-807,
+ * whose points are not the old grid's).  Host-only. */
+cgs_status cgs_grid_subbox(const cgs_grid *grid, const int32_t *lo, const int32_t *count, cgs_grid *out);
 
 /* CudaGridSearcher::search / search_by_value (src/gpu/cudaGridSearch.cuh:553-556,
  * 736-765) with the functor chosen by opts->cost, fused with
@@ -197,7 +209,104 @@ cgs_status cgs_batch_results(cgs_batch *batch, cgs_argmin *out, void *stream);
 /* Total work items and grid points of the planned batch (diagnostics). */
 cgs_status cgs_batch_info(const cgs_batch *batch, int64_t *items, int64_t *points, int64_t *pixel_evals);
 
+/* Local refinement with full-grid fallback.  No reference counterpart (the
+ * reference's coarse-to-fine loop, src/cpu/multispectral_ImageMatcher.cu:751-807,
+ * builds new grids instead).  Each pair searches the box of its grid within
+ * `radius` samples of its centre on every axis (3^8 points for radius 1 on
+ * an 8-D grid; the box is cut at the grid's faces).  A box minimum on a box
+ * face that is not a grid face re-centres the box there, up to max_steps
+ * boxes; the pair then falls back (outcome BOUNDARY).  A minimum inside its
+ * box is a local minimum of the grid; it is kept when its cost is at most
+ * `threshold`, else the pair falls back (outcome THRESHOLD).  With
+ * fallback != 0 each pair that falls back is searched over its whole grid
+ * in the same call, so its answer equals cgs_search's.  Box points are the
+ * grid's own points (cgs_grid_subbox), so every cost is bit-identical to
+ * the full search's cost of that point.  All pairs still moving share one
+ * batched launch per step. */
+typedef struct {
+    int32_t radius;        /* 1..16 */
+    int32_t max_steps;     /* boxes per pair before a boundary fallback: 1..1024 */
+    float threshold;       /* largest accepted local-minimum cost (+inf: any); not NaN */
+    int32_t fallback;      /* != 0: run the full-grid searches of the pairs that fall back */
+} cgs_refine_opts;
+
+typedef enum {
+    CGS_REFINE_LOCAL = 0,               /* answer = the local minimum */
+    CGS_REFINE_FALLBACK_BOUNDARY = 1,   /* still on a box face after max_steps boxes */
+    CGS_REFINE_FALLBACK_THRESHOLD = 2   /* local minimum costs more than threshold */
+} cgs_refine_outcome;
+
+typedef struct {
+    int32_t index;          /* grid index of the answer (-1: none, when a fallback was not run) */
+    float value;            /* its cost */
+    int32_t steps;          /* boxes searched */
+    int32_t outcome;        /* cgs_refine_outcome */
+    int32_t local_index;    /* grid index of the last box minimum */
+    float local_value;
+    int64_t points;         /* grid points evaluated for this pair, fallback included */
+} cgs_refine_result;
+
+/* pairs: as for cgs_batch_plan (result pointers are ignored); centres:
+ * n x grid.dim sample indices (clamped to the grid); out: n results.
+ * Replaces any plan the batch held.  Synchronous. */
+cgs_status cgs_batch_refine(cgs_batch *batch, const cgs_batch_pair *pairs, int32_t n, const int32_t *centres,
+                            const cgs_refine_opts *opts, cgs_refine_result *out, void *stream);
+
 cgs_status cgs_batch_destroy(cgs_batch *batch);
+
+/* ------------------------------------------------------------------------
+ * Proposal network.  No reference counterpart: the reference always searches
+ * the full grid (src/gpu/cudaGridSearch.cuh:736-765).  A small CNN regresses
+ * the homography's 4 corner offsets from a downsampled pair; a local grid
+ * around its proposal then replaces most of the full search.
+ *
+ * Fixed architecture: input 2 x 128 x 128 bf16, NHWC (channel 0 = moving,
+ * 1 = fixed); six 3x3 convolutions, zero padding 1, channels
+ * 32-32-64-64-128-128, stride 2 on the 2nd, 4th and 6th, ReLU after each;
+ * global average pooling; a fully connected layer to 8 outputs.  BF16
+ * weights and activations, FP32 accumulation (matrix cores), FP32 biases,
+ * FC, loss and optimiser.
+ *
+ * Parameters are one FP32 vector of cgs_cnn_param_count() values: for each
+ * conv layer its weight [Cout][ky][kx][Cin] then its bias [Cout]; then the
+ * FC weight [8][128] and bias [8].
+ * ---------------------------------------------------------------------- */
+typedef struct cgs_cnn_s cgs_cnn;
+
+int64_t cgs_cnn_param_count(void);
+
+/* Allocates a network (all parameters zero) for batches of up to max_batch
+ * (1..4096) pairs on the current device. */
+cgs_status cgs_cnn_create(int32_t max_batch, cgs_cnn **out);
+
+cgs_status cgs_cnn_destroy(cgs_cnn *net);
+
+/* Host FP32 parameter vectors in and out.  set_params also resets the
+ * optimiser state. */
+cgs_status cgs_cnn_set_params(cgs_cnn *net, const float *params);
+cgs_status cgs_cnn_get_params(const cgs_cnn *net, float *params);
+
+/* The network input of n pairs: each one-channel 128x128 or 256x256 image
+ * (uint8 or float) is reduced to 128x128 (2x2 box mean), standardised to
+ * zero mean and unit variance and rounded to bf16.  input: device
+ * uint16_t[n][128][128][2] (bf16 bit patterns).  Synchronous. */
+cgs_status cgs_cnn_prepare(const cgs_image *moved, const cgs_image *fixed, int32_t n, uint16_t *input,
+                           void *stream);
+
+/* Forward pass of n pairs (n <= max_batch).  offsets: device float[n][8]
+ * or NULL.  Keeps the activations for cgs_cnn_backward.  Asynchronous. */
+cgs_status cgs_cnn_forward(cgs_cnn *net, const uint16_t *input, int32_t n, float *offsets, void *stream);
+
+/* Gradient of the mean squared error sum((out - target)^2) / (8n) of the
+ * last forward pass with respect to every parameter; target: device
+ * float[n][8].  loss_out (host, may be NULL: then asynchronous) receives
+ * the loss.  Read the gradient with cgs_cnn_get_grads. */
+cgs_status cgs_cnn_backward(cgs_cnn *net, const float *target, float *loss_out, void *stream);
+cgs_status cgs_cnn_get_grads(const cgs_cnn *net, float *grads);
+
+/* One Adam step (bias-corrected) on the master weights with the last
+ * gradient, then the bf16 working copies are refreshed.  Asynchronous. */
+cgs_status cgs_cnn_adam_step(cgs_cnn *net, float lr, float beta1, float beta2, float eps, void *stream);
 
 /* Duration in ms of the calling thread's last timed search kernel (flags &
  * CGS_FLAG_TIME_KERNEL), measured with HIP events around that launch; -1

@@ -195,3 +195,58 @@ def argmin_reference_sim(r):
 
 def max_threads():
     return int(lib().orc_max_threads())
+
+
+# ------------------------------------------------------- proposal network
+
+_CNN_SO = os.path.join(_HERE, "_build", "libcnn_oracle.so")
+_cnn = None
+
+
+def cnn_lib():
+    global _cnn
+    if _cnn is None:
+        if not os.path.exists(_CNN_SO):
+            build()
+        L = ctypes.CDLL(_CNN_SO)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.cnn_param_count.restype = ctypes.c_int64
+        L.cnn_round_bf16.argtypes = [ctypes.c_double]
+        L.cnn_round_bf16.restype = ctypes.c_double
+        L.cnn_loss_grad.argtypes = [dp, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.cnn_loss_grad.restype = ctypes.c_double
+        _cnn = L
+    return _cnn
+
+
+def cnn_param_count():
+    return int(cnn_lib().cnn_param_count())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def cnn_loss_grad(params, x, target=None, emulate=True, want_grad=True):
+    """Network forward (and backward) in binary64.  x: [n, S, S, 2];
+    target: [n, 8] or None.  emulate: round weights, activations and
+    inter-layer gradients to bf16 where the GPU stores bf16.
+    Returns (loss, out [n, 8], grad [P] or None)."""
+    p = np.ascontiguousarray(params, np.float64)
+    xx = np.ascontiguousarray(x, np.float64)
+    n, side = xx.shape[0], xx.shape[1]
+    out = np.zeros((n, 8), np.float64)
+    t = None if target is None else np.ascontiguousarray(target, np.float64)
+    g = np.zeros(cnn_param_count(), np.float64) if (want_grad and t is not None) else None
+    loss = cnn_lib().cnn_loss_grad(_dp(p), _dp(xx), _dp(t) if t is not None else None, n, side, 1 if emulate else 0,
+                                   _dp(out), _dp(g) if g is not None else None)
+    return loss, out, g
+
+
+def round_bf16(a):
+    """binary64 -> binary32 -> bf16 (nearest even), as float64."""
+    f = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    nan = (f & 0x7FFFFFFF) > 0x7F800000
+    r = ((f + 0x7FFF + ((f >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    out = r.view(np.float32).astype(np.float64)
+    return np.where(nan, np.asarray(a, np.float64), out)
