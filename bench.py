@@ -29,6 +29,8 @@ PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
 FLOP_PER_PE = {("nearest", "sad"): 17 + 3, ("bilinear", "ncc"): 17 + 13 + 6, ("bilinear", "mi"): 17 + 13 + 2,
                ("bilinear", "sad"): 17 + 13 + 3}
 BATCH_WORKLOAD = "batch_ragged_256"
+PROPOSAL_WORKLOAD = "proposal_1024"
+PROPOSAL_WEIGHTS = os.path.join(ROOT, "cugridsearch_amd", "weights", "proposal_c5.npz")
 
 
 def parse():
@@ -148,6 +150,63 @@ def run_batch(args):
     return line
 
 
+def run_proposal(args):
+    """Learned proposal + local refinement on held-out 256x256 pairs (6^8 NCC
+    grid): pairs/s of the whole pipeline (input preparation, network,
+    3^8 boxes, full-grid fallbacks) at its agreement with the plain
+    full-grid argmin, which is searched first as the judge and as the plain
+    pairs/s; plus the ablation without the network."""
+    import torch
+    from cugridsearch_amd import cnn
+    from cugridsearch_amd import proposal as P
+
+    n = args.pairs if args.pairs != 256 else 1024
+    with np.load(PROPOSAL_WEIGHTS, allow_pickle=False) as z:
+        threshold = float(z["threshold"])
+    net = cnn.ProposalNet.load(PROPOSAL_WEIGHTS, max_batch=256)
+    g = P.grid()
+    _, images = P.upload_pairs(range(n), heldout=True, keep_images=True)
+    judge, t_full = P.run_full_grid(images, g)
+    rep, _, _ = P.evaluate(net, images, g, threshold, judge=judge)
+    for _ in range(max(0, args.warmup - 1)):
+        P.run_learned(net, images, g, threshold)
+    times, agree, pts = [], [], []
+    for _ in range(args.steps):
+        res, t, _ = P.run_learned(net, images, g, threshold)
+        times.append(t)
+        agree.append(float((res["index"] == judge).mean()))
+        pts.append(int(res["points"].sum()))
+    dt = float(np.mean(times))
+    flop = np.mean(pts) * 256 * 256 * FLOP_PER_PE[("bilinear", "ncc")]
+    achieved = flop / dt / 1e12
+    line = {
+        "metric": METRIC,
+        "value": n / dt,
+        "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1000.0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 network / f32 search", "data": "synthetic",
+        "config": {"workload": PROPOSAL_WORKLOAD, "pairs": n, "image": "256x256 float32", "grid_points": g.numElements(),
+                   "cost": "ncc", "local_box": "3^8", "threshold": threshold, "weights": os.path.basename(PROPOSAL_WEIGHTS)},
+        "argmin_agreement": float(np.mean(agree)),
+        "full_grid_pairs_per_s": n / t_full,
+        "speedup_vs_full_grid": t_full / dt,
+        "evaluation": rep,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_VALU_TFLOPS, "traffic": None,
+                     "note": "search FLOP of all boxes and fallbacks / whole-pipeline time"},
+    }
+    if not args.no_cpu:
+        m, f = images[0][0].tensor.cpu().numpy(), images[0][1].tensor.cpu().numpy()
+        cb = cpu_baseline(m, f, dict(cost="ncc", sampler="bilinear"), P.GRID_START, P.GRID_END, P.GRID_COUNT,
+                          256 * 256, args.cpu_seconds)
+        cb["value"] = cb["value"] * 1e9 / (g.numElements() * 256 * 256)
+        cb["unit"] = "pairs/s (plain full grid)"
+        cb["sample"] = "held-out pair 0: " + cb["sample"]
+        line["cpu_baseline"] = cb
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -157,6 +216,9 @@ def main():
         sys.exit(2)
     if args.workload == BATCH_WORKLOAD:
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.workload == PROPOSAL_WORKLOAD:
+        print(json.dumps(run_proposal(args)), flush=True)
         return
     import torch
     from cugridsearch_amd import _lib as L

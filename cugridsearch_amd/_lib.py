@@ -19,6 +19,7 @@ PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION = range(4)
 TIE_LOWEST, TIE_REFERENCE = 0, 1
 PIX_U8, PIX_F32 = 0, 1
 MAX_DIM = 8
+ABI_VERSION = 3          # CGS_ABI_VERSION of include/cgs.h these layouts follow
 FLAG_TIME_KERNEL = 1
 
 
@@ -77,6 +78,7 @@ REFINE_LOCAL, REFINE_FALLBACK_BOUNDARY, REFINE_FALLBACK_THRESHOLD = range(3)
 _P = ctypes.POINTER
 SIGNATURES = {
     "cgs_version": (ctypes.c_char_p, []),
+    "cgs_abi_version": (ctypes.c_int32, []),
     "cgs_default_opts": (None, [_P(SearchOpts)]),
     "cgs_grid_init": (ctypes.c_int, [_P(Grid), ctypes.c_int32, _P(ctypes.c_float), _P(ctypes.c_float),
                                      _P(ctypes.c_float)]),
@@ -134,6 +136,9 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.cgs_abi_version() != ABI_VERSION:
+            raise ImportError(f"{LIB_PATH} has ABI version {L.cgs_abi_version()}, this binding expects "
+                              f"{ABI_VERSION}: rebuild with __graft_entry__.build()")
         _lib = L
     return _lib
 

@@ -326,7 +326,9 @@ using namespace cgs;
 
 extern "C" {
 
-const char *cgs_version(void) { return "cgs 0.2.0 gfx950"; }
+const char *cgs_version(void) { return "cgs 0.3.0 gfx950"; }
+
+int32_t cgs_abi_version(void) { return CGS_ABI_VERSION; }
 
 void cgs_default_opts(cgs_search_opts *o) {
     if (!o) return;

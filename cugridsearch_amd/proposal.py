@@ -397,13 +397,75 @@ def main(argv=None):
     tr.add_argument("--seed", type=int, default=1)
     tr.add_argument("--workers", type=int, default=12)
     tr.add_argument("--out", required=True)
+    ev = sub.add_parser("evaluate", help="learned pipeline vs full grid on held-out pairs (GPU)")
+    ev.add_argument("--weights", required=True)
+    ev.add_argument("--pairs", type=int, default=1024)
+    ev.add_argument("--max-steps", type=int, default=8)
+    ev.add_argument("--threshold", type=float, default=None, help="default: the one saved with the weights")
+    ev.add_argument("--workers", type=int, default=12)
+    ev.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "label":
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         label(a.first, a.count, a.seconds, a.out, a.heldout, a.workers)
     elif a.cmd == "train":
         train_main(a)
+    elif a.cmd == "evaluate":
+        evaluate_main(a)
     return 0
+
+
+def evaluate(net, images, g, threshold, max_steps=8, judge=None, log=None):
+    """Agreement with the full-grid argmin and pairs/s of: the plain full
+    grid (the judge), the learned pipeline, the same refinement started from
+    the identity (no network), and the network's nearest grid point alone."""
+    n = len(images)
+    if judge is None:
+        judge, t_full = run_full_grid(images, g)
+    else:
+        t_full = None
+    run_learned(net, images[:min(8, n)], g, threshold, max_steps)        # first-call set-up outside the timing
+    res, t_learn, off = run_learned(net, images, g, threshold, max_steps)
+    abl, t_abl = run_without_network(images, g, threshold, max_steps)
+    count = g.getAxisSampleCounts().astype(int)
+    inc = np.cumprod(np.r_[1, count[:-1]])
+    cnn_idx = (centres_from_offsets(off, g) * inc).sum(1)
+    full_pts = g.numElements()
+
+    def summary(r, t):
+        return dict(agree=float((r["index"] == judge).mean()), seconds=t, pairs_per_s=n / t,
+                    local=float((r["outcome"] == 0).mean()), fallback_boundary=float((r["outcome"] == 1).mean()),
+                    fallback_threshold=float((r["outcome"] == 2).mean()), mean_steps=float(r["steps"].mean()),
+                    points_per_pair=float(r["points"].mean()), points_vs_full=float(r["points"].mean() / full_pts))
+
+    rep = dict(pairs=n, grid_points=full_pts, threshold=float(threshold), max_steps=max_steps,
+               learned=summary(res, t_learn), without_network=summary(abl, t_abl),
+               network_alone_agree=float((cnn_idx == judge).mean()))
+    if t_full is not None:
+        rep["full_grid"] = dict(seconds=t_full, pairs_per_s=n / t_full)
+        rep["speedup_learned_vs_full"] = t_full / t_learn
+        rep["speedup_without_network_vs_full"] = t_full / t_abl
+    if log:
+        log("[evaluate] " + str(rep))
+    return rep, res, judge
+
+
+def evaluate_main(a):
+    import json
+    from .cnn import ProposalNet
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    net = ProposalNet.load(a.weights, max_batch=256)
+    if a.threshold is not None:
+        thr = a.threshold
+    else:
+        with np.load(a.weights, allow_pickle=False) as z:
+            thr = float(z["threshold"]) if "threshold" in z else float("inf")
+    g = grid()
+    _, images = upload_pairs(range(a.pairs), heldout=True, workers=a.workers, keep_images=True, log=log)
+    rep, _, _ = evaluate(net, images, g, thr, a.max_steps, log=log)
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(rep, f, indent=1)
 
 
 def train_main(a):

@@ -21,6 +21,11 @@ extern "C" {
 
 #define CGS_MAX_DIM 8
 
+/* Layout version of the structures below; cgs_abi_version() returns the
+ * value the library was built with, so a caller compiled against another
+ * header can refuse to run instead of passing mis-sized structures. */
+#define CGS_ABI_VERSION 3
+
 typedef enum {
     CGS_OK = 0,
     CGS_ERR_INVALID_ARGUMENT = 1,   /* null pointer, bad dimension, bad enum */
@@ -114,6 +119,9 @@ typedef struct {
 
 /* Library version string. */
 const char *cgs_version(void);
+
+/* CGS_ABI_VERSION of the library's build. */
+int32_t cgs_abi_version(void);
 
 /* Fills the options with the defaults (SAD, bilinear, centred, lowest-index
  * ties, 32 MI bins). */

@@ -145,6 +145,8 @@ struct CudaGrid {
     float _start[D] = {}, _end[D] = {}, _count[D];
 
     CudaGrid() {
+        if (cgs_abi_version() != CGS_ABI_VERSION)
+            throw cgs_error(CGS_ERR_UNSUPPORTED, "libcgs was built with another cgs.h (ABI version mismatch)");
         for (unsigned d = 0; d < D; ++d) _count[d] = 1.0f;
         update();
     }
