@@ -30,6 +30,8 @@ FLOP_PER_PE = {("nearest", "sad"): 17 + 3, ("bilinear", "ncc"): 17 + 13 + 6, ("b
                ("bilinear", "sad"): 17 + 13 + 3}
 BATCH_WORKLOAD = "batch_ragged_256"
 PROPOSAL_WORKLOAD = "proposal_1024"
+CNN_WORKLOAD = "cnn_train_step"
+PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
 PROPOSAL_WEIGHTS = os.path.join(ROOT, "cugridsearch_amd", "weights", "proposal_c5.npz")
 
 
@@ -161,18 +163,17 @@ def run_proposal(args):
     from cugridsearch_amd import proposal as P
 
     n = args.pairs if args.pairs != 256 else 1024
-    with np.load(PROPOSAL_WEIGHTS, allow_pickle=False) as z:
-        threshold = float(z["threshold"])
+    threshold, modes = P.saved_settings(PROPOSAL_WEIGHTS)
     net = cnn.ProposalNet.load(PROPOSAL_WEIGHTS, max_batch=256)
     g = P.grid()
     _, images = P.upload_pairs(range(n), heldout=True, keep_images=True)
     judge, t_full = P.run_full_grid(images, g)
-    rep, _, _ = P.evaluate(net, images, g, threshold, judge=judge)
+    rep, _, _ = P.evaluate(net, images, g, threshold, judge=judge, modes=modes)
     for _ in range(max(0, args.warmup - 1)):
-        P.run_learned(net, images, g, threshold)
+        P.run_learned(net, images, g, threshold, modes=modes)
     times, agree, pts = [], [], []
     for _ in range(args.steps):
-        res, t, _ = P.run_learned(net, images, g, threshold)
+        res, t, _ = P.run_learned(net, images, g, threshold, modes=modes)
         times.append(t)
         agree.append(float((res["index"] == judge).mean()))
         pts.append(int(res["points"].sum()))
@@ -187,7 +188,8 @@ def run_proposal(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16 network / f32 search", "data": "synthetic",
         "config": {"workload": PROPOSAL_WORKLOAD, "pairs": n, "image": "256x256 float32", "grid_points": g.numElements(),
-                   "cost": "ncc", "local_box": "3^8", "threshold": threshold, "weights": os.path.basename(PROPOSAL_WEIGHTS)},
+                   "cost": "ncc", "local_box": "3^8", "starts": list(modes), "threshold": threshold,
+                   "weights": os.path.basename(PROPOSAL_WEIGHTS)},
         "argmin_agreement": float(np.mean(agree)),
         "full_grid_pairs_per_s": n / t_full,
         "speedup_vs_full_grid": t_full / dt,
@@ -207,6 +209,64 @@ def run_proposal(args):
     return line
 
 
+def cnn_flop_per_pair():
+    """Algorithmic FLOP of one forward pass (2 per multiply-add of the six
+    convolutions and the FC layer); a training step is 3x (forward, data
+    gradient, weight gradient)."""
+    from cugridsearch_amd import cnn
+    side, fl = cnn.SIDE, 0
+    for cin, cout, st in zip(cnn.CIN, cnn.COUT, cnn.STRIDE):
+        side //= st
+        fl += 2 * side * side * cout * 9 * cin
+    return fl + 2 * cnn.FEAT * cnn.NOUT
+
+
+def run_cnn(args):
+    """One training step of the proposal network (forward, backward, Adam) on
+    a batch of synthetic inputs resident in HBM."""
+    import torch
+    from cugridsearch_amd import cnn
+    B = 128
+    g = torch.Generator(device="cpu")
+    g.manual_seed(0)
+    x = (torch.randn((B, cnn.SIDE, cnn.SIDE, 2), generator=g).to(torch.bfloat16).view(torch.int16)).cuda()
+    t = (0.5 * torch.randn((B, 8), generator=g)).cuda()
+    net = cnn.ProposalNet(max_batch=B, seed=1)
+    stream = torch.cuda.Stream()
+
+    def step():
+        net.forward(x, stream=stream)
+        net.backward(t, stream=stream, want_loss=False)
+        net.adam_step(1e-4, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    gpu_ms = e0.elapsed_time(e1) / args.steps
+    flop = 3 * cnn_flop_per_pair() * B
+    achieved = flop / (gpu_ms / 1000.0) / 1e12
+    return {
+        "metric": METRIC, "value": B / dt, "unit": "training pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1000.0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+        "config": {"workload": CNN_WORKLOAD, "batch": B, "input": "2x128x128", "convs": "32-32-64-64-128-128",
+                   "flop_per_step": flop},
+        "gpu_ms_per_step": gpu_ms,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_BF16_MFMA_TFLOPS, "traffic": None,
+                     "note": "whole training step (all kernels) on its stream"},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -216,6 +276,9 @@ def main():
         sys.exit(2)
     if args.workload == BATCH_WORKLOAD:
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.workload == CNN_WORKLOAD:
+        print(json.dumps(run_cnn(args)), flush=True)
         return
     if args.workload == PROPOSAL_WORKLOAD:
         print(json.dumps(run_proposal(args)), flush=True)

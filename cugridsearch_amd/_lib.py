@@ -19,7 +19,7 @@ PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION = range(4)
 TIE_LOWEST, TIE_REFERENCE = 0, 1
 PIX_U8, PIX_F32 = 0, 1
 MAX_DIM = 8
-ABI_VERSION = 3          # CGS_ABI_VERSION of include/cgs.h these layouts follow
+ABI_VERSION = 4          # CGS_ABI_VERSION of include/cgs.h these layouts follow
 FLAG_TIME_KERNEL = 1
 
 
@@ -62,7 +62,7 @@ class TileConfig(ctypes.Structure):
 
 class RefineOpts(ctypes.Structure):
     _fields_ = [("radius", ctypes.c_int32), ("max_steps", ctypes.c_int32), ("threshold", ctypes.c_float),
-                ("fallback", ctypes.c_int32)]
+                ("fallback", ctypes.c_int32), ("starts", ctypes.c_int32)]
 
 
 class RefineResult(ctypes.Structure):

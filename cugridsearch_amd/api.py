@@ -322,16 +322,21 @@ class BatchSearcher:
 
     def refine(self, pairs, centres, radius=1, max_steps=8, threshold=float("inf"), fallback=True,
                tie_mode=L.TIE_LOWEST, stream=None):
-        """Local refinement around per-pair centres (sample indices, n x dim)
-        with full-grid fallback (cgs_batch_refine).  pairs: (functor, grid,
-        moved, fixed, ignored).  Returns a numpy record array with fields
-        index, value, steps, outcome, local_index, local_value, points.
-        Drops any plan the batch held."""
+        """Local refinement around per-pair centres with full-grid fallback
+        (cgs_batch_refine).  pairs: (functor, grid, moved, fixed, ignored);
+        centres: n x dim sample indices, or n x starts x dim for several
+        descents per pair.  Returns a numpy record array with fields index,
+        value, steps, outcome, local_index, local_value, points.  Drops any
+        plan the batch held."""
         arr, keep = self._pair_array(pairs, tie_mode)
         c = np.ascontiguousarray(centres, np.int32)
-        if c.shape != (len(pairs), pairs[0][1].getDimension()):
-            raise ValueError("centres must be n x grid dimension")
-        o = L.RefineOpts(int(radius), int(max_steps), float(threshold), 1 if fallback else 0)
+        dim = pairs[0][1].getDimension()
+        if c.ndim == 2:
+            c = c[:, None, :]
+        if c.ndim != 3 or c.shape[0] != len(pairs) or c.shape[2] != dim:
+            raise ValueError("centres must be n x dim or n x starts x dim")
+        c = np.ascontiguousarray(c)
+        o = L.RefineOpts(int(radius), int(max_steps), float(threshold), 1 if fallback else 0, int(c.shape[1]))
         out = (L.RefineResult * len(pairs))()
         L.check(L.lib().cgs_batch_refine(self._h, arr, len(pairs), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                          ctypes.byref(o), out, _stream_ptr(stream)))

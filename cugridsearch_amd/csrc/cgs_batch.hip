@@ -273,8 +273,9 @@ cgs_status enqueue_staging(cgs_batch *b, hipStream_t s) {
     return CGS_OK;
 }
 
-// Descriptor of pair i searching g (the pair's grid or a box of it) into key slot `slot`.
-BatchDesc describe(const cgs_batch *b, int i, const cgs_grid &g, int slot, float *result) {
+// Descriptor of pair i searching g (the pair's grid or a box of it); its
+// key slot is its position in the vector handed to set_descs.
+BatchDesc describe(const cgs_batch *b, int i, const cgs_grid &g, float *result) {
     const cgs_batch_pair &p = b->pairs[i];
     BatchDesc D;
     std::memset(&D, 0, sizeof(D));
@@ -293,7 +294,7 @@ BatchDesc describe(const cgs_batch *b, int i, const cgs_grid &g, int slot, float
     D.P.cx = (float) p.moved.width / 2;
     D.P.cy = (float) p.moved.height / 2;
     D.P.result = result;
-    D.P.key = (unsigned long long *) b->keys.ptr + slot;
+    D.P.key = nullptr;   // set by set_descs once the key buffer is sized
     D.P.tie_mode = p.opts.tie_mode;
     D.P.bins = p.opts.mi_bins;
     D.kind = batch_kind(p.opts.cost);
@@ -303,10 +304,18 @@ BatchDesc describe(const cgs_batch *b, int i, const cgs_grid &g, int slot, float
 // Groups the slot-ordered descriptors by kind (slot order kept inside a
 // kind), numbers their work items and uploads them.
 cgs_status set_descs(cgs_batch *b, std::vector<BatchDesc> &by_slot, const std::vector<int> &slot_pair) {
+    // every slot has its own descriptor and key
+    const size_t slots = std::max<size_t>(1, by_slot.size());
+    cgs_status st;
+    if ((st = b->descs.ensure(sizeof(BatchDesc) * slots)) != CGS_OK ||
+        (st = b->keys.ensure(sizeof(unsigned long long) * slots)) != CGS_OK ||
+        (st = b->keys_host.ensure(sizeof(unsigned long long) * slots, true)) != CGS_OK)
+        return st;
     uint64_t kitems[kBatchKinds] = {};
     int64_t points = 0, pe = 0;
     for (size_t k = 0; k < by_slot.size(); ++k) {
         BatchDesc &D = by_slot[k];
+        D.P.key = (unsigned long long *) b->keys.ptr + k;
         const int gpt = D.kind == kBatchMI ? b->tiles.mi_gpt : b->tiles.fast_gpt;
         D.item_begin = (uint32_t) kitems[D.kind];
         kitems[D.kind] += (uint64_t) ((D.P.grid.n + gpt - 1) / gpt);
@@ -385,7 +394,7 @@ cgs_status cgs_batch_plan(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, 
     std::vector<BatchDesc> by_slot(n);
     std::vector<int> slot_pair(n);
     for (int32_t i = 0; i < n; ++i) {
-        by_slot[i] = describe(b, i, pairs[i].grid, i, pairs[i].result);   // keys in the caller's pair order
+        by_slot[i] = describe(b, i, pairs[i].grid, pairs[i].result);   // keys in the caller's pair order
         slot_pair[i] = i;
     }
     return set_descs(b, by_slot, slot_pair);
@@ -434,55 +443,89 @@ cgs_status cgs_batch_info(const cgs_batch *b, int64_t *items, int64_t *points, i
     return CGS_OK;
 }
 
-// Local refinement: every pair searches a (2r+1)^D box of its grid around
-// its centre; a pair whose box minimum lies on a box face that is not a
-// grid face re-centres there and searches again, up to max_steps boxes.
-// All pairs still moving share one batched launch per step; the staging
-// (tap tables, float images) is done once.
+// Local refinement: every (pair, start) descent searches a (2r+1)^D box of
+// the pair's grid around its centre; a descent whose box minimum lies on a
+// box face that is not a grid face re-centres there and searches again, up
+// to max_steps boxes.  A descent that reaches a centre another start of the
+// same pair already holds stops (it would repeat that start's boxes).  Each
+// pair keeps the lowest local minimum over its starts (lower index on
+// ties).  All descents still moving share one batched launch per step; the
+// staging (tap tables, float images) is done once.
 cgs_status cgs_batch_refine(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, const int32_t *centres,
                             const cgs_refine_opts *ro, cgs_refine_result *out, void *stream) {
     clear_error();
     if (!b || !pairs || n <= 0 || !centres || !ro || !out)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_refine: bad arguments");
-    if (ro->radius < 1 || ro->radius > 16 || ro->max_steps < 1 || ro->max_steps > 1024 || std::isnan(ro->threshold))
-        return set_error(CGS_ERR_INVALID_ARGUMENT, "batch_refine: radius 1..16, max_steps 1..1024, threshold not NaN");
+    if (ro->radius < 1 || ro->radius > 16 || ro->max_steps < 1 || ro->max_steps > 1024 || std::isnan(ro->threshold) ||
+        ro->starts < 1 || ro->starts > 64)
+        return set_error(CGS_ERR_INVALID_ARGUMENT,
+                         "batch_refine: radius 1..16, max_steps 1..1024, starts 1..64, threshold not NaN");
     hipStream_t s = (hipStream_t) stream;
     cgs_tile_config t;
     cgs_status st = parse_tiles(nullptr, &t);
     if (st != CGS_OK) return st;
     if ((st = stage_alloc(b, pairs, n, t)) != CGS_OK) return st;
     if ((st = enqueue_staging(b, s)) != CGS_OK) return st;
-    struct State {
+    const int S = ro->starts;
+    enum { kMoving, kLocalMin, kOnFace, kMerged };
+    struct Descent {
         int32_t c[CGS_MAX_DIM];
-        bool moving;
+        int state;
+        int32_t index;      // grid index of its last box minimum
+        float value;
     };
-    std::vector<State> S(n);
+    std::vector<Descent> D((size_t) n * S);
     for (int32_t i = 0; i < n; ++i) {
         const cgs_grid &g = pairs[i].grid;
-        for (int a = 0; a < g.dim; ++a)
-            S[i].c[a] = std::min(std::max(centres[(size_t) i * g.dim + a], 0), (int32_t) g.count[a] - 1);
-        S[i].moving = true;
+        for (int k = 0; k < S; ++k) {
+            Descent &d = D[(size_t) i * S + k];
+            for (int a = 0; a < g.dim; ++a)
+                d.c[a] = std::min(std::max(centres[((size_t) i * S + k) * g.dim + a], 0), (int32_t) g.count[a] - 1);
+            d.state = kMoving;
+            d.index = -1;
+            d.value = INFINITY;
+        }
         out[i] = cgs_refine_result{-1, INFINITY, 0, CGS_REFINE_LOCAL, -1, INFINITY, 0};
     }
+    auto same_centre = [](const Descent &x, const Descent &y, int dim) {
+        for (int a = 0; a < dim; ++a)
+            if (x.c[a] != y.c[a]) return false;
+        return true;
+    };
     std::vector<BatchDesc> descs;
-    std::vector<int> slot_pair;
-    std::vector<cgs_grid> boxes(n);
+    std::vector<int> slot_pair, slot_descent;
+    std::vector<cgs_grid> boxes((size_t) n * S);
     std::vector<cgs_argmin> r;
     for (int step = 1; step <= ro->max_steps; ++step) {
         descs.clear();
         slot_pair.clear();
+        slot_descent.clear();
         for (int32_t i = 0; i < n; ++i) {
-            if (!S[i].moving) continue;
             const cgs_grid &g = pairs[i].grid;
-            int32_t lo[CGS_MAX_DIM], cnt[CGS_MAX_DIM];
-            for (int a = 0; a < g.dim; ++a) {
-                lo[a] = std::max(0, S[i].c[a] - ro->radius);
-                const int32_t hi = std::min((int32_t) g.count[a] - 1, S[i].c[a] + ro->radius);
-                cnt[a] = hi - lo[a] + 1;
+            for (int k = 0; k < S; ++k) {
+                Descent &d = D[(size_t) i * S + k];
+                if (d.state != kMoving) continue;
+                bool dup = false;
+                for (int q = 0; q < k && !dup; ++q) {
+                    const Descent &e = D[(size_t) i * S + q];
+                    dup = (e.state == kMoving || e.state == kLocalMin) && same_centre(d, e, g.dim);
+                }
+                if (dup) {
+                    d.state = kMerged;
+                    continue;
+                }
+                int32_t lo[CGS_MAX_DIM], cnt[CGS_MAX_DIM];
+                for (int a = 0; a < g.dim; ++a) {
+                    lo[a] = std::max(0, d.c[a] - ro->radius);
+                    const int32_t hi = std::min((int32_t) g.count[a] - 1, d.c[a] + ro->radius);
+                    cnt[a] = hi - lo[a] + 1;
+                }
+                cgs_grid &box = boxes[(size_t) i * S + k];
+                if ((st = cgs_grid_subbox(&g, lo, cnt, &box)) != CGS_OK) return st;
+                descs.push_back(describe(b, i, box, nullptr));
+                slot_pair.push_back(i);
+                slot_descent.push_back(i * S + k);
             }
-            if ((st = cgs_grid_subbox(&g, lo, cnt, &boxes[i])) != CGS_OK) return st;
-            descs.push_back(describe(b, i, boxes[i], (int) descs.size(), nullptr));
-            slot_pair.push_back(i);
         }
         if (descs.empty()) break;
         if ((st = set_descs(b, descs, slot_pair)) != CGS_OK) return st;
@@ -490,13 +533,12 @@ cgs_status cgs_batch_refine(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n
         if ((st = read_keys(b, s, r)) != CGS_OK) return st;
         for (size_t k = 0; k < r.size(); ++k) {
             const int32_t i = slot_pair[k];
-            const cgs_grid &g = pairs[i].grid, &box = boxes[i];
-            cgs_refine_result &o = out[i];
-            o.steps = step;
-            o.points += box.num_elements;
+            Descent &d = D[slot_descent[k]];
+            const cgs_grid &g = pairs[i].grid, &box = boxes[slot_descent[k]];
+            out[i].steps += 1;
+            out[i].points += box.num_elements;
             if (r[k].index < 0) {   // no finite cost in the box
-                S[i].moving = false;
-                o.outcome = CGS_REFINE_FALLBACK_BOUNDARY;
+                d.state = kOnFace;
                 continue;
             }
             int32_t rem = r[k].index, full = 0;
@@ -507,28 +549,46 @@ cgs_status cgs_batch_refine(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n
                 const int32_t lo = box.offset[a] - g.offset[a], c = lo + j;
                 const int32_t hi = lo + (int32_t) box.count[a] - 1;
                 if ((c == lo && lo > 0) || (c == hi && hi < (int32_t) g.count[a] - 1)) face = true;
-                S[i].c[a] = c;
+                d.c[a] = c;
                 full += c * g.increments[a];
             }
-            o.local_index = full;
-            o.local_value = r[k].value;
-            if (!face) {
-                S[i].moving = false;
-                o.outcome = r[k].value <= ro->threshold ? CGS_REFINE_LOCAL : CGS_REFINE_FALLBACK_THRESHOLD;
-            } else if (step == ro->max_steps) {
-                S[i].moving = false;
-                o.outcome = CGS_REFINE_FALLBACK_BOUNDARY;
-            }
+            d.index = full;
+            d.value = r[k].value;
+            if (!face) d.state = kLocalMin;
+            else if (step == ro->max_steps) d.state = kOnFace;
         }
     }
     descs.clear();
     slot_pair.clear();
     for (int32_t i = 0; i < n; ++i) {
-        if (out[i].outcome == CGS_REFINE_LOCAL) {
-            out[i].index = out[i].local_index;
-            out[i].value = out[i].local_value;
+        cgs_refine_result &o = out[i];
+        bool any_local = false;
+        for (int k = 0; k < S; ++k) {
+            const Descent &d = D[(size_t) i * S + k];
+            if (d.state != kLocalMin) continue;
+            if (!any_local || d.value < o.local_value || (d.value == o.local_value && d.index < o.local_index)) {
+                o.local_value = d.value;
+                o.local_index = d.index;
+            }
+            any_local = true;
+        }
+        if (!any_local) {
+            o.outcome = CGS_REFINE_FALLBACK_BOUNDARY;
+            for (int k = 0; k < S; ++k) {   // report the best box minimum seen
+                const Descent &d = D[(size_t) i * S + k];
+                if (d.index >= 0 && (o.local_index < 0 || d.value < o.local_value)) {
+                    o.local_value = d.value;
+                    o.local_index = d.index;
+                }
+            }
+        } else {
+            o.outcome = o.local_value <= ro->threshold ? CGS_REFINE_LOCAL : CGS_REFINE_FALLBACK_THRESHOLD;
+        }
+        if (o.outcome == CGS_REFINE_LOCAL) {
+            o.index = o.local_index;
+            o.value = o.local_value;
         } else if (ro->fallback) {
-            descs.push_back(describe(b, i, pairs[i].grid, (int) descs.size(), nullptr));
+            descs.push_back(describe(b, i, pairs[i].grid, nullptr));
             slot_pair.push_back(i);
         }
     }

@@ -322,22 +322,40 @@ __global__ void reduce_slices_kernel(const float *part, int slices, int cout, in
     grad[i] = s;
 }
 
-// Bias gradient: column sums of dz [P][C], first per block of rows, then in block order.
-__global__ void colsum_partial_kernel(const uint16_t *dz, int P, int C, int rows_per_block, float *part) {
-    const int c = threadIdx.x;
-    if (c >= C) return;
-    const int r0 = blockIdx.x * rows_per_block, r1 = min(P, r0 + rows_per_block);
+// Bias gradient: column sums of dz [P][C] (C divides 256).  Block b sums
+// rows [b*R, b*R + R): its 256 threads are 256/C row lanes x C channels, so
+// each load instruction reads 512 contiguous bytes; the row lanes are then
+// added in LDS in a fixed order.  A second kernel adds the blocks' partial
+// sums, one workgroup per channel, in a fixed tree order (deterministic).
+constexpr int kColRows = 1024;
+
+__global__ __launch_bounds__(256) void colsum_partial_kernel(const uint16_t *dz, int P, int C, float *part) {
+    __shared__ float red[256];
+    const int tid = threadIdx.x, c = tid % C, lane_row = tid / C, nr = 256 / C;
+    const int r0 = blockIdx.x * kColRows, r1 = min(P, r0 + kColRows);
     float s = 0.0f;
-    for (int r = r0; r < r1; ++r) s += bf2f(dz[(size_t) r * C + c]);
-    part[(size_t) blockIdx.x * C + c] = s;
+    for (int r = r0 + lane_row; r < r1; r += nr) s += bf2f(dz[(size_t) r * C + c]);
+    red[tid] = s;
+    __syncthreads();
+    if (tid < C) {
+        float t = 0.0f;
+        for (int k = 0; k < nr; ++k) t += red[k * C + tid];
+        part[(size_t) blockIdx.x * C + tid] = t;
+    }
 }
 
-__global__ void colsum_final_kernel(const float *part, int blocks, int C, float *out) {
-    const int c = threadIdx.x;
-    if (c >= C) return;
+__global__ __launch_bounds__(256) void colsum_final_kernel(const float *part, int blocks, int C, float *out) {
+    __shared__ float red[256];
+    const int c = blockIdx.x, tid = threadIdx.x;
     float s = 0.0f;
-    for (int b = 0; b < blocks; ++b) s += part[(size_t) b * C + c];
-    out[c] = s;
+    for (int b = tid; b < blocks; b += 256) s += part[(size_t) b * C + c];
+    red[tid] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) out[c] = red[0];
 }
 
 // ------------------------------------------------------------- small passes
@@ -533,8 +551,6 @@ struct cgs_cnn_s {
 
 namespace {
 
-constexpr int kColBlocks = 64;
-
 // split-K slices of the weight gradient: about 2 waves of workgroups per CU
 int wgrad_slices(int l, int n, int cus, int tiles) {
     const int P = n * hout(l) * hout(l);
@@ -586,10 +602,10 @@ hipError_t launch_weight(const uint16_t *dz, const uint16_t *x, float *part, flo
     const int cnt = kCout[L] * ktrue(L);
     hipLaunchKernelGGL(reduce_slices_kernel, dim3((cnt + 255) / 256), dim3(256), 0, s, (const float *) part, z,
                        kCout[L], Kp, ktrue(L), grad);
-    const int rows = (P + kColBlocks - 1) / kColBlocks;
-    hipLaunchKernelGGL(colsum_partial_kernel, dim3(kColBlocks), dim3(kCout[L]), 0, s, dz, P, kCout[L], rows, colpart);
-    hipLaunchKernelGGL(colsum_final_kernel, dim3(1), dim3(kCout[L]), 0, s, (const float *) colpart, kColBlocks,
-                       kCout[L], gbias);
+    const int blocks = (P + kColRows - 1) / kColRows;
+    hipLaunchKernelGGL(colsum_partial_kernel, dim3(blocks), dim3(256), 0, s, dz, P, kCout[L], colpart);
+    hipLaunchKernelGGL(colsum_final_kernel, dim3(kCout[L]), dim3(256), 0, s, (const float *) colpart, blocks, kCout[L],
+                       gbias);
     return hipGetLastError();
 }
 
@@ -655,7 +671,10 @@ cgs_status cgs_cnn_create(int32_t max_batch, cgs_cnn **out) {
     }
     need(c->dz[0], dzmax * 2); need(c->dz[1], dzmax * 2);
     need(c->part, part_floats(max_batch, c->cus) * 4);
-    need(c->colpart, (size_t) kColBlocks * 128 * 4);
+    size_t colmax = 0;
+    for (int l = 0; l < kLayers; ++l)
+        colmax = std::max(colmax, (size_t) ((max_batch * hout(l) * hout(l) + kColRows - 1) / kColRows) * kCout[l]);
+    need(c->colpart, colmax * 4);
     need(c->feat, (size_t) max_batch * kFeat * 4);
     need(c->out, (size_t) max_batch * kOut * 4);
     need(c->dout, (size_t) max_batch * kOut * 4);

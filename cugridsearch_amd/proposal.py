@@ -309,6 +309,38 @@ def centres_from_offsets(off, g, w=SIZE, h=SIZE):
     return np.stack([nearest_centre(offsets_to_params(o, w, h), start, res, count) for o in off])
 
 
+def lattice_coords(off, g, w=SIZE, h=SIZE):
+    """Continuous per-axis sample coordinates (p - start) / res of the
+    proposals, [n, dim]."""
+    start, res = g.start_point().astype(np.float64), g.resolution().astype(np.float64)
+    return np.stack([(offsets_to_params(o, w, h) - start) / np.where(res == 0, 1, res) for o in off])
+
+
+START_MODES = ("nearest", "opposite", "identity")
+
+
+def make_centres(off, g, modes=("nearest", "opposite", "identity"), w=SIZE, h=SIZE):
+    """Descent starts per pair, [n, len(modes), dim]: "nearest" = the grid
+    point nearest the proposal; "opposite" = the far corner of the lattice
+    cell holding the proposal (floor and ceil swapped on every axis);
+    "identity" = the grid point nearest the identity homography."""
+    count = g.getAxisSampleCounts().astype(np.int64)
+    u = lattice_coords(off, g, w, h)
+    near = np.clip(np.rint(u), 0, count - 1).astype(np.int32)
+    out = []
+    for m in modes:
+        if m == "nearest":
+            out.append(near)
+        elif m == "opposite":
+            other = np.where(np.rint(u) > u, np.floor(u), np.ceil(u))
+            out.append(np.clip(other, 0, count - 1).astype(np.int32))
+        elif m == "identity":
+            out.append(identity_centres(len(off), g))
+        else:
+            raise ValueError(f"unknown start mode {m!r}")
+    return np.stack(out, axis=1)
+
+
 def identity_centres(n, g):
     """Centres without a network: the grid point nearest the identity."""
     c = nearest_centre([0, 1, 1, 0, 0, 0, 0, 0], g.start_point(), g.resolution(), g.getAxisSampleCounts())
@@ -316,6 +348,7 @@ def identity_centres(n, g):
 
 
 def refine(images, g, centres, threshold=float("inf"), max_steps=8, fallback=True):
+    """Batched refinement of all pairs; centres [n, dim] or [n, starts, dim]."""
     from .api import BatchSearcher, calcNCC
     b = BatchSearcher()
     return b.refine([(calcNCC, g, m, f, None) for m, f in images], centres, radius=1, max_steps=max_steps,
@@ -337,10 +370,10 @@ def calibrate_threshold(values, wrong, max_wrong_rate):
     return float(v[k])
 
 
-def run_learned(net, images, g, threshold, max_steps=8, batch=256):
-    """The learned pipeline on resident pairs: inputs, network, centres,
-    local refinement with full-grid fallback.  Returns (results, seconds,
-    offsets)."""
+def run_learned(net, images, g, threshold, max_steps=8, batch=256, modes=("nearest",)):
+    """The learned pipeline on resident pairs: inputs, network, descent
+    starts (make_centres modes), local refinement with full-grid fallback.
+    Returns (results, seconds, offsets)."""
     import torch
     from .cnn import ProposalNet
     torch.cuda.synchronize()
@@ -350,18 +383,34 @@ def run_learned(net, images, g, threshold, max_steps=8, batch=256):
         part = images[b:b + batch]
         ProposalNet.prepare([m for m, _ in part], [f for _, f in part], out=X[b:b + len(part)])
     off = predict_offsets(net, X, batch)
-    centres = centres_from_offsets(off, g)
-    res = refine(images, g, centres, threshold, max_steps)
+    res = refine(images, g, make_centres(off, g, modes), threshold, max_steps)
     torch.cuda.synchronize()
     return res, time.perf_counter() - t0, off
 
 
-def run_without_network(images, g, threshold, max_steps=8):
-    """Ablation: the same refinement and fallback started from the identity."""
+def fixed_centres(n, g, starts):
+    """Data-independent starts: the corners of the lattice cell holding the
+    identity homography -- its nearest grid point, the opposite corner, then
+    corners alternating per axis -- [n, starts, dim]."""
+    count = g.getAxisSampleCounts().astype(np.int64)
+    start, res = g.start_point().astype(np.float64), g.resolution().astype(np.float64)
+    u = (np.array([0, 1, 1, 0, 0, 0, 0, 0], np.float64)[:len(count)] - start) / np.where(res == 0, 1, res)
+    near, far = np.rint(u), np.where(np.rint(u) > u, np.floor(u), np.ceil(u))
+    cands = [near, far]
+    for k in range(starts):
+        mask = ((np.arange(len(count)) >> k) & 1).astype(bool)
+        cands.append(np.where(mask, far, near))
+    out = [np.clip(c, 0, count - 1).astype(np.int32) for c in cands[:starts]]
+    return np.tile(np.stack(out)[None], (n, 1, 1))
+
+
+def run_without_network(images, g, threshold, max_steps=8, starts=1):
+    """Ablation: the same refinement and fallback from as many fixed starts
+    (fixed_centres) instead of the network's."""
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = refine(images, g, identity_centres(len(images), g), threshold, max_steps)
+    res = refine(images, g, fixed_centres(len(images), g, starts), threshold, max_steps)
     torch.cuda.synchronize()
     return res, time.perf_counter() - t0
 
@@ -393,7 +442,8 @@ def main(argv=None):
     tr.add_argument("--batch", type=int, default=128)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--val", type=int, default=1024, help="labelled pairs held back for validation / threshold")
-    tr.add_argument("--max-wrong", type=float, default=0.004)
+    tr.add_argument("--max-wrong", type=float, default=0.001)
+    tr.add_argument("--modes", default="nearest,opposite", help="descent starts (make_centres modes)")
     tr.add_argument("--seed", type=int, default=1)
     tr.add_argument("--workers", type=int, default=12)
     tr.add_argument("--out", required=True)
@@ -402,6 +452,7 @@ def main(argv=None):
     ev.add_argument("--pairs", type=int, default=1024)
     ev.add_argument("--max-steps", type=int, default=8)
     ev.add_argument("--threshold", type=float, default=None, help="default: the one saved with the weights")
+    ev.add_argument("--modes", default=None, help="descent starts; default: the ones saved with the weights")
     ev.add_argument("--workers", type=int, default=12)
     ev.add_argument("--out", default=None)
     a = ap.parse_args(argv)
@@ -415,18 +466,18 @@ def main(argv=None):
     return 0
 
 
-def evaluate(net, images, g, threshold, max_steps=8, judge=None, log=None):
+def evaluate(net, images, g, threshold, max_steps=8, judge=None, log=None, modes=("nearest",)):
     """Agreement with the full-grid argmin and pairs/s of: the plain full
-    grid (the judge), the learned pipeline, the same refinement started from
-    the identity (no network), and the network's nearest grid point alone."""
+    grid (the judge), the learned pipeline, the same refinement from as many
+    fixed starts (no network), and the network's nearest grid point alone."""
     n = len(images)
     if judge is None:
         judge, t_full = run_full_grid(images, g)
     else:
         t_full = None
-    run_learned(net, images[:min(8, n)], g, threshold, max_steps)        # first-call set-up outside the timing
-    res, t_learn, off = run_learned(net, images, g, threshold, max_steps)
-    abl, t_abl = run_without_network(images, g, threshold, max_steps)
+    run_learned(net, images[:min(8, n)], g, threshold, max_steps, modes=modes)   # first-call set-up untimed
+    res, t_learn, off = run_learned(net, images, g, threshold, max_steps, modes=modes)
+    abl, t_abl = run_without_network(images, g, threshold, max_steps, starts=len(modes))
     count = g.getAxisSampleCounts().astype(int)
     inc = np.cumprod(np.r_[1, count[:-1]])
     cnn_idx = (centres_from_offsets(off, g) * inc).sum(1)
@@ -438,7 +489,7 @@ def evaluate(net, images, g, threshold, max_steps=8, judge=None, log=None):
                     fallback_threshold=float((r["outcome"] == 2).mean()), mean_steps=float(r["steps"].mean()),
                     points_per_pair=float(r["points"].mean()), points_vs_full=float(r["points"].mean() / full_pts))
 
-    rep = dict(pairs=n, grid_points=full_pts, threshold=float(threshold), max_steps=max_steps,
+    rep = dict(pairs=n, grid_points=full_pts, threshold=float(threshold), max_steps=max_steps, starts=list(modes),
                learned=summary(res, t_learn), without_network=summary(abl, t_abl),
                network_alone_agree=float((cnn_idx == judge).mean()))
     if t_full is not None:
@@ -450,19 +501,27 @@ def evaluate(net, images, g, threshold, max_steps=8, judge=None, log=None):
     return rep, res, judge
 
 
+def saved_settings(path):
+    """(threshold, descent-start modes) stored with trained weights."""
+    with np.load(path, allow_pickle=False) as z:
+        thr = float(z["threshold"]) if "threshold" in z else float("inf")
+        modes = tuple(str(z["modes"]).split(",")) if "modes" in z else ("nearest",)
+    return thr, modes
+
+
 def evaluate_main(a):
     import json
     from .cnn import ProposalNet
     log = lambda m: print(m, flush=True)  # noqa: E731
     net = ProposalNet.load(a.weights, max_batch=256)
+    thr, modes = saved_settings(a.weights)
     if a.threshold is not None:
         thr = a.threshold
-    else:
-        with np.load(a.weights, allow_pickle=False) as z:
-            thr = float(z["threshold"]) if "threshold" in z else float("inf")
+    if a.modes is not None:
+        modes = tuple(a.modes.split(","))
     g = grid()
     _, images = upload_pairs(range(a.pairs), heldout=True, workers=a.workers, keep_images=True, log=log)
-    rep, _, _ = evaluate(net, images, g, thr, a.max_steps, log=log)
+    rep, _, _ = evaluate(net, images, g, thr, a.max_steps, log=log, modes=modes)
     if a.out:
         with open(a.out, "w") as f:
             json.dump(rep, f, indent=1)
@@ -489,16 +548,18 @@ def train_main(a):
     # from the full-grid label are the errors a threshold has to catch
     del X, Xtr, Ytr
     _, vimages = upload_pairs(ks[:nval], workers=a.workers, keep_images=True)
-    vres, vsec, _ = run_learned(net, vimages, g, threshold=float("inf"))
+    modes = tuple(a.modes.split(","))
+    vres, vsec, _ = run_learned(net, vimages, g, threshold=float("inf"), modes=modes)
     local = vres["outcome"] == 0
     wrong = vres["local_index"] != idx[:nval]
     thr = calibrate_threshold(vres["local_value"][local], wrong[local], a.max_wrong)
     meta = dict(train_pairs=int(len(ks) - nval), val_pairs=int(nval), epochs_run=len(hist),
                 history=[list(map(float, h)) for h in hist], threshold=thr,
                 val_local_rate=float(local.mean()), val_local_wrong=float((local & wrong).mean()),
-                val_agree_inf=float((vres["index"] == idx[:nval]).mean()), labels=[os.path.basename(p) for p in a.labels])
+                val_agree_inf=float((vres["index"] == idx[:nval]).mean()), labels=[os.path.basename(p) for p in a.labels],
+                modes=list(modes), val_seconds=vsec)
     log("[train] " + json.dumps({k: v for k, v in meta.items() if k != "history"}))
-    net.save(a.out, threshold=np.float32(thr), meta=json.dumps(meta))
+    net.save(a.out, threshold=np.float32(thr), modes=",".join(modes), meta=json.dumps(meta))
     log(f"[train] saved {a.out}")
 
 

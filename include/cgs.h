@@ -24,7 +24,7 @@ extern "C" {
 /* Layout version of the structures below; cgs_abi_version() returns the
  * value the library was built with, so a caller compiled against another
  * header can refuse to run instead of passing mis-sized structures. */
-#define CGS_ABI_VERSION 3
+#define CGS_ABI_VERSION 4
 
 typedef enum {
     CGS_OK = 0,
@@ -229,13 +229,18 @@ cgs_status cgs_batch_info(const cgs_batch *batch, int64_t *items, int64_t *point
  * fallback != 0 each pair that falls back is searched over its whole grid
  * in the same call, so its answer equals cgs_search's.  Box points are the
  * grid's own points (cgs_grid_subbox), so every cost is bit-identical to
- * the full search's cost of that point.  All pairs still moving share one
+ * the full search's cost of that point.  With starts > 1 every pair runs
+ * that many descents from its own centres and keeps the lowest local
+ * minimum (lower index on ties); a descent that reaches a centre an earlier
+ * start of the pair holds stops there.  It falls back (BOUNDARY) only when
+ * no descent found a local minimum.  All descents still moving share one
  * batched launch per step. */
 typedef struct {
     int32_t radius;        /* 1..16 */
-    int32_t max_steps;     /* boxes per pair before a boundary fallback: 1..1024 */
+    int32_t max_steps;     /* boxes per descent before it counts as stuck on a face: 1..1024 */
     float threshold;       /* largest accepted local-minimum cost (+inf: any); not NaN */
     int32_t fallback;      /* != 0: run the full-grid searches of the pairs that fall back */
+    int32_t starts;        /* descents per pair, each from its own centre: 1..64 */
 } cgs_refine_opts;
 
 typedef enum {
@@ -247,7 +252,7 @@ typedef enum {
 typedef struct {
     int32_t index;          /* grid index of the answer (-1: none, when a fallback was not run) */
     float value;            /* its cost */
-    int32_t steps;          /* boxes searched */
+    int32_t steps;          /* boxes searched, over all starts */
     int32_t outcome;        /* cgs_refine_outcome */
     int32_t local_index;    /* grid index of the last box minimum */
     float local_value;
@@ -255,7 +260,7 @@ typedef struct {
 } cgs_refine_result;
 
 /* pairs: as for cgs_batch_plan (result pointers are ignored); centres:
- * n x grid.dim sample indices (clamped to the grid); out: n results.
+ * n x starts x grid.dim sample indices (clamped to the grid); out: n results.
  * Replaces any plan the batch held.  Synchronous. */
 cgs_status cgs_batch_refine(cgs_batch *batch, const cgs_batch_pair *pairs, int32_t n, const int32_t *centres,
                             const cgs_refine_opts *opts, cgs_refine_result *out, void *stream);

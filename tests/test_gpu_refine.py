@@ -124,3 +124,25 @@ def test_refine_rejects_bad_options(gpu):
     for kw in (dict(radius=0), dict(max_steps=0), dict(threshold=float("nan"))):
         with pytest.raises(L.CgsError):
             b.refine([(fn, g, m, f, None)], np.zeros((1, 8), np.int32), **kw)
+
+
+def test_refine_multi_start_keeps_the_best(gpu):
+    """Several starts per pair answer the lowest of the local minima the
+    same starts reach one at a time (lower index on ties)."""
+    pairs = small_pairs(5, seed=25)
+    fulls = [full(fn, g, m, f) for fn, g, m, f in pairs]
+    rng = np.random.default_rng(9)
+    centres = rng.integers(0, 4, (len(pairs), 3, 8)).astype(np.int32)
+    arg = [(fn, g, m, f, None) for fn, g, m, f in pairs]
+    multi = A.BatchSearcher().refine(arg, centres, max_steps=6, fallback=False)
+    singles = [A.BatchSearcher().refine(arg, centres[:, k, :], max_steps=6, fallback=False) for k in range(3)]
+    for i, ((fn, g, m, f), (cost, _)) in enumerate(zip(pairs, fulls)):
+        found = [(s[i]["value"], s[i]["index"]) for s in singles if s[i]["outcome"] == L.REFINE_LOCAL]
+        if not found:
+            assert multi[i]["outcome"] == L.REFINE_FALLBACK_BOUNDARY
+            continue
+        v, idx = min(found)
+        assert multi[i]["outcome"] == L.REFINE_LOCAL, fn.name
+        assert (np.float32(multi[i]["value"]), multi[i]["index"]) == (np.float32(v), idx)
+        assert np.float32(multi[i]["value"]).view(np.uint32) == cost[idx].view(np.uint32)
+        assert multi[i]["steps"] <= sum(s[i]["steps"] for s in singles)
