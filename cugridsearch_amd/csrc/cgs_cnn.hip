@@ -99,6 +99,17 @@ __device__ __host__ inline float bf2f(uint16_t h) {
 __device__ inline bool positive(uint16_t h) { return h != 0 && (h & 0x8000u) == 0; }
 
 // ------------------------------------------------------------ GEMM skeleton
+//
+// A problem type supplies, for an MT x NT tile and the 32-wide reduction
+// chunk at k0, the tile contents in 16-byte pieces (8 bf16): loadA / loadB
+// read a thread's pieces from global memory into registers, storeA /
+// storeB put them into the LDS tiles (reduction index contiguous per row),
+// k_range gives the blockIdx.z slice of the reduction and store() takes one
+// accumulator.  The skeleton issues the loads of chunk k+1 before it
+// multiplies chunk k, so global-memory latency overlaps the matrix work.
+
+template <int T>
+constexpr int pieces() { return (T * 4 + 255) / 256; }   // 16-byte pieces per thread of a T x 32 tile
 
 template <class Prob, int MT, int NT>
 __global__ __launch_bounds__(256) void igemm_kernel(Prob pb) {
@@ -117,10 +128,19 @@ __global__ __launch_bounds__(256) void igemm_kernel(Prob pb) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const int lrow = lane & 15, lk = (lane >> 4) * 8;
+    uint4 ra[pieces<MT>()], rb[pieces<NT>()];
+    if (kbeg < kend) {
+        pb.template loadA<MT>(ra, m0, kbeg, tid);
+        pb.template loadB<NT>(rb, n0, kbeg, tid);
+    }
     for (int k0 = kbeg; k0 < kend; k0 += 32) {
-        pb.template fillA<MT>(As, m0, k0, tid);
-        pb.template fillB<NT>(Bs, n0, k0, tid);
+        pb.template storeA<MT>(As, ra, tid);
+        pb.template storeB<NT>(Bs, rb, tid);
         __syncthreads();
+        if (k0 + 32 < kend) {
+            pb.template loadA<MT>(ra, m0, k0 + 32, tid);
+            pb.template loadB<NT>(rb, n0, k0 + 32, tid);
+        }
         bf16x8 a[FM], b[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i)
@@ -146,6 +166,29 @@ __global__ __launch_bounds__(256) void igemm_kernel(Prob pb) {
 
 __device__ inline void put8(uint16_t *dst, uint4 v) { *(uint4 *) dst = v; }
 
+__device__ inline uint4 pack8(const uint16_t *e) {
+    return make_uint4(e[0] | (uint32_t) e[1] << 16, e[2] | (uint32_t) e[3] << 16, e[4] | (uint32_t) e[5] << 16,
+                      e[6] | (uint32_t) e[7] << 16);
+}
+
+// Writes the 8 values of a piece down a column of a [row][kLDK] LDS tile
+// (rows r .. r+7, column px): the transposing store of the weight gradient.
+__device__ inline void put8_column(uint16_t *T, int r, int px, uint4 v) {
+    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) T[(r + q) * kLDK + px] = (uint16_t) (w4[q >> 1] >> ((q & 1) * 16));
+}
+
+// Row-major pieces (row r = t / 4, reduction offset kb = 8 (t % 4)) of a T x 32 tile.
+template <int T>
+__device__ inline void store_rows(uint16_t *S, const uint4 *r, int tid) {
+#pragma unroll
+    for (int j = 0; j < pieces<T>(); ++j) {
+        const int t = tid + 256 * j;
+        if (t < T * 4) put8(&S[(t >> 2) * kLDK + (t & 3) * 8], r[j]);
+    }
+}
+
 // Forward: out[m = (b, oy, ox)][co] = relu(bias + sum_{tap, ci} x[b][oy*S+ky-1][ox*S+kx-1][ci] w[co][tap][ci]).
 template <int L>
 struct FwdProb {
@@ -165,135 +208,186 @@ struct FwdProb {
         return x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
     }
     template <int MT>
-    __device__ void fillA(uint16_t *As, int m0, int k0, int tid) const {
-        for (int t = tid; t < MT * 4; t += 256) {
-            const int r = t >> 2, kb = (t & 3) * 8, m = m0 + r;
+    __device__ void loadA(uint4 *r, int m0, int k0, int tid) const {
+#pragma unroll
+        for (int j = 0; j < pieces<MT>(); ++j) {
+            const int t = tid + 256 * j, m = m0 + (t >> 2), k = k0 + (t & 3) * 8;
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (m < M) {
+            if (t < MT * 4 && m < M) {
                 const int b = m / (Ho * Ho), rem = m - b * Ho * Ho, oy = rem / Ho, ox = rem - oy * Ho;
-                const int k = k0 + kb;
                 if constexpr (Cin % 8 == 0) {
                     const int tap = k / Cin, ci = k - tap * Cin;
                     const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
                     if (iy >= 0 && iy < Hi && ix >= 0 && ix < Hi)
                         v = *(const uint4 *) &x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
+                } else if constexpr (Cin == 2) {
+                    // a piece is 4 taps x both channels: one 4-byte load per tap
+                    uint32_t e[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int tap = k / 2 + q;
+                        const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
+                        e[q] = (tap < 9 && iy >= 0 && iy < Hi && ix >= 0 && ix < Hi)
+                                   ? *(const uint32_t *) &x[(((size_t) b * Hi + iy) * Hi + ix) * 2]
+                                   : 0u;
+                    }
+                    v = make_uint4(e[0], e[1], e[2], e[3]);
                 } else {
                     uint16_t e[8];
 #pragma unroll
                     for (int q = 0; q < 8; ++q) e[q] = tap_value(b, oy, ox, k + q);
-                    v = make_uint4(e[0] | (uint32_t) e[1] << 16, e[2] | (uint32_t) e[3] << 16,
-                                   e[4] | (uint32_t) e[5] << 16, e[6] | (uint32_t) e[7] << 16);
+                    v = pack8(e);
                 }
             }
-            put8(&As[r * kLDK + kb], v);
+            r[j] = v;
         }
     }
     template <int NT>
-    __device__ void fillB(uint16_t *Bs, int n0, int k0, int tid) const {
-        for (int t = tid; t < NT * 4; t += 256) {
-            const int r = t >> 2, kb = (t & 3) * 8;
-            put8(&Bs[r * kLDK + kb], *(const uint4 *) &w[(size_t) (n0 + r) * Kp + k0 + kb]);
+    __device__ void loadB(uint4 *r, int n0, int k0, int tid) const {
+#pragma unroll
+        for (int j = 0; j < pieces<NT>(); ++j) {
+            const int t = tid + 256 * j;
+            r[j] = t < NT * 4 ? *(const uint4 *) &w[(size_t) (n0 + (t >> 2)) * Kp + k0 + (t & 3) * 8]
+                              : make_uint4(0, 0, 0, 0);
         }
     }
+    template <int MT>
+    __device__ void storeA(uint16_t *As, const uint4 *r, int tid) const { store_rows<MT>(As, r, tid); }
+    template <int NT>
+    __device__ void storeB(uint16_t *Bs, const uint4 *r, int tid) const { store_rows<NT>(Bs, r, tid); }
     __device__ void store(int m, int n, float v) const {
         if (m < M) y[(size_t) m * Cout + n] = f2bf(fmaxf(v + bias[n], 0.0f));
     }
 };
 
-// Data gradient: dx[m = (b, iy, ix)][ci] = sum_{tap, co} dz[b][oy][ox][co] w[co][tap][ci]
+// Data gradient: dx[(b, iy, ix)][ci] = sum_{tap, co} dz[b][oy][ox][co] w[co][tap][ci]
 // over the output pixels with oy*S + ky - 1 = iy; stored as the previous
-// layer's pre-activation gradient dz_prev = dx * (x > 0).
-template <int L>
+// layer's pre-activation gradient dz_prev = dx * (x > 0).  For stride 2 the
+// input pixels are split by parity class (PY, PX) = (iy & 1, ix & 1): only
+// the taps with ky = 1 (PY = 0) or ky in {0, 2} (PY = 1), and likewise kx,
+// reach such a pixel, so each class is its own GEMM over 1, 2, 2 or 4 taps
+// instead of 9.  For stride 1 the single class (0, 0) holds every pixel and
+// every tap.  Rows m enumerate the class's pixels (b, a, c) with
+// iy = S a + PY, ix = S c + PX.
+template <int L, int PY, int PX>
 struct DataProb {
     static constexpr int Cin = kCin[L], Cout = kCout[L], S = kStride[L], Hi = kHin[L], Ho = hout(L);
+    static constexpr int Hc = Hi / S;                                 // class pixels per side
+    static constexpr int NY = S == 1 ? 3 : (PY == 0 ? 1 : 2), NX = S == 1 ? 3 : (PX == 0 ? 1 : 2);
+    static constexpr int Ktaps = NY * NX;
     const uint16_t *dz;     // [n][Ho][Ho][Cout]
     const uint16_t *wt;     // [9][Cin][Cout]
     const uint16_t *x;      // [n][Hi][Hi][Cin] (this layer's input = previous ReLU output)
     uint16_t *dzp;          // [n][Hi][Hi][Cin]
-    int M;                  // n * Hi * Hi
-    __device__ void k_range(int, int &b, int &e) const { b = 0; e = 9 * Cout; }
+    int M;                  // n * Hc * Hc
+    // the class's t-th tap: ky, kx
+    __device__ static int tap_ky(int t) { return S == 1 ? t / 3 : (PY == 0 ? 1 : 2 * (t / NX)); }
+    __device__ static int tap_kx(int t) { return S == 1 ? t % 3 : (PX == 0 ? 1 : 2 * (t % NX)); }
+    __device__ void k_range(int, int &b, int &e) const { b = 0; e = Ktaps * Cout; }
+    __device__ void pixel(int m, int &b, int &iy, int &ix) const {
+        b = m / (Hc * Hc);
+        const int rem = m - b * Hc * Hc, a = rem / Hc, c = rem - a * Hc;
+        iy = S * a + PY;
+        ix = S * c + PX;
+    }
     template <int MT>
-    __device__ void fillA(uint16_t *As, int m0, int k0, int tid) const {
-        const int tap = k0 / Cout, co0 = k0 - tap * Cout;   // a chunk never straddles a tap (Cout % 32 == 0)
-        const int ky = tap / 3, kx = tap % 3;
-        for (int t = tid; t < MT * 4; t += 256) {
-            const int r = t >> 2, kb = (t & 3) * 8, m = m0 + r;
+    __device__ void loadA(uint4 *r, int m0, int k0, int tid) const {
+        const int t9 = k0 / Cout, co0 = k0 - t9 * Cout;   // a chunk never straddles a tap (Cout % 32 == 0)
+        const int ky = tap_ky(t9), kx = tap_kx(t9);
+#pragma unroll
+        for (int j = 0; j < pieces<MT>(); ++j) {
+            const int t = tid + 256 * j, m = m0 + (t >> 2), kb = (t & 3) * 8;
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (m < M) {
-                const int b = m / (Hi * Hi), rem = m - b * Hi * Hi, iy = rem / Hi, ix = rem - iy * Hi;
-                const int ty = iy + 1 - ky, tx = ix + 1 - kx;
-                if (ty >= 0 && tx >= 0 && ty % S == 0 && tx % S == 0 && ty / S < Ho && tx / S < Ho)
+            if (t < MT * 4 && m < M) {
+                int b, iy, ix;
+                pixel(m, b, iy, ix);
+                const int ty = iy + 1 - ky, tx = ix + 1 - kx;   // divisible by S by the class's choice of taps
+                if (ty >= 0 && tx >= 0 && ty / S < Ho && tx / S < Ho)
                     v = *(const uint4 *) &dz[(((size_t) b * Ho + ty / S) * Ho + tx / S) * Cout + co0 + kb];
             }
-            put8(&As[r * kLDK + kb], v);
+            r[j] = v;
         }
     }
     template <int NT>
-    __device__ void fillB(uint16_t *Bs, int n0, int k0, int tid) const {
-        const int tap = k0 / Cout, co0 = k0 - tap * Cout;
-        for (int t = tid; t < NT * 4; t += 256) {
-            const int r = t >> 2, kb = (t & 3) * 8;
-            put8(&Bs[r * kLDK + kb], *(const uint4 *) &wt[((size_t) tap * Cin + n0 + r) * Cout + co0 + kb]);
+    __device__ void loadB(uint4 *r, int n0, int k0, int tid) const {
+        const int t9 = k0 / Cout, co0 = k0 - t9 * Cout;
+        const int tap = tap_ky(t9) * 3 + tap_kx(t9);
+#pragma unroll
+        for (int j = 0; j < pieces<NT>(); ++j) {
+            const int t = tid + 256 * j;
+            r[j] = t < NT * 4 ? *(const uint4 *) &wt[((size_t) tap * Cin + n0 + (t >> 2)) * Cout + co0 + (t & 3) * 8]
+                              : make_uint4(0, 0, 0, 0);
         }
     }
+    template <int MT>
+    __device__ void storeA(uint16_t *As, const uint4 *r, int tid) const { store_rows<MT>(As, r, tid); }
+    template <int NT>
+    __device__ void storeB(uint16_t *Bs, const uint4 *r, int tid) const { store_rows<NT>(Bs, r, tid); }
     __device__ void store(int m, int n, float v) const {
         if (m < M) {
-            const size_t i = (size_t) m * Cin + n;
+            int b, iy, ix;
+            pixel(m, b, iy, ix);
+            const size_t i = (((size_t) b * Hi + iy) * Hi + ix) * Cin + n;
             dzp[i] = positive(x[i]) ? f2bf(v) : (uint16_t) 0;
         }
     }
 };
 
-// Weight gradient: dw[co][k = (tap, ci)] = sum_p dz[p][co] x[shift_tap(p)][ci],
-// the pixel sum split into slices of `slice` pixels (blockIdx.z); partial
-// sums go to part[z][co][Kp].
-template <int L>
+// Columns of the weight-gradient GEMM: the 9*Cin weight columns, then one
+// column of ones whose product with dz is the bias gradient, padded to a
+// multiple of the tile width NT.
+constexpr int wgrad_cols(int l, int nt) { return (ktrue(l) + 1 + nt - 1) / nt * nt; }
+
+// Weight gradient: dw[co][k = (tap, ci)] = sum_p dz[p][co] x[shift_tap(p)][ci]
+// for k < 9 Cin, and db[co] = sum_p dz[p][co] in column 9 Cin (B holds 1
+// there); the pixel sum split into slices of `slice` pixels (blockIdx.z),
+// partial sums in part[z][co][Kw].
+template <int L, int NT>
 struct WeightProb {
     static constexpr int Cin = kCin[L], Cout = kCout[L], S = kStride[L], Hi = kHin[L], Ho = hout(L);
-    static constexpr int Kt = ktrue(L), Kp = kpad(L);
+    static constexpr int Kt = ktrue(L), Kw = wgrad_cols(L, NT);
     const uint16_t *dz;     // [P][Cout]
     const uint16_t *x;      // [n][Hi][Hi][Cin]
-    float *part;            // [slices][Cout][Kp]
+    float *part;            // [slices][Cout][Kw]
     int P;                  // n * Ho * Ho
     int slice;              // pixels per slice, multiple of 32
     __device__ void k_range(int z, int &b, int &e) const {
         b = z * slice;
         e = min(P, b + slice);
     }
+    // piece t of a T x 32 tile: pixel px = t % 32 of the chunk, rows o .. o+7 with o = 8 (t / 32)
     template <int MT>
-    __device__ void fillA(uint16_t *As, int m0, int k0, int tid) const {
-        // As[co][px] = dz[k0 + px][m0 + co]: 16-byte loads along co, transposed into LDS
-        for (int t = tid; t < 32 * (MT / 8); t += 256) {
-            const int px = t % 32, o = (t / 32) * 8, p = k0 + px;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (p < P) v = *(const uint4 *) &dz[(size_t) p * Cout + m0 + o];
-            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+    __device__ void loadA(uint4 *r, int m0, int k0, int tid) const {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) As[(o + q) * kLDK + px] = (uint16_t) (w4[q >> 1] >> ((q & 1) * 16));
+        for (int j = 0; j < pieces<MT>(); ++j) {
+            const int t = tid + 256 * j, p = k0 + t % 32, o = (t / 32) * 8;
+            r[j] = (t < MT * 4 && p < P) ? *(const uint4 *) &dz[(size_t) p * Cout + m0 + o] : make_uint4(0, 0, 0, 0);
         }
     }
-    template <int NT>
-    __device__ void fillB(uint16_t *Bs, int n0, int k0, int tid) const {
-        // Bs[n][px] = x at pixel k0+px shifted by tap(n), channel ci(n)
-        for (int t = tid; t < 32 * (NT / 8); t += 256) {
-            const int px = t % 32, o = (t / 32) * 8, p = k0 + px;
-            uint16_t e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (p < P) {
-                const int b = p / (Ho * Ho), rem = p - b * Ho * Ho, oy = rem / Ho, ox = rem - oy * Ho;
-                const int n = n0 + o;
-                if constexpr (Cin % 8 == 0) {
-                    const int tap = n / Cin, ci = n - tap * Cin;
-                    const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
-                    if (n < Kt && iy >= 0 && iy < Hi && ix >= 0 && ix < Hi) {
-                        const uint4 v = *(const uint4 *) &x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
-                        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+    template <int NT_>
+    __device__ void loadB(uint4 *r, int n0, int k0, int tid) const {
+        constexpr uint16_t kOne = 0x3F80;   // bf16 1.0
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) e[q] = (uint16_t) (w4[q >> 1] >> ((q & 1) * 16));
+        for (int j = 0; j < pieces<NT_>(); ++j) {
+            const int t = tid + 256 * j, p = k0 + t % 32, n = n0 + (t / 32) * 8;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (t < NT_ * 4 && p < P) {
+                const int b = p / (Ho * Ho), rem = p - b * Ho * Ho, oy = rem / Ho, ox = rem - oy * Ho;
+                if constexpr (Cin % 8 == 0) {
+                    if (n < Kt) {
+                        const int tap = n / Cin, ci = n - tap * Cin;
+                        const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
+                        if (iy >= 0 && iy < Hi && ix >= 0 && ix < Hi)
+                            v = *(const uint4 *) &x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
+                    } else if (n == Kt) {
+                        v.x = kOne;
                     }
                 } else {
+                    uint16_t e[8];
 #pragma unroll
                     for (int q = 0; q < 8; ++q) {
                         const int k = n + q;
+                        e[q] = k == Kt ? kOne : (uint16_t) 0;
                         if (k < Kt) {
                             const int tap = k / Cin, ci = k - tap * Cin;
                             const int iy = oy * S + tap / 3 - 1, ix = ox * S + tap % 3 - 1;
@@ -301,61 +395,56 @@ struct WeightProb {
                                 e[q] = x[(((size_t) b * Hi + iy) * Hi + ix) * Cin + ci];
                         }
                     }
+                    v = pack8(e);
                 }
             }
+            r[j] = v;
+        }
+    }
+    template <int MT>
+    __device__ void storeA(uint16_t *As, const uint4 *r, int tid) const {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) Bs[(o + q) * kLDK + px] = e[q];
+        for (int j = 0; j < pieces<MT>(); ++j) {
+            const int t = tid + 256 * j;
+            if (t < MT * 4) put8_column(As, (t / 32) * 8, t % 32, r[j]);
+        }
+    }
+    template <int NT_>
+    __device__ void storeB(uint16_t *Bs, const uint4 *r, int tid) const {
+#pragma unroll
+        for (int j = 0; j < pieces<NT_>(); ++j) {
+            const int t = tid + 256 * j;
+            if (t < NT_ * 4) put8_column(Bs, (t / 32) * 8, t % 32, r[j]);
         }
     }
     __device__ void store(int m, int n, float v) const {
-        part[((size_t) blockIdx.z * Cout + m) * Kp + n] = v;
+        part[((size_t) blockIdx.z * Cout + m) * Kw + n] = v;
     }
 };
 
-// grad[co][k] (k < Kt) = sum over slices of part[z][co][k], in slice order.
-__global__ void reduce_slices_kernel(const float *part, int slices, int cout, int kp, int kt, float *grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cout * kt) return;
-    const int co = i / kt, k = i - co * kt;
-    float s = 0.0f;
-    for (int z = 0; z < slices; ++z) s += part[((size_t) z * cout + co) * kp + k];
-    grad[i] = s;
-}
-
-// Bias gradient: column sums of dz [P][C] (C divides 256).  Block b sums
-// rows [b*R, b*R + R): its 256 threads are 256/C row lanes x C channels, so
-// each load instruction reads 512 contiguous bytes; the row lanes are then
-// added in LDS in a fixed order.  A second kernel adds the blocks' partial
-// sums, one workgroup per channel, in a fixed tree order (deterministic).
-constexpr int kColRows = 1024;
-
-__global__ __launch_bounds__(256) void colsum_partial_kernel(const uint16_t *dz, int P, int C, float *part) {
+// Adds the slices' partial sums: output o = (co, k) for k <= Kt; 64 outputs
+// per workgroup, 4 lanes per output each adding every 4th slice, the lanes
+// then combined in a fixed order (deterministic).  k < Kt goes to the weight
+// gradient [co][Kt], k == Kt to the bias gradient.
+__global__ __launch_bounds__(256) void reduce_slices_kernel(const float *part, int slices, int cout, int kw, int kt,
+                                                           float *grad_w, float *grad_b) {
     __shared__ float red[256];
-    const int tid = threadIdx.x, c = tid % C, lane_row = tid / C, nr = 256 / C;
-    const int r0 = blockIdx.x * kColRows, r1 = min(P, r0 + kColRows);
+    const int tid = threadIdx.x, lane = tid >> 6, o = blockIdx.x * 64 + (tid & 63);
+    const int outs = cout * (kt + 1);
     float s = 0.0f;
-    for (int r = r0 + lane_row; r < r1; r += nr) s += bf2f(dz[(size_t) r * C + c]);
+    int co = 0, k = 0;
+    if (o < outs) {
+        co = o / (kt + 1);
+        k = o - co * (kt + 1);
+        for (int z = lane; z < slices; z += 4) s += part[((size_t) z * cout + co) * kw + k];
+    }
     red[tid] = s;
     __syncthreads();
-    if (tid < C) {
-        float t = 0.0f;
-        for (int k = 0; k < nr; ++k) t += red[k * C + tid];
-        part[(size_t) blockIdx.x * C + tid] = t;
+    if (tid < 64 && o < outs) {
+        const float t = (red[tid] + red[tid + 64]) + (red[tid + 128] + red[tid + 192]);
+        if (k < kt) grad_w[(size_t) co * kt + k] = t;
+        else grad_b[co] = t;
     }
-}
-
-__global__ __launch_bounds__(256) void colsum_final_kernel(const float *part, int blocks, int C, float *out) {
-    __shared__ float red[256];
-    const int c = blockIdx.x, tid = threadIdx.x;
-    float s = 0.0f;
-    for (int b = tid; b < blocks; b += 256) s += part[(size_t) b * C + c];
-    red[tid] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) red[tid] += red[tid + w];
-        __syncthreads();
-    }
-    if (tid == 0) out[c] = red[0];
 }
 
 // ------------------------------------------------------------- small passes
@@ -432,11 +521,10 @@ __global__ __launch_bounds__(kFeat) void gap_fc_kernel(const uint16_t *y6, const
     }
 }
 
-// Mean squared error over n x 8 outputs and the FC gradients.  One workgroup.
+// Mean squared error over n x 8 outputs and its gradient.  One workgroup.
 // loss = sum (out - target)^2 / (8n); dout = 2 (out - target) / (8n).
-__global__ __launch_bounds__(256) void loss_fc_kernel(const float *out, const float *target, const float *feat,
-                                                     const float *fcw, int n, float *dout, float *dfeat,
-                                                     float *grad_fcw, float *grad_fcb, float *loss) {
+__global__ __launch_bounds__(256) void loss_kernel(const float *out, const float *target, int n, float *dout,
+                                                  float *loss) {
     __shared__ float red[256];
     const float scale = 1.0f / (kOut * (float) n);
     float s = 0.0f;
@@ -452,22 +540,29 @@ __global__ __launch_bounds__(256) void loss_fc_kernel(const float *out, const fl
         __syncthreads();
     }
     if (threadIdx.x == 0) *loss = red[0] * scale;
-    for (int i = threadIdx.x; i < kOut * kFeat; i += 256) {
+}
+
+// FC gradients: threads [0, 1024) the weight gradient sum_b dout[b][j] feat[b][c],
+// [1024, 1032) the bias gradient, then one thread per (b, c) of the pooled
+// features' gradient sum_j dout[b][j] fcw[j][c].
+__global__ __launch_bounds__(256) void fc_grad_kernel(const float *dout, const float *feat, const float *fcw, int n,
+                                                     float *grad_fcw, float *grad_fcb, float *dfeat) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    constexpr int W = kOut * kFeat;
+    if (i < W) {
         const int j = i / kFeat, c = i % kFeat;
         float g = 0.0f;
         for (int b = 0; b < n; ++b) g = fmaf(dout[b * kOut + j], feat[(size_t) b * kFeat + c], g);
         grad_fcw[i] = g;
-    }
-    if (threadIdx.x < kOut) {
+    } else if (i < W + kOut) {
         float g = 0.0f;
-        for (int b = 0; b < n; ++b) g += dout[b * kOut + threadIdx.x];
-        grad_fcb[threadIdx.x] = g;
-    }
-    for (int i = threadIdx.x; i < n * kFeat; i += 256) {
-        const int b = i / kFeat, c = i % kFeat;
+        for (int b = 0; b < n; ++b) g += dout[b * kOut + (i - W)];
+        grad_fcb[i - W] = g;
+    } else if (i < W + kOut + n * kFeat) {
+        const int k = i - W - kOut, b = k / kFeat, c = k % kFeat;
         float g = 0.0f;
         for (int j = 0; j < kOut; ++j) g = fmaf(dout[b * kOut + j], fcw[j * kFeat + c], g);
-        dfeat[i] = g;
+        dfeat[k] = g;
     }
 }
 
@@ -535,7 +630,7 @@ struct cgs_cnn_s {
     const uint16_t *last_x = nullptr;
     DevBuf master, grad, adam_m, adam_v, fcw, w[kLayers], wt[kLayers];
     DevBuf act[kLayers];            // outputs of the conv layers
-    DevBuf dz[2], part, colpart, feat, out, dout, dfeat, loss, loss_host;
+    DevBuf dz[2], part, feat, out, dout, dfeat, loss, loss_host;
     std::vector<float> host_master;
     ~cgs_cnn_s() {
         auto rel = [](DevBuf &b, bool host) {
@@ -544,7 +639,7 @@ struct cgs_cnn_s {
         };
         rel(master, false); rel(grad, false); rel(adam_m, false); rel(adam_v, false); rel(fcw, false);
         for (int l = 0; l < kLayers; ++l) { rel(w[l], false); rel(wt[l], false); rel(act[l], false); }
-        rel(dz[0], false); rel(dz[1], false); rel(part, false); rel(colpart, false); rel(feat, false);
+        rel(dz[0], false); rel(dz[1], false); rel(part, false); rel(feat, false);
         rel(out, false); rel(dout, false); rel(dfeat, false); rel(loss, false); rel(loss_host, true);
     }
 };
@@ -569,7 +664,7 @@ constexpr int fwd_nt(int l) { return kCout[l]; }
 constexpr int dat_mt(int l) { return kCin[l] >= 64 ? 64 : 128; }
 constexpr int dat_nt(int l) { return kCin[l]; }
 constexpr int wgt_mt(int l) { return kCout[l] >= 128 ? 128 : kCout[l]; }
-constexpr int wgt_nt(int l) { return kpad(l) % 64 == 0 ? 64 : 32; }
+constexpr int wgt_nt(int l) { return l == 0 ? 32 : 64; }
 
 template <int L>
 hipError_t launch_fwd(const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *y, int n, hipStream_t s) {
@@ -580,32 +675,41 @@ hipError_t launch_fwd(const uint16_t *x, const uint16_t *w, const float *bias, u
     return hipGetLastError();
 }
 
-template <int L>
-hipError_t launch_data(const uint16_t *dz, const uint16_t *wt, const uint16_t *x, uint16_t *dzp, int n,
-                       hipStream_t s) {
+template <int L, int PY, int PX>
+hipError_t launch_data_class(const uint16_t *dz, const uint16_t *wt, const uint16_t *x, uint16_t *dzp, int n,
+                             hipStream_t s) {
     constexpr int MT = dat_mt(L), NT = dat_nt(L);
-    DataProb<L> pb{dz, wt, x, dzp, n * kHin[L] * kHin[L]};
-    hipLaunchKernelGGL((igemm_kernel<DataProb<L>, MT, NT>), dim3((pb.M + MT - 1) / MT, kCin[L] / NT, 1), dim3(256),
-                       0, s, pb);
+    using Prob = DataProb<L, PY, PX>;
+    Prob pb{dz, wt, x, dzp, n * Prob::Hc * Prob::Hc};
+    hipLaunchKernelGGL((igemm_kernel<Prob, MT, NT>), dim3((pb.M + MT - 1) / MT, kCin[L] / NT, 1), dim3(256), 0, s, pb);
     return hipGetLastError();
 }
 
 template <int L>
-hipError_t launch_weight(const uint16_t *dz, const uint16_t *x, float *part, float *grad, float *colpart,
-                         float *gbias, int n, int cus, hipStream_t s) {
-    constexpr int MT = wgt_mt(L), NT = wgt_nt(L), Kp = kpad(L);
-    const int tiles = (kCout[L] / MT) * (Kp / NT);
+hipError_t launch_data(const uint16_t *dz, const uint16_t *wt, const uint16_t *x, uint16_t *dzp, int n,
+                       hipStream_t s) {
+    hipError_t e = launch_data_class<L, 0, 0>(dz, wt, x, dzp, n, s);
+    if constexpr (kStride[L] == 2) {
+        if (e == hipSuccess) e = launch_data_class<L, 0, 1>(dz, wt, x, dzp, n, s);
+        if (e == hipSuccess) e = launch_data_class<L, 1, 0>(dz, wt, x, dzp, n, s);
+        if (e == hipSuccess) e = launch_data_class<L, 1, 1>(dz, wt, x, dzp, n, s);
+    }
+    return e;
+}
+
+template <int L>
+hipError_t launch_weight(const uint16_t *dz, const uint16_t *x, float *part, float *grad_w, float *grad_b, int n,
+                         int cus, hipStream_t s) {
+    constexpr int MT = wgt_mt(L), NT = wgt_nt(L), Kw = wgrad_cols(L, NT);
+    const int tiles = (kCout[L] / MT) * (Kw / NT);
     const int z = wgrad_slices(L, n, cus, tiles);
     const int P = n * hout(L) * hout(L);
-    WeightProb<L> pb{dz, x, part, P, wgrad_slice_len(L, n, z)};
-    hipLaunchKernelGGL((igemm_kernel<WeightProb<L>, MT, NT>), dim3(kCout[L] / MT, Kp / NT, z), dim3(256), 0, s, pb);
-    const int cnt = kCout[L] * ktrue(L);
-    hipLaunchKernelGGL(reduce_slices_kernel, dim3((cnt + 255) / 256), dim3(256), 0, s, (const float *) part, z,
-                       kCout[L], Kp, ktrue(L), grad);
-    const int blocks = (P + kColRows - 1) / kColRows;
-    hipLaunchKernelGGL(colsum_partial_kernel, dim3(blocks), dim3(256), 0, s, dz, P, kCout[L], colpart);
-    hipLaunchKernelGGL(colsum_final_kernel, dim3(kCout[L]), dim3(256), 0, s, (const float *) colpart, blocks, kCout[L],
-                       gbias);
+    WeightProb<L, NT> pb{dz, x, part, P, wgrad_slice_len(L, n, z)};
+    hipLaunchKernelGGL((igemm_kernel<WeightProb<L, NT>, MT, NT>), dim3(kCout[L] / MT, Kw / NT, z), dim3(256), 0, s,
+                       pb);
+    const int outs = kCout[L] * (ktrue(L) + 1);
+    hipLaunchKernelGGL(reduce_slices_kernel, dim3((outs + 63) / 64), dim3(256), 0, s, (const float *) part, z,
+                       kCout[L], Kw, ktrue(L), grad_w, grad_b);
     return hipGetLastError();
 }
 
@@ -614,8 +718,9 @@ size_t act_elems(int l, int n) { return (size_t) n * hout(l) * hout(l) * kCout[l
 size_t part_floats(int n, int cus) {
     size_t mx = 0;
     auto one = [&](int l, int mt, int nt) {
-        const int tiles = (kCout[l] / mt) * (kpad(l) / nt);
-        mx = std::max(mx, (size_t) wgrad_slices(l, n, cus, tiles) * kCout[l] * kpad(l));
+        const int kw = (ktrue(l) + 1 + nt - 1) / nt * nt;
+        const int tiles = (kCout[l] / mt) * (kw / nt);
+        mx = std::max(mx, (size_t) wgrad_slices(l, n, cus, tiles) * kCout[l] * kw);
     };
     one(0, wgt_mt(0), wgt_nt(0)); one(1, wgt_mt(1), wgt_nt(1)); one(2, wgt_mt(2), wgt_nt(2));
     one(3, wgt_mt(3), wgt_nt(3)); one(4, wgt_mt(4), wgt_nt(4)); one(5, wgt_mt(5), wgt_nt(5));
@@ -671,10 +776,6 @@ cgs_status cgs_cnn_create(int32_t max_batch, cgs_cnn **out) {
     }
     need(c->dz[0], dzmax * 2); need(c->dz[1], dzmax * 2);
     need(c->part, part_floats(max_batch, c->cus) * 4);
-    size_t colmax = 0;
-    for (int l = 0; l < kLayers; ++l)
-        colmax = std::max(colmax, (size_t) ((max_batch * hout(l) * hout(l) + kColRows - 1) / kColRows) * kCout[l]);
-    need(c->colpart, colmax * 4);
     need(c->feat, (size_t) max_batch * kFeat * 4);
     need(c->out, (size_t) max_batch * kOut * 4);
     need(c->dout, (size_t) max_batch * kOut * 4);
@@ -795,24 +896,27 @@ cgs_status cgs_cnn_backward(cgs_cnn *c, const float *target, float *loss_out, vo
     auto A = [&](int l) { return (const uint16_t *) c->act[l].ptr; };
     auto WT = [&](int l) { return (const uint16_t *) c->wt[l].ptr; };
     uint16_t *dz[2] = {(uint16_t *) c->dz[0].ptr, (uint16_t *) c->dz[1].ptr};
-    float *part = (float *) c->part.ptr, *colp = (float *) c->colpart.ptr;
-    hipLaunchKernelGGL(loss_fc_kernel, dim3(1), dim3(256), 0, s, (const float *) c->out.ptr, target,
-                       (const float *) c->feat.ptr, (const float *) c->fcw.ptr, n, (float *) c->dout.ptr,
-                       (float *) c->dfeat.ptr, g + kOff.fcw, g + kOff.fcb, (float *) c->loss.ptr);
+    float *part = (float *) c->part.ptr;
+    hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(256), 0, s, (const float *) c->out.ptr, target, n,
+                       (float *) c->dout.ptr, (float *) c->loss.ptr);
+    const int fc_threads = kOut * kFeat + kOut + n * kFeat;
+    hipLaunchKernelGGL(fc_grad_kernel, dim3((fc_threads + 255) / 256), dim3(256), 0, s, (const float *) c->dout.ptr,
+                       (const float *) c->feat.ptr, (const float *) c->fcw.ptr, n, g + kOff.fcw, g + kOff.fcb,
+                       (float *) c->dfeat.ptr);
     hipLaunchKernelGGL(gap_backward_kernel, dim3(512), dim3(256), 0, s, A(5), (const float *) c->dfeat.ptr, n, dz[1]);
     HIP_TRY(hipGetLastError());
     // dz of layer l lives in dz[l & 1]
-    HIP_TRY(launch_weight<5>(dz[1], A(4), part, g + kOff.w[5], colp, g + kOff.b[5], n, c->cus, s));
+    HIP_TRY(launch_weight<5>(dz[1], A(4), part, g + kOff.w[5], g + kOff.b[5], n, c->cus, s));
     HIP_TRY(launch_data<5>(dz[1], WT(5), A(4), dz[0], n, s));
-    HIP_TRY(launch_weight<4>(dz[0], A(3), part, g + kOff.w[4], colp, g + kOff.b[4], n, c->cus, s));
+    HIP_TRY(launch_weight<4>(dz[0], A(3), part, g + kOff.w[4], g + kOff.b[4], n, c->cus, s));
     HIP_TRY(launch_data<4>(dz[0], WT(4), A(3), dz[1], n, s));
-    HIP_TRY(launch_weight<3>(dz[1], A(2), part, g + kOff.w[3], colp, g + kOff.b[3], n, c->cus, s));
+    HIP_TRY(launch_weight<3>(dz[1], A(2), part, g + kOff.w[3], g + kOff.b[3], n, c->cus, s));
     HIP_TRY(launch_data<3>(dz[1], WT(3), A(2), dz[0], n, s));
-    HIP_TRY(launch_weight<2>(dz[0], A(1), part, g + kOff.w[2], colp, g + kOff.b[2], n, c->cus, s));
+    HIP_TRY(launch_weight<2>(dz[0], A(1), part, g + kOff.w[2], g + kOff.b[2], n, c->cus, s));
     HIP_TRY(launch_data<2>(dz[0], WT(2), A(1), dz[1], n, s));
-    HIP_TRY(launch_weight<1>(dz[1], A(0), part, g + kOff.w[1], colp, g + kOff.b[1], n, c->cus, s));
+    HIP_TRY(launch_weight<1>(dz[1], A(0), part, g + kOff.w[1], g + kOff.b[1], n, c->cus, s));
     HIP_TRY(launch_data<1>(dz[1], WT(1), A(0), dz[0], n, s));
-    HIP_TRY(launch_weight<0>(dz[0], c->last_x, part, g + kOff.w[0], colp, g + kOff.b[0], n, c->cus, s));
+    HIP_TRY(launch_weight<0>(dz[0], c->last_x, part, g + kOff.w[0], g + kOff.b[0], n, c->cus, s));
     if (loss_out) {
         HIP_TRY(hipMemcpyAsync(c->loss_host.ptr, c->loss.ptr, 4, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));

@@ -316,14 +316,15 @@ def lattice_coords(off, g, w=SIZE, h=SIZE):
     return np.stack([(offsets_to_params(o, w, h) - start) / np.where(res == 0, 1, res) for o in off])
 
 
-START_MODES = ("nearest", "opposite", "identity")
+START_MODES = ("nearest", "opposite", "identity", "identity_far")
 
 
 def make_centres(off, g, modes=("nearest", "opposite", "identity"), w=SIZE, h=SIZE):
     """Descent starts per pair, [n, len(modes), dim]: "nearest" = the grid
     point nearest the proposal; "opposite" = the far corner of the lattice
     cell holding the proposal (floor and ceil swapped on every axis);
-    "identity" = the grid point nearest the identity homography."""
+    "identity" = the grid point nearest the identity homography;
+    "identity_far" = the far corner of the lattice cell holding the identity."""
     count = g.getAxisSampleCounts().astype(np.int64)
     u = lattice_coords(off, g, w, h)
     near = np.clip(np.rint(u), 0, count - 1).astype(np.int32)
@@ -336,6 +337,8 @@ def make_centres(off, g, modes=("nearest", "opposite", "identity"), w=SIZE, h=SI
             out.append(np.clip(other, 0, count - 1).astype(np.int32))
         elif m == "identity":
             out.append(identity_centres(len(off), g))
+        elif m == "identity_far":
+            out.append(fixed_centres(len(off), g, 2)[:, 1, :])
         else:
             raise ValueError(f"unknown start mode {m!r}")
     return np.stack(out, axis=1)
@@ -355,19 +358,15 @@ def refine(images, g, centres, threshold=float("inf"), max_steps=8, fallback=Tru
                     threshold=threshold, fallback=fallback)
 
 
-def calibrate_threshold(values, wrong, max_wrong_rate):
-    """Largest accepted-cost threshold whose accepted wrong local minima stay
-    within max_wrong_rate of all pairs (costs: lower is better)."""
-    order = np.argsort(values)
-    v, w = np.asarray(values)[order], np.asarray(wrong)[order]
-    allowed = max_wrong_rate * len(v)
-    cum = np.cumsum(w)
-    ok = np.where(cum <= allowed)[0]
-    if len(ok) == 0:
-        return float(v[0]) - 1.0
-    k = ok[-1]
-    # accept everything up to v[k]; stop before the next wrong one
-    return float(v[k])
+def calibrate_threshold(values, fallback_rate):
+    """Accepted-cost threshold that sends the worst-matching fallback_rate of
+    pairs to the full grid: the (1 - fallback_rate) quantile of the local
+    minima's costs (lower is better), taken over validation pairs."""
+    v = np.sort(np.asarray(values, np.float64))
+    if len(v) == 0 or fallback_rate <= 0:
+        return float("inf")
+    k = int(math.ceil((1.0 - fallback_rate) * len(v))) - 1
+    return float(v[min(max(k, 0), len(v) - 1)])
 
 
 def run_learned(net, images, g, threshold, max_steps=8, batch=256, modes=("nearest",)):
@@ -442,7 +441,8 @@ def main(argv=None):
     tr.add_argument("--batch", type=int, default=128)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--val", type=int, default=1024, help="labelled pairs held back for validation / threshold")
-    tr.add_argument("--max-wrong", type=float, default=0.001)
+    tr.add_argument("--fallback-rate", type=float, default=0.01,
+                    help="share of validation pairs whose local minimum the threshold sends to the full grid")
     tr.add_argument("--modes", default="nearest,opposite", help="descent starts (make_centres modes)")
     tr.add_argument("--seed", type=int, default=1)
     tr.add_argument("--workers", type=int, default=12)
@@ -552,12 +552,14 @@ def train_main(a):
     vres, vsec, _ = run_learned(net, vimages, g, threshold=float("inf"), modes=modes)
     local = vres["outcome"] == 0
     wrong = vres["local_index"] != idx[:nval]
-    thr = calibrate_threshold(vres["local_value"][local], wrong[local], a.max_wrong)
+    thr = calibrate_threshold(vres["local_value"][local], a.fallback_rate)
+    accepted = local & (vres["local_value"] <= thr)
     meta = dict(train_pairs=int(len(ks) - nval), val_pairs=int(nval), epochs_run=len(hist),
-                history=[list(map(float, h)) for h in hist], threshold=thr,
+                history=[list(map(float, h)) for h in hist], threshold=thr, fallback_rate=a.fallback_rate,
                 val_local_rate=float(local.mean()), val_local_wrong=float((local & wrong).mean()),
-                val_agree_inf=float((vres["index"] == idx[:nval]).mean()), labels=[os.path.basename(p) for p in a.labels],
-                modes=list(modes), val_seconds=vsec)
+                val_agree_inf=float((vres["index"] == idx[:nval]).mean()),
+                val_agree_threshold=float(((accepted & ~wrong) | ~accepted).mean()),
+                labels=[os.path.basename(p) for p in a.labels], modes=list(modes), val_seconds=vsec)
     log("[train] " + json.dumps({k: v for k, v in meta.items() if k != "history"}))
     net.save(a.out, threshold=np.float32(thr), modes=",".join(modes), meta=json.dumps(meta))
     log(f"[train] saved {a.out}")

@@ -162,12 +162,13 @@ def test_nearest_centre_and_index_axes():
 
 
 def test_calibrate_threshold():
-    v = np.array([-0.99, -0.98, -0.97, -0.96, -0.95, -0.9])
-    wrong = np.array([0, 0, 1, 0, 1, 1])
-    assert P.calibrate_threshold(v, wrong, 0.0) == -0.98
-    assert P.calibrate_threshold(v, wrong, 1 / 6) == -0.96
-    assert P.calibrate_threshold(v, wrong, 1.0) == -0.9
-    assert P.calibrate_threshold(v, np.ones(6), 0.0) < -0.99
+    v = np.array([-0.99, -0.98, -0.97, -0.96, -0.95, -0.9, -0.94, -0.93, -0.92, -0.91])
+    assert P.calibrate_threshold(v, 0.0) == np.inf
+    assert P.calibrate_threshold(v, 0.1) == -0.91          # the worst tenth falls back
+    assert P.calibrate_threshold(v, 0.3) == -0.93
+    assert (v > P.calibrate_threshold(v, 0.2)).sum() == 2
+    assert P.calibrate_threshold(v, 1.0) == -0.99
+    assert P.calibrate_threshold([], 0.1) == np.inf
 
 
 def test_network_layout():
