@@ -115,8 +115,9 @@ template <class Prob, int MT, int NT>
 __global__ __launch_bounds__(256) void igemm_kernel(Prob pb) {
     static_assert(MT % 32 == 0 && NT % 32 == 0, "tiles are multiples of 32");
     constexpr int WM = MT / 2, WN = NT / 2, FM = WM / 16, FN = WN / 16;
-    __shared__ __attribute__((aligned(16))) uint16_t As[MT * kLDK];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[NT * kLDK];
+    constexpr int kOperands = (MT + NT) * kLDK, kTile = MT * (NT + 8);   // operand tiles; bf16 output tile
+    __shared__ __attribute__((aligned(16))) uint16_t lds[kOperands > kTile ? kOperands : kTile];
+    uint16_t *const As = lds, *const Bs = lds + MT * kLDK;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = wave >> 1, wn = wave & 1;
     const int m0 = blockIdx.x * MT, n0 = blockIdx.y * NT;
@@ -155,13 +156,34 @@ __global__ __launch_bounds__(256) void igemm_kernel(Prob pb) {
         __syncthreads();
     }
     // D[i][j] of a 16x16 block sits in lane j + 16*(i/4), element i%4
+    if constexpr (Prob::kTileStore) {
+        // bf16 results go through LDS so that global stores are 16-byte
+        // row pieces: each lane writes its values into an MT x NT tile,
+        // then each thread stores whole 8-column pieces
+        uint16_t *T = lds;   // the operand tiles are free after the loop (last barrier above)
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
+            for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
-                pb.store(m0 + wm * WM + i * 16 + (lane >> 4) * 4 + r, n0 + wn * WN + j * 16 + lrow, acc[i][j][r]);
+                for (int r = 0; r < 4; ++r) {
+                    const int row = wm * WM + i * 16 + (lane >> 4) * 4 + r, col = wn * WN + j * 16 + lrow;
+                    T[row * (NT + 8) + col] = pb.value(n0 + col, acc[i][j][r]);
+                }
+        __syncthreads();
+        for (int t = tid; t < MT * NT / 8; t += 256) {
+            const int row = t / (NT / 8), c8 = (t % (NT / 8)) * 8;
+            pb.store8(m0 + row, n0 + c8, *(const uint4 *) &T[row * (NT + 8) + c8]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    pb.store(m0 + wm * WM + i * 16 + (lane >> 4) * 4 + r, n0 + wn * WN + j * 16 + lrow, acc[i][j][r]);
+    }
 }
 
 __device__ inline void put8(uint16_t *dst, uint4 v) { *(uint4 *) dst = v; }
@@ -255,8 +277,10 @@ struct FwdProb {
     __device__ void storeA(uint16_t *As, const uint4 *r, int tid) const { store_rows<MT>(As, r, tid); }
     template <int NT>
     __device__ void storeB(uint16_t *Bs, const uint4 *r, int tid) const { store_rows<NT>(Bs, r, tid); }
-    __device__ void store(int m, int n, float v) const {
-        if (m < M) y[(size_t) m * Cout + n] = f2bf(fmaxf(v + bias[n], 0.0f));
+    static constexpr bool kTileStore = true;
+    __device__ uint16_t value(int n, float v) const { return f2bf(fmaxf(v + bias[n], 0.0f)); }
+    __device__ void store8(int m, int n, uint4 v) const {
+        if (m < M) *(uint4 *) &y[(size_t) m * Cout + n] = v;
     }
 };
 
@@ -323,12 +347,23 @@ struct DataProb {
     __device__ void storeA(uint16_t *As, const uint4 *r, int tid) const { store_rows<MT>(As, r, tid); }
     template <int NT>
     __device__ void storeB(uint16_t *Bs, const uint4 *r, int tid) const { store_rows<NT>(Bs, r, tid); }
-    __device__ void store(int m, int n, float v) const {
+    static constexpr bool kTileStore = true;
+    __device__ uint16_t value(int, float v) const { return f2bf(v); }
+    __device__ void store8(int m, int n, uint4 v) const {
         if (m < M) {
             int b, iy, ix;
             pixel(m, b, iy, ix);
             const size_t i = (((size_t) b * Hi + iy) * Hi + ix) * Cin + n;
-            dzp[i] = positive(x[i]) ? f2bf(v) : (uint16_t) 0;
+            const uint4 xm = *(const uint4 *) &x[i];
+            const uint32_t xs[4] = {xm.x, xm.y, xm.z, xm.w};
+            uint32_t ds[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t lo = positive((uint16_t) xs[q]) ? (ds[q] & 0xFFFFu) : 0u;
+                const uint32_t hi = positive((uint16_t) (xs[q] >> 16)) ? (ds[q] & 0xFFFF0000u) : 0u;
+                ds[q] = lo | hi;
+            }
+            *(uint4 *) &dzp[i] = make_uint4(ds[0], ds[1], ds[2], ds[3]);
         }
     }
 };
@@ -417,6 +452,7 @@ struct WeightProb {
             if (t < NT_ * 4) put8_column(Bs, (t / 32) * 8, t % 32, r[j]);
         }
     }
+    static constexpr bool kTileStore = false;
     __device__ void store(int m, int n, float v) const {
         part[((size_t) blockIdx.z * Cout + m) * Kw + n] = v;
     }
