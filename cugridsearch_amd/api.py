@@ -240,6 +240,22 @@ class CudaGridSearcher:
         self._result.last_argmin = (out.value, out.index)
         return out.value, out.index
 
+    def covariance_by_value(self, covar=None, stream=None):
+        """CudaGridSearcher::covariance_by_value (src/gpu/cudaGridSearch.cuh:768-794)
+        over the result tensor of the last search: adds the dim x dim
+        covariance of the cost surface around its minimum into covar (a
+        float32 numpy array, created as zeros when None) and returns it."""
+        d = self._grid.getDimension()
+        c = np.zeros((d, d), np.float32)
+        m = L.Argmin()
+        L.check(L.lib().cgs_covariance(ctypes.byref(self._grid._g), ctypes.c_void_p(self._result.data.data_ptr()),
+                                       c.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(m),
+                                       _stream_ptr(stream)))
+        if covar is None:
+            return c
+        covar += c
+        return covar
+
     # The reference's launch-policy variants compute the same costs; here they
     # all run the fused kernel.  STREAM_BLOCK_DIM / numCores / numThreads are
     # accepted for signature compatibility

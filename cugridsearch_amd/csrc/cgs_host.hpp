@@ -37,6 +37,7 @@ struct Workspace {
     std::mutex mu;
     DevBuf mov, quads, sincos, sincos_host, hist, key, key_host;
     DevBuf prep, prep_host;     // proposal-network input descriptors
+    DevBuf cov, cov_host;       // covariance partial sums
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     // HIP events around one launch on the caller's stream; elapsed_ms()
     // stores the duration for cgs_last_kernel_ms() (after a stream sync).

@@ -172,6 +172,20 @@ cgs_status cgs_eval_point(const cgs_image *moved, const cgs_image *fixed, const 
                           int32_t dim, const float *params, float *cost_out, uint32_t *hist_out,
                           void *stream);
 
+/* CudaGridSearcher::covariance_by_value (src/gpu/cudaGridSearch.cuh:768-794,
+ * kernel calculateCovariance_by_value :317-419) over a cost tensor `result`
+ * (device, grid->num_elements floats in grid order).  The minimum is the
+ * reference's: the lowest index of the smallest negative cost, else index 0
+ * with value 0 (returned in min_out, host, may be NULL).  For every other
+ * point i with f = result[i] (0 read as -1e-9), dx = p(i) - p(min),
+ * len = |dx| and s = (log|min| - log|f|) / len >= 0, the sum of g g^T with
+ * g = dx s / len goes to covar (host, dim x dim, row-major, overwritten;
+ * the reference adds into the caller's matrix, which the facade does).
+ * Per-point arithmetic in binary32 as the reference writes it, log as the
+ * binary64 log rounded to binary32, sums in binary64.  Synchronous. */
+cgs_status cgs_covariance(const cgs_grid *grid, const float *result, float *covar, cgs_argmin *min_out,
+                          void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched ragged search.  No reference counterpart: the reference searches
  * one pair per CudaGridSearcher call (src/gpu/cudaGridSearch.cuh:736-765).

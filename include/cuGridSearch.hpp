@@ -246,6 +246,18 @@ struct CudaGridSearcher {
 
     // The launch-policy variants compute the same costs with the one fused
     // kernel; their tuning arguments are accepted and unused.
+    // CudaGridSearcher::covariance_by_value (src/gpu/cudaGridSearch.cuh:768-794):
+    // adds the D x D covariance around the result's minimum into covar_matrix
+    // (host, row-major; the reference copies (D+1)^2 values into a D^2 array,
+    // here exactly D^2 are touched).
+    void covariance_by_value(float *covar_matrix) {
+        std::cout << "CudaGrid has " << _grid->numElements() << " search samples." << std::endl;
+        float c[CGS_MAX_DIM * CGS_MAX_DIM];
+        cgs_check(cgs_covariance(&_grid->_g, (const float *) _result->data(), c, nullptr, stream),
+                  "covariance_by_value");
+        for (unsigned i = 0; i < D * D; ++i) covar_matrix[i] += c[i];
+    }
+
     template <typename Img> void search_by_value_stream(const CgsFunctor &f, int, int, Img &m, Img &x) {
         search_by_value(f, m, x);
     }

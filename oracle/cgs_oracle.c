@@ -496,3 +496,47 @@ int orc_max_threads(void) {
     return 1;
 #endif
 }
+
+/* CudaGridSearcher::covariance_by_value's kernel, calculateCovariance_by_value
+ * (src/gpu/cudaGridSearch.cuh:317-419), in its own loop order: the minimum
+ * scan with strict < from (value 0, index 0) (:331-339), then every other
+ * point: f == 0 read as -1e-9 (:346), dx = p - p_min, len = sqrt of the
+ * axis-ordered binary32 sum of dx^2 (:349-356), slope = (log|min| - log|f|)
+ * * (1 / len) skipped when < 0 (:357-362), g = dx * slope / len (:365-367),
+ * covar += g g^T (:369-394).  Binary32 per point with every operation
+ * rounded on its own; log declared as the binary64 log rounded to binary32;
+ * the sums in binary64 (the reference adds into binary32).  covar: D x D,
+ * overwritten.  Returns the minimum's index; *min_value gets its value. */
+int64_t orc_covariance(int D, const float *start, const float *res, const float *count, const float *result,
+                       int64_t n, double *covar, float *min_value) {
+    int64_t mi = 0;
+    float mv = 0.0f;
+    for (int64_t i = 0; i < n; ++i)
+        if (result[i] < mv) {
+            mv = result[i];
+            mi = i;
+        }
+    *min_value = mv;
+    for (int k = 0; k < D * D; ++k) covar[k] = 0.0;
+    float xmin[ORC_MAXD], p[ORC_MAXD], dx[ORC_MAXD], g[ORC_MAXD];
+    orc_grid_point(D, start, res, count, mi, xmin);
+    const float logmin = (float) log((double) fabsf(mv));
+    for (int64_t i = 0; i < n; ++i) {
+        if (i == mi) continue;
+        orc_grid_point(D, start, res, count, i, p);
+        float f = result[i];
+        if (f == 0.0f) f = -1e-9f;
+        float len2 = 0.0f;
+        for (int d = 0; d < D; ++d) {
+            dx[d] = p[d] - xmin[d];
+            len2 = len2 + dx[d] * dx[d];
+        }
+        const float len = sqrtf(len2);
+        const float slope = (logmin - (float) log((double) fabsf(f))) * (1.0f / len);
+        if (slope < 0.0f) continue;
+        for (int d = 0; d < D; ++d) g[d] = dx[d] * slope / len;
+        for (int r = 0; r < D; ++r)
+            for (int c = 0; c < D; ++c) covar[r * D + c] += (double) (g[r] * g[c]);
+    }
+    return mi;
+}

@@ -64,6 +64,8 @@ def lib():
         L.orc_argmin.argtypes = [fp, i64, ctypes.c_int, fp, ctypes.POINTER(i64)]
         L.orc_argmin_reference_sim.argtypes = [fp, i64, fp, ctypes.POINTER(i64)]
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_covariance.argtypes = [ctypes.c_int, fp, fp, fp, fp, i64, ctypes.POINTER(ctypes.c_double), fp]
+        L.orc_covariance.restype = i64
         _lib = L
     return _lib
 
@@ -191,6 +193,20 @@ def argmin_reference_sim(r):
     i = ctypes.c_int64()
     lib().orc_argmin_reference_sim(_f32p(a), len(a), ctypes.byref(v), ctypes.byref(i))
     return v.value, i.value
+
+
+def covariance(start, res, count, result):
+    """calculateCovariance_by_value restated: (D x D covariance, min index, min value)."""
+    s = np.ascontiguousarray(start, np.float32)
+    r = np.ascontiguousarray(res, np.float32)
+    c = np.ascontiguousarray(count, np.float32)
+    v = np.ascontiguousarray(result, np.float32)
+    D = len(c)
+    out = np.zeros(D * D, np.float64)
+    mv = ctypes.c_float()
+    mi = lib().orc_covariance(D, _f32p(s), _f32p(r), _f32p(c), _f32p(v), len(v),
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(mv))
+    return out.reshape(D, D), int(mi), mv.value
 
 
 def max_threads():

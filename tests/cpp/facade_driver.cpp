@@ -67,12 +67,19 @@ static int run(std::ifstream &spec, const std::string &fn, const std::string &sa
     grid_precision min_grid_point[grid_dimension];
     grid.getGridPoint(min_grid_point, min_value_index1d);
 
+    // covariance_by_value adds into the caller's matrix (src/cpu/multispectral_ImageMatcher.cu:818-925
+    // zeroes it first)
+    float covar[grid_dimension * grid_dimension] = {};
+    searcher.covariance_by_value(covar);
+    double covar_trace = 0.0;
+    for (unsigned d = 0; d < grid_dimension; ++d) covar_trace += covar[d * grid_dimension + d];
+
     const std::vector<func_precision> costs = func_values.values();
     std::ofstream(out_path, std::ios::binary).write((const char *) costs.data(), costs.size() * sizeof(float));
     std::printf("{\"fused_index\": %d, \"fused_value\": %.9g, \"extrema_index\": %d, \"extrema_value\": %.9g, "
-                "\"num_elements\": %u, \"point0\": %.9g}\n",
+                "\"num_elements\": %u, \"point0\": %.9g, \"covar_trace\": %.9g}\n",
                 searcher.last_argmin.index, (double) searcher.last_argmin.value, min_value_index1d,
-                (double) min_value, grid.numElements(), (double) min_grid_point[0]);
+                (double) min_value, grid.numElements(), (double) min_grid_point[0], covar_trace);
     (void) hipFree(image_mov.data());
     (void) hipFree(image_fix.data());
     (void) hipFree(grid.data());

@@ -56,6 +56,8 @@ def test_facade_ncc_and_reference_ties(gpu, oracle, tmp_path):
     vo, io = oracle.argmin(want, oracle.TIE_REFERENCE)
     assert abs(want[res["fused_index"]] - vo) <= 2e-5 * abs(vo)
     assert res["fused_index"] == res["extrema_index"]
+    cov, _, _ = oracle.covariance(s, r, c2, costs)      # covariance_by_value over the same tensor
+    assert abs(res["covar_trace"] - np.trace(cov)) <= 1e-5 * abs(np.trace(cov))
 
 
 def test_facade_mi(gpu, oracle, tmp_path):

@@ -208,3 +208,30 @@ def test_invalid_arguments_raise(gpu):
         A.CudaGridSearcher(grid, res).search_by_value(A.calcNCC, m, m)   # 2-D grid, 8-D functor
     with pytest.raises(L.CgsError):
         A.CudaGridSearcher(grid, res).search_by_value(A.calcMIstream.with_(bins=100), m, m)
+
+
+COV_RTOL = 1e-5   # binary64 sums in another order; log from ocml vs libm, both rounded to binary32
+
+
+@pytest.mark.parametrize("name", ["ncc", "sad"])
+def test_covariance_by_value(gpu, oracle, name):
+    """covariance_by_value over a search's cost tensor against the oracle's
+    restatement of calculateCovariance_by_value; SAD costs are positive, so
+    the reference's scan keeps index 0 / value 0 and the matrix is zero."""
+    m, f, _ = synth.make_pair(61, 48, sigma=2.0, perturb=2.0, dtype=np.float32)
+    s, e, c = synth.grid_ranges(4, 3.0)
+    fn = A.calcNCC if name == "ncc" else A.centredAbsoluteDifference
+    got, (v, i), grid = gpu_search(fn, m, f, s, e, c)
+    res = A.CudaTensor(grid.getAxisSampleCounts())
+    searcher = A.CudaGridSearcher(grid, res)
+    searcher.search_by_value(fn, A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f))
+    base = np.eye(8, dtype=np.float32)
+    cov = searcher.covariance_by_value(base.copy())
+    c2, r = oracle.grid_update(s, e, c)
+    want, mi, mv = oracle.covariance(s, r, c2, res.data.cpu().numpy())
+    scale = np.abs(want).max()
+    if name == "ncc":
+        assert mi == i and scale > 0
+    else:
+        assert mi == 0 and mv == 0.0 and scale == 0
+    assert np.allclose(cov - base, want, rtol=COV_RTOL, atol=COV_RTOL * scale)

@@ -327,3 +327,72 @@ def test_argmin_lowest_and_specials(oracle):
 def test_reference_rank_is_a_bijection(oracle):
     ranks = sorted(oracle.reference_rank(i) for i in range(4096))
     assert ranks == list(range(4096))
+
+
+# ------------------------------------------------ covariance around the argmin
+
+def _cov_numpy(start, res, count, result):
+    """Independent restatement of calculateCovariance_by_value
+    (src/gpu/cudaGridSearch.cuh:317-419) in numpy: binary32 per point,
+    binary64 sums."""
+    f32 = np.float32
+    D = len(count)
+    n = len(result)
+    neg = np.where(result < 0)[0]
+    mi = int(neg[np.argmin(result[neg])]) if len(neg) else 0
+    mv = f32(result[mi]) if len(neg) else f32(0.0)
+    idx = np.stack(np.unravel_index(np.arange(n), [int(c) for c in count[::-1]])[::-1], 1)
+    pts = np.array([[np.fma(f32(i), f32(res[a]), f32(start[a])) if hasattr(np, "fma") else
+                     f32(np.float64(f32(i)) * np.float64(res[a]) + np.float64(start[a])) for a, i in enumerate(row)]
+                    for row in idx], np.float32)
+    logmin = f32(np.log(np.float64(abs(mv))))
+    C = np.zeros((D, D))
+    for i in range(n):
+        if i == mi:
+            continue
+        f = f32(result[i]) if result[i] != 0 else f32(-1e-9)
+        dx = (pts[i] - pts[mi]).astype(np.float32)
+        l2 = f32(0)
+        for d in range(D):
+            l2 = f32(l2 + f32(dx[d] * dx[d]))
+        ln = np.sqrt(l2, dtype=np.float32)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            s = f32(f32(logmin - f32(np.log(np.float64(abs(f))))) * f32(f32(1) / ln))
+        if s < 0:
+            continue
+        g = np.array([f32(f32(dx[d] * s) / ln) for d in range(D)], np.float32)
+        C += np.outer(g, g).astype(np.float32).astype(np.float64)
+    return C, mi
+
+
+def test_covariance_matches_numpy_restatement(oracle):
+    rng = np.random.default_rng(41)
+    start = np.array([-1.0, 0.5, -2.0], np.float32)
+    end = np.array([1.0, 1.5, 2.0], np.float32)
+    count = np.array([5, 4, 3], np.float32)
+    c2, r = oracle.grid_update(start, end, count)
+    n = oracle.grid_num_elements(c2)
+    result = -np.abs(rng.normal(1.0, 0.3, n)).astype(np.float32)
+    result[7] = 0.0                                   # read as -1e-9
+    C, mi, mv = oracle.covariance(start, r, c2, result)
+    want, wmi = _cov_numpy(start, r, c2, result)
+    assert mi == wmi == int(np.argmin(result)) and mv == result.min()
+    assert np.allclose(C, want, rtol=1e-6, atol=1e-9 * np.abs(want).max())
+    assert np.allclose(C, C.T) and np.linalg.eigvalsh(C).min() >= -1e-6 * np.abs(C).max()
+
+
+def test_covariance_closed_forms(oracle):
+    # one axis: C = sum over the other points of slope^2, slope = (log|min| - log|f|) / |dx|
+    start, end, count = np.array([0.0], np.float32), np.array([4.0], np.float32), np.array([5.0], np.float32)
+    c2, r = oracle.grid_update(start, end, count)
+    result = np.array([-0.5, -2.0, -1.0, -0.25, -4.0], np.float32)
+    C, mi, mv = oracle.covariance(start, r, c2, result)
+    x = np.arange(5.0)
+    with np.errstate(divide="ignore", invalid="ignore"):   # 0/0 at the minimum itself, which is excluded
+        s = (np.log(4.0) - np.log(np.abs(result.astype(np.float64)))) / np.abs(x - 4.0)
+    keep = (np.arange(5) != 4) & (s >= 0)
+    assert mi == 4 and mv == -4.0
+    assert np.isclose(C[0, 0], np.sum(s[keep] ** 2), rtol=1e-6)
+    # no negative cost: the scan keeps (index 0, value 0), log 0 = -inf, every slope < 0 -> zero matrix
+    C, mi, mv = oracle.covariance(start, r, c2, np.array([3.0, 1.0, 2.0, 5.0, 4.0], np.float32))
+    assert mi == 0 and mv == 0.0 and not C.any()
