@@ -14,6 +14,7 @@ STATUS_NAMES = {0: "CGS_OK", 1: "CGS_ERR_INVALID_ARGUMENT", 2: "CGS_ERR_GRID_TOO
                 4: "CGS_ERR_HIP", 5: "CGS_ERR_NO_DEVICE", 6: "CGS_ERR_TOO_LARGE"}
 
 COST_SAD, COST_NCC, COST_MI, COST_SQD, COST_SAD_SUM, COST_MI_PLAIN = range(6)
+COST_NCC_STREAM, COST_SQD_STREAM, COST_SAD_STREAM = 6, 7, 8
 SAMPLER_NEAREST, SAMPLER_BILINEAR = 0, 1
 PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION = range(4)
 TIE_LOWEST, TIE_REFERENCE = 0, 1

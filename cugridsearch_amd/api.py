@@ -65,6 +65,13 @@ calcNCC = Functor("calcNCC", L.COST_NCC, L.PARAM_CENTRED)
 calcMIstream = Functor("calcMIstream", L.COST_MI, L.PARAM_CENTRED, bins=64)
 # src/gpu/cudaErrorFunction_mi.cuh:8-115
 calcMI = Functor("calcMI", L.COST_MI_PLAIN, L.PARAM_CENTRED, bins=64)
+# the thread-per-row "stream" functors: the same sums, their own final combine
+# src/gpu/cudaErrorFunctionsStreams.cuh:31-83 (2-D translation; no overlap -> 0)
+averageAbsoluteDifference_stream = Functor("averageAbsoluteDifference_stream", L.COST_SAD_STREAM, L.PARAM_TRANSLATION)
+# src/gpu/cudaErrorFunctionsStreams.cuh:85-163 (0 unless both sums of squares are positive)
+calcNCCstream = Functor("calcNCCstream", L.COST_NCC_STREAM, L.PARAM_CENTRED)
+# src/gpu/cudaErrorFunctionsStreams.cuh:283-351 (no overlap sentinel: 0/0)
+calcSQDstream = Functor("calcSQDstream", L.COST_SQD_STREAM, L.PARAM_CENTRED)
 # the centred-homography SAD used by the search workloads
 centredAbsoluteDifference = Functor("centredAbsoluteDifference", L.COST_SAD, L.PARAM_CENTRED)
 

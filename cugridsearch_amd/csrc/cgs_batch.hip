@@ -144,8 +144,8 @@ namespace {
 
 int batch_kind(int cost) {
     switch (cost) {
-    case CGS_COST_NCC: return kBatchNCC;
-    case CGS_COST_SQD: return kBatchSQD;
+    case CGS_COST_NCC: case CGS_COST_NCC_STREAM: return kBatchNCC;
+    case CGS_COST_SQD: case CGS_COST_SQD_STREAM: return kBatchSQD;
     case CGS_COST_MI: case CGS_COST_MI_PLAIN: return kBatchMI;
     default: return kBatchSAD;
     }

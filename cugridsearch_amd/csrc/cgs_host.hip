@@ -94,7 +94,7 @@ cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const c
     if ((st = validate_image(fixed, "fixed")) != CGS_OK) return st;
     if (moved->channels != fixed->channels)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "moved/fixed channel counts differ");
-    if (o->cost < CGS_COST_SAD || o->cost > CGS_COST_MI_PLAIN) return set_error(CGS_ERR_INVALID_ARGUMENT, "bad cost");
+    if (o->cost < CGS_COST_SAD || o->cost > CGS_COST_SAD_STREAM) return set_error(CGS_ERR_INVALID_ARGUMENT, "bad cost");
     if (o->sampler != CGS_SAMPLER_NEAREST && o->sampler != CGS_SAMPLER_BILINEAR)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "bad sampler");
     if (o->param_mode < CGS_PARAM_CENTRED || o->param_mode > CGS_PARAM_TRANSLATION)

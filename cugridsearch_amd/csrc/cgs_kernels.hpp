@@ -167,10 +167,13 @@ template <int FAM>
 __device__ __forceinline__ float finish_cost(int cost, const double *t, bool exact_int) {
     if constexpr (FAM == kFamSAD) {
         if (cost == CGS_COST_SAD_SUM) return (float) t[0];
+        if (cost == CGS_COST_SAD_STREAM && t[3] == 0.0) return 0.0f;
         if (exact_int) return (float) t[0] / (float) t[3];
         return (float) (t[0] / t[3]);
     } else {
         if (cost == CGS_COST_NCC) return t[3] > 0.0 ? (float) (-t[2] / sqrt(t[0] * t[1])) : 0.0f;
+        if (cost == CGS_COST_NCC_STREAM) return (t[0] > 0.0 && t[1] > 0.0) ? (float) (-t[2] / sqrt(t[0] * t[1])) : 0.0f;
+        if (cost == CGS_COST_SQD_STREAM) return (float) (t[2] / sqrt(t[0] * t[1]));
         return t[3] > 0.0 ? (float) (t[2] / sqrt(t[0] * t[1])) : 123123.0f;
     }
 }
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(kThreads) void search_exact_kernel(const SearchPara
                         const double md = (double) m, fd = (double) f;
                         acc0[g] = fma(fd, fd, acc0[g]);
                         acc1[g] = fma(md, md, acc1[g]);
-                        if (P.cost == CGS_COST_NCC) acc2[g] = fma(fd, md, acc2[g]);
+                        if (P.cost == CGS_COST_NCC || P.cost == CGS_COST_NCC_STREAM) acc2[g] = fma(fd, md, acc2[g]);
                         else acc2[g] = fma(md - fd, md - fd, acc2[g]);
                     } else {
                         atomicAdd(&s_hist[g * B2 + mi_bin(f, P.bins) * P.bins + mi_bin(m, P.bins)], 1u);

@@ -132,7 +132,8 @@ static hipError_t exact_gpt(const SearchParams &P, int mpix, int fpix, hipStream
 }
 
 static int family_of(int cost) {
-    if (cost == CGS_COST_NCC || cost == CGS_COST_SQD) return kFamNCC;
+    if (cost == CGS_COST_NCC || cost == CGS_COST_SQD || cost == CGS_COST_NCC_STREAM || cost == CGS_COST_SQD_STREAM)
+        return kFamNCC;
     if (cost == CGS_COST_MI || cost == CGS_COST_MI_PLAIN) return kFamMI;
     return kFamSAD;
 }
@@ -177,8 +178,8 @@ static hipError_t fast_pix(const SearchParams &P, int fpix, int gpt, hipStream_t
 }
 
 static hipError_t dispatch_fast(const SearchParams &P, int fpix, int gpt, hipStream_t stream) {
-    if (P.cost == CGS_COST_NCC) return fast_pix<1>(P, fpix, gpt, stream);
-    if (P.cost == CGS_COST_SQD) return fast_pix<2>(P, fpix, gpt, stream);
+    if (P.cost == CGS_COST_NCC || P.cost == CGS_COST_NCC_STREAM) return fast_pix<1>(P, fpix, gpt, stream);
+    if (P.cost == CGS_COST_SQD || P.cost == CGS_COST_SQD_STREAM) return fast_pix<2>(P, fpix, gpt, stream);
     return fast_pix<0>(P, fpix, gpt, stream);
 }
 

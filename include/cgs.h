@@ -44,7 +44,12 @@ typedef enum {
     CGS_COST_MI = 2,        /* calcMIstream (-2MI/(Hx+Hy)) src/gpu/cudaErrorFunction_miStreams.cuh:212-368 */
     CGS_COST_SQD = 3,       /* calcSQD                    src/gpu/cudaErrorFunctions.cuh:378-446 */
     CGS_COST_SAD_SUM = 4,   /* sumOfAbsoluteDifferences   src/gpu/cudaErrorFunctions.cuh:89-114  */
-    CGS_COST_MI_PLAIN = 5   /* calcMI (-MI)               src/gpu/cudaErrorFunction_mi.cuh:8-115  */
+    CGS_COST_MI_PLAIN = 5,  /* calcMI (-MI)               src/gpu/cudaErrorFunction_mi.cuh:8-115  */
+    /* the thread-per-row "stream" functors: same sums, their own final combine */
+    CGS_COST_NCC_STREAM = 6, /* calcNCCstream: 0 unless both sums > 0   src/gpu/cudaErrorFunctionsStreams.cuh:85-163 */
+    CGS_COST_SQD_STREAM = 7, /* calcSQDstream: no overlap sentinel      src/gpu/cudaErrorFunctionsStreams.cuh:283-351 */
+    CGS_COST_SAD_STREAM = 8  /* averageAbsoluteDifference_stream: no overlap -> 0
+                                                              src/gpu/cudaErrorFunctionsStreams.cuh:31-83 */
 } cgs_cost;
 
 /* CudaImage samplers (include/cudaTensor.cuh:557-575). */

@@ -67,6 +67,10 @@ constexpr CgsFunctor calcSQD{CGS_COST_SQD, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINE
 constexpr CgsFunctor calcNCC{CGS_COST_NCC, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
 constexpr CgsFunctor calcMIstream{CGS_COST_MI, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
 constexpr CgsFunctor calcMI{CGS_COST_MI_PLAIN, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+// the thread-per-row "stream" functors (src/gpu/cudaErrorFunctionsStreams.cuh)
+constexpr CgsFunctor averageAbsoluteDifference_stream{CGS_COST_SAD_STREAM, CGS_PARAM_TRANSLATION, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcNCCstream{CGS_COST_NCC_STREAM, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcSQDstream{CGS_COST_SQD_STREAM, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
 // centred-homography SAD (the search workloads' SAD)
 constexpr CgsFunctor centredAbsoluteDifference{CGS_COST_SAD, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
 

@@ -65,7 +65,8 @@
 #define ORC_MAXD 8
 
 enum { ORC_COST_SAD = 0, ORC_COST_NCC = 1, ORC_COST_MI = 2, ORC_COST_SQD = 3,
-       ORC_COST_SAD_SUM = 4, ORC_COST_MI_PLAIN = 5 };
+       ORC_COST_SAD_SUM = 4, ORC_COST_MI_PLAIN = 5, ORC_COST_NCC_STREAM = 6, ORC_COST_SQD_STREAM = 7,
+       ORC_COST_SAD_STREAM = 8 };
 enum { ORC_NEAREST = 0, ORC_BILINEAR = 1 };
 enum { ORC_PARAM_CENTRED = 0, ORC_PARAM_NONCENTRED = 1, ORC_PARAM_RAW_H = 2,
        ORC_PARAM_TRANSLATION = 3 };
@@ -335,15 +336,15 @@ float orc_cost_point(const orc_image *moved, const orc_image *fixed, int cost, i
                 const float f = pix(fixed, ((int64_t) y * fixed->width + x) * C + c);
                 a.count++;
                 switch (cost) {
-                case ORC_COST_SAD: case ORC_COST_SAD_SUM:
+                case ORC_COST_SAD: case ORC_COST_SAD_SUM: case ORC_COST_SAD_STREAM:
                     if (exact_int) a.isum += (int64_t) fabsf(m - f);
                     else a.sum += (double) fabsf(m - f);
                     break;
-                case ORC_COST_NCC: case ORC_COST_SQD: {
+                case ORC_COST_NCC: case ORC_COST_SQD: case ORC_COST_NCC_STREAM: case ORC_COST_SQD_STREAM: {
                     const double md = (double) m / 255.0, fd = (double) f / 255.0;
                     a.i1 += fd * fd;
                     a.i2 += md * md;
-                    if (cost == ORC_COST_NCC) a.ic += fd * md;
+                    if (cost == ORC_COST_NCC || cost == ORC_COST_NCC_STREAM) a.ic += fd * md;
                     else a.ic += (md - fd) * (md - fd);
                     break;
                 }
@@ -367,6 +368,17 @@ float orc_cost_point(const orc_image *moved, const orc_image *fixed, int cost, i
         break;
     case ORC_COST_SQD:
         out = a.count > 0 ? (float) (a.ic / sqrt(a.i1 * a.i2)) : 123123.0f;
+        break;
+    case ORC_COST_SAD_STREAM:   /* averageAbsoluteDifference_stream: no overlap -> 0 (cudaErrorFunctionsStreams.cuh:71-78) */
+        if (a.count == 0) out = 0.0f;
+        else if (exact_int) out = (float) a.isum / (float) a.count;
+        else out = (float) (a.sum / (double) a.count);
+        break;
+    case ORC_COST_NCC_STREAM:   /* calcNCCstream: 0 unless both sums are positive (cudaErrorFunctionsStreams.cuh:158-161) */
+        out = (a.i1 > 0 && a.i2 > 0) ? (float) (-a.ic / sqrt(a.i1 * a.i2)) : 0.0f;
+        break;
+    case ORC_COST_SQD_STREAM:   /* calcSQDstream: no sentinel (cudaErrorFunctionsStreams.cuh:337-350) */
+        out = (float) (a.ic / sqrt(a.i1 * a.i2));
         break;
     default: {
         /* calcMIstream (cudaErrorFunction_miStreams.cuh:306-365) in binary64 */

@@ -44,7 +44,8 @@ def close(a, b, rtol, atol=0.0):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     both_nan = np.isnan(a) & np.isnan(b)
-    ok = both_nan | (np.abs(a - b) <= rtol * np.abs(b) + atol)
+    with np.errstate(invalid="ignore"):
+        ok = both_nan | (a == b) | (np.abs(a - b) <= rtol * np.abs(b) + atol)
     return bool(ok.all()), np.where(~ok)[0][:10]
 
 
@@ -173,6 +174,49 @@ def test_functor_modes(gpu, oracle, name, fn, dtype, channels, rtol):
         assert ok, (bad, got[bad], want[bad])
         vo, _ = oracle.argmin(want)
         assert abs(want[i] - vo) <= 2 * rtol * abs(vo)
+
+
+def test_stream_functor_finish_rules(gpu, oracle):
+    """calcNCCstream / calcSQDstream / averageAbsoluteDifference_stream on grids
+    whose shifts reach the special cases: no overlap (SQD stream 0/0 = NaN,
+    SAD stream 0) and an overlap on the zero half of the fixed image (NCC stream
+    0 where calcNCC gives NaN).  Single search and the batch both match the oracle."""
+    m, f, _ = synth.make_pair(44, 40, 36, sigma=2.0, perturb=3.0, dtype=np.uint8)
+    f = f.copy()
+    f[:, :24] = 0
+    s, e, c = synth.grid_ranges(3, 3.0)
+    s[4], e[4], c[4] = -60.0, 60.0, 7
+    cases = [(A.calcNCCstream, s, e, c), (A.calcSQDstream, s, e, c),
+             (A.calcNCCstream.with_(sampler=L.SAMPLER_NEAREST), s, e, c),
+             (A.averageAbsoluteDifference_stream, [-50, -50], [50, 50], [21, 21]),
+             (A.averageAbsoluteDifference_stream.with_(sampler=L.SAMPLER_NEAREST), [-50, -50], [50, 50], [21, 21])]
+    pairs, wants = [], []
+    for fn, s_, e_, c_ in cases:
+        got, (v, i), grid = gpu_search(fn, m, f, s_, e_, c_)
+        want = oracle_search(oracle, fn, m, f, s_, e_, c_)
+        exact = fn.cost == L.COST_SAD_STREAM and fn.sampler == L.SAMPLER_NEAREST
+        if exact:
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        else:
+            ok, bad = close(got, want, NCC_RTOL, atol=1e-30)
+            assert ok, (fn.name, bad, got[bad], want[bad])
+        vo, io = oracle.argmin(want)
+        assert (v, i) == (vo, io) or abs(want[i] - vo) <= 2 * NCC_RTOL * abs(vo)
+        # the special cases are reached
+        if fn.cost == L.COST_NCC_STREAM:
+            plain_ncc = oracle_search(oracle, A.calcNCC.with_(sampler=fn.sampler), m, f, s_, e_, c_)
+            assert np.any(np.isnan(plain_ncc) & (want == 0.0))
+        elif fn.cost == L.COST_SQD_STREAM:
+            assert np.isnan(want).any() and np.isfinite(want).any()
+        else:
+            assert (want == 0.0).sum() > 1
+        pairs.append((fn, grid, A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f), A.CudaTensor(grid.getAxisSampleCounts())))
+        wants.append(got)
+    b = A.BatchSearcher()
+    b.plan(pairs)
+    b.run(use_graph=False)
+    for (fn, _, _, _, res), want in zip(pairs, wants):
+        assert np.array_equal(res.data.cpu().numpy().view(np.uint32), want.view(np.uint32)), fn.name
 
 
 def test_driver_images_translation(gpu, oracle):

@@ -218,6 +218,45 @@ def test_sad_ncc_sqd_translation_brute_force(oracle, sampler):
         assert abs(sqd - want) <= 1e-6 * abs(want)
 
 
+def test_stream_finish_rules_brute_force(oracle):
+    """The stream functors' own final combines (cudaErrorFunctionsStreams.cuh:71-78,
+    158-161, 337-350) on integer translations, against numpy sums: a zero
+    left half of the fixed image makes an overlap with i1 = 0, a shift past
+    the image makes no overlap at all."""
+    moved, fixed = _shift_pair(seed=4)
+    fixed[:, :12] = 0
+    T = oracle.PARAM_TRANSLATION
+    for tx, ty in [(0, 0), (3, -2), (14, 1), (-20, 0), (30, 0), (0, -25)]:
+        f, m = _numpy_translation_terms(moved, fixed, tx, ty)
+        p = [tx, ty]
+        i1, i2 = (f * f).sum(), (m * m).sum()
+        sad_s, aux = oracle.cost_point(moved, fixed, oracle.COST_SAD_STREAM, 0, p, mode=T)
+        assert aux[4] == len(f)
+        want = np.float32(0) if len(f) == 0 else np.float32(np.float32(np.abs(m - f).sum()) / np.float32(len(f)))
+        assert sad_s == want, (tx, ty)
+        ncc_s, _ = oracle.cost_point(moved, fixed, oracle.COST_NCC_STREAM, 0, p, mode=T)
+        ncc, _ = oracle.cost_point(moved, fixed, oracle.COST_NCC, 0, p, mode=T)
+        if i1 > 0 and i2 > 0:
+            want = -(f * m).sum() / math.sqrt(i1 * i2)
+            assert abs(ncc_s - want) <= 1e-6 * abs(want) and ncc_s == ncc
+        else:
+            assert ncc_s == 0.0
+            assert (len(f) == 0 and ncc == 0.0) or (len(f) > 0 and math.isnan(ncc))
+        sqd_s, _ = oracle.cost_point(moved, fixed, oracle.COST_SQD_STREAM, 0, p, mode=T)
+        sqd, _ = oracle.cost_point(moved, fixed, oracle.COST_SQD, 0, p, mode=T)
+        if len(f) == 0:
+            assert math.isnan(sqd_s) and sqd == np.float32(123123.0)
+        elif i1 > 0:
+            want = ((m - f) ** 2).sum() / math.sqrt(i1 * i2)
+            assert abs(sqd_s - want) <= 1e-6 * abs(want) and sqd_s == sqd
+        else:
+            assert math.isinf(sqd_s) and sqd_s == sqd
+    # the shifts above do reach the special cases
+    assert len(_numpy_translation_terms(moved, fixed, 30, 0)[0]) == 0
+    f, _ = _numpy_translation_terms(moved, fixed, 14, 1)
+    assert len(f) > 0 and (f * f).sum() == 0
+
+
 def test_ncc_invariants(oracle):
     m, f, _ = synth.make_pair(5, 32, dtype=np.float32)
     ident = [0, 1, 1, 0, 0, 0, 0, 0]
