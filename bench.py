@@ -29,6 +29,10 @@ NONPACKED_FP32_TFLOPS = 78.6    # the same without packed FP32, which this build
 # NCC 6, MI 2, SAD 3
 FLOP_PER_PE = {("nearest", "sad"): 17 + 3, ("bilinear", "ncc"): 17 + 13 + 6, ("bilinear", "mi"): 17 + 13 + 2,
                ("bilinear", "sad"): 17 + 13 + 3}
+# line-traversal functors, per sample: the point on the line 4 (calcMIAlt 8 with
+# its rescale), two bilinear samples 2 x 13, then NCC/SQD 6 or MI 2
+FLOP_PER_PE_ALT = {"calcSQDAlt": 4 + 26 + 6, "calcNCCAlt": 4 + 26 + 6, "calcNCCstreamAlt": 4 + 26 + 6,
+                   "calcMIstreamAlt": 4 + 26 + 2, "calcMIAlt": 8 + 26 + 2}
 BATCH_WORKLOAD = "batch_ragged_256"
 PROPOSAL_WORKLOAD = "proposal_1024"
 CNN_WORKLOAD = "cnn_train_step"
@@ -42,6 +46,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="bilinear_ncc_256")
+    ap.add_argument("--functor", default=None,
+                    help="run the workload's images and grid through another functor (e.g. calcNCCAlt)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--pairs", type=int, default=256, help="pairs of the ragged batch workload")
@@ -49,19 +55,18 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(m, f, spec, start, end, count, pe_per_point, target_s):
+def cpu_baseline(m, f, fn, start, end, count, pe_per_point, target_s):
     from oracle import oracle as O
     threads = min(16, os.cpu_count() or 1)
     c2, r = O.grid_update(start, end, count)
-    cost = {"sad": O.COST_SAD, "ncc": O.COST_NCC, "mi": O.COST_MI}[spec["cost"]]
-    samp = {"nearest": O.NEAREST, "bilinear": O.BILINEAR}[spec["sampler"]]
+    kw = dict(mode=fn.param_mode, bins=fn.bins, nthreads=threads)
     n = 256
     t = time.perf_counter()
-    O.search(m, f, cost, samp, start, r, c2, bins=spec.get("bins", 32), i0=0, i1=n, nthreads=threads)
+    O.search(m, f, fn.cost, fn.sampler, start, r, c2, i0=0, i1=n, **kw)
     dt = time.perf_counter() - t
     n = int(min(O.grid_num_elements(c2), max(256, n * target_s / max(dt, 1e-6))))
     t = time.perf_counter()
-    O.search(m, f, cost, samp, start, r, c2, bins=spec.get("bins", 32), i0=0, i1=n, nthreads=threads)
+    O.search(m, f, fn.cost, fn.sampler, start, r, c2, i0=0, i1=n, **kw)
     dt = time.perf_counter() - t
     return {"value": n * pe_per_point / dt / 1e9, "unit": "Gpixel-evals/s", "cores": threads, "kind": "port",
             "sample": f"grid indices [0, {n}) of the same workload ({n * pe_per_point:.3e} pixel-evals, "
@@ -148,8 +153,9 @@ def run_batch(args):
     if not args.no_cpu:
         sp = specs[0]
         m, f = sp["moving"], sp["fixed"]
-        line["cpu_baseline"] = cpu_baseline(m, f, dict(cost=sp["cost"], sampler="bilinear", bins=32), sp["start"],
-                                            sp["end"], sp["count"], f.shape[0] * f.shape[1], args.cpu_seconds)
+        fn0 = {"sad": A.centredAbsoluteDifference, "ncc": A.calcNCC, "mi": A.calcMIstream}[sp["cost"]].with_(bins=32)
+        line["cpu_baseline"] = cpu_baseline(m, f, fn0, sp["start"], sp["end"], sp["count"], f.shape[0] * f.shape[1],
+                                            args.cpu_seconds)
         line["cpu_baseline"]["sample"] = "first pair of the batch: " + line["cpu_baseline"]["sample"]
     return line
 
@@ -161,6 +167,7 @@ def run_proposal(args):
     full-grid argmin, which is searched first as the judge and as the plain
     pairs/s; plus the ablation without the network."""
     import torch
+    from cugridsearch_amd import api as A
     from cugridsearch_amd import cnn
     from cugridsearch_amd import proposal as P
 
@@ -202,8 +209,7 @@ def run_proposal(args):
     }
     if not args.no_cpu:
         m, f = images[0][0].tensor.cpu().numpy(), images[0][1].tensor.cpu().numpy()
-        cb = cpu_baseline(m, f, dict(cost="ncc", sampler="bilinear"), P.GRID_START, P.GRID_END, P.GRID_COUNT,
-                          256 * 256, args.cpu_seconds)
+        cb = cpu_baseline(m, f, A.calcNCC, P.GRID_START, P.GRID_END, P.GRID_COUNT, 256 * 256, args.cpu_seconds)
         cb["value"] = cb["value"] * 1e9 / (g.numElements() * 256 * 256)
         cb["unit"] = "pairs/s (plain full grid)"
         cb["sample"] = "held-out pair 0: " + cb["sample"]
@@ -291,9 +297,16 @@ def main():
     from cugridsearch_amd import synth
 
     m, f, start, end, count, spec = synth.workload(args.workload)
-    fn = {"ncc": A.calcNCC, "sad": A.centredAbsoluteDifference, "mi": A.calcMIstream}[spec["cost"]]
-    fn = fn.with_(sampler=L.SAMPLER_NEAREST if spec["sampler"] == "nearest" else L.SAMPLER_BILINEAR,
-                  bins=spec.get("bins", 32))
+    if args.functor:
+        fn = getattr(A, args.functor)
+        if fn.cost < L.COST_SQD_ALT:
+            raise SystemExit("--functor takes a line-traversal functor (calcSQDAlt, calcNCCAlt, ...)")
+    else:
+        fn = {"ncc": A.calcNCC, "sad": A.centredAbsoluteDifference, "mi": A.calcMIstream}[spec["cost"]]
+        fn = fn.with_(sampler=L.SAMPLER_NEAREST if spec["sampler"] == "nearest" else L.SAMPLER_BILINEAR,
+                      bins=spec.get("bins", 32))
+    if fn.cost == L.COST_MI_ALT:   # calcMIAlt's exact identity has no qualifying row: step the grid off it
+        start[4], end[4] = start[4] + 0.3, end[4] + 0.3
     dev = torch.device("cuda:0")
     grid = A.CudaGrid(8)
     grid.setStartPoint(start)
@@ -331,18 +344,19 @@ def main():
 
     matched = None
     gold_path = os.path.join(ROOT, "tests", "golden", "workloads.json")
-    if os.path.exists(gold_path):
+    if os.path.exists(gold_path) and not args.functor:
         gold = json.load(open(gold_path)).get(args.workload)
         if gold:
             rtol = 0.0 if spec["sampler"] == "nearest" else 1e-5
             matched = bool(out.index == gold["argmin_index"] or
                            abs(out.value - gold["argmin_value"]) <= rtol * abs(gold["argmin_value"]))
 
-    flop = pe * FLOP_PER_PE[(spec["sampler"], spec["cost"])]
+    flop_pe = FLOP_PER_PE_ALT[fn.name] if args.functor else FLOP_PER_PE[(spec["sampler"], spec["cost"])]
+    flop = pe * flop_pe
     achieved = flop / (kernel_ms / 1000.0) / 1e12
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"search_{args.workload}_traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and not args.functor:
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     line = {
         "metric": METRIC,
@@ -351,9 +365,10 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": args.workload, "image": f"{f.shape[1]}x{f.shape[0]} {f.dtype}",
-                   "grid_points": n, "levels": spec["levels"], "cost": spec["cost"], "sampler": spec["sampler"],
-                   "pixel_evals_per_step": pe},
+        "config": {"workload": args.workload + (f"+{fn.name}" if args.functor else ""),
+                   "image": f"{f.shape[1]}x{f.shape[0]} {f.dtype}",
+                   "grid_points": n, "levels": spec["levels"], "cost": fn.name if args.functor else spec["cost"],
+                   "sampler": spec["sampler"], "pixel_evals_per_step": pe},
         "pairs_per_s": 1000.0 / ms_per_step,
         "argmin_matched": matched,
         "argmin": {"index": out.index, "value": out.value},
@@ -361,10 +376,10 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_VALU_TFLOPS, "traffic": traffic,
                      "frac_of_nonpacked_fp32": achieved / NONPACKED_FP32_TFLOPS,
-                     "flop_per_pixel_eval": FLOP_PER_PE[(spec["sampler"], spec["cost"])]},
+                     "flop_per_pixel_eval": flop_pe},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(m, f, spec, start, end, count, pe_per_point, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(m, f, fn, start, end, count, pe_per_point, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

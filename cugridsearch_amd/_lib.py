@@ -15,12 +15,14 @@ STATUS_NAMES = {0: "CGS_OK", 1: "CGS_ERR_INVALID_ARGUMENT", 2: "CGS_ERR_GRID_TOO
 
 COST_SAD, COST_NCC, COST_MI, COST_SQD, COST_SAD_SUM, COST_MI_PLAIN = range(6)
 COST_NCC_STREAM, COST_SQD_STREAM, COST_SAD_STREAM = 6, 7, 8
+COST_SQD_ALT, COST_NCC_ALT, COST_NCC_STREAM_ALT, COST_MI_STREAM_ALT, COST_MI_ALT = 9, 10, 11, 12, 13
+MI_COSTS = (COST_MI, COST_MI_PLAIN, COST_MI_STREAM_ALT, COST_MI_ALT)
 SAMPLER_NEAREST, SAMPLER_BILINEAR = 0, 1
-PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION = range(4)
+PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION, PARAM_NORM = range(5)
 TIE_LOWEST, TIE_REFERENCE = 0, 1
 PIX_U8, PIX_F32 = 0, 1
 MAX_DIM = 8
-ABI_VERSION = 4          # CGS_ABI_VERSION of include/cgs.h these layouts follow
+ABI_VERSION = 5          # CGS_ABI_VERSION of include/cgs.h these layouts follow
 FLAG_TIME_KERNEL = 1
 
 

@@ -72,6 +72,18 @@ averageAbsoluteDifference_stream = Functor("averageAbsoluteDifference_stream", L
 calcNCCstream = Functor("calcNCCstream", L.COST_NCC_STREAM, L.PARAM_CENTRED)
 # src/gpu/cudaErrorFunctionsStreams.cuh:283-351 (no overlap sentinel: 0/0)
 calcSQDstream = Functor("calcSQDstream", L.COST_SQD_STREAM, L.PARAM_CENTRED)
+# line traversal: each fixed row walks its line through the moving image in
+# unit lambda steps, with an overlap test and a sentinel
+# src/gpu/cudaErrorFunctions.cuh:226-300 (123123)
+calcSQDAlt = Functor("calcSQDAlt", L.COST_SQD_ALT, L.PARAM_CENTRED)
+# src/gpu/cudaErrorFunctions.cuh:302-376 (123123)
+calcNCCAlt = Functor("calcNCCAlt", L.COST_NCC_ALT, L.PARAM_CENTRED)
+# src/gpu/cudaErrorFunctionsStreams.cuh:165-281 (0)
+calcNCCstreamAlt = Functor("calcNCCstreamAlt", L.COST_NCC_STREAM_ALT, L.PARAM_CENTRED)
+# src/gpu/cudaErrorFunction_miStreams.cuh:9-210 (64 bins, 123123)
+calcMIstreamAlt = Functor("calcMIstreamAlt", L.COST_MI_STREAM_ALT, L.PARAM_CENTRED, bins=64)
+# src/gpu/cudaErrorFunction_mi.cuh:117-255: normalised H and lines, -MI * fixed area (64 bins)
+calcMIAlt = Functor("calcMIAlt", L.COST_MI_ALT, L.PARAM_NORM, bins=64)
 # the centred-homography SAD used by the search workloads
 centredAbsoluteDifference = Functor("centredAbsoluteDifference", L.COST_SAD, L.PARAM_CENTRED)
 
@@ -286,7 +298,7 @@ def eval_point(functor, img_moved, img_fixed, params, stream=None):
     cost = ctypes.c_float()
     hist = None
     hp = None
-    if functor.cost in (L.COST_MI, L.COST_MI_PLAIN):
+    if functor.cost in L.MI_COSTS:
         hist = np.zeros(functor.bins * functor.bins, np.uint32)
         hp = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
     p = [float(v) for v in params]

@@ -1,0 +1,539 @@
+// cgs_alt.hip -- the line-traversal ("Alt") costs of the reference:
+// calcSQDAlt, calcNCCAlt (src/gpu/cudaErrorFunctions.cuh:226-376),
+// calcNCCstreamAlt (src/gpu/cudaErrorFunctionsStreams.cuh:165-281),
+// calcMIstreamAlt (src/gpu/cudaErrorFunction_miStreams.cuh:9-210) and
+// calcMIAlt (src/gpu/cudaErrorFunction_mi.cuh:117-255).
+//
+// These functors do not warp pixel by pixel.  Each fixed row maps to a line
+// p0 + lambda v through the moving image (parametricAssignValues,
+// src/gpu/cudaImageFunctions.cuh:315-441); the row's samples sit at lambda =
+// start, start + step, ... below the line's end, with lambda accumulated in
+// binary32 and the fixed image sampled at (row, lambda).
+//
+// Two kernels, one grid point per wavefront (4 per workgroup):
+//   alt_step_kernel  -- the four unit-step functors.  The wavefront walks the
+//     rows in order; its 64 lanes take 64 consecutive lambda steps of a row,
+//     so moving-image taps along the line and fixed-image taps along the row
+//     are neighbours across lanes.  Lane j's lambda is the binary32 sequence
+//     lambda += 1 evaluated in closed form (advance_unit).  Step k of a row
+//     accumulates into lane k mod 64, and the lanes combine in xor-butterfly
+//     order: the sums are exactly the oracle's.
+//   alt_norm_kernel  -- calcMIAlt, whose lambda step 1/(W_f - 1) rounds at
+//     every step: a lane per row, walking its row sequentially as the
+//     reference does.  Row values and the fixed-area table come from the host.
+// MI costs build the joint histogram of their wavefront in LDS.  Compiled
+// with -ffp-contract=off (see cgs_device.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "cgs.h"
+#include "cgs_device.hpp"
+#include "cgs_host.hpp"
+#include "cgs_kernels.hpp"
+
+namespace cgs {
+
+struct AltParams {
+    GridDev grid;
+    const float2 *sincos;   // (cos, sin) per level of axis 0
+    int cost;
+    const void *mov;        // moving image + zero pad
+    const void *fix;        // fixed image + zero pad
+    int mw, mh, fw, fh, C;
+    int bins;
+    const float *rows;      // calcMIAlt: the row values
+    int nrows;
+    const float *area;      // calcMIAlt: fixed area after n steps
+    int narea;
+    float *result;
+    unsigned long long *key;
+    int tie_mode;
+    uint32_t *hist_dump;    // histogram of grid point 0 (diagnostics)
+};
+
+struct AltLine {
+    float ls, le, vx, vy, px, py;
+    bool in;
+};
+
+// parametricAssignValues (src/gpu/cudaImageFunctions.cuh:315-376): the
+// reference's expressions in their association order, repeated factors
+// named once (dt, a0, aw, gy, gx).
+__device__ __forceinline__ AltLine line_centred(const float *h, float MW, float MH, float FW, float FH, float r) {
+    const float h11 = h[0], h12 = h[1], h13 = h[2], h21 = h[3], h22 = h[4], h23 = h[5], h31 = h[6], h32 = h[7];
+    AltLine L;
+    const float dt = h11*h22 - h12*h21 - h11*h23*h32 + h12*h23*h31 + h13*h21*h32 - h13*h22*h31;
+    const float a0 = h11*h22 - h12*h21 - h11*h32*r + h12*h31*r;
+    const float aw = h11*h22 - h12*h21 + FW*h21*h32 - FW*h22*h31 - h11*h32*r + h12*h31*r;
+    const float gy = h21 - h31*r, gx = h22 - h32*r;
+    L.vx = (gx*(dt))/(a0*aw);
+    L.vy = -(gy*(dt))/(a0*aw);
+    const float q = h12*h21 - h11*h22 + r*(h11*h32 - h12*h31);
+    L.px = (h13*h22 - h12*h23 + r*(h12 - h13*h32))/q;
+    L.py = -(h13*h21 - h11*h23 + r*(h11 - h13*h31))/q;
+    const float lt = -((h11*h23 - h13*h21 - h11*r + h13*h31*r)*(aw))/(gy*(dt));
+    const float xt = -(h23 - r)/gy;
+    const float lb = -((aw)*(h11*h23 - h13*h21 - h11*r + h11*h22*MH - h12*h21*MH + h13*h31*r - h11*h32*MH*r
+                             + h12*h31*MH*r))/(gy*(dt));
+    const float xb = -(h23 - r + h22*MH - h32*MH*r)/gy;
+    const float ll = -((h12*h23 - h13*h22 - h12*r + h13*h32*r)*(aw))/(gx*(dt));
+    const float yl = -(h23 - r)/gx;
+    const float lr = -((aw)*(h12*h23 - h13*h22 - h12*r - h11*h22*MW + h12*h21*MW + h13*h32*r + h11*h32*MW*r
+                             - h12*h31*MW*r))/(gx*(dt));
+    const float yr = -(h23 - r + h21*MW - h31*MW*r)/gx;
+    L.ls = 0.0f;
+    L.le = FW;
+    L.in = false;
+    if (xt >= 0 && xt < MW && lt >= 0 && lt < FW) { if (L.vy > 0) L.ls = lt; else L.le = lt; L.in = true; }
+    if (xb >= 0 && xb < MW && lb >= 0 && lb < FW) { if (L.vy < 0) L.ls = lb; else L.le = lb; L.in = true; }
+    if (yl >= 0 && yl < MH && ll >= 0 && ll < FH) { if (L.vx > 0) L.ls = ll; else L.le = ll; L.in = true; }
+    if (yr >= 0 && yr < MH && lr >= 0 && lr < FH) { if (L.vx < 0) L.ls = lr; else L.le = lr; L.in = true; }
+    return L;
+}
+
+// parametricAssignValuesNorm (src/gpu/cudaImageFunctions.cuh:378-441),
+// column -0.5; ty / tx are the quotients shared with p0 = (tx, -ty).
+__device__ __forceinline__ AltLine line_norm(const float *h, float row) {
+    const float h11 = h[0], h12 = h[1], h13 = h[2], h21 = h[3], h22 = h[4], h23 = h[5], h31 = h[6], h32 = h[7];
+    const float col = -0.5f;
+    AltLine L;
+    const float dt = h11*h22 - h12*h21 - h11*h23*h32 + h12*h23*h31 + h13*h21*h32 - h13*h22*h31;
+    L.vy = -(h21 - h31*row)/dt;
+    L.vx = (h22 - h32*row)/dt;
+    const float q = h11*h22 - h12*h21 + col*(h21*h32 - h22*h31) - row*(h11*h32 - h12*h31);
+    const float tx = (h12*h23 - h13*h22 + col*(h22 - h23*h32) - row*(h12 - h13*h32))/q;
+    const float ty = (h11*h23 - h13*h21 + col*(h21 - h23*h31) - row*(h11 - h13*h31))/q;
+    L.px = tx;
+    L.py = -ty;
+    const float gy = h21 - h31*row, gx = h22 - h32*row;
+    const float lt = -((ty - 0.5f)*(dt))/gy;
+    const float xt = (h22 - 2*h23 + 2*row - h32*row)/(2*gy);
+    const float lb = -((ty + 0.5f)*(dt))/gy;
+    const float xb = -(h22 + 2*h23 - 2*row - h32*row)/(2*gy);
+    const float ll = -((tx + 0.5f)*(dt))/gx;
+    const float yl = (h21 - 2*h23 + 2*row - h31*row)/(2*gx);
+    const float lr = -((tx - 0.5f)*(dt))/gx;
+    const float yr = -(h21 + 2*h23 - 2*row - h31*row)/(2*gx);
+    L.ls = 0.0f;
+    L.le = 1.0f;
+    L.in = false;
+    if (xt > -0.5f && xt < 0.5f && lt >= 0 && lt < 1) { if (L.vy > 0) L.ls = lt; else L.le = lt; L.in = true; }
+    if (xb > -0.5f && xb < 0.5f && lb > 0 && lb < 1) { if (L.vy < 0) L.ls = lb; else L.le = lb; L.in = true; }
+    if (yl > -0.5f && yl < 0.5f && ll > 0 && ll < 1) { if (L.vx > 0) L.ls = ll; else L.le = ll; L.in = true; }
+    if (yr > -0.5f && yr < 0.5f && lr > 0 && lr < 1) { if (L.vx < 0) L.ls = lr; else L.le = lr; L.in = true; }
+    return L;
+}
+
+// parametersToHomographyNorm (src/gpu/cudaImageFunctions.cuh:181-214): integer
+// size ratios and centres, the moving centre's y taken from the width.
+__device__ __forceinline__ void homography_norm(const float *p, float c, float s, int mw, int mh, int fw, int fh,
+                                                float *h) {
+    const float x_ratio = (float) (fw / mw), y_ratio = (float) (fh / mh);
+    const float cfx = (float) (fw / 2), cfy = (float) (fh / 2), cmx = (float) (mw / 2), cmy = (float) (mw / 2);
+    const float scaleX = p[1] / x_ratio, scaleY = p[2] / y_ratio, shearXY = p[3];
+    h[0] = scaleX * c;
+    h[1] = scaleY * shearXY * c - scaleY * s;
+    h[2] = (p[4] - cfx + cmx) / (float) fw;
+    h[3] = scaleX * s;
+    h[4] = scaleY * shearXY * s + scaleY * c;
+    h[5] = (p[5] - cfy + cmy) / (float) fh;
+    h[6] = p[6];
+    h[7] = p[7];
+}
+
+// The value lambda reaches after r steps of lambda = lambda + 1.0f from x
+// (x >= 0 or -0, x < 2^23).  Inside a binade [2^e, 2^(e+1)) every +1 is
+// exact, so the steps that stay below the binade's top B are taken at once
+// (x + t, exact); the step that crosses B is taken as the one rounded add
+// x + 1.0f; below 1 every step crosses.
+__device__ __forceinline__ float advance_unit(float x, int r) {
+    while (r > 0) {
+        if (!(x >= 1.0f)) {
+            x = x + 1.0f;
+            r -= 1;
+            continue;
+        }
+        const float B = __uint_as_float((__float_as_uint(x) & 0x7F800000u) + 0x00800000u);
+        const float t = ceilf(B - x) - 1.0f;   // B - x is exact (Sterbenz)
+        if ((float) r <= t) {
+            x = x + (float) r;
+            r = 0;
+        } else {
+            x = x + t;
+            x = x + 1.0f;
+            r -= (int) t + 1;
+        }
+    }
+    return x;
+}
+
+// CudaImage::valueAt (include/cudaTensor.cuh:552-575): the bilinear value,
+// 0 outside the image.  img is a zero-padded copy.
+template <typename T>
+__device__ __forceinline__ float value_at(const T *img, float X, float Y, int c, int C, int W, int H) {
+    if (!(X >= 0.0f && Y >= 0.0f && X < (float) W && Y < (float) H)) return 0.0f;
+    return sample_bilinear(img, X, Y, c, C, W);
+}
+
+// MI of a wavefront's joint histogram h (row = fixed bin) over n samples,
+// binary64: mi = sum pxy (log2 pxy - (log2 px + log2 py)), hx / hy the
+// marginal entropies.  Every lane gets the totals.  Lane k handles column k.
+__device__ __forceinline__ void wave_mi(const uint32_t *h, int bins, double n, double *mi_out, double *hx_out,
+                                        double *hy_out) {
+    const int lane = threadIdx.x & 63;
+    double px = 0.0, py = 0.0;
+    if (lane < bins) {
+        uint32_t rs = 0, cs = 0;
+        for (int j = 0; j < bins; ++j) {
+            rs += h[lane * bins + j];
+            cs += h[j * bins + lane];
+        }
+        px = rs / n;
+        py = cs / n;
+    }
+    const double lpx = px > 0 ? log2(px) : 0.0, lpy = py > 0 ? log2(py) : 0.0;
+    double hx = px > 0 ? -(px * lpx) : 0.0, hy = py > 0 ? -(py * lpy) : 0.0;
+    double mi = 0.0;
+    for (int i = 0; i < bins; ++i) {
+        const double lpx_i = __shfl(lpx, i), px_i = __shfl(px, i);
+        if (lane < bins) {
+            const double pxy = h[i * bins + lane] / n;
+            if (pxy > 0 && px_i > 0 && py > 0) mi += pxy * (log2(pxy) - (lpx_i + lpy));
+        }
+    }
+    *mi_out = wave_sum(mi);
+    *hx_out = wave_sum(hx);
+    *hy_out = wave_sum(hy);
+}
+
+// The overlap test of the unit-step functors (src/gpu/cudaErrorFunctions.cuh:296, 372).
+__device__ __forceinline__ bool alt_overlap(float mov_area, float fix_area, const AltParams &P, float scale_y) {
+    return (double) (mov_area / ((float) (P.mw * P.mh) * scale_y)) > 0.25 &&
+           (double) (fix_area / (float) (P.fw * P.fh)) > 0.35;
+}
+
+__device__ __forceinline__ void alt_grid_point(const AltParams &P, int32_t gi, float *p, float *c, float *s) {
+    int32_t i0;
+    grid_point(P.grid, gi, p, &i0);
+    const float2 cs = P.sincos[i0];
+    *c = cs.x;
+    *s = cs.y;
+}
+
+// Writes the workgroup's 4 costs and folds their minimum into the argmin key
+// (thread 0).
+__device__ __forceinline__ void alt_emit(const AltParams &P, int32_t base, const float *s_cost) {
+    unsigned long long best = ~0ull;
+    for (int g = 0; g < 4; ++g) {
+        const int32_t gi = base + g;
+        if (gi < P.grid.n) {
+            if (P.result) P.result[gi] = s_cost[g];
+            const unsigned long long k = argmin_key(s_cost[g], (uint32_t) gi, P.tie_mode);
+            best = k < best ? k : best;
+        }
+    }
+    if (best != ~0ull && best < __hip_atomic_load(P.key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(P.key, best);
+}
+
+template <typename MT, typename FT, bool MI>
+__global__ __launch_bounds__(kThreads) void alt_step_kernel(const AltParams P) {
+    extern __shared__ uint32_t s_hist[];
+    __shared__ float s_cost[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t base = (int32_t) blockIdx.x * 4;
+    const int32_t gi = base + wave;
+    const bool live = gi < P.grid.n;
+    const int B2 = P.bins * P.bins;
+    uint32_t *hist = s_hist + wave * B2;
+    if constexpr (MI) {
+        for (int i = lane; i < B2; i += 64) hist[i] = 0u;
+    }
+    __syncthreads();
+
+    const MT *mov = (const MT *) P.mov;
+    const FT *fix = (const FT *) P.fix;
+    const int C = P.C;
+    const float MW = (float) P.mw, MH = (float) P.mh, FW = (float) P.fw, FH = (float) P.fh;
+    float p[kMaxDim];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, mov_area = 0.0;
+    int32_t steps = 0;
+    if (live) {
+        float c, s;
+        alt_grid_point(P, gi, p, &c, &s);
+        float h[8];
+        homography_centred(p, c, s, MW / 2, MH / 2, h);
+        for (int row = 0; row < P.fh; ++row) {
+            const AltLine L = line_centred(h, MW, MH, FW, FH, (float) row);
+            if (!L.in) continue;
+            const float mag = sqrtf(L.vx * L.vx + L.vy * L.vy);
+            const float fr = (float) row;
+            float x0 = L.ls;
+            int32_t nrow = 0;
+            while (true) {
+                const float lam = advance_unit(x0, lane);
+                const bool ok = lam < L.le && lam < FW;
+                const unsigned long long m = __ballot(ok);
+                if (m == 0ull) break;
+                if (ok) {
+                    const float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
+                    for (int ch = 0; ch < C; ++ch) {
+                        const float mv = value_at(mov, nx, ny, ch, C, P.mw, P.mh);
+                        const float fv = value_at(fix, lam, fr, ch, C, P.fw, P.fh);
+                        if constexpr (MI) {
+                            atomicAdd(&hist[mi_bin(fv, P.bins) * P.bins + mi_bin(mv, P.bins)], 1u);
+                        } else {
+                            const double md = (double) mv, fd = (double) fv;
+                            if (P.cost == CGS_COST_SQD_ALT) s0 = fma(md - fd, md - fd, s0);
+                            else s0 = fma(md, fd, s0);
+                            s1 = fma(fd, fd, s1);
+                            s2 = fma(md, md, s2);
+                        }
+                    }
+                }
+                nrow += __popcll(m);
+                if (!(m >> 63)) break;   // valid lanes form a prefix: the row ended inside this chunk
+                x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lam), 63)) + 1.0f;
+            }
+            if (nrow > 0) mov_area += (double) nrow * (double) mag;
+            steps += nrow;
+        }
+    }
+    __syncthreads();
+    float cost = 0.0f;
+    if (live) {
+        const bool pass = alt_overlap((float) mov_area, (float) steps, P, p[2]);
+        if constexpr (MI) {
+            if (pass) {
+                double mi, hx, hy;
+                wave_mi(hist, P.bins, (double) steps * (double) C, &mi, &hx, &hy);
+                cost = (float) (-2.0 * mi / (hx + hy));
+            } else {
+                cost = 123123.0f;
+            }
+            if (P.hist_dump && gi == 0)
+                for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[i];
+        } else {
+            const double S0 = wave_sum(s0), S1 = wave_sum(s1), S2 = wave_sum(s2);
+            if (!pass) cost = P.cost == CGS_COST_NCC_STREAM_ALT ? 0.0f : 123123.0f;
+            else if (P.cost == CGS_COST_SQD_ALT) cost = (float) (S0 / sqrt(S1 * S2));
+            else cost = (float) (-S0 / sqrt(S1 * S2));
+        }
+    }
+    if (lane == 0) s_cost[wave] = cost;
+    __syncthreads();
+    if (threadIdx.x == 0) alt_emit(P, base, s_cost);
+}
+
+template <typename MT, typename FT>
+__global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
+    extern __shared__ uint32_t s_hist[];
+    __shared__ float s_cost[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t base = (int32_t) blockIdx.x * 4;
+    const int32_t gi = base + wave;
+    const bool live = gi < P.grid.n;
+    const int B2 = P.bins * P.bins;
+    uint32_t *hist = s_hist + wave * B2;
+    for (int i = lane; i < B2; i += 64) hist[i] = 0u;
+    __syncthreads();
+
+    const MT *mov = (const MT *) P.mov;
+    const FT *fix = (const FT *) P.fix;
+    const int C = P.C;
+    int32_t steps = 0;
+    if (live) {
+        float p[kMaxDim], c, s;
+        alt_grid_point(P, gi, p, &c, &s);
+        float h[8];
+        homography_norm(p, c, s, P.mw, P.mh, P.fw, P.fh, h);
+        const float sc = 1 / ((float) (P.fw - 1));
+        const int max_steps = P.fw + 2;   // lambda < 1 in steps of 1/(W_f - 1): at most W_f of them
+        for (int k = lane; k < P.nrows; k += 64) {
+            const float row = P.rows[k];
+            const AltLine L = line_norm(h, row);
+            if (!L.in) continue;
+            const float ri = (float) (((double) row + 0.5) * (double) (P.fh - 1));
+            int n = 0;
+            for (float lam = L.ls; lam < L.le && lam < 1 && n < max_steps; lam = lam + sc, ++n) {
+                float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
+                nx = (float) (((double) nx + 0.5) * (double) P.mw);
+                ny = (float) (((double) ny + 0.5) * (double) P.mh);
+                const float li = lam * (float) (P.fw - 1);
+                for (int ch = 0; ch < C; ++ch) {
+                    const float fv = value_at(fix, li, ri, ch, C, P.fw, P.fh);
+                    const float mv = value_at(mov, nx, ny, ch, C, P.mw, P.mh);
+                    atomicAdd(&hist[mi_bin(fv, P.bins) * P.bins + mi_bin(mv, P.bins)], 1u);
+                }
+            }
+            steps += n;
+        }
+    }
+    __syncthreads();
+    float cost = 0.0f;
+    if (live) {
+        const int32_t total = (int32_t) wave_sum((double) steps);
+        const float fa = P.area[total < P.narea ? total : P.narea - 1];
+        double mi = 0.0, hx, hy;
+        if (total > 0) wave_mi(hist, P.bins, (double) total * (double) C, &mi, &hx, &hy);
+        cost = (float) (-mi * (double) fa);
+        if (P.hist_dump && gi == 0)
+            for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[i];
+    }
+    if (lane == 0) s_cost[wave] = cost;
+    __syncthreads();
+    if (threadIdx.x == 0) alt_emit(P, base, s_cost);
+}
+
+// ------------------------------------------------------------------ host
+
+template <typename T>
+__global__ void alt_pad_kernel(const T *src, T *dst, size_t n, size_t total) {
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t) gridDim.x * blockDim.x)
+        dst[i] = i < n ? src[i] : (T) 0;
+}
+
+__global__ void alt_key_kernel(unsigned long long *key) {
+    if (threadIdx.x == 0) *key = ~0ull;
+}
+
+static hipError_t pad_image(const cgs_image *im, void *dst, hipStream_t s) {
+    const size_t n = (size_t) im->width * im->height * im->channels;
+    const size_t total = n + mov_pad_elems(im->width, im->channels);
+    if (im->pixtype == CGS_PIX_U8)
+        hipLaunchKernelGGL(alt_pad_kernel<uint8_t>, dim3(256), dim3(256), 0, s, (const uint8_t *) im->data,
+                           (uint8_t *) dst, n, total);
+    else
+        hipLaunchKernelGGL(alt_pad_kernel<float>, dim3(256), dim3(256), 0, s, (const float *) im->data,
+                           (float *) dst, n, total);
+    return hipGetLastError();
+}
+
+template <typename MT, typename FT>
+static hipError_t alt_launch(const AltParams &P, hipStream_t s) {
+    const int blocks = (P.grid.n + 3) / 4;
+    const size_t lds = 4 * (size_t) P.bins * P.bins * sizeof(uint32_t);
+    if (P.cost == CGS_COST_MI_ALT)
+        hipLaunchKernelGGL((alt_norm_kernel<MT, FT>), dim3(blocks), dim3(kThreads), lds, s, P);
+    else if (P.cost == CGS_COST_MI_STREAM_ALT)
+        hipLaunchKernelGGL((alt_step_kernel<MT, FT, true>), dim3(blocks), dim3(kThreads), lds, s, P);
+    else
+        hipLaunchKernelGGL((alt_step_kernel<MT, FT, false>), dim3(blocks), dim3(kThreads), 0, s, P);
+    return hipGetLastError();
+}
+
+static hipError_t alt_dispatch(const AltParams &P, int mpix, int fpix, hipStream_t s) {
+    if (mpix == CGS_PIX_U8 && fpix == CGS_PIX_U8) return alt_launch<uint8_t, uint8_t>(P, s);
+    if (mpix == CGS_PIX_F32 && fpix == CGS_PIX_F32) return alt_launch<float, float>(P, s);
+    if (mpix == CGS_PIX_U8) return alt_launch<uint8_t, float>(P, s);
+    return alt_launch<float, uint8_t>(P, s);
+}
+
+// calcMIAlt's host tables, in the reference's own loops: the row values
+// (row = -0.5f; row += 1/(float)(H_f - 1) while row < 0.5,
+// src/gpu/cudaErrorFunction_mi.cuh:162) and the fixed area after n steps
+// (fix_area += 1.0 / (W_f H_f) through binary64, rounded to binary32, :198)
+// for n up to rows * (W_f + 2).
+static void mi_alt_tables(int fw, int fh, std::vector<float> &rows, std::vector<float> &area) {
+    rows.clear();
+    const float step = 1 / ((float) (fh - 1));
+    for (float row = -0.5f; row < 0.5; row += step) rows.push_back(row);
+    const size_t n = rows.size() * ((size_t) fw + 2) + 1;
+    area.resize(n);
+    const double d = 1.0 / (double) (fw * fh);
+    float fa = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        area[i] = fa;
+        fa = (float) ((double) fa + d);
+    }
+}
+
+cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
+                          const cgs_search_opts *o, float *result, cgs_argmin *out, hipStream_t stream,
+                          uint32_t *hist_dump_host) {
+    if (grid->dim != 8) return set_error(CGS_ERR_INVALID_ARGUMENT, "line-traversal costs take 8 parameters");
+    const size_t esz_m = moved->pixtype == CGS_PIX_U8 ? 1 : 4, esz_f = fixed->pixtype == CGS_PIX_U8 ? 1 : 4;
+    const size_t nm = (size_t) moved->width * moved->height * moved->channels + mov_pad_elems(moved->width, moved->channels);
+    const size_t nf = (size_t) fixed->width * fixed->height * fixed->channels + mov_pad_elems(fixed->width, fixed->channels);
+    const int levels0 = (int) grid->count[0];
+    const bool mi = is_mi_cost(o->cost);
+    const size_t B2 = (size_t) o->mi_bins * o->mi_bins;
+    cgs_status st;
+    if ((st = ws->key.ensure(64)) != CGS_OK || (st = ws->key_host.ensure(64, true)) != CGS_OK ||
+        (st = ws->mov.ensure(nm * esz_m)) != CGS_OK || (st = ws->fixp.ensure(nf * esz_f)) != CGS_OK ||
+        (st = ws->sincos.ensure(8 * (size_t) levels0)) != CGS_OK ||
+        (st = ws->sincos_host.ensure(8 * (size_t) levels0, true)) != CGS_OK)
+        return st;
+    if (hist_dump_host && mi && (st = ws->hist.ensure(B2 * sizeof(uint32_t))) != CGS_OK) return st;
+    HIP_TRY(pad_image(moved, ws->mov.ptr, stream));
+    HIP_TRY(pad_image(fixed, ws->fixp.ptr, stream));
+
+    AltParams P{};
+    if (o->cost == CGS_COST_MI_ALT) {
+        if (fixed->width < 2 || fixed->height < 2)
+            return set_error(CGS_ERR_INVALID_ARGUMENT, "calcMIAlt needs a fixed image of at least 2 x 2");
+        std::vector<float> rows, area;
+        mi_alt_tables(fixed->width, fixed->height, rows, area);
+        if (area.size() > (size_t) INT32_MAX) return set_error(CGS_ERR_TOO_LARGE, "calcMIAlt: fixed image too large");
+        const size_t bytes = sizeof(float) * (rows.size() + area.size());
+        if (ws->alt_tab_fw != fixed->width || ws->alt_tab_fh != fixed->height) {
+            if ((st = ws->alt_tab.ensure(bytes)) != CGS_OK || (st = ws->alt_tab_host.ensure(bytes, true)) != CGS_OK)
+                return st;
+            HIP_TRY(hipStreamSynchronize(stream));   // the pinned staging buffer may still feed an earlier copy
+            float *h = (float *) ws->alt_tab_host.ptr;
+            std::copy(rows.begin(), rows.end(), h);
+            std::copy(area.begin(), area.end(), h + rows.size());
+            HIP_TRY(hipMemcpyAsync(ws->alt_tab.ptr, h, bytes, hipMemcpyHostToDevice, stream));
+            ws->alt_tab_fw = fixed->width;
+            ws->alt_tab_fh = fixed->height;
+        }
+        P.rows = (const float *) ws->alt_tab.ptr;
+        P.nrows = (int) rows.size();
+        P.area = P.rows + rows.size();
+        P.narea = (int) area.size();
+    }
+    {
+        float *h = (float *) ws->sincos_host.ptr;
+        for (int i = 0; i < levels0; ++i) {
+            const float theta = fmaf((float) (grid->offset[0] + i), grid->resolution[0], grid->start[0]);
+            h[2 * i] = (float) std::cos((double) theta);
+            h[2 * i + 1] = (float) std::sin((double) theta);
+        }
+        HIP_TRY(hipMemcpyAsync(ws->sincos.ptr, h, sizeof(float) * 2 * levels0, hipMemcpyHostToDevice, stream));
+    }
+    unsigned long long *key = (unsigned long long *) ws->key.ptr;
+    hipLaunchKernelGGL(alt_key_kernel, dim3(1), dim3(64), 0, stream, key);
+
+    fill_grid_dev(grid, &P.grid);
+    P.sincos = (const float2 *) ws->sincos.ptr;
+    P.cost = o->cost;
+    P.mov = ws->mov.ptr;
+    P.fix = ws->fixp.ptr;
+    P.mw = moved->width;
+    P.mh = moved->height;
+    P.fw = fixed->width;
+    P.fh = fixed->height;
+    P.C = moved->channels;
+    P.bins = mi ? o->mi_bins : 0;
+    P.result = result;
+    P.key = key;
+    P.tie_mode = o->tie_mode;
+    P.hist_dump = (hist_dump_host && mi) ? (uint32_t *) ws->hist.ptr : nullptr;
+    const bool timed = (o->flags & CGS_FLAG_TIME_KERNEL) != 0;
+    if (timed) HIP_TRY(ws->record_start(stream));
+    HIP_TRY(alt_dispatch(P, moved->pixtype, fixed->pixtype, stream));
+    if (timed) HIP_TRY(ws->record_stop(stream));
+    HIP_TRY(hipMemcpyAsync(ws->key_host.ptr, key, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    if (P.hist_dump)
+        HIP_TRY(hipMemcpyAsync(hist_dump_host, ws->hist.ptr, B2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (timed) HIP_TRY(ws->elapsed_ms());
+    if (out) *out = decode_key(*(unsigned long long *) ws->key_host.ptr, o->tie_mode);
+    return CGS_OK;
+}
+
+}  // namespace cgs

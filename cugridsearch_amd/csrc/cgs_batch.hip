@@ -199,6 +199,8 @@ cgs_status stage_alloc(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, con
     for (int32_t i = 0; i < n; ++i) {
         const cgs_batch_pair &p = pairs[i];
         if ((st = validate_search(&p.grid, &p.moved, &p.fixed, &p.opts)) != CGS_OK) return st;
+        if (is_alt_cost(p.opts.cost))
+            return set_error(CGS_ERR_UNSUPPORTED, "line-traversal costs run through cgs_search, not the batch");
         if (p.opts.sampler != CGS_SAMPLER_BILINEAR || p.moved.channels != 1 || p.moved.width > 65536 ||
             p.moved.height > 65536)
             return set_error(CGS_ERR_UNSUPPORTED, "batch pairs must be one-channel bilinear searches");

@@ -94,15 +94,26 @@ cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const c
     if ((st = validate_image(fixed, "fixed")) != CGS_OK) return st;
     if (moved->channels != fixed->channels)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "moved/fixed channel counts differ");
-    if (o->cost < CGS_COST_SAD || o->cost > CGS_COST_SAD_STREAM) return set_error(CGS_ERR_INVALID_ARGUMENT, "bad cost");
+    if (o->cost < CGS_COST_SAD || o->cost > CGS_COST_MI_ALT) return set_error(CGS_ERR_INVALID_ARGUMENT, "bad cost");
     if (o->sampler != CGS_SAMPLER_NEAREST && o->sampler != CGS_SAMPLER_BILINEAR)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "bad sampler");
-    if (o->param_mode < CGS_PARAM_CENTRED || o->param_mode > CGS_PARAM_TRANSLATION)
+    if (o->param_mode < CGS_PARAM_CENTRED || o->param_mode > CGS_PARAM_NORM)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "bad param mode");
     if (o->tie_mode != CGS_TIE_LOWEST && o->tie_mode != CGS_TIE_REFERENCE)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "bad tie mode");
-    if ((o->cost == CGS_COST_MI || o->cost == CGS_COST_MI_PLAIN) && (o->mi_bins < 8 || o->mi_bins > 64))
+    if (is_mi_cost(o->cost) && (o->mi_bins < 8 || o->mi_bins > 64))
         return set_error(CGS_ERR_INVALID_ARGUMENT, "mi_bins must be in [8, 64]");
+    if (is_alt_cost(o->cost)) {
+        // the line-traversal functors sample through valueAt (bilinear) and
+        // fix their own parametrisation
+        if (o->sampler != CGS_SAMPLER_BILINEAR)
+            return set_error(CGS_ERR_UNSUPPORTED, "line-traversal costs sample bilinearly only");
+        if (o->param_mode != (o->cost == CGS_COST_MI_ALT ? CGS_PARAM_NORM : CGS_PARAM_CENTRED))
+            return set_error(CGS_ERR_INVALID_ARGUMENT,
+                             "line-traversal costs: calcMIAlt takes CGS_PARAM_NORM, the others CGS_PARAM_CENTRED");
+    } else if (o->param_mode == CGS_PARAM_NORM) {
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "CGS_PARAM_NORM belongs to calcMIAlt only");
+    }
     if (o->tile_gpt != 0 && o->tile_gpt != 2 && o->tile_gpt != 4 && o->tile_gpt != 8)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "tile_gpt must be 0, 2, 4 or 8");
     const int need = o->param_mode == CGS_PARAM_TRANSLATION ? 2 : 8;

@@ -38,6 +38,9 @@ struct Workspace {
     DevBuf mov, quads, sincos, sincos_host, hist, key, key_host;
     DevBuf prep, prep_host;     // proposal-network input descriptors
     DevBuf cov, cov_host;       // covariance partial sums
+    DevBuf fixp;                // zero-padded fixed image (line-traversal costs)
+    DevBuf alt_tab, alt_tab_host;   // calcMIAlt's row values and fixed-area table
+    int alt_tab_fw = 0, alt_tab_fh = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     // HIP events around one launch on the caller's stream; elapsed_ms()
     // stores the duration for cgs_last_kernel_ms() (after a stream sync).
@@ -49,6 +52,17 @@ struct Workspace {
 void set_last_kernel_ms(float ms);
 
 cgs_status workspace_for_current_device(Workspace **out);
+
+inline bool is_alt_cost(int cost) { return cost >= CGS_COST_SQD_ALT && cost <= CGS_COST_MI_ALT; }
+inline bool is_mi_cost(int cost) {
+    return cost == CGS_COST_MI || cost == CGS_COST_MI_PLAIN || cost == CGS_COST_MI_STREAM_ALT || cost == CGS_COST_MI_ALT;
+}
+
+// The line-traversal search (cgs_alt.hip).  Called by cgs_search with the
+// options validated and ws->mu held.
+cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
+                          const cgs_search_opts *o, float *result, cgs_argmin *out, hipStream_t stream,
+                          uint32_t *hist_dump_host);
 
 cgs_status validate_image(const cgs_image *im, const char *name);
 cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,

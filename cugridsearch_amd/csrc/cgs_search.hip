@@ -229,6 +229,7 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     st = workspace_for_current_device(&ws);
     if (st != CGS_OK) return st;
     std::lock_guard<std::mutex> lock(ws->mu);
+    if (is_alt_cost(o->cost)) return run_alt_search(ws, grid, moved, fixed, o, result, out, stream, hist_dump_host);
 
     const KernelKind kind = kernel_kind(moved, o);
     const bool fast = kind == kKernelFast;
@@ -359,7 +360,7 @@ cgs_status cgs_eval_point(const cgs_image *moved, const cgs_image *fixed, const 
     if (st != CGS_OK) return st;
     cgs_argmin am;
     st = run_search(&g, moved, fixed, opts, nullptr, &am, (hipStream_t) stream,
-                    (opts && (opts->cost == CGS_COST_MI || opts->cost == CGS_COST_MI_PLAIN)) ? hist_out : nullptr);
+                    (opts && is_mi_cost(opts->cost)) ? hist_out : nullptr);
     if (st == CGS_OK) *cost_out = am.index < 0 ? NAN : am.value;
     return st;
 }

@@ -59,7 +59,7 @@ def plain_key(functor, moved, fixed):
 
 
 def family(functor):
-    if functor.sampler != L.SAMPLER_BILINEAR:
+    if functor.sampler != L.SAMPLER_BILINEAR or functor.cost >= L.COST_SQD_ALT:   # line traversal: no tiles
         return None
     return "mi" if functor.cost in (L.COST_MI, L.COST_MI_PLAIN) else "fast"
 

@@ -24,7 +24,7 @@ extern "C" {
 /* Layout version of the structures below; cgs_abi_version() returns the
  * value the library was built with, so a caller compiled against another
  * header can refuse to run instead of passing mis-sized structures. */
-#define CGS_ABI_VERSION 4
+#define CGS_ABI_VERSION 5
 
 typedef enum {
     CGS_OK = 0,
@@ -48,8 +48,16 @@ typedef enum {
     /* the thread-per-row "stream" functors: same sums, their own final combine */
     CGS_COST_NCC_STREAM = 6, /* calcNCCstream: 0 unless both sums > 0   src/gpu/cudaErrorFunctionsStreams.cuh:85-163 */
     CGS_COST_SQD_STREAM = 7, /* calcSQDstream: no overlap sentinel      src/gpu/cudaErrorFunctionsStreams.cuh:283-351 */
-    CGS_COST_SAD_STREAM = 8  /* averageAbsoluteDifference_stream: no overlap -> 0
+    CGS_COST_SAD_STREAM = 8, /* averageAbsoluteDifference_stream: no overlap -> 0
                                                               src/gpu/cudaErrorFunctionsStreams.cuh:31-83 */
+    /* line-traversal ("Alt") costs: each fixed row walks its line through the
+       moving image in unit lambda steps; bilinear only; their own overlap test */
+    CGS_COST_SQD_ALT = 9,         /* calcSQDAlt (123123)        src/gpu/cudaErrorFunctions.cuh:226-300 */
+    CGS_COST_NCC_ALT = 10,        /* calcNCCAlt (123123)        src/gpu/cudaErrorFunctions.cuh:302-376 */
+    CGS_COST_NCC_STREAM_ALT = 11, /* calcNCCstreamAlt (0)       src/gpu/cudaErrorFunctionsStreams.cuh:165-281 */
+    CGS_COST_MI_STREAM_ALT = 12,  /* calcMIstreamAlt (123123)   src/gpu/cudaErrorFunction_miStreams.cuh:9-210 */
+    CGS_COST_MI_ALT = 13          /* calcMIAlt, -MI * fixed area, CGS_PARAM_NORM only
+                                                              src/gpu/cudaErrorFunction_mi.cuh:117-255 */
 } cgs_cost;
 
 /* CudaImage samplers (include/cudaTensor.cuh:557-575). */
@@ -64,7 +72,9 @@ typedef enum {
                                    src/gpu/cudaImageFunctions.cuh:216-252, inverse warp :277-285 */
     CGS_PARAM_NONCENTRED = 1,   /* averageAbsoluteDifference's H  src/gpu/cudaErrorFunctions.cuh:50-64 */
     CGS_PARAM_RAW_H = 2,        /* averageAbsoluteDifferenceH, forward  src/gpu/cudaErrorFunctions.cuh:194-224 */
-    CGS_PARAM_TRANSLATION = 3   /* 2-D (tx, ty)  src/gpu/cudaErrorFunctions.cuh:89-114 */
+    CGS_PARAM_TRANSLATION = 3,  /* 2-D (tx, ty)  src/gpu/cudaErrorFunctions.cuh:89-114 */
+    CGS_PARAM_NORM = 4          /* parametersToHomographyNorm (calcMIAlt only)
+                                   src/gpu/cudaImageFunctions.cuh:181-214 */
 } cgs_param_mode;
 
 /* Argmin tie order. */

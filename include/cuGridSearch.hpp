@@ -71,6 +71,14 @@ constexpr CgsFunctor calcMI{CGS_COST_MI_PLAIN, CGS_PARAM_CENTRED, CGS_SAMPLER_BI
 constexpr CgsFunctor averageAbsoluteDifference_stream{CGS_COST_SAD_STREAM, CGS_PARAM_TRANSLATION, CGS_SAMPLER_BILINEAR, 64};
 constexpr CgsFunctor calcNCCstream{CGS_COST_NCC_STREAM, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
 constexpr CgsFunctor calcSQDstream{CGS_COST_SQD_STREAM, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+// the line-traversal functors (src/gpu/cudaErrorFunctions.cuh:226-376,
+// cudaErrorFunctionsStreams.cuh:165-281, cudaErrorFunction_miStreams.cuh:9-210,
+// cudaErrorFunction_mi.cuh:117-255)
+constexpr CgsFunctor calcSQDAlt{CGS_COST_SQD_ALT, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcNCCAlt{CGS_COST_NCC_ALT, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcNCCstreamAlt{CGS_COST_NCC_STREAM_ALT, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcMIstreamAlt{CGS_COST_MI_STREAM_ALT, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
+constexpr CgsFunctor calcMIAlt{CGS_COST_MI_ALT, CGS_PARAM_NORM, CGS_SAMPLER_BILINEAR, 64};
 // centred-homography SAD (the search workloads' SAD)
 constexpr CgsFunctor centredAbsoluteDifference{CGS_COST_SAD, CGS_PARAM_CENTRED, CGS_SAMPLER_BILINEAR, 64};
 

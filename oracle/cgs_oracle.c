@@ -37,6 +37,11 @@
  *                   src/gpu/cudaErrorFunction_mi.cuh:8-115 (plain -MI)
  *   argmin          include/cudaTensor.cuh:215-265,
  *                   include/cudaTensorKernels.cuh:133-220
+ *   line traversal  src/gpu/cudaErrorFunctions.cuh:226-376 (calcSQDAlt, calcNCCAlt)
+ *                   src/gpu/cudaErrorFunctionsStreams.cuh:165-281 (calcNCCstreamAlt)
+ *                   src/gpu/cudaErrorFunction_miStreams.cuh:9-210 (calcMIstreamAlt)
+ *                   src/gpu/cudaErrorFunction_mi.cuh:117-255 (calcMIAlt)
+ *                   src/gpu/cudaImageFunctions.cuh:181-214, 315-441 (norm H, lines)
  *
  * Declared decisions where the reference is undefined (each is pinned by a
  * test in tests/test_oracle_*.py):
@@ -66,10 +71,11 @@
 
 enum { ORC_COST_SAD = 0, ORC_COST_NCC = 1, ORC_COST_MI = 2, ORC_COST_SQD = 3,
        ORC_COST_SAD_SUM = 4, ORC_COST_MI_PLAIN = 5, ORC_COST_NCC_STREAM = 6, ORC_COST_SQD_STREAM = 7,
-       ORC_COST_SAD_STREAM = 8 };
+       ORC_COST_SAD_STREAM = 8, ORC_COST_SQD_ALT = 9, ORC_COST_NCC_ALT = 10, ORC_COST_NCC_STREAM_ALT = 11,
+       ORC_COST_MI_STREAM_ALT = 12, ORC_COST_MI_ALT = 13 };
 enum { ORC_NEAREST = 0, ORC_BILINEAR = 1 };
 enum { ORC_PARAM_CENTRED = 0, ORC_PARAM_NONCENTRED = 1, ORC_PARAM_RAW_H = 2,
-       ORC_PARAM_TRANSLATION = 3 };
+       ORC_PARAM_TRANSLATION = 3, ORC_PARAM_NORM = 4 };
 enum { ORC_TIE_LOWEST = 0, ORC_TIE_REFERENCE = 1 };
 enum { ORC_PIX_U8 = 0, ORC_PIX_F32 = 1 };
 
@@ -311,9 +317,13 @@ static inline int mi_bin(float v, int bins) {
 /* Evaluates one grid point.  hist (bins*bins uint32, row = fixed bin,
  * column = moving bin) is filled for the MI costs when non-NULL or needed.
  * Returns the cost as the product writes it (binary32). */
+static float cost_alt(const orc_image *moved, const orc_image *fixed, int cost, const float *params,
+                      int bins, uint32_t *hist_out, double *aux_out);
+
 float orc_cost_point(const orc_image *moved, const orc_image *fixed, int cost, int sampler,
                      int mode, const float *params, int bins, uint32_t *hist_out,
                      double *aux_out) {
+    if (cost >= ORC_COST_SQD_ALT) return cost_alt(moved, fixed, cost, params, bins, hist_out, aux_out);
     float coef[9];
     const float cx = (float) moved->width / 2, cy = (float) moved->height / 2;
     orc_warp_coeffs(mode, params, cx, cy, coef);
@@ -414,6 +424,308 @@ float orc_cost_point(const orc_image *moved, const orc_image *fixed, int cost, i
     }
     if (is_mi && !hist_out) free(hist);
     return out;
+}
+
+/* ------------------------------------------------- line-traversal costs */
+
+/* MI of a joint histogram (row = fixed bin, column = moving bin) of n
+ * samples, binary64: mi = sum pxy (log2 pxy - (log2 px + log2 py)) over the
+ * cells where all three are positive; hx, hy the marginal entropies. */
+static void mi_of_hist(const uint32_t *hist, int bins, double n, double *mi_out, double *hx_out, double *hy_out) {
+    double mi = 0, hx = 0, hy = 0;
+    double *px = (double *) calloc(bins, sizeof(double));
+    double *py = (double *) calloc(bins, sizeof(double));
+    for (int i = 0; i < bins; i++)
+        for (int j = 0; j < bins; j++) {
+            px[i] += hist[i * bins + j];
+            py[j] += hist[i * bins + j];
+        }
+    for (int i = 0; i < bins; i++)
+        for (int j = 0; j < bins; j++) {
+            const double pxy = hist[i * bins + j] / n, pxi = px[i] / n, pyj = py[j] / n;
+            if (pxy > 0 && pxi > 0 && pyj > 0) mi += pxy * (log2(pxy) - (log2(pxi) + log2(pyj)));
+        }
+    for (int i = 0; i < bins; i++) {
+        const double pxi = px[i] / n, pyi = py[i] / n;
+        if (pxi > 0) hx -= pxi * log2(pxi);
+        if (pyi > 0) hy -= pyi * log2(pyi);
+    }
+    free(px); free(py);
+    *mi_out = mi; *hx_out = hx; *hy_out = hy;
+}
+
+/* One fixed row's line through the moving image: the row's samples sit at
+ * p0 + lambda v for lambda = ls, ls + step, ... while lambda < le (and the
+ * row's own end); in = some image border crossing qualified.  out[7] =
+ * {ls, le, vx, vy, px, py, in}. */
+typedef struct { float ls, le, vx, vy, px, py; int in; } orc_line;
+
+/* parametricAssignValues (cudaImageFunctions.cuh:315-376).  Each expression
+ * is the reference's, in its association order; the factors it repeats are
+ * named once here (dt: the shared determinant-like term; a0 / aw: the
+ * row's denominators at column 0 and at column f_width; gy / gx: the
+ * crossing denominators of the horizontal / vertical borders). */
+static void line_centred(const float *h, int mw, int mh, int fw, int fh, int row, orc_line *L) {
+    const float h11 = h[0], h12 = h[1], h13 = h[2], h21 = h[3], h22 = h[4], h23 = h[5];
+    const float h31 = h[6], h32 = h[7];
+    const float r = (float) row, FW = (float) fw, MW = (float) mw, MH = (float) mh, FH = (float) fh;
+    const float dt = h11*h22 - h12*h21 - h11*h23*h32 + h12*h23*h31 + h13*h21*h32 - h13*h22*h31;
+    const float a0 = h11*h22 - h12*h21 - h11*h32*r + h12*h31*r;
+    const float aw = h11*h22 - h12*h21 + FW*h21*h32 - FW*h22*h31 - h11*h32*r + h12*h31*r;
+    const float gy = h21 - h31*r, gx = h22 - h32*r;
+    L->vx = (gx*(dt))/(a0*aw);
+    L->vy = -(gy*(dt))/(a0*aw);
+    const float q = h12*h21 - h11*h22 + r*(h11*h32 - h12*h31);
+    L->px = (h13*h22 - h12*h23 + r*(h12 - h13*h32))/q;
+    L->py = -(h13*h21 - h11*h23 + r*(h11 - h13*h31))/q;
+    const float lt = -((h11*h23 - h13*h21 - h11*r + h13*h31*r)*(aw))/(gy*(dt));
+    const float xt = -(h23 - r)/gy;
+    const float lb = -((aw)*(h11*h23 - h13*h21 - h11*r + h11*h22*MH - h12*h21*MH + h13*h31*r - h11*h32*MH*r
+                             + h12*h31*MH*r))/(gy*(dt));
+    const float xb = -(h23 - r + h22*MH - h32*MH*r)/gy;
+    const float ll = -((h12*h23 - h13*h22 - h12*r + h13*h32*r)*(aw))/(gx*(dt));
+    const float yl = -(h23 - r)/gx;
+    const float lr = -((aw)*(h12*h23 - h13*h22 - h12*r - h11*h22*MW + h12*h21*MW + h13*h32*r + h11*h32*MW*r
+                             - h12*h31*MW*r))/(gx*(dt));
+    const float yr = -(h23 - r + h21*MW - h31*MW*r)/gx;
+    L->ls = 0.0f;
+    L->le = FW;
+    L->in = 0;
+    if (xt >= 0 && xt < MW && lt >= 0 && lt < FW) { if (L->vy > 0) L->ls = lt; else L->le = lt; L->in = 1; }
+    if (xb >= 0 && xb < MW && lb >= 0 && lb < FW) { if (L->vy < 0) L->ls = lb; else L->le = lb; L->in = 1; }
+    if (yl >= 0 && yl < MH && ll >= 0 && ll < FH) { if (L->vx > 0) L->ls = ll; else L->le = ll; L->in = 1; }
+    if (yr >= 0 && yr < MH && lr >= 0 && lr < FH) { if (L->vx < 0) L->ls = lr; else L->le = lr; L->in = 1; }
+}
+
+/* parametricAssignValuesNorm (cudaImageFunctions.cuh:378-441), normalised
+ * coordinates (the image spans [-0.5, 0.5]), column -0.5.  ty / tx are the
+ * quotients the border crossings share with p0 (p0 = (tx, -ty)). */
+static void line_norm(const float *h, float row, orc_line *L) {
+    const float h11 = h[0], h12 = h[1], h13 = h[2], h21 = h[3], h22 = h[4], h23 = h[5];
+    const float h31 = h[6], h32 = h[7];
+    const float col = -0.5f;
+    const float dt = h11*h22 - h12*h21 - h11*h23*h32 + h12*h23*h31 + h13*h21*h32 - h13*h22*h31;
+    L->vy = -(h21 - h31*row)/dt;
+    L->vx = (h22 - h32*row)/dt;
+    const float q = h11*h22 - h12*h21 + col*(h21*h32 - h22*h31) - row*(h11*h32 - h12*h31);
+    const float tx = (h12*h23 - h13*h22 + col*(h22 - h23*h32) - row*(h12 - h13*h32))/q;
+    const float ty = (h11*h23 - h13*h21 + col*(h21 - h23*h31) - row*(h11 - h13*h31))/q;
+    L->px = tx;
+    L->py = -ty;
+    const float gy = h21 - h31*row, gx = h22 - h32*row;
+    const float lt = -((ty - 0.5f)*(dt))/gy;
+    const float xt = (h22 - 2*h23 + 2*row - h32*row)/(2*gy);
+    const float lb = -((ty + 0.5f)*(dt))/gy;
+    const float xb = -(h22 + 2*h23 - 2*row - h32*row)/(2*gy);
+    const float ll = -((tx + 0.5f)*(dt))/gx;
+    const float yl = (h21 - 2*h23 + 2*row - h31*row)/(2*gx);
+    const float lr = -((tx - 0.5f)*(dt))/gx;
+    const float yr = -(h21 + 2*h23 - 2*row - h31*row)/(2*gx);
+    L->ls = 0.0f;
+    L->le = 1.0f;
+    L->in = 0;
+    if (xt > -0.5f && xt < 0.5f && lt >= 0 && lt < 1) { if (L->vy > 0) L->ls = lt; else L->le = lt; L->in = 1; }
+    if (xb > -0.5f && xb < 0.5f && lb > 0 && lb < 1) { if (L->vy < 0) L->ls = lb; else L->le = lb; L->in = 1; }
+    if (yl > -0.5f && yl < 0.5f && ll > 0 && ll < 1) { if (L->vx > 0) L->ls = ll; else L->le = ll; L->in = 1; }
+    if (yr > -0.5f && yr < 0.5f && lr > 0 && lr < 1) { if (L->vx < 0) L->ls = lr; else L->le = lr; L->in = 1; }
+}
+
+/* parametersToHomographyNorm (cudaImageFunctions.cuh:181-214) with its
+ * integer quotients f_w / m_w, f_h / m_h, f_w / 2, f_h / 2, m_w / 2 and the
+ * moving-image centre's y taken from the width (Q11). */
+static void homography_norm(const float *p, int mw, int mh, int fw, int fh, float *h) {
+    const float c = (float) cos((double) p[0]), s = (float) sin((double) p[0]);
+    const float x_ratio = (float) (fw / mw), y_ratio = (float) (fh / mh);
+    const float cfx = (float) (fw / 2), cfy = (float) (fh / 2), cmx = (float) (mw / 2), cmy = (float) (mw / 2);
+    const float scaleX = p[1] / x_ratio, scaleY = p[2] / y_ratio, shearXY = p[3];
+    h[0] = scaleX * c;
+    h[1] = scaleY * shearXY * c - scaleY * s;
+    h[2] = (p[4] - cfx + cmx) / (float) fw;
+    h[3] = scaleX * s;
+    h[4] = scaleY * shearXY * s + scaleY * c;
+    h[5] = (p[5] - cfy + cmy) / (float) fh;
+    h[6] = p[6];
+    h[7] = p[7];
+}
+
+void orc_alt_homography(int norm, const float *params, int mw, int mh, int fw, int fh, float *h) {
+    if (norm) homography_norm(params, mw, mh, fw, fh, h);
+    else homography_centred(params, (float) mw / 2, (float) mh / 2, h);
+}
+
+void orc_alt_line(int norm, const float *h, int mw, int mh, int fw, int fh, float row, float *out) {
+    orc_line L;
+    if (norm) line_norm(h, row, &L);
+    else line_centred(h, mw, mh, fw, fh, (int) row, &L);
+    out[0] = L.ls; out[1] = L.le; out[2] = L.vx; out[3] = L.vy; out[4] = L.px; out[5] = L.py;
+    out[6] = (float) L.in;
+}
+
+/* Sum of 64 lane partial sums in the xor-butterfly order: at each distance
+ * o = 32, 16, ..., 1 every lane i adds lane i ^ o's value. */
+static double lane_tree(const double *p) {
+    double a[64], b[64];
+    memcpy(a, p, sizeof(a));
+    for (int o = 32; o >= 1; o >>= 1) {
+        for (int i = 0; i < 64; i++) b[i] = a[i] + a[i ^ o];
+        memcpy(a, b, sizeof(a));
+    }
+    return a[0];
+}
+
+/* The reference's overlap test of the step-1 Alt functors
+ * (cudaErrorFunctions.cuh:296, 372; cudaErrorFunctionsStreams.cuh:272;
+ * cudaErrorFunction_miStreams.cuh:199): moved area / (W_m H_m scaleY) > 0.25
+ * and fixed area / (W_f H_f) > 0.35, in binary32 with the double compare. */
+static int alt_overlap(float mov_area, float fix_area, int mw, int mh, int fw, int fh, float scale_y) {
+    return (double) (mov_area / ((float) (mw * mh) * scale_y)) > 0.25 &&
+           (double) (fix_area / (float) (fw * fh)) > 0.35;
+}
+
+/* calcSQDAlt / calcNCCAlt / calcNCCstreamAlt / calcMIstreamAlt: one line
+ * per fixed row (centred H), unit lambda steps from the line's start,
+ * accumulated as lambda += 1 in binary32; both images sampled by the gated
+ * bilinear valueAt, the fixed one at (row, lambda).  Declared sums: step k of
+ * a row goes to lane k mod 64; each lane adds, in row and step order,
+ * fma(m, f) (or fma(m - f, m - f) for SQD), fma(m, m), fma(f, f) in binary64
+ * (the 1/255 scaling of the reference cancels in every one of these costs);
+ * the lanes combine in lane_tree order.  Moved area: binary64 sum over rows
+ * of steps * |v| (binary32 |v|), then rounded to binary32; fixed area: the
+ * step count.  MI: joint counts of every sample. */
+static float cost_alt_step(const orc_image *moved, const orc_image *fixed, int cost, const float *params,
+                           int bins, uint32_t *hist, double *aux) {
+    const int mw = moved->width, mh = moved->height, fw = fixed->width, fh = fixed->height, C = fixed->channels;
+    float h[8];
+    homography_centred(params, (float) mw / 2, (float) mh / 2, h);
+    double s0[64], s1[64], s2[64];
+    for (int i = 0; i < 64; i++) s0[i] = s1[i] = s2[i] = 0.0;
+    double mov = 0.0;
+    int64_t steps = 0, count = 0;
+    const int is_mi = cost == ORC_COST_MI_STREAM_ALT;
+    for (int row = 0; row < fh; row++) {
+        orc_line L;
+        line_centred(h, mw, mh, fw, fh, row, &L);
+        if (!L.in) continue;
+        const float mag = sqrtf(L.vx * L.vx + L.vy * L.vy);
+        int64_t k = 0;
+        for (float lam = L.ls; lam < L.le && lam < (float) fw; lam += 1.0f, k++) {
+            const float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
+            const int lane = (int) (k & 63);
+            for (int c = 0; c < C; c++) {
+                const float m = orc_sample_bilinear(moved, ny, nx, c);
+                const float f = orc_sample_bilinear(fixed, (float) row, lam, c);
+                if (is_mi) {
+                    hist[mi_bin(f, bins) * bins + mi_bin(m, bins)]++;
+                } else {
+                    const double md = (double) m, fd = (double) f;
+                    if (cost == ORC_COST_SQD_ALT) s0[lane] = fma(md - fd, md - fd, s0[lane]);
+                    else s0[lane] = fma(md, fd, s0[lane]);
+                    s1[lane] = fma(fd, fd, s1[lane]);
+                    s2[lane] = fma(md, md, s2[lane]);
+                }
+            }
+            count += C;
+        }
+        if (k > 0) mov += (double) k * (double) mag;
+        steps += k;
+    }
+    const float mov_area = (float) mov, fix_area = (float) steps;
+    const int pass = alt_overlap(mov_area, fix_area, mw, mh, fw, fh, params[2]);
+    const double S0 = lane_tree(s0), S1 = lane_tree(s1), S2 = lane_tree(s2);
+    double mi = 0, hx = 0, hy = 0;
+    float out;
+    if (is_mi) {
+        if (pass) mi_of_hist(hist, bins, (double) count, &mi, &hx, &hy);
+        out = pass ? (float) (-2.0 * mi / (hx + hy)) : 123123.0f;
+    } else if (cost == ORC_COST_SQD_ALT) {
+        out = pass ? (float) (S0 / sqrt(S1 * S2)) : 123123.0f;
+    } else {
+        out = pass ? (float) (-S0 / sqrt(S1 * S2)) : (cost == ORC_COST_NCC_ALT ? 123123.0f : 0.0f);
+    }
+    if (aux) {
+        aux[0] = (double) steps; aux[1] = S1; aux[2] = S2; aux[3] = S0; aux[4] = (double) count;
+        aux[5] = mi; aux[6] = (double) mov_area; aux[7] = (double) fix_area; aux[8] = pass;
+    }
+    return out;
+}
+
+/* Number of rows calcMIAlt visits and their values: row = -0.5f, then
+ * row += 1 / (float)(H_f - 1) in binary32 while row < 0.5
+ * (cudaErrorFunction_mi.cuh:162).  rows may be NULL; at most cap values
+ * are written.  Returns the count. */
+static float mi_alt_row_step(int fh) { return 1 / ((float) (fh - 1)); }
+
+int64_t orc_mi_alt_rows(int fh, float *rows, int64_t cap) {
+    const float step = mi_alt_row_step(fh);
+    int64_t k = 0;
+    for (float row = -0.5f; row < 0.5; row += step, k++)
+        if (rows && k < cap) rows[k] = row;
+    return k;
+}
+
+/* calcMIAlt's fixed area after n steps: fix_area += 1.0 / (W_f H_f), a
+ * binary64 add rounded to binary32 at every step (cudaErrorFunction_mi.cuh:198). */
+float orc_mi_alt_area(int fw, int fh, int64_t n) {
+    const double d = 1.0 / (double) (fw * fh);
+    float fa = 0.0f;
+    for (int64_t i = 0; i < n; i++) fa = (float) ((double) fa + d);
+    return fa;
+}
+
+/* calcMIAlt (cudaErrorFunction_mi.cuh:117-255): normalised H and lines,
+ * rows and lambda stepped in binary32 by 1/(H_f - 1) and 1/(W_f - 1);
+ * the sample points scaled back to pixels through binary64 ((x + 0.5) * W);
+ * the moving sample is the gated bilinear value, so a point outside the
+ * moving image counts in bin 0 (Q3).  Cost -MI * fix_area, MI in binary64
+ * from the exact counts. */
+static float cost_mi_alt(const orc_image *moved, const orc_image *fixed, const float *params, int bins,
+                         uint32_t *hist, double *aux) {
+    const int mw = moved->width, mh = moved->height, fw = fixed->width, fh = fixed->height, C = fixed->channels;
+    float h[8];
+    homography_norm(params, mw, mh, fw, fh, h);
+    const float sr = mi_alt_row_step(fh), sc = 1 / ((float) (fw - 1));
+    int64_t steps = 0, count = 0;
+    for (float row = -0.5f; row < 0.5; row += sr) {
+        orc_line L;
+        line_norm(h, row, &L);
+        if (!L.in) continue;
+        for (float lam = L.ls; lam < L.le && lam < 1; lam += sc) {
+            float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
+            nx = (float) (((double) nx + 0.5) * (double) mw);
+            ny = (float) (((double) ny + 0.5) * (double) mh);
+            const float ri = (float) (((double) row + 0.5) * (double) (fh - 1));
+            const float li = lam * (float) (fw - 1);
+            for (int c = 0; c < C; c++) {
+                const float f = orc_sample_bilinear(fixed, ri, li, c);
+                const float m = orc_sample_bilinear(moved, ny, nx, c);
+                hist[mi_bin(f, bins) * bins + mi_bin(m, bins)]++;
+                count++;
+            }
+            steps++;
+        }
+    }
+    const float fa = orc_mi_alt_area(fw, fh, steps);
+    double mi = 0, hx = 0, hy = 0;
+    if (count > 0) mi_of_hist(hist, bins, (double) count, &mi, &hx, &hy);
+    if (aux) {
+        aux[0] = (double) steps; aux[1] = aux[2] = aux[3] = 0.0; aux[4] = (double) count;
+        aux[5] = mi; aux[6] = 0.0; aux[7] = (double) fa; aux[8] = 1.0;
+    }
+    return (float) (-mi * (double) fa);
+}
+
+static float cost_alt(const orc_image *moved, const orc_image *fixed, int cost, const float *params,
+                      int bins, uint32_t *hist_out, double *aux_out) {
+    const int is_mi = cost == ORC_COST_MI_STREAM_ALT || cost == ORC_COST_MI_ALT;
+    uint32_t *hist = NULL;
+    if (is_mi) {
+        hist = hist_out ? hist_out : (uint32_t *) malloc(sizeof(uint32_t) * bins * bins);
+        memset(hist, 0, sizeof(uint32_t) * bins * bins);
+    }
+    const float v = cost == ORC_COST_MI_ALT ? cost_mi_alt(moved, fixed, params, bins, hist, aux_out)
+                                            : cost_alt_step(moved, fixed, cost, params, bins, hist, aux_out);
+    if (is_mi && !hist_out) free(hist);
+    return v;
 }
 
 /* Evaluates grid indices [i0, i1) into result[0 .. i1-i0).  Threads: OpenMP

@@ -15,8 +15,10 @@ _SO = os.path.join(_HERE, "_build", "libcgs_oracle.so")
 
 COST_SAD, COST_NCC, COST_MI, COST_SQD, COST_SAD_SUM, COST_MI_PLAIN = range(6)
 COST_NCC_STREAM, COST_SQD_STREAM, COST_SAD_STREAM = 6, 7, 8
+COST_SQD_ALT, COST_NCC_ALT, COST_NCC_STREAM_ALT, COST_MI_STREAM_ALT, COST_MI_ALT = 9, 10, 11, 12, 13
+ALT_COSTS = (COST_SQD_ALT, COST_NCC_ALT, COST_NCC_STREAM_ALT, COST_MI_STREAM_ALT, COST_MI_ALT)
 NEAREST, BILINEAR = 0, 1
-PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION = range(4)
+PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION, PARAM_NORM = range(5)
 TIE_LOWEST, TIE_REFERENCE = 0, 1
 PIX_U8, PIX_F32 = 0, 1
 
@@ -67,6 +69,13 @@ def lib():
         L.orc_max_threads.restype = ctypes.c_int
         L.orc_covariance.argtypes = [ctypes.c_int, fp, fp, fp, fp, i64, ctypes.POINTER(ctypes.c_double), fp]
         L.orc_covariance.restype = i64
+        L.orc_alt_homography.argtypes = [ctypes.c_int, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
+        L.orc_alt_line.argtypes = [ctypes.c_int, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_float, fp]
+        L.orc_mi_alt_rows.argtypes = [ctypes.c_int, fp, i64]
+        L.orc_mi_alt_rows.restype = i64
+        L.orc_mi_alt_area.argtypes = [ctypes.c_int, ctypes.c_int, i64]
+        L.orc_mi_alt_area.restype = ctypes.c_float
         _lib = L
     return _lib
 
@@ -139,6 +148,35 @@ def warp_point(coef, x, y, mw=1e30, mh=1e30):
     return out, bool(inside)
 
 
+def alt_homography(params, mw, mh, fw, fh, norm=False):
+    """H of the line-traversal costs: parametersToHomography (centred on the
+    moving image) or parametersToHomographyNorm (norm=True)."""
+    p = np.zeros(8, np.float32)
+    p[:len(params)] = params
+    h = np.zeros(8, np.float32)
+    lib().orc_alt_homography(int(norm), _f32p(p), mw, mh, fw, fh, _f32p(h))
+    return h
+
+
+def alt_line(h, mw, mh, fw, fh, row, norm=False):
+    """One fixed row's line: dict(ls, le, v, p0, inside)."""
+    hh = np.ascontiguousarray(h, np.float32)
+    out = np.zeros(7, np.float32)
+    lib().orc_alt_line(int(norm), _f32p(hh), mw, mh, fw, fh, float(row), _f32p(out))
+    return dict(ls=out[0], le=out[1], v=out[2:4].copy(), p0=out[4:6].copy(), inside=bool(out[6]))
+
+
+def mi_alt_rows(fh):
+    n = int(lib().orc_mi_alt_rows(fh, None, 0))
+    rows = np.zeros(n, np.float32)
+    lib().orc_mi_alt_rows(fh, _f32p(rows), n)
+    return rows
+
+
+def mi_alt_area(fw, fh, n):
+    return np.float32(lib().orc_mi_alt_area(fw, fh, int(n)))
+
+
 def sample(img, y, x, sampler, c=0):
     im = image(img)
     f = lib().orc_sample_nearest if sampler == NEAREST else lib().orc_sample_bilinear
@@ -147,7 +185,9 @@ def sample(img, y, x, sampler, c=0):
 
 def cost_point(moved, fixed, cost, sampler, params, mode=PARAM_CENTRED, bins=32, want_hist=False):
     """Returns (cost, aux) with aux = [sum, i1, i2, ic, count, mi, hx, hy, mi_cost]
-    (and the bins x bins histogram when want_hist)."""
+    (and the bins x bins histogram when want_hist).  For the line-traversal
+    costs aux = [steps, sum f^2, sum m^2, cross term, count, mi, moved area,
+    fixed area, overlap test passed]."""
     m, f = image(moved), image(fixed)
     p = np.zeros(8, np.float32)
     p[:len(params)] = params

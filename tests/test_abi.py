@@ -90,3 +90,27 @@ def test_search_rejects_bad_arguments_before_touching_a_device():
     img = L.Image(None, 8, 8, 1, L.PIX_U8)
     assert L.lib().cgs_search(ctypes.byref(g), ctypes.byref(img), ctypes.byref(img), ctypes.byref(o), None, None,
                               None) == 1
+
+
+def test_line_traversal_options_are_validated():
+    """The line-traversal costs take the bilinear sampler and their own
+    parametrisation (checked before any device call)."""
+    st, g = _grid(8, *synth.grid_ranges(2, 1.0))
+    buf = (ctypes.c_uint8 * 64)()
+    img = L.Image(ctypes.cast(buf, ctypes.c_void_p), 8, 8, 1, L.PIX_U8)
+
+    def search(cost, sampler=L.SAMPLER_BILINEAR, mode=L.PARAM_CENTRED):
+        o = L.SearchOpts()
+        L.lib().cgs_default_opts(ctypes.byref(o))
+        o.cost, o.sampler, o.param_mode, o.mi_bins = cost, sampler, mode, 64
+        return L.lib().cgs_search(ctypes.byref(g), ctypes.byref(img), ctypes.byref(img), ctypes.byref(o), None,
+                                  None, None), L.lib().cgs_last_error().decode()
+
+    for cost in (L.COST_SQD_ALT, L.COST_NCC_ALT, L.COST_NCC_STREAM_ALT, L.COST_MI_STREAM_ALT):
+        assert search(cost, sampler=L.SAMPLER_NEAREST)[0] == 3                 # CGS_ERR_UNSUPPORTED
+        assert search(cost, mode=L.PARAM_NORM)[0] == 1
+        assert search(cost, mode=L.PARAM_NONCENTRED)[0] == 1
+    assert search(L.COST_MI_ALT, mode=L.PARAM_CENTRED)[0] == 1
+    st, msg = search(L.COST_NCC, mode=L.PARAM_NORM)
+    assert st == 1 and "calcMIAlt" in msg
+    assert search(14)[0] == 1
