@@ -10,19 +10,24 @@
 // start, start + step, ... below the line's end, with lambda accumulated in
 // binary32 and the fixed image sampled at (row, lambda).
 //
-// Two kernels, one grid point per wavefront (4 per workgroup):
-//   alt_step_kernel  -- the four unit-step functors.  The wavefront walks the
-//     rows in order; its 64 lanes take 64 consecutive lambda steps of a row,
-//     so moving-image taps along the line and fixed-image taps along the row
-//     are neighbours across lanes.  Lane j's lambda is the binary32 sequence
-//     lambda += 1 evaluated in closed form (advance_unit).  Step k of a row
-//     accumulates into lane k mod 64, and the lanes combine in xor-butterfly
-//     order: the sums are exactly the oracle's.
-//   alt_norm_kernel  -- calcMIAlt, whose lambda step 1/(W_f - 1) rounds at
-//     every step: a lane per row, walking its row sequentially as the
-//     reference does.  Row values and the fixed-area table come from the host.
-// MI costs build the joint histogram of their wavefront in LDS.  Compiled
-// with -ffp-contract=off (see cgs_device.hpp).
+// Rows go in blocks of 64: lane r sets up row rb + r's line, then the
+// wavefront walks the block's qualifying rows in order, taking 64
+// consecutive lambda steps of a row per pass (lane j: step j), so the
+// moving-image taps along the line and the fixed-image taps along the row
+// are neighbours across lanes.  Lambda is the reference's binary32 sequence,
+// evaluated in closed form per lane (cgs_lambda.hpp).
+//   alt_step_kernel     calcSQDAlt / calcNCCAlt / calcNCCstreamAlt, one grid
+//     point per wavefront.  Step k of a row accumulates into lane k mod 64
+//     and the lanes combine in xor-butterfly order, as the oracle declares,
+//     so the costs are the oracle's bit for bit.
+//   alt_step_mi_kernel  calcMIstreamAlt, and
+//   alt_norm_kernel     calcMIAlt (normalised H, rows from a host table,
+//     lambda += 1/(W_f - 1) in runs inside one binade): one grid point per
+//     workgroup, the four wavefronts taking row blocks in turn into one LDS
+//     joint histogram.
+// The moved area is summed per row block in butterfly order and the block
+// sums in block order (the oracle's rule).  Compiled with -ffp-contract=off
+// (see cgs_device.hpp).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +39,7 @@
 #include "cgs_device.hpp"
 #include "cgs_host.hpp"
 #include "cgs_kernels.hpp"
+#include "cgs_lambda.hpp"
 
 namespace cgs {
 
@@ -145,30 +151,9 @@ __device__ __forceinline__ void homography_norm(const float *p, float c, float s
     h[7] = p[7];
 }
 
-// The value lambda reaches after r steps of lambda = lambda + 1.0f from x
-// (x >= 0 or -0, x < 2^23).  Inside a binade [2^e, 2^(e+1)) every +1 is
-// exact, so the steps that stay below the binade's top B are taken at once
-// (x + t, exact); the step that crosses B is taken as the one rounded add
-// x + 1.0f; below 1 every step crosses.
-__device__ __forceinline__ float advance_unit(float x, int r) {
-    while (r > 0) {
-        if (!(x >= 1.0f)) {
-            x = x + 1.0f;
-            r -= 1;
-            continue;
-        }
-        const float B = __uint_as_float((__float_as_uint(x) & 0x7F800000u) + 0x00800000u);
-        const float t = ceilf(B - x) - 1.0f;   // B - x is exact (Sterbenz)
-        if ((float) r <= t) {
-            x = x + (float) r;
-            r = 0;
-        } else {
-            x = x + t;
-            x = x + 1.0f;
-            r -= (int) t + 1;
-        }
-    }
-    return x;
+// v of lane j (j wavefront-uniform).
+__device__ __forceinline__ float lane_value(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
 // CudaImage::valueAt (include/cudaTensor.cuh:552-575): the bilinear value,
@@ -177,6 +162,20 @@ template <typename T>
 __device__ __forceinline__ float value_at(const T *img, float X, float Y, int c, int C, int W, int H) {
     if (!(X >= 0.0f && Y >= 0.0f && X < (float) W && Y < (float) H)) return 0.0f;
     return sample_bilinear(img, X, Y, c, C, W);
+}
+
+// valueAt(row, X) of the fixed image at an integral row inside the image and
+// 0 <= X < W (row_off = row * W * C): ceil(row) == row, so the bottom lerp is
+// the top one and the value is fmaf(0, top, 1 * top) -- top itself, or NaN
+// where top is infinite, exactly as the full form.
+template <typename T>
+__device__ __forceinline__ float row_value(const T *img, float X, int row_off, int c, int C) {
+    const float fx = floorf(X);
+    const float ax = X - fx;
+    const int b = row_off + (int) fx * C + c;
+    const float tl = (float) img[b], tr = (float) img[b + (ax != 0.0f ? C : 0)];
+    const float top = fmaf(ax, tr, (1.0f - ax) * tl);
+    return fmaf(0.0f, top, top);
 }
 
 // MI of a wavefront's joint histogram h (row = fixed bin) over n samples,
@@ -224,11 +223,11 @@ __device__ __forceinline__ void alt_grid_point(const AltParams &P, int32_t gi, f
     *s = cs.y;
 }
 
-// Writes the workgroup's 4 costs and folds their minimum into the argmin key
-// (thread 0).
-__device__ __forceinline__ void alt_emit(const AltParams &P, int32_t base, const float *s_cost) {
+// Writes the workgroup's costs (grid points base .. base + count - 1) and
+// folds their minimum into the argmin key (thread 0).
+__device__ __forceinline__ void alt_emit(const AltParams &P, int32_t base, const float *s_cost, int count) {
     unsigned long long best = ~0ull;
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < count; ++g) {
         const int32_t gi = base + g;
         if (gi < P.grid.n) {
             if (P.result) P.result[gi] = s_cost[g];
@@ -240,153 +239,256 @@ __device__ __forceinline__ void alt_emit(const AltParams &P, int32_t base, const
         atomicMin(P.key, best);
 }
 
-template <typename MT, typename FT, bool MI>
+// Walks one row's unit-step line: 64 consecutive lambda steps per pass, lane
+// j taking step j (advance_unit); calls at(lam, nx, ny) for the lane's step
+// while it is below the line's end and the row's width.  Returns the row's
+// step count (wavefront-uniform).
+template <typename F>
+__device__ __forceinline__ int32_t walk_unit_row(float ls, float le, float FW, float vx, float vy, float px,
+                                                 float py, F &&at) {
+    const int lane = threadIdx.x & 63;
+    const float flane = (float) lane;
+    float x0 = ls;
+    int32_t nrow = 0;
+    while (true) {
+        // x0 + 63 inside x0's binade [1, 2^23): every step is the exact x0 + j
+        const bool inside = x0 >= 1.0f && x0 + 63.0f < u2f((f2u(x0) & 0x7F800000u) + 0x00800000u);
+        const float lam = inside ? x0 + flane : advance_unit(x0, lane);
+        const bool ok = lam < le && lam < FW;
+        const unsigned long long m = __ballot(ok);
+        if (m == 0ull) break;
+        if (ok) at(lam, lam * vx + px, lam * vy + py);
+        nrow += __popcll(m);
+        if (!(m >> 63)) break;   // valid lanes form a prefix: the row ended in this pass
+        x0 = lane_value(lam, 63) + 1.0f;
+    }
+    return nrow;
+}
+
+// The line of row rb + lane (centred H) and its |v|; in = false past the image.
+__device__ __forceinline__ AltLine block_line(const float *h, const AltParams &P, int rb, float *mag) {
+    const int lane = threadIdx.x & 63;
+    AltLine L{};
+    *mag = 0.0f;
+    if (rb + lane < P.fh) {
+        L = line_centred(h, (float) P.mw, (float) P.mh, (float) P.fw, (float) P.fh, (float) (rb + lane));
+        *mag = sqrtf(L.vx * L.vx + L.vy * L.vy);
+    }
+    return L;
+}
+
+// calcSQDAlt / calcNCCAlt / calcNCCstreamAlt: one grid point per wavefront.
+template <typename MT, typename FT>
 __global__ __launch_bounds__(kThreads) void alt_step_kernel(const AltParams P) {
-    extern __shared__ uint32_t s_hist[];
     __shared__ float s_cost[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int32_t base = (int32_t) blockIdx.x * 4;
     const int32_t gi = base + wave;
-    const bool live = gi < P.grid.n;
-    const int B2 = P.bins * P.bins;
-    uint32_t *hist = s_hist + wave * B2;
-    if constexpr (MI) {
-        for (int i = lane; i < B2; i += 64) hist[i] = 0u;
-    }
-    __syncthreads();
-
     const MT *mov = (const MT *) P.mov;
     const FT *fix = (const FT *) P.fix;
     const int C = P.C;
-    const float MW = (float) P.mw, MH = (float) P.mh, FW = (float) P.fw, FH = (float) P.fh;
-    float p[kMaxDim];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, mov_area = 0.0;
-    int32_t steps = 0;
-    if (live) {
-        float c, s;
+    float cost = 0.0f;
+    if (gi < P.grid.n) {
+        float p[kMaxDim], c, s, h[8];
         alt_grid_point(P, gi, p, &c, &s);
-        float h[8];
-        homography_centred(p, c, s, MW / 2, MH / 2, h);
-        for (int row = 0; row < P.fh; ++row) {
-            const AltLine L = line_centred(h, MW, MH, FW, FH, (float) row);
-            if (!L.in) continue;
-            const float mag = sqrtf(L.vx * L.vx + L.vy * L.vy);
-            const float fr = (float) row;
-            float x0 = L.ls;
-            int32_t nrow = 0;
-            while (true) {
-                const float lam = advance_unit(x0, lane);
-                const bool ok = lam < L.le && lam < FW;
-                const unsigned long long m = __ballot(ok);
-                if (m == 0ull) break;
-                if (ok) {
-                    const float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
-                    for (int ch = 0; ch < C; ++ch) {
-                        const float mv = value_at(mov, nx, ny, ch, C, P.mw, P.mh);
-                        const float fv = value_at(fix, lam, fr, ch, C, P.fw, P.fh);
-                        if constexpr (MI) {
-                            atomicAdd(&hist[mi_bin(fv, P.bins) * P.bins + mi_bin(mv, P.bins)], 1u);
-                        } else {
-                            const double md = (double) mv, fd = (double) fv;
+        homography_centred(p, c, s, (float) P.mw / 2, (float) P.mh / 2, h);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, mov_area = 0.0;
+        int32_t steps = 0;
+        for (int rb = 0; rb < P.fh; rb += 64) {
+            // lane r sets up row rb + r's line; the wavefront walks the block's
+            // qualifying rows in order, reading each line from its lane
+            float magr;
+            const AltLine Lr = block_line(h, P, rb, &magr);
+            double term = 0.0;   // lane r: row rb + r's steps * |v|
+            unsigned long long rows_in = __ballot(Lr.in);
+            while (rows_in) {
+                const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
+                rows_in &= rows_in - 1;
+                const float fr = (float) (rb + j);
+                const int row_off = (rb + j) * P.fw * C;
+                const int32_t nrow = walk_unit_row(
+                    lane_value(Lr.ls, j), lane_value(Lr.le, j), (float) P.fw, lane_value(Lr.vx, j),
+                    lane_value(Lr.vy, j), lane_value(Lr.px, j), lane_value(Lr.py, j),
+                    [&](float lam, float nx, float ny) {
+                        for (int ch = 0; ch < C; ++ch) {
+                            const double md = (double) value_at(mov, nx, ny, ch, C, P.mw, P.mh);
+                            const double fd = (double) row_value(fix, lam, row_off, ch, C);
                             if (P.cost == CGS_COST_SQD_ALT) s0 = fma(md - fd, md - fd, s0);
                             else s0 = fma(md, fd, s0);
                             s1 = fma(fd, fd, s1);
                             s2 = fma(md, md, s2);
                         }
-                    }
-                }
-                nrow += __popcll(m);
-                if (!(m >> 63)) break;   // valid lanes form a prefix: the row ended inside this chunk
-                x0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lam), 63)) + 1.0f;
+                    });
+                if (lane == j && nrow > 0) term = (double) nrow * (double) magr;
+                steps += nrow;
             }
-            if (nrow > 0) mov_area += (double) nrow * (double) mag;
-            steps += nrow;
+            mov_area += wave_sum(term);
         }
-    }
-    __syncthreads();
-    float cost = 0.0f;
-    if (live) {
         const bool pass = alt_overlap((float) mov_area, (float) steps, P, p[2]);
-        if constexpr (MI) {
-            if (pass) {
-                double mi, hx, hy;
-                wave_mi(hist, P.bins, (double) steps * (double) C, &mi, &hx, &hy);
-                cost = (float) (-2.0 * mi / (hx + hy));
-            } else {
-                cost = 123123.0f;
-            }
-            if (P.hist_dump && gi == 0)
-                for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[i];
-        } else {
-            const double S0 = wave_sum(s0), S1 = wave_sum(s1), S2 = wave_sum(s2);
-            if (!pass) cost = P.cost == CGS_COST_NCC_STREAM_ALT ? 0.0f : 123123.0f;
-            else if (P.cost == CGS_COST_SQD_ALT) cost = (float) (S0 / sqrt(S1 * S2));
-            else cost = (float) (-S0 / sqrt(S1 * S2));
-        }
+        const double S0 = wave_sum(s0), S1 = wave_sum(s1), S2 = wave_sum(s2);
+        if (!pass) cost = P.cost == CGS_COST_NCC_STREAM_ALT ? 0.0f : 123123.0f;
+        else if (P.cost == CGS_COST_SQD_ALT) cost = (float) (S0 / sqrt(S1 * S2));
+        else cost = (float) (-S0 / sqrt(S1 * S2));
     }
     if (lane == 0) s_cost[wave] = cost;
     __syncthreads();
-    if (threadIdx.x == 0) alt_emit(P, base, s_cost);
+    if (threadIdx.x == 0) alt_emit(P, base, s_cost, 4);
 }
 
-template <typename MT, typename FT>
-__global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
-    extern __shared__ uint32_t s_hist[];
-    __shared__ float s_cost[4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int32_t base = (int32_t) blockIdx.x * 4;
-    const int32_t gi = base + wave;
-    const bool live = gi < P.grid.n;
-    const int B2 = P.bins * P.bins;
-    uint32_t *hist = s_hist + wave * B2;
-    for (int i = lane; i < B2; i += 64) hist[i] = 0u;
-    __syncthreads();
+// MI costs: one grid point per workgroup, the four wavefronts taking blocks
+// of 64 rows in turn (wavefront w: blocks w, w + 4, ...) into one LDS joint
+// histogram.  LDS: bins^2 counters, then one binary64 moved-area sum per row
+// block (summed in block order by thread 0).
+__device__ __forceinline__ uint32_t *alt_lds_hist() {
+    extern __shared__ uint32_t s_dyn[];
+    return s_dyn;
+}
 
+// calcMIstreamAlt.
+template <typename MT, typename FT>
+__global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P) {
+    __shared__ int32_t s_steps[4];
+    __shared__ float s_cost;
+    uint32_t *hist = alt_lds_hist();
+    const int B2 = P.bins * P.bins;
+    double *s_mov = (double *) (hist + ((B2 + 1) & ~1));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t gi = (int32_t) blockIdx.x;
+    for (int i = threadIdx.x; i < B2; i += kThreads) hist[i] = 0u;
+    __syncthreads();
     const MT *mov = (const MT *) P.mov;
     const FT *fix = (const FT *) P.fix;
     const int C = P.C;
+    float p[kMaxDim], c, s, h[8];
+    alt_grid_point(P, gi, p, &c, &s);
+    homography_centred(p, c, s, (float) P.mw / 2, (float) P.mh / 2, h);
     int32_t steps = 0;
-    if (live) {
-        float p[kMaxDim], c, s;
-        alt_grid_point(P, gi, p, &c, &s);
-        float h[8];
-        homography_norm(p, c, s, P.mw, P.mh, P.fw, P.fh, h);
-        const float sc = 1 / ((float) (P.fw - 1));
-        const int max_steps = P.fw + 2;   // lambda < 1 in steps of 1/(W_f - 1): at most W_f of them
-        for (int k = lane; k < P.nrows; k += 64) {
-            const float row = P.rows[k];
-            const AltLine L = line_norm(h, row);
-            if (!L.in) continue;
-            const float ri = (float) (((double) row + 0.5) * (double) (P.fh - 1));
-            int n = 0;
-            for (float lam = L.ls; lam < L.le && lam < 1 && n < max_steps; lam = lam + sc, ++n) {
-                float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
-                nx = (float) (((double) nx + 0.5) * (double) P.mw);
-                ny = (float) (((double) ny + 0.5) * (double) P.mh);
-                const float li = lam * (float) (P.fw - 1);
-                for (int ch = 0; ch < C; ++ch) {
-                    const float fv = value_at(fix, li, ri, ch, C, P.fw, P.fh);
-                    const float mv = value_at(mov, nx, ny, ch, C, P.mw, P.mh);
-                    atomicAdd(&hist[mi_bin(fv, P.bins) * P.bins + mi_bin(mv, P.bins)], 1u);
-                }
-            }
-            steps += n;
+    for (int blk = wave; blk * 64 < P.fh; blk += 4) {
+        const int rb = blk * 64;
+        float magr;
+        const AltLine Lr = block_line(h, P, rb, &magr);
+        double term = 0.0;
+        unsigned long long rows_in = __ballot(Lr.in);
+        while (rows_in) {
+            const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
+            rows_in &= rows_in - 1;
+            const float fr = (float) (rb + j);
+            const int row_off = (rb + j) * P.fw * C;
+            const int32_t nrow = walk_unit_row(
+                lane_value(Lr.ls, j), lane_value(Lr.le, j), (float) P.fw, lane_value(Lr.vx, j), lane_value(Lr.vy, j),
+                lane_value(Lr.px, j), lane_value(Lr.py, j), [&](float lam, float nx, float ny) {
+                    for (int ch = 0; ch < C; ++ch) {
+                        const float mv = value_at(mov, nx, ny, ch, C, P.mw, P.mh);
+                        const float fv = row_value(fix, lam, row_off, ch, C);
+                        atomicAdd(&hist[mi_bin(fv, P.bins) * P.bins + mi_bin(mv, P.bins)], 1u);
+                    }
+                });
+            if (lane == j && nrow > 0) term = (double) nrow * (double) magr;
+            steps += nrow;
         }
+        const double bsum = wave_sum(term);
+        if (lane == 0) s_mov[blk] = bsum;
+    }
+    if (lane == 0) s_steps[wave] = steps;
+    __syncthreads();
+    if (wave == 0) {
+        double mov_area = 0.0;
+        for (int blk = 0; blk * 64 < P.fh; ++blk) mov_area += s_mov[blk];   // block order
+        const int32_t total = s_steps[0] + s_steps[1] + s_steps[2] + s_steps[3];
+        float cost = 123123.0f;
+        if (alt_overlap((float) mov_area, (float) total, P, p[2])) {
+            double mi, hx, hy;
+            wave_mi(hist, P.bins, (double) total * (double) C, &mi, &hx, &hy);
+            cost = (float) (-2.0 * mi / (hx + hy));
+        }
+        if (P.hist_dump && gi == 0)
+            for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[i];
+        if (lane == 0) s_cost = cost;
     }
     __syncthreads();
-    float cost = 0.0f;
-    if (live) {
-        const int32_t total = (int32_t) wave_sum((double) steps);
+    if (threadIdx.x == 0) alt_emit(P, gi, &s_cost, 1);
+}
+
+// calcMIAlt: rows from the host's table, lambda += 1/(W_f - 1) in runs that
+// stay inside one binade with one increment (step_run), lane j taking step j
+// of the run; the fixed area after n steps comes from the host's table.
+template <typename MT, typename FT>
+__global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
+    __shared__ int32_t s_steps[4];
+    __shared__ float s_cost;
+    uint32_t *hist = alt_lds_hist();
+    const int B2 = P.bins * P.bins;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t gi = (int32_t) blockIdx.x;
+    for (int i = threadIdx.x; i < B2; i += kThreads) hist[i] = 0u;
+    __syncthreads();
+    const MT *mov = (const MT *) P.mov;
+    const FT *fix = (const FT *) P.fix;
+    const int C = P.C;
+    float p[kMaxDim], c, s, h[8];
+    alt_grid_point(P, gi, p, &c, &s);
+    homography_norm(p, c, s, P.mw, P.mh, P.fw, P.fh, h);
+    const float sc = 1 / ((float) (P.fw - 1));
+    const float wm1 = (float) (P.fw - 1);
+    const int max_runs = P.fw + 3;   // lambda < 1 in steps of 1/(W_f - 1): at most W_f steps, >= 1 per run
+    int32_t steps = 0;
+    for (int blk = wave; blk * 64 < P.nrows; blk += 4) {
+        const int k = blk * 64 + lane;
+        AltLine Lr{};
+        float rir = 0.0f;
+        if (k < P.nrows) {
+            const float row = P.rows[k];
+            Lr = line_norm(h, row);
+            rir = (float) (((double) row + 0.5) * (double) (P.fh - 1));
+        }
+        unsigned long long rows_in = __ballot(Lr.in);
+        while (rows_in) {
+            const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
+            rows_in &= rows_in - 1;
+            const float le = lane_value(Lr.le, j);
+            const float vx = lane_value(Lr.vx, j), vy = lane_value(Lr.vy, j);
+            const float px = lane_value(Lr.px, j), py = lane_value(Lr.py, j);
+            const float ri = lane_value(rir, j);
+            float x0 = lane_value(Lr.ls, j);
+            for (int run_i = 0; run_i < max_runs; ++run_i) {
+                int32_t a, d;
+                float U;
+                const int run = __builtin_amdgcn_readfirstlane(step_run(x0, sc, &a, &d, &U));
+                const float lam = lane == 0 ? x0 : (float) (a + lane * d) * U;
+                const bool ok = lane < run && lam < le && lam < 1;
+                const unsigned long long m = __ballot(ok);
+                if (m == 0ull) break;
+                if (ok) {
+                    float nx = lam * vx + px, ny = lam * vy + py;
+                    nx = (float) (((double) nx + 0.5) * (double) P.mw);
+                    ny = (float) (((double) ny + 0.5) * (double) P.mh);
+                    const float li = lam * wm1;
+                    for (int ch = 0; ch < C; ++ch) {
+                        const float fv = value_at(fix, li, ri, ch, C, P.fw, P.fh);
+                        const float mv = value_at(mov, nx, ny, ch, C, P.mw, P.mh);
+                        atomicAdd(&hist[mi_bin(fv, P.bins) * P.bins + mi_bin(mv, P.bins)], 1u);
+                    }
+                }
+                const int nok = __popcll(m);
+                steps += nok;
+                if (nok < run) break;   // valid lanes form a prefix: the row ended in this run
+                x0 = lane_value(lam, run - 1) + sc;
+            }
+        }
+    }
+    if (lane == 0) s_steps[wave] = steps;
+    __syncthreads();
+    if (wave == 0) {
+        const int32_t total = s_steps[0] + s_steps[1] + s_steps[2] + s_steps[3];
         const float fa = P.area[total < P.narea ? total : P.narea - 1];
         double mi = 0.0, hx, hy;
         if (total > 0) wave_mi(hist, P.bins, (double) total * (double) C, &mi, &hx, &hy);
-        cost = (float) (-mi * (double) fa);
         if (P.hist_dump && gi == 0)
             for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[i];
+        if (lane == 0) s_cost = (float) (-mi * (double) fa);
     }
-    if (lane == 0) s_cost[wave] = cost;
     __syncthreads();
-    if (threadIdx.x == 0) alt_emit(P, base, s_cost);
+    if (threadIdx.x == 0) alt_emit(P, gi, &s_cost, 1);
 }
 
 // ------------------------------------------------------------------ host
@@ -415,14 +517,14 @@ static hipError_t pad_image(const cgs_image *im, void *dst, hipStream_t s) {
 
 template <typename MT, typename FT>
 static hipError_t alt_launch(const AltParams &P, hipStream_t s) {
-    const int blocks = (P.grid.n + 3) / 4;
-    const size_t lds = 4 * (size_t) P.bins * P.bins * sizeof(uint32_t);
+    const int B2 = P.bins * P.bins;
+    const size_t lds = sizeof(uint32_t) * (size_t) ((B2 + 1) & ~1) + sizeof(double) * (size_t) ((P.fh + 63) / 64);
     if (P.cost == CGS_COST_MI_ALT)
-        hipLaunchKernelGGL((alt_norm_kernel<MT, FT>), dim3(blocks), dim3(kThreads), lds, s, P);
+        hipLaunchKernelGGL((alt_norm_kernel<MT, FT>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
     else if (P.cost == CGS_COST_MI_STREAM_ALT)
-        hipLaunchKernelGGL((alt_step_kernel<MT, FT, true>), dim3(blocks), dim3(kThreads), lds, s, P);
+        hipLaunchKernelGGL((alt_step_mi_kernel<MT, FT>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
     else
-        hipLaunchKernelGGL((alt_step_kernel<MT, FT, false>), dim3(blocks), dim3(kThreads), 0, s, P);
+        hipLaunchKernelGGL((alt_step_kernel<MT, FT>), dim3((P.grid.n + 3) / 4), dim3(kThreads), 0, s, P);
     return hipGetLastError();
 }
 
@@ -469,6 +571,10 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
         (st = ws->sincos_host.ensure(8 * (size_t) levels0, true)) != CGS_OK)
         return st;
     if (hist_dump_host && mi && (st = ws->hist.ensure(B2 * sizeof(uint32_t))) != CGS_OK) return st;
+    if (mi && sizeof(uint32_t) * B2 + sizeof(double) * ((size_t) (fixed->height + 63) / 64) > 64 * 1024)
+        return set_error(CGS_ERR_TOO_LARGE, "line-traversal MI: the histogram and row-block sums exceed 64 KiB of LDS");
+    unsigned long long *key = (unsigned long long *) ws->key.ptr;
+    hipLaunchKernelGGL(alt_key_kernel, dim3(1), dim3(64), 0, stream, key);
     HIP_TRY(pad_image(moved, ws->mov.ptr, stream));
     HIP_TRY(pad_image(fixed, ws->fixp.ptr, stream));
 
@@ -505,8 +611,6 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
         }
         HIP_TRY(hipMemcpyAsync(ws->sincos.ptr, h, sizeof(float) * 2 * levels0, hipMemcpyHostToDevice, stream));
     }
-    unsigned long long *key = (unsigned long long *) ws->key.ptr;
-    hipLaunchKernelGGL(alt_key_kernel, dim3(1), dim3(64), 0, stream, key);
 
     fill_grid_dev(grid, &P.grid);
     P.sincos = (const float2 *) ws->sincos.ptr;

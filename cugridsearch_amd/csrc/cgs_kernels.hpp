@@ -59,18 +59,19 @@ __device__ __forceinline__ float sample_nearest(const MT *img, float X, float Y,
 }
 
 // valueAt_bilinear (include/cudaTensor.cuh:561-575), each lerp declared as
-// fmaf(a, q, (1-a)*p).  The right/bottom taps
-// are read at floor+1; where the coordinate is integral the reference reads
-// ceil == floor instead, and that tap's weight is exactly 0, so the value is
-// the same.  ceil(x) == W lands on the next row through the linear index and
-// ceil(y) == H on the zero pad, as in the reference's linear addressing.
+// fmaf(a, q, (1-a)*p).  The right/bottom taps are read at ceil, as the
+// reference does: at an integral coordinate that is the floor tap itself
+// (weight 0, and a non-finite neighbour does not leak in).  ceil(x) == W
+// lands on the next row through the linear index and ceil(y) == H on the
+// zero pad, as in the reference's linear addressing.
 template <typename MT>
 __device__ __forceinline__ float sample_bilinear(const MT *img, float X, float Y, int c, int C, int W) {
     const float fx = floorf(X), fy = floorf(Y);
-    const int b = ((int) fy * W + (int) fx) * C + c;
-    const float tl = (float) img[b], tr = (float) img[b + C];
-    const float bl = (float) img[b + C * W], br = (float) img[b + C * W + C];
     const float ax = X - fx, ay = Y - fy;
+    const int b = ((int) fy * W + (int) fx) * C + c;
+    const int dx = ax != 0.0f ? C : 0, dy = ay != 0.0f ? C * W : 0;
+    const float tl = (float) img[b], tr = (float) img[b + dx];
+    const float bl = (float) img[b + dy], br = (float) img[b + dy + dx];
     const float top = fmaf(ax, tr, (1.0f - ax) * tl);
     const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
     return fmaf(ay, bottom, (1.0f - ay) * top);

@@ -1,0 +1,97 @@
+// cgs_lambda.hpp -- closed forms of the binary32 lambda sequences of the
+// line-traversal costs, so that a lane can start at step j of a row without
+// walking steps 0..j-1.
+//
+//   unit step   lambda = lambda + 1.0f     (src/gpu/cudaErrorFunctions.cuh:270, 346)
+//   norm step   lambda = lambda + sc, sc = 1 / (float)(W_f - 1)
+//                                          (src/gpu/cudaErrorFunction_mi.cuh:177)
+//
+// Both give exactly the values of the sequential loop; the host build (g++,
+// -ffp-contract=off) is checked against that loop by
+// tests/test_lambda_steps.py.  Within a binade [2^e, 2^(e+1)) with ulp U,
+// x + s for x on the U grid rounds to x + delta with one delta per binade
+// (delta = s rounded to the U grid), so runs of steps inside a binade are
+// taken at once in integer units of U, and the step that leaves the binade
+// is taken as the literal rounded add.
+#pragma once
+
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#define CGS_HD __host__ __device__
+#else
+#define CGS_HD
+#endif
+
+namespace cgs {
+
+CGS_HD inline uint32_t f2u(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+CGS_HD inline float u2f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// lambda after r steps of lambda = lambda + 1.0f from x (x >= 0 or -0,
+// x < 2^23).  Every +1 inside a binade of x >= 1 is exact; below 1 every step
+// leaves the binade.
+CGS_HD inline float advance_unit(float x, int r) {
+    while (r > 0) {
+        if (!(x >= 1.0f)) {
+            x = x + 1.0f;
+            r -= 1;
+            continue;
+        }
+        const float B = u2f((f2u(x) & 0x7F800000u) + 0x00800000u);   // top of x's binade
+        const float t = __builtin_ceilf(B - x) - 1.0f;                // steps staying below B; B - x exact
+        if ((float) r <= t) {
+            x = x + (float) r;
+            r = 0;
+        } else {
+            x = x + t;
+            x = x + 1.0f;
+            r -= (int) t + 1;
+        }
+    }
+    return x;
+}
+
+// A run of the lambda = lambda + s sequence (x >= 0 or -0, s > 0, x and s
+// normal) that a wavefront takes in one pass: returns n in [1, 64] such that
+// step j < n of the run is (float) (a + j * d) * U (step 0 is x itself), all
+// inside x's binade with one increment d * U; the step after the run is the
+// literal add of s to step n - 1.  When s is an odd multiple of U / 2 every
+// sum is a tie and ties-to-even picks the even one of the two neighbouring
+// grid points: from an even a the increment is the even one of
+// floor(s / U) and floor(s / U) + 1 at every step, so the run goes on; from an
+// odd a the first step is taken alone.  A run of 1 also covers x == 0 and a
+// step that leaves the binade.
+CGS_HD inline int step_run(float x, float s, int32_t *a, int32_t *d, float *U) {
+    *a = 0;
+    *d = 0;
+    *U = 1.0f;
+    if (x == 0.0f) return 1;
+    const uint32_t bits = f2u(x);
+    const uint32_t expo = bits & 0x7F800000u;
+    const float u = u2f(expo - (23u << 23));
+    const float B = u2f(expo + 0x00800000u);
+    const float y = x + s;
+    const float delta = y - x;
+    if (!(y < B) || delta == 0.0f) return 1;
+    const float err = s - delta;                                   // exact, |err| <= U / 2
+    const int32_t am = (int32_t) ((bits & 0x007FFFFFu) | 0x00800000u);  // x / U
+    if ((err == 0.5f * u || err == -0.5f * u) && (am & 1)) return 1;
+    *a = am;
+    *d = (int32_t) (delta / u);
+    *U = u;
+    const int32_t room = (0x00FFFFFF - *a) / *d;
+    return room + 1 < 64 ? room + 1 : 64;
+}
+
+}  // namespace cgs

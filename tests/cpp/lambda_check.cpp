@@ -1,0 +1,68 @@
+// Checks the closed forms of cgs_lambda.hpp against the sequential binary32
+// loops they replace.  Prints "<mismatches> <comparisons>".
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "cgs_lambda.hpp"
+
+static bool same(float a, float b) { return std::memcmp(&a, &b, 4) == 0; }
+
+int main() {
+    std::mt19937 rng(12345);
+    std::uniform_real_distribution<float> u01(0.0f, 1.0f);
+    long bad = 0, n = 0, runs512 = 0;
+    // lambda += 1.0f from starts in [0, 65536), including 0, -0, integers and
+    // values just below powers of two
+    std::vector<float> starts = {0.0f, -0.0f, 1.0f, 0.5f, 0.999999940f, 63.9999962f, 255.5f, 1023.99994f};
+    for (int i = 0; i < 4000; ++i) {
+        const int e = (int) (rng() % 17);
+        starts.push_back(u01(rng) * (float) (1 << e));
+    }
+    for (float x0 : starts) {
+        float seq = x0;
+        for (int k = 0; k <= 600; ++k) {
+            ++n;
+            if (!same(cgs::advance_unit(x0, k), seq)) ++bad;
+            seq = seq + 1.0f;
+        }
+    }
+    // lambda += 1/(float)(W - 1) from starts in [0, 1), walked in runs (the
+    // kernel's passes); runs counts the passes of the starts at 0
+    std::vector<int> widths;
+    for (int w = 2; w <= 300; ++w) widths.push_back(w);
+    for (int i = 0; i < 300; ++i) widths.push_back(300 + (int) (rng() % 8000));
+    for (int w : {256, 257, 384, 385, 511, 512, 513, 640, 1000, 1024, 1025, 4096, 4097, 6001}) widths.push_back(w);
+    for (int w : widths) {
+        const float s = 1 / ((float) (w - 1));
+        std::vector<float> xs = {0.0f, -0.0f, s, 0.5f * s, 0.25f, 0.5f, 0.499999970f};
+        for (int i = 0; i < 6; ++i) xs.push_back(u01(rng));
+        for (int i = 0; i < 3; ++i) xs.push_back(u01(rng) * 1e-3f);
+        for (float x0 : xs) {
+            float seq = x0;
+            std::vector<float> want;
+            for (int k = 0; seq < 1.0f && k < w + 4; ++k) {
+                want.push_back(seq);
+                seq = seq + s;
+            }
+            float base = x0;
+            size_t k = 0;
+            while (k < want.size()) {
+                if (x0 == 0.0f && w == 512) ++runs512;
+                int32_t a, d;
+                float U;
+                const int run = cgs::step_run(base, s, &a, &d, &U);
+                float last = base;
+                for (int j = 0; j < run && k < want.size(); ++j, ++k) {
+                    last = j == 0 ? base : (float) (a + j * d) * U;
+                    ++n;
+                    if (!same(last, want[k])) ++bad;
+                }
+                base = last + s;
+            }
+        }
+    }
+    std::printf("%ld %ld %ld\n", bad, n, runs512);
+    return 0;
+}

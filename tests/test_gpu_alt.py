@@ -130,3 +130,26 @@ def test_alt_batch_refused(gpu):
     b = A.BatchSearcher()
     with pytest.raises(Exception, match="line-traversal"):
         b.plan([(A.calcNCCAlt, grid, A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f), None)])
+
+
+@pytest.mark.parametrize("fn", [A.calcNCCAlt, A.calcMIstreamAlt], ids=["calcNCCAlt", "calcMIstreamAlt"])
+def test_alt_nonfinite_fixed_image(gpu, oracle, fn):
+    """A NaN and an infinity in the fixed image.  valueAt reads its taps at
+    floor and ceil, so at the integral row (and integral lambda) a non-finite
+    neighbour does not leak in, while an infinite pixel itself gives
+    1 * inf + 0 * inf = NaN; the kernels follow that bit for bit (MI bins a
+    NaN into bin 0)."""
+    m, f, _ = synth.make_pair(12, 40, 36, sigma=2.0, perturb=3.0, dtype=np.float32)
+    f = f.copy()
+    f[20, 11] = np.nan
+    f[7, 30] = np.inf
+    s, e, c = grid_for(fn)
+    got, _, _ = run(fn, m, f, s, e, c)
+    want = want_of(oracle, fn, m, f, s, e, c)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    assert np.isnan(want).any() or fn.cost != L.COST_NCC_ALT   # MI bins a NaN into bin 0
+    fin = ~np.isnan(want)
+    if fn.cost == L.COST_NCC_ALT:
+        assert np.array_equal(got[fin].view(np.uint32), want[fin].view(np.uint32))
+    else:
+        assert np.all((got[fin] == want[fin]) | (np.abs(got[fin] - want[fin]) <= MI_RTOL * np.abs(want[fin])))

@@ -232,6 +232,25 @@ def test_stream_functor_finish_rules(gpu, oracle):
         assert np.array_equal(res.data.cpu().numpy().view(np.uint32), want.view(np.uint32)), fn.name
 
 
+def test_exact_kernel_nonfinite_pixels(gpu, oracle):
+    """Three-channel float images (the exact kernel) with a NaN in the moving
+    image's last row and an infinity in its last column, under integer
+    translations: the bilinear taps at floor and ceil keep them out of the
+    samples next to them, as the reference's valueAt does, so the shifts that
+    leave that row and column out of the overlap have finite costs."""
+    planes = [synth.make_pair(60 + k, 32, 28, sigma=2.0, perturb=3.0, dtype=np.float32) for k in range(3)]
+    m = np.stack([p[0] for p in planes], -1)
+    f = np.stack([p[1] for p in planes], -1)
+    m[27, 12, 1] = np.nan
+    m[9, 31, 0] = np.inf
+    fn = A.calcNCC.with_(param_mode=L.PARAM_TRANSLATION)
+    got, _, _ = gpu_search(fn, m, f, [-2, -2], [2, 2], [5, 5])
+    want = oracle_search(oracle, fn, m, f, [-2, -2], [2, 2], [5, 5])
+    assert np.array_equal(np.isnan(got), np.isnan(want)) and np.isnan(want).any() and (~np.isnan(want)).any()
+    ok = close_mask(got, want, NCC_RTOL)
+    assert ok.all(), np.where(~ok)[0][:8]
+
+
 def test_driver_images_translation(gpu, oracle):
     d = json.load(open(os.path.join(HERE, "golden", "driver_images.json")))
     Aimg = np.array(d["imageA"], np.uint8).reshape(6, 6) * 255

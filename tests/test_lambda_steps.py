@@ -1,0 +1,21 @@
+"""The closed-form lambda sequences of the line-traversal kernels
+(csrc/cgs_lambda.hpp) against the literal binary32 loops, compiled for the
+host with g++ -ffp-contract=off.  CPU only."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+def test_closed_form_lambda_matches_sequential_loops(tmp_path):
+    exe = str(tmp_path / "lambda_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+                    "-I", os.path.join(ROOT, "cugridsearch_amd", "csrc"), "-o", exe,
+                    os.path.join(HERE, "cpp", "lambda_check.cpp")], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    bad, n, runs512 = int(out[0]), int(out[1]), int(out[2])
+    assert n > 2_000_000 and bad == 0, (bad, n)
+    # a 512-wide row (step 1/511, a tie in one binade) from lambda = 0 (twice:
+    # +0 and -0) goes in a few runs, not one pass per step
+    assert runs512 <= 2 * 16, runs512
