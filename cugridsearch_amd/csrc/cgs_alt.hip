@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "cgs.h"
@@ -47,8 +48,10 @@ struct AltParams {
     GridDev grid;
     const float2 *sincos;   // (cos, sin) per level of axis 0
     int cost;
-    const void *mov;        // moving image + zero pad
-    const void *fix;        // fixed image + zero pad
+    const void *mov;        // moving image + zero pad (several channels)
+    const void *fix;        // fixed image + zero pad (several channels)
+    const void *mq;         // moving image's exact tap table (one channel)
+    const void *fq;         // fixed image's exact tap table (one channel)
     int mw, mh, fw, fh, C;
     int bins;
     const float *rows;      // calcMIAlt: the row values
@@ -239,10 +242,78 @@ __device__ __forceinline__ void alt_emit(const AltParams &P, int32_t base, const
         atomicMin(P.key, best);
 }
 
+// Image samplers of the kernels.  valueAt (include/cudaTensor.cuh:552-575)
+// reads its taps at floor and ceil; at(X, Y) is that value, 0 outside the
+// image; at_row(X, row) the value at an integral row inside the image with
+// 0 <= X < W, where ceil(row) == row makes the bottom lerp the top one, so
+// the value is fmaf(0, top, 1 * top).
+//
+// TabSampler: one-channel images through their exact tap table, one load per
+// sample (entry L + W + 1 holds p[L], p[L+1], p[L+W], p[L+W+1], 0 outside);
+// the ceil taps are picked by selects.  Out-of-image lanes read past the
+// table, where the buffer's range check returns zeros.
+template <typename T>
+struct TabSampler {
+    __amdgpu_buffer_rsrc_t rsrc;
+    int w;
+    float W, H;
+    __device__ TabSampler(const void *table, int width, int height) {
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void *) uniform(table), (short) 0,
+            (int) (((size_t) width * height + 2 * (size_t) width + 2) * TapTable<T>::kBytes), 0x00020000);
+        w = width;
+        W = (float) width;
+        H = (float) height;
+    }
+    __device__ __forceinline__ float at(float X, float Y, int) const {
+        const bool in = X >= 0.0f && Y >= 0.0f && X < W && Y < H;
+        const float fx = floorf(X), fy = floorf(Y);
+        const float ax = X - fx, ay = Y - fy;
+        const uint32_t off = in ? (uint32_t) (((int) fy * w + (int) fx + w + 1) * TapTable<T>::kBytes) : kOffsetOut;
+        const float4 q = TapTable<T>::load(rsrc, off);
+        const float tr = ax != 0.0f ? q.y : q.x;
+        const float bl = ay != 0.0f ? q.z : q.x;
+        const float br = ay != 0.0f ? (ax != 0.0f ? q.w : q.z) : tr;
+        const float top = fmaf(ax, tr, (1.0f - ax) * q.x);
+        const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
+        const float v = fmaf(ay, bottom, (1.0f - ay) * top);
+        return in ? v : 0.0f;
+    }
+    __device__ __forceinline__ float at_row(float X, int row, int) const {
+        const float fx = floorf(X), ax = X - fx;
+        const float4 q = TapTable<T>::load(rsrc, (uint32_t) ((row * w + (int) fx + w + 1) * TapTable<T>::kBytes));
+        const float tr = ax != 0.0f ? q.y : q.x;
+        const float top = fmaf(ax, tr, (1.0f - ax) * q.x);
+        return fmaf(0.0f, top, top);
+    }
+};
+
+// RawSampler: any channel count, on a zero-padded copy of the image.
+template <typename T>
+struct RawSampler {
+    const T *img;
+    int C, w, h;
+    __device__ RawSampler(const void *p, int channels, int width, int height)
+        : img((const T *) p), C(channels), w(width), h(height) {}
+    __device__ __forceinline__ float at(float X, float Y, int c) const { return value_at(img, X, Y, c, C, w, h); }
+    __device__ __forceinline__ float at_row(float X, int row, int c) const {
+        return row_value(img, X, row * w * C, c, C);
+    }
+};
+
+template <typename T, bool ONE>
+using Sampler = typename std::conditional<ONE, TabSampler<T>, RawSampler<T>>::type;
+
+template <typename T, bool ONE>
+__device__ __forceinline__ Sampler<T, ONE> make_sampler(const void *tab, const void *raw, int C, int w, int h) {
+    if constexpr (ONE) return TabSampler<T>(tab, w, h);
+    else return RawSampler<T>(raw, C, w, h);
+}
+
 // Walks one row's unit-step line: 64 consecutive lambda steps per pass, lane
-// j taking step j (advance_unit); calls at(lam, nx, ny) for the lane's step
-// while it is below the line's end and the row's width.  Returns the row's
-// step count (wavefront-uniform).
+// j taking step j; calls at(lam, nx, ny) for the lane's step while it is
+// below the line's end and the row's width.  Returns the row's step count
+// (wavefront-uniform).
 template <typename F>
 __device__ __forceinline__ int32_t walk_unit_row(float ls, float le, float FW, float vx, float vy, float px,
                                                  float py, F &&at) {
@@ -277,16 +348,17 @@ __device__ __forceinline__ AltLine block_line(const float *h, const AltParams &P
     return L;
 }
 
-// calcSQDAlt / calcNCCAlt / calcNCCstreamAlt: one grid point per wavefront.
-template <typename MT, typename FT>
+// calcSQDAlt (SQD true) / calcNCCAlt / calcNCCstreamAlt: one grid point per
+// wavefront.  ONE: one-channel images through tap tables.
+template <typename MT, typename FT, bool SQD, bool ONE>
 __global__ __launch_bounds__(kThreads) void alt_step_kernel(const AltParams P) {
     __shared__ float s_cost[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int32_t base = (int32_t) blockIdx.x * 4;
     const int32_t gi = base + wave;
-    const MT *mov = (const MT *) P.mov;
-    const FT *fix = (const FT *) P.fix;
-    const int C = P.C;
+    const auto mov = make_sampler<MT, ONE>(P.mq, P.mov, P.C, P.mw, P.mh);
+    const auto fix = make_sampler<FT, ONE>(P.fq, P.fix, P.C, P.fw, P.fh);
+    const int C = ONE ? 1 : P.C;
     float cost = 0.0f;
     if (gi < P.grid.n) {
         float p[kMaxDim], c, s, h[8];
@@ -304,16 +376,15 @@ __global__ __launch_bounds__(kThreads) void alt_step_kernel(const AltParams P) {
             while (rows_in) {
                 const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
                 rows_in &= rows_in - 1;
-                const float fr = (float) (rb + j);
-                const int row_off = (rb + j) * P.fw * C;
+                const int row = rb + j;
                 const int32_t nrow = walk_unit_row(
                     lane_value(Lr.ls, j), lane_value(Lr.le, j), (float) P.fw, lane_value(Lr.vx, j),
                     lane_value(Lr.vy, j), lane_value(Lr.px, j), lane_value(Lr.py, j),
                     [&](float lam, float nx, float ny) {
                         for (int ch = 0; ch < C; ++ch) {
-                            const double md = (double) value_at(mov, nx, ny, ch, C, P.mw, P.mh);
-                            const double fd = (double) row_value(fix, lam, row_off, ch, C);
-                            if (P.cost == CGS_COST_SQD_ALT) s0 = fma(md - fd, md - fd, s0);
+                            const double md = (double) mov.at(nx, ny, ch);
+                            const double fd = (double) fix.at_row(lam, row, ch);
+                            if constexpr (SQD) s0 = fma(md - fd, md - fd, s0);
                             else s0 = fma(md, fd, s0);
                             s1 = fma(fd, fd, s1);
                             s2 = fma(md, md, s2);
@@ -327,7 +398,7 @@ __global__ __launch_bounds__(kThreads) void alt_step_kernel(const AltParams P) {
         const bool pass = alt_overlap((float) mov_area, (float) steps, P, p[2]);
         const double S0 = wave_sum(s0), S1 = wave_sum(s1), S2 = wave_sum(s2);
         if (!pass) cost = P.cost == CGS_COST_NCC_STREAM_ALT ? 0.0f : 123123.0f;
-        else if (P.cost == CGS_COST_SQD_ALT) cost = (float) (S0 / sqrt(S1 * S2));
+        else if (SQD) cost = (float) (S0 / sqrt(S1 * S2));
         else cost = (float) (-S0 / sqrt(S1 * S2));
     }
     if (lane == 0) s_cost[wave] = cost;
@@ -338,14 +409,14 @@ __global__ __launch_bounds__(kThreads) void alt_step_kernel(const AltParams P) {
 // MI costs: one grid point per workgroup, the four wavefronts taking blocks
 // of 64 rows in turn (wavefront w: blocks w, w + 4, ...) into one LDS joint
 // histogram.  LDS: bins^2 counters, then one binary64 moved-area sum per row
-// block (summed in block order by thread 0).
+// block (summed in block order).
 __device__ __forceinline__ uint32_t *alt_lds_hist() {
     extern __shared__ uint32_t s_dyn[];
     return s_dyn;
 }
 
 // calcMIstreamAlt.
-template <typename MT, typename FT>
+template <typename MT, typename FT, bool ONE>
 __global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P) {
     __shared__ int32_t s_steps[4];
     __shared__ float s_cost;
@@ -356,9 +427,10 @@ __global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P
     const int32_t gi = (int32_t) blockIdx.x;
     for (int i = threadIdx.x; i < B2; i += kThreads) hist[i] = 0u;
     __syncthreads();
-    const MT *mov = (const MT *) P.mov;
-    const FT *fix = (const FT *) P.fix;
-    const int C = P.C;
+    const auto mov = make_sampler<MT, ONE>(P.mq, P.mov, P.C, P.mw, P.mh);
+    const auto fix = make_sampler<FT, ONE>(P.fq, P.fix, P.C, P.fw, P.fh);
+    const int C = ONE ? 1 : P.C;
+    const int bins = P.bins;
     float p[kMaxDim], c, s, h[8];
     alt_grid_point(P, gi, p, &c, &s);
     homography_centred(p, c, s, (float) P.mw / 2, (float) P.mh / 2, h);
@@ -372,15 +444,14 @@ __global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P
         while (rows_in) {
             const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
             rows_in &= rows_in - 1;
-            const float fr = (float) (rb + j);
-            const int row_off = (rb + j) * P.fw * C;
+            const int row = rb + j;
             const int32_t nrow = walk_unit_row(
                 lane_value(Lr.ls, j), lane_value(Lr.le, j), (float) P.fw, lane_value(Lr.vx, j), lane_value(Lr.vy, j),
                 lane_value(Lr.px, j), lane_value(Lr.py, j), [&](float lam, float nx, float ny) {
                     for (int ch = 0; ch < C; ++ch) {
-                        const float mv = value_at(mov, nx, ny, ch, C, P.mw, P.mh);
-                        const float fv = row_value(fix, lam, row_off, ch, C);
-                        atomicAdd(&hist[mi_bin(fv, P.bins) * P.bins + mi_bin(mv, P.bins)], 1u);
+                        const float mv = mov.at(nx, ny, ch);
+                        const float fv = fix.at_row(lam, row, ch);
+                        atomicAdd(&hist[mi_bin(fv, bins) * bins + mi_bin(mv, bins)], 1u);
                     }
                 });
             if (lane == j && nrow > 0) term = (double) nrow * (double) magr;
@@ -412,7 +483,7 @@ __global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P
 // calcMIAlt: rows from the host's table, lambda += 1/(W_f - 1) in runs that
 // stay inside one binade with one increment (step_run), lane j taking step j
 // of the run; the fixed area after n steps comes from the host's table.
-template <typename MT, typename FT>
+template <typename MT, typename FT, bool ONE>
 __global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
     __shared__ int32_t s_steps[4];
     __shared__ float s_cost;
@@ -422,14 +493,16 @@ __global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
     const int32_t gi = (int32_t) blockIdx.x;
     for (int i = threadIdx.x; i < B2; i += kThreads) hist[i] = 0u;
     __syncthreads();
-    const MT *mov = (const MT *) P.mov;
-    const FT *fix = (const FT *) P.fix;
-    const int C = P.C;
+    const auto mov = make_sampler<MT, ONE>(P.mq, P.mov, P.C, P.mw, P.mh);
+    const auto fix = make_sampler<FT, ONE>(P.fq, P.fix, P.C, P.fw, P.fh);
+    const int C = ONE ? 1 : P.C;
+    const int bins = P.bins;
     float p[kMaxDim], c, s, h[8];
     alt_grid_point(P, gi, p, &c, &s);
     homography_norm(p, c, s, P.mw, P.mh, P.fw, P.fh, h);
     const float sc = 1 / ((float) (P.fw - 1));
     const float wm1 = (float) (P.fw - 1);
+    const double dmw = (double) P.mw, dmh = (double) P.mh;
     const int max_runs = P.fw + 3;   // lambda < 1 in steps of 1/(W_f - 1): at most W_f steps, >= 1 per run
     int32_t steps = 0;
     for (int blk = wave; blk * 64 < P.nrows; blk += 4) {
@@ -460,13 +533,13 @@ __global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
                 if (m == 0ull) break;
                 if (ok) {
                     float nx = lam * vx + px, ny = lam * vy + py;
-                    nx = (float) (((double) nx + 0.5) * (double) P.mw);
-                    ny = (float) (((double) ny + 0.5) * (double) P.mh);
+                    nx = (float) (((double) nx + 0.5) * dmw);
+                    ny = (float) (((double) ny + 0.5) * dmh);
                     const float li = lam * wm1;
                     for (int ch = 0; ch < C; ++ch) {
-                        const float fv = value_at(fix, li, ri, ch, C, P.fw, P.fh);
-                        const float mv = value_at(mov, nx, ny, ch, C, P.mw, P.mh);
-                        atomicAdd(&hist[mi_bin(fv, P.bins) * P.bins + mi_bin(mv, P.bins)], 1u);
+                        const float fv = fix.at(li, ri, ch);
+                        const float mv = mov.at(nx, ny, ch);
+                        atomicAdd(&hist[mi_bin(fv, bins) * bins + mi_bin(mv, bins)], 1u);
                     }
                 }
                 const int nok = __popcll(m);
@@ -515,24 +588,42 @@ static hipError_t pad_image(const cgs_image *im, void *dst, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename MT, typename FT>
+// Exact tap table of a one-channel image.
+static hipError_t tap_table(const cgs_image *im, void *dst, hipStream_t s) {
+    if (im->pixtype == CGS_PIX_U8)
+        hipLaunchKernelGGL(tap_u8_kernel, dim3(1024), dim3(256), 0, s, (const uint8_t *) im->data, (uint32_t *) dst,
+                           im->width, im->height);
+    else
+        hipLaunchKernelGGL(tap_f32_kernel, dim3(1024), dim3(256), 0, s, (const float *) im->data, (float4 *) dst,
+                           im->width, im->height);
+    return hipGetLastError();
+}
+
+template <typename MT, typename FT, bool ONE>
 static hipError_t alt_launch(const AltParams &P, hipStream_t s) {
     const int B2 = P.bins * P.bins;
     const size_t lds = sizeof(uint32_t) * (size_t) ((B2 + 1) & ~1) + sizeof(double) * (size_t) ((P.fh + 63) / 64);
     if (P.cost == CGS_COST_MI_ALT)
-        hipLaunchKernelGGL((alt_norm_kernel<MT, FT>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
+        hipLaunchKernelGGL((alt_norm_kernel<MT, FT, ONE>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
     else if (P.cost == CGS_COST_MI_STREAM_ALT)
-        hipLaunchKernelGGL((alt_step_mi_kernel<MT, FT>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
+        hipLaunchKernelGGL((alt_step_mi_kernel<MT, FT, ONE>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
+    else if (P.cost == CGS_COST_SQD_ALT)
+        hipLaunchKernelGGL((alt_step_kernel<MT, FT, true, ONE>), dim3((P.grid.n + 3) / 4), dim3(kThreads), 0, s, P);
     else
-        hipLaunchKernelGGL((alt_step_kernel<MT, FT>), dim3((P.grid.n + 3) / 4), dim3(kThreads), 0, s, P);
+        hipLaunchKernelGGL((alt_step_kernel<MT, FT, false, ONE>), dim3((P.grid.n + 3) / 4), dim3(kThreads), 0, s, P);
     return hipGetLastError();
 }
 
+template <typename MT, typename FT>
+static hipError_t alt_launch_c(const AltParams &P, hipStream_t s) {
+    return P.C == 1 ? alt_launch<MT, FT, true>(P, s) : alt_launch<MT, FT, false>(P, s);
+}
+
 static hipError_t alt_dispatch(const AltParams &P, int mpix, int fpix, hipStream_t s) {
-    if (mpix == CGS_PIX_U8 && fpix == CGS_PIX_U8) return alt_launch<uint8_t, uint8_t>(P, s);
-    if (mpix == CGS_PIX_F32 && fpix == CGS_PIX_F32) return alt_launch<float, float>(P, s);
-    if (mpix == CGS_PIX_U8) return alt_launch<uint8_t, float>(P, s);
-    return alt_launch<float, uint8_t>(P, s);
+    if (mpix == CGS_PIX_U8 && fpix == CGS_PIX_U8) return alt_launch_c<uint8_t, uint8_t>(P, s);
+    if (mpix == CGS_PIX_F32 && fpix == CGS_PIX_F32) return alt_launch_c<float, float>(P, s);
+    if (mpix == CGS_PIX_U8) return alt_launch_c<uint8_t, float>(P, s);
+    return alt_launch_c<float, uint8_t>(P, s);
 }
 
 // calcMIAlt's host tables, in the reference's own loops: the row values
@@ -564,19 +655,31 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
     const int levels0 = (int) grid->count[0];
     const bool mi = is_mi_cost(o->cost);
     const size_t B2 = (size_t) o->mi_bins * o->mi_bins;
+    const bool one = moved->channels == 1;
+    const size_t tm = one ? ((size_t) moved->width * moved->height + 2 * (size_t) moved->width + 2) *
+                                (moved->pixtype == CGS_PIX_U8 ? 4 : 16) : 0;
+    const size_t tf = one ? ((size_t) fixed->width * fixed->height + 2 * (size_t) fixed->width + 2) *
+                                (fixed->pixtype == CGS_PIX_U8 ? 4 : 16) : 0;
     cgs_status st;
     if ((st = ws->key.ensure(64)) != CGS_OK || (st = ws->key_host.ensure(64, true)) != CGS_OK ||
-        (st = ws->mov.ensure(nm * esz_m)) != CGS_OK || (st = ws->fixp.ensure(nf * esz_f)) != CGS_OK ||
         (st = ws->sincos.ensure(8 * (size_t) levels0)) != CGS_OK ||
         (st = ws->sincos_host.ensure(8 * (size_t) levels0, true)) != CGS_OK)
+        return st;
+    if (one ? ((st = ws->quads.ensure(tm)) != CGS_OK || (st = ws->fixq.ensure(tf)) != CGS_OK)
+            : ((st = ws->mov.ensure(nm * esz_m)) != CGS_OK || (st = ws->fixp.ensure(nf * esz_f)) != CGS_OK))
         return st;
     if (hist_dump_host && mi && (st = ws->hist.ensure(B2 * sizeof(uint32_t))) != CGS_OK) return st;
     if (mi && sizeof(uint32_t) * B2 + sizeof(double) * ((size_t) (fixed->height + 63) / 64) > 64 * 1024)
         return set_error(CGS_ERR_TOO_LARGE, "line-traversal MI: the histogram and row-block sums exceed 64 KiB of LDS");
     unsigned long long *key = (unsigned long long *) ws->key.ptr;
     hipLaunchKernelGGL(alt_key_kernel, dim3(1), dim3(64), 0, stream, key);
-    HIP_TRY(pad_image(moved, ws->mov.ptr, stream));
-    HIP_TRY(pad_image(fixed, ws->fixp.ptr, stream));
+    if (one) {
+        HIP_TRY(tap_table(moved, ws->quads.ptr, stream));
+        HIP_TRY(tap_table(fixed, ws->fixq.ptr, stream));
+    } else {
+        HIP_TRY(pad_image(moved, ws->mov.ptr, stream));
+        HIP_TRY(pad_image(fixed, ws->fixp.ptr, stream));
+    }
 
     AltParams P{};
     if (o->cost == CGS_COST_MI_ALT) {
@@ -615,8 +718,10 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
     fill_grid_dev(grid, &P.grid);
     P.sincos = (const float2 *) ws->sincos.ptr;
     P.cost = o->cost;
-    P.mov = ws->mov.ptr;
-    P.fix = ws->fixp.ptr;
+    P.mov = one ? nullptr : ws->mov.ptr;
+    P.fix = one ? nullptr : ws->fixp.ptr;
+    P.mq = one ? ws->quads.ptr : nullptr;
+    P.fq = one ? ws->fixq.ptr : nullptr;
     P.mw = moved->width;
     P.mh = moved->height;
     P.fw = fixed->width;

@@ -38,7 +38,7 @@ struct Workspace {
     DevBuf mov, quads, sincos, sincos_host, hist, key, key_host;
     DevBuf prep, prep_host;     // proposal-network input descriptors
     DevBuf cov, cov_host;       // covariance partial sums
-    DevBuf fixp;                // zero-padded fixed image (line-traversal costs)
+    DevBuf fixp, fixq;          // line-traversal costs: zero-padded fixed image, its tap table
     DevBuf alt_tab, alt_tab_host;   // calcMIAlt's row values and fixed-area table
     int alt_tab_fw = 0, alt_tab_fh = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;

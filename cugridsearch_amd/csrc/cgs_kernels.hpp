@@ -600,6 +600,30 @@ __device__ __forceinline__ float div_by(float n, Recip R) {
     return fmaf(rem, R.r, q);
 }
 
+// Exact-bilinear tap tables of a one-channel image (the MI and line-traversal
+// kernels): entry j (tap L = j - (W+1))
+// holds p[L], p[L+1], p[L+W], p[L+W+1] (0 outside the image) -- four bytes
+// packed in a word for uint8 images, a float4 for float images.
+static __global__ void tap_u8_kernel(const uint8_t *src, uint32_t *dst, int W, int H) {
+    const long n = (long) W * H;
+    const long total = n + 2L * W + 2;
+    for (long j = (long) blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long) gridDim.x * blockDim.x) {
+        const long L = j - (W + 1);
+        auto P = [&](long i) { return (i >= 0 && i < n) ? (uint32_t) src[i] : 0u; };
+        dst[j] = P(L) | (P(L + 1) << 8) | (P(L + W) << 16) | (P(L + W + 1) << 24);
+    }
+}
+
+static __global__ void tap_f32_kernel(const float *src, float4 *dst, int W, int H) {
+    const long n = (long) W * H;
+    const long total = n + 2L * W + 2;
+    for (long j = (long) blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long) gridDim.x * blockDim.x) {
+        const long L = j - (W + 1);
+        auto P = [&](long i) { return (i >= 0 && i < n) ? src[i] : 0.0f; };
+        dst[j] = make_float4(P(L), P(L + 1), P(L + W), P(L + W + 1));
+    }
+}
+
 template <typename MT> struct TapTable;
 template <> struct TapTable<uint8_t> {
     static constexpr int kBytes = 4;

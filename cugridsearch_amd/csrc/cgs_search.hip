@@ -59,29 +59,6 @@ __global__ void quad_kernel(const T *src, float4 *dst, int W, int H) {
     }
 }
 
-// Exact-bilinear tap tables for the MI kernel: entry j (tap L = j - (W+1))
-// holds p[L], p[L+1], p[L+W], p[L+W+1] (0 outside the image) -- four bytes
-// packed in a word for uint8 images, a float4 for float images.
-__global__ void tap_u8_kernel(const uint8_t *src, uint32_t *dst, int W, int H) {
-    const long n = (long) W * H;
-    const long total = n + 2L * W + 2;
-    for (long j = (long) blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long) gridDim.x * blockDim.x) {
-        const long L = j - (W + 1);
-        auto P = [&](long i) { return (i >= 0 && i < n) ? (uint32_t) src[i] : 0u; };
-        dst[j] = P(L) | (P(L + 1) << 8) | (P(L + W) << 16) | (P(L + W + 1) << 24);
-    }
-}
-
-__global__ void tap_f32_kernel(const float *src, float4 *dst, int W, int H) {
-    const long n = (long) W * H;
-    const long total = n + 2L * W + 2;
-    for (long j = (long) blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long) gridDim.x * blockDim.x) {
-        const long L = j - (W + 1);
-        auto P = [&](long i) { return (i >= 0 && i < n) ? src[i] : 0.0f; };
-        dst[j] = make_float4(P(L), P(L + 1), P(L + W), P(L + W + 1));
-    }
-}
-
 // find_extrema over an arbitrary device array: grid-stride keys, workgroup
 // minimum, one atomicMin per workgroup.
 __global__ __launch_bounds__(256) void extrema_kernel(const float *v, int64_t n, int tie_mode,
