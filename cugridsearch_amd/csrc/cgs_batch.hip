@@ -299,6 +299,7 @@ BatchDesc describe(const cgs_batch *b, int i, const cgs_grid &g, float *result) 
     D.P.key = nullptr;   // set by set_descs once the key buffer is sized
     D.P.tie_mode = p.opts.tie_mode;
     D.P.bins = p.opts.mi_bins;
+    D.P.ceil_taps = p.moved.pixtype == CGS_PIX_F32;
     D.kind = batch_kind(p.opts.cost);
     return D;
 }

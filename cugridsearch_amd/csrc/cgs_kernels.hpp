@@ -40,6 +40,8 @@ struct SearchParams {
     int tie_mode;
     int bins;
     uint32_t *hist_dump;    // histogram of grid point 0 of block 0 (diagnostics)
+    int ceil_taps;          // MI kernel on a float tap table: read valueAt's ceil taps (a float
+                            // source image, which may hold non-finite pixels); 0 for uint8 sources
 };
 
 constexpr int kThreads = 256;
@@ -669,6 +671,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void *) uniform(P.quads), (short) 0, (int) (((size_t) mw * mh + 2 * (size_t) mw + 2) * EB), 0x00020000);
     const int bins = uniform(P.bins), B2 = bins * bins;
+    const bool ceil_taps = uniform(P.ceil_taps) != 0;
     const float fbins = (float) bins, bscale = fbins / 256.0f;
     const uint32_t dummy = (uint32_t) (GPT * B2 + lane);
 
@@ -735,8 +738,18 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
             }
 #pragma unroll
             for (int g = 0; g < GPT; ++g) {
-                const float top = fmaf(ax[g], q[g].y, (1.0f - ax[g]) * q[g].x);
-                const float bottom = fmaf(ax[g], q[g].w, (1.0f - ax[g]) * q[g].z);
+                float tr = q[g].y, bl = q[g].z, br = q[g].w;
+                if (std::is_same<MT, float>::value && ceil_taps) {
+                    // valueAt's ceil taps: at an integral coordinate the floor tap
+                    // itself, so a non-finite neighbour stays out (taps from a
+                    // uint8 image are finite and non-negative: there the weight-0
+                    // tap adds exactly +0 and the selects change nothing)
+                    tr = ax[g] != 0.0f ? q[g].y : q[g].x;
+                    bl = ay[g] != 0.0f ? q[g].z : q[g].x;
+                    br = ay[g] != 0.0f ? (ax[g] != 0.0f ? q[g].w : q[g].z) : tr;
+                }
+                const float top = fmaf(ax[g], tr, (1.0f - ax[g]) * q[g].x);
+                const float bottom = fmaf(ax[g], br, (1.0f - ax[g]) * bl);
                 const float v = fmaf(ay[g], bottom, (1.0f - ay[g]) * top);
                 const uint32_t h = (uint32_t) (g * B2) + rowbin + (uint32_t) mi_bin_fast<POW2>(v, bins, fbins, bscale);
                 atomicAdd(&hist[mask_select(in[g], dummy, h)], 1u);

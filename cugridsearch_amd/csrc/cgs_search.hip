@@ -284,6 +284,7 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     P.tie_mode = o->tie_mode;
     P.bins = o->mi_bins;
     P.hist_dump = hist_dump_host ? (uint32_t *) ws->hist.ptr : nullptr;
+    P.ceil_taps = moved->pixtype == CGS_PIX_F32;
     const bool timed = (o->flags & CGS_FLAG_TIME_KERNEL) != 0;
     if (timed) HIP_TRY(ws->record_start(stream));
     if (kind == kKernelFast) HIP_TRY(dispatch_fast(P, fixed->pixtype, o->tile_gpt, stream));
