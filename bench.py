@@ -36,6 +36,7 @@ FLOP_PER_PE_ALT = {"calcSQDAlt": 4 + 26 + 6, "calcNCCAlt": 4 + 26 + 6, "calcNCCs
 BATCH_WORKLOAD = "batch_ragged_256"
 PROPOSAL_WORKLOAD = "proposal_1024"
 CNN_WORKLOAD = "cnn_train_step"
+MATCHER_WORKLOADS = {"matcher_256": 256, "matcher_512": 512}
 PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
 PROPOSAL_WEIGHTS = os.path.join(ROOT, "cugridsearch_amd", "weights", "proposal_c5.npz")
 
@@ -275,6 +276,51 @@ def run_cnn(args):
     }
 
 
+def run_matcher(args):
+    """The multispectral matcher end to end on one synthetic uint8 pair (the
+    reference CLI's grid, filter, coarse calcMIAlt search, covariance search
+    and covariance): pairs/s, with the searches' share."""
+    import io
+    import torch
+    from cugridsearch_amd import matcher as Mt
+    from cugridsearch_amd import synth
+    side = MATCHER_WORKLOADS[args.workload]
+    m, f, _ = synth.make_pair(0x5EED, side, sigma=3.0, perturb=side / 32.0, dtype=np.uint8, gamma=True, noise=4.0)
+    for _ in range(args.warmup):
+        Mt.run(f, m, out=io.StringIO())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = Mt.run(f, m, out=io.StringIO())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    num, start, end = Mt.coarse_grid(side, side, side, side)
+    from cugridsearch_amd import api as A
+    g = A.CudaGrid(8)
+    g.setStartPoint(start)
+    g.setEndPoint(end)
+    g.setNumSamples(num)
+    pe = g.numElements() * side * side
+    line = {
+        "metric": METRIC, "value": 1.0 / dt, "unit": "image pairs/s (matcher end to end)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1000.0,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 images, f32/f64 arithmetic", "data": "synthetic",
+        "config": {"workload": args.workload, "image": f"{side}x{side} uint8", "coarse_grid_points": g.numElements(),
+                   "functor": "calcMIAlt, 64 bins", "depth": Mt.DEPTH},
+        "coarse_search_gpixel_evals_per_s": pe / r["seconds"] / 1e9,
+        "coarse_search_s": r["seconds"],
+        "argmin": {"index": int(r["argmin"][1]), "value": float(r["argmin"][0])},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        fil = O.image_filter(m, Mt.UNSHARP_3X3)
+        line["cpu_baseline"] = cpu_baseline(fil, f, A.calcMIAlt, np.array(start, np.float32), np.array(end, np.float32),
+                                            np.array(num, np.float32), side * side, args.cpu_seconds)
+        line["cpu_baseline"]["sample"] = "coarse calcMIAlt search only: " + line["cpu_baseline"]["sample"]
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -287,6 +333,9 @@ def main():
         return
     if args.workload == CNN_WORKLOAD:
         print(json.dumps(run_cnn(args)), flush=True)
+        return
+    if args.workload in MATCHER_WORKLOADS:
+        print(json.dumps(run_matcher(args)), flush=True)
         return
     if args.workload == PROPOSAL_WORKLOAD:
         print(json.dumps(run_proposal(args)), flush=True)

@@ -92,6 +92,9 @@ SIGNATURES = {
     "cgs_find_extrema": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _P(Argmin),
                                         ctypes.c_void_p]),
     "cgs_covariance": (ctypes.c_int, [_P(Grid), ctypes.c_void_p, _P(ctypes.c_float), _P(Argmin), ctypes.c_void_p]),
+    "cgs_image_filter": (ctypes.c_int, [_P(Image), _P(Image), _P(ctypes.c_float), ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_void_p]),
+    "cgs_image_quantize": (ctypes.c_int, [_P(Image), ctypes.c_float, ctypes.c_void_p]),
     "cgs_eval_point": (ctypes.c_int, [_P(Image), _P(Image), _P(SearchOpts), ctypes.c_int32, _P(ctypes.c_float),
                                       _P(ctypes.c_float), _P(ctypes.c_uint32), ctypes.c_void_p]),
     "cgs_batch_create": (ctypes.c_int, [_P(ctypes.c_void_p)]),

@@ -157,15 +157,16 @@ class CudaGrid:
     def offset(self):
         return np.array(self._g.offset[:self._dim], np.int32)
 
-    def display(self, name=""):
+    def display(self, name="", echo=True):
         """CudaMatrix::display format (include/cudaTensor.cuh:395-415): the four
-        rows start / end / resolution / count."""
+        rows start / end / resolution / count.  Prints it (echo) and returns it."""
         rows = [self.start_point(), self.end_point(), self.resolution(), self.getAxisSampleCounts()]
         lines = [f"Matrix {name} 4 x {self._dim} elements of f", ""]
         for r in rows:
             lines.append("{ " + ", ".join(f"{v:g}" for v in r) + " }")
         text = "\n".join(lines) + "\n"
-        print(text)
+        if echo:
+            print(text)
         return text
 
 
@@ -223,6 +224,27 @@ class CudaImage:
 
     def channels(self):
         return self._img.channels
+
+
+def filter_image(img, filt, out=None, stream=None):
+    """CudaImage::filter with every channel filtered (include/cudaTensor.cuh:577-593,
+    filterProcess include/cudaTensorKernels.cuh:353-392).  filt: (fh, fw)
+    array of at most 256 taps.  Returns out (a new CudaImage when None)."""
+    f = np.ascontiguousarray(filt, np.float32)
+    if f.ndim != 2:
+        raise ValueError("filter must be a (rows, cols) array")
+    if out is None:
+        out = CudaImage(_torch().empty_like(img.tensor))
+    L.check(L.lib().cgs_image_filter(ctypes.byref(img._img), ctypes.byref(out._img), L.floats(f.reshape(-1)),
+                                     f.shape[1], f.shape[0], _stream_ptr(stream)))
+    return out
+
+
+def quantize_image(img, mod, stream=None):
+    """transformPixel (src/cpu/multispectral_ImageMatcher.cu:411-430), in place
+    on a uint8 CudaImage."""
+    L.check(L.lib().cgs_image_quantize(ctypes.byref(img._img), float(mod), _stream_ptr(stream)))
+    return img
 
 
 def make_opts(functor, tie_mode=L.TIE_LOWEST):

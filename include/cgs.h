@@ -135,6 +135,25 @@ typedef struct {
 /* Library version string. */
 const char *cgs_version(void);
 
+/* ---------------------------------------------------------- preprocessing
+ * The multispectral matcher's image preparation
+ * (src/cpu/multispectral_ImageMatcher.cu:568-632).                        */
+
+/* CudaImage::filter with every channel filtered (include/cudaTensor.cuh:577-593,
+ * kernel filterProcess include/cudaTensorKernels.cuh:353-392): out(x, y) =
+ * sum of filter[r * fw + c] * in(x - (fw-1)/2 + c, y - (fh-1)/2 + r), a tap
+ * outside the image reading the centre pixel; uint8 output truncated and
+ * saturated to [0, 255].  filter: host array of fw * fh <= 256 floats, row
+ * major.  in and out: same shape and pixel type, device memory, no aliasing.
+ * Asynchronous on stream. */
+cgs_status cgs_image_filter(const cgs_image *in, const cgs_image *out, const float *filter, int32_t fw, int32_t fh,
+                            void *stream);
+
+/* transformPixel (src/cpu/multispectral_ImageMatcher.cu:411-430) in place on
+ * a uint8 image: each byte read as a signed char, (int)(v / mod), then
+ * (int)(v * mod), stored modulo 256.  mod >= 2^-20.  Asynchronous. */
+cgs_status cgs_image_quantize(const cgs_image *img, float mod, void *stream);
+
 /* CGS_ABI_VERSION of the library's build. */
 int32_t cgs_abi_version(void);
 

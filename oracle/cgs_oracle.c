@@ -589,9 +589,11 @@ static int alt_overlap(float mov_area, float fix_area, int mw, int mh, int fw, i
  * a row goes to lane k mod 64; each lane adds, in row and step order,
  * fma(m, f) (or fma(m - f, m - f) for SQD), fma(m, m), fma(f, f) in binary64
  * (the 1/255 scaling of the reference cancels in every one of these costs);
- * the lanes combine in lane_tree order.  Moved area: binary64 sum over rows
- * of steps * |v| (binary32 |v|), then rounded to binary32; fixed area: the
- * step count.  MI: joint counts of every sample. */
+ * the lanes combine in lane_tree order.  Moved area: the binary64 terms
+ * steps * |v| (binary32 |v|) of each block of 64 rows are summed in
+ * lane_tree order (row r of the block as lane r), the block sums added in
+ * block order, the total rounded to binary32; fixed area: the step count.
+ * MI: joint counts of every sample. */
 static float cost_alt_step(const orc_image *moved, const orc_image *fixed, int cost, const float *params,
                            int bins, uint32_t *hist, double *aux) {
     const int mw = moved->width, mh = moved->height, fw = fixed->width, fh = fixed->height, C = fixed->channels;
@@ -599,35 +601,38 @@ static float cost_alt_step(const orc_image *moved, const orc_image *fixed, int c
     homography_centred(params, (float) mw / 2, (float) mh / 2, h);
     double s0[64], s1[64], s2[64];
     for (int i = 0; i < 64; i++) s0[i] = s1[i] = s2[i] = 0.0;
-    double mov = 0.0;
+    double mov = 0.0, mov_block[64];
     int64_t steps = 0, count = 0;
     const int is_mi = cost == ORC_COST_MI_STREAM_ALT;
     for (int row = 0; row < fh; row++) {
+        if (row % 64 == 0) memset(mov_block, 0, sizeof(mov_block));
         orc_line L;
         line_centred(h, mw, mh, fw, fh, row, &L);
-        if (!L.in) continue;
-        const float mag = sqrtf(L.vx * L.vx + L.vy * L.vy);
-        int64_t k = 0;
-        for (float lam = L.ls; lam < L.le && lam < (float) fw; lam += 1.0f, k++) {
-            const float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
-            const int lane = (int) (k & 63);
-            for (int c = 0; c < C; c++) {
-                const float m = orc_sample_bilinear(moved, ny, nx, c);
-                const float f = orc_sample_bilinear(fixed, (float) row, lam, c);
-                if (is_mi) {
-                    hist[mi_bin(f, bins) * bins + mi_bin(m, bins)]++;
-                } else {
-                    const double md = (double) m, fd = (double) f;
-                    if (cost == ORC_COST_SQD_ALT) s0[lane] = fma(md - fd, md - fd, s0[lane]);
-                    else s0[lane] = fma(md, fd, s0[lane]);
-                    s1[lane] = fma(fd, fd, s1[lane]);
-                    s2[lane] = fma(md, md, s2[lane]);
+        if (L.in) {
+            const float mag = sqrtf(L.vx * L.vx + L.vy * L.vy);
+            int64_t k = 0;
+            for (float lam = L.ls; lam < L.le && lam < (float) fw; lam += 1.0f, k++) {
+                const float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
+                const int lane = (int) (k & 63);
+                for (int c = 0; c < C; c++) {
+                    const float m = orc_sample_bilinear(moved, ny, nx, c);
+                    const float f = orc_sample_bilinear(fixed, (float) row, lam, c);
+                    if (is_mi) {
+                        hist[mi_bin(f, bins) * bins + mi_bin(m, bins)]++;
+                    } else {
+                        const double md = (double) m, fd = (double) f;
+                        if (cost == ORC_COST_SQD_ALT) s0[lane] = fma(md - fd, md - fd, s0[lane]);
+                        else s0[lane] = fma(md, fd, s0[lane]);
+                        s1[lane] = fma(fd, fd, s1[lane]);
+                        s2[lane] = fma(md, md, s2[lane]);
+                    }
                 }
+                count += C;
             }
-            count += C;
+            if (k > 0) mov_block[row % 64] = (double) k * (double) mag;
+            steps += k;
         }
-        if (k > 0) mov += (double) k * (double) mag;
-        steps += k;
+        if (row % 64 == 63 || row == fh - 1) mov += lane_tree(mov_block);
     }
     const float mov_area = (float) mov, fix_area = (float) steps;
     const int pass = alt_overlap(mov_area, fix_area, mw, mh, fw, fh, params[2]);
@@ -726,6 +731,53 @@ static float cost_alt(const orc_image *moved, const orc_image *fixed, int cost, 
                                             : cost_alt_step(moved, fixed, cost, params, bins, hist, aux_out);
     if (is_mi && !hist_out) free(hist);
     return v;
+}
+
+/* ---------------------------------------------------------- preprocessing */
+
+/* filterProcess (include/cudaTensorKernels.cuh:353-392) with every channel
+ * filtered, as CudaImage::filter launches it: output (x, y) = the running sum
+ * over filter rows r, columns c of filter[r][c] * in(y - (fh-1)/2 + r,
+ * x - (fw-1)/2 + c), an outside tap reading the centre pixel; the sum is not
+ * reset between channels; "pixVal += f * p" as fmaf(f, p, pixVal); uint8
+ * output truncated toward zero and saturated to [0, 255]. */
+void orc_image_filter(const orc_image *in, void *out, const float *filter, int fw, int fh) {
+    const int W = in->width, H = in->height, C = in->channels;
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            float acc = 0.0f;
+            for (int c = 0; c < C; c++) {
+                for (int r = 0; r < fh; r++)
+                    for (int k = 0; k < fw; k++) {
+                        const int yy = y - (fh - 1) / 2 + r, xx = x - (fw - 1) / 2 + k;
+                        const int inside = xx >= 0 && yy >= 0 && xx < W && yy < H;
+                        const int64_t at = inside ? ((int64_t) yy * W + xx) * C + c : ((int64_t) y * W + x) * C + c;
+                        acc = fmaf(filter[r * fw + k], pix(in, at), acc);
+                    }
+                const int64_t o = ((int64_t) y * W + x) * C + c;
+                if (in->pixtype == ORC_PIX_U8) {
+                    float t = truncf(acc);
+                    if (!(t >= 0.0f)) t = 0.0f;   /* negatives and NaN */
+                    if (t > 255.0f) t = 255.0f;
+                    ((uint8_t *) out)[o] = (uint8_t) t;
+                } else {
+                    ((float *) out)[o] = acc;
+                }
+            }
+        }
+}
+
+/* transformPixel (src/cpu/multispectral_ImageMatcher.cu:411-430), in place:
+ * char pixel = byte (signed); int temp = pixel; temp /= mod; temp *= mod;
+ * byte = (uint8_t) temp.  The int /= float and int *= float go through
+ * binary32 and truncate toward zero. */
+void orc_image_quantize(uint8_t *img, int64_t n, float mod) {
+    for (int64_t i = 0; i < n; i++) {
+        int t = (int) (int8_t) img[i];
+        t = (int) ((float) t / mod);
+        t = (int) ((float) t * mod);
+        img[i] = (uint8_t) t;
+    }
 }
 
 /* Evaluates grid indices [i0, i1) into result[0 .. i1-i0).  Threads: OpenMP

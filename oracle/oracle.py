@@ -76,6 +76,8 @@ def lib():
         L.orc_mi_alt_rows.restype = i64
         L.orc_mi_alt_area.argtypes = [ctypes.c_int, ctypes.c_int, i64]
         L.orc_mi_alt_area.restype = ctypes.c_float
+        L.orc_image_filter.argtypes = [ctypes.POINTER(_Image), ctypes.c_void_p, fp, ctypes.c_int, ctypes.c_int]
+        L.orc_image_quantize.argtypes = [ctypes.POINTER(ctypes.c_uint8), i64, ctypes.c_float]
         _lib = L
     return _lib
 
@@ -175,6 +177,22 @@ def mi_alt_rows(fh):
 
 def mi_alt_area(fw, fh, n):
     return np.float32(lib().orc_mi_alt_area(fw, fh, int(n)))
+
+
+def image_filter(img, filt):
+    """filterProcess with every channel filtered; filt is a (fh, fw) array."""
+    im = image(img)
+    f = np.ascontiguousarray(filt, np.float32)
+    out = np.zeros_like(np.ascontiguousarray(img))
+    lib().orc_image_filter(ctypes.byref(im), out.ctypes.data, _f32p(f), f.shape[1], f.shape[0])
+    return out
+
+
+def image_quantize(img, mod):
+    """transformPixel on a uint8 image (a copy is returned)."""
+    out = np.array(img, np.uint8, copy=True, order="C")
+    lib().orc_image_quantize(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), out.size, float(mod))
+    return out
 
 
 def sample(img, y, x, sampler, c=0):
