@@ -161,12 +161,17 @@ hipError_t launch_kind(cgs_batch *b, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int KIND>
+hipError_t launch_fast(cgs_batch *b, hipStream_t s) {
+    return b->tiles.fast_gpt == 6 ? launch_kind<KIND, 6>(b, s) : launch_kind<KIND, 4>(b, s);
+}
+
 hipError_t launch_batch_any(cgs_batch *b, hipStream_t s) {
     const bool m4 = b->tiles.mi_gpt == 4;
     hipError_t e = hipSuccess;
-    if (b->kind_count[kBatchSAD] && e == hipSuccess) e = launch_kind<kBatchSAD, 4>(b, s);
-    if (b->kind_count[kBatchNCC] && e == hipSuccess) e = launch_kind<kBatchNCC, 4>(b, s);
-    if (b->kind_count[kBatchSQD] && e == hipSuccess) e = launch_kind<kBatchSQD, 4>(b, s);
+    if (b->kind_count[kBatchSAD] && e == hipSuccess) e = launch_fast<kBatchSAD>(b, s);
+    if (b->kind_count[kBatchNCC] && e == hipSuccess) e = launch_fast<kBatchNCC>(b, s);
+    if (b->kind_count[kBatchSQD] && e == hipSuccess) e = launch_fast<kBatchSQD>(b, s);
     if (b->kind_count[kBatchMI] && e == hipSuccess)
         e = m4 ? launch_kind<kBatchMI, 4>(b, s) : launch_kind<kBatchMI, 2>(b, s);
     return e;
@@ -180,7 +185,7 @@ cgs_status parse_tiles(const cgs_tile_config *tiles, cgs_tile_config *out) {
         if (tiles->wg_per_cu) t.wg_per_cu = tiles->wg_per_cu;
         if (tiles->items_per_claim) t.items_per_claim = tiles->items_per_claim;
     }
-    if (t.fast_gpt != 4 || (t.mi_gpt != 2 && t.mi_gpt != 4) || t.wg_per_cu < 1 ||
+    if ((t.fast_gpt != 4 && t.fast_gpt != 6) || (t.mi_gpt != 2 && t.mi_gpt != 4) || t.wg_per_cu < 1 ||
         t.wg_per_cu > 8 || t.items_per_claim < 1 || t.items_per_claim > 64)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "batch: bad tile configuration");
     *out = t;
@@ -282,6 +287,8 @@ BatchDesc describe(const cgs_batch *b, int i, const cgs_grid &g, float *result) 
     BatchDesc D;
     std::memset(&D, 0, sizeof(D));
     fill_grid_dev(&g, &D.P.grid);
+    int order[kMaxDim];
+    if (work_order(g.dim, p.opts.param_mode, order)) set_work_order(&D.P.grid, order);
     const bool has_theta = p.opts.param_mode == CGS_PARAM_CENTRED || p.opts.param_mode == CGS_PARAM_NONCENTRED;
     D.P.sincos = has_theta ? (const float2 *) b->sincos[i].ptr + (g.offset[0] - p.grid.offset[0]) : nullptr;
     D.P.param_mode = p.opts.param_mode;

@@ -20,6 +20,9 @@ struct GridDev {
     float res[kMaxDim];
     int32_t incr[kMaxDim];
     int32_t off[kMaxDim];     // first sample index per axis (grid sub-boxes)
+    int32_t count[kMaxDim];   // samples per axis
+    int32_t wincr[kMaxDim];   // stride of each axis in work-index order
+    int32_t permuted;         // 0: the work index is the grid index
     int32_t dim;
     int32_t n;
 };
@@ -43,6 +46,30 @@ __device__ __forceinline__ void grid_point(const GridDev &g, int32_t index, floa
     for (int a = 0; a < kMaxDim; ++a)
         p[a] = (a < g.dim) ? fmaf((float) (idx[a] + g.off[a]), g.res[a], g.start[a]) : 0.0f;
     *i0 = idx[0];
+}
+
+// A workgroup's grid points are consecutive work indices.  The work index
+// enumerates the grid with its own axis order (wincr), so that points sharing
+// a workgroup can be chosen for locality of their taps; the grid index, which
+// orders results and ties, is recovered from the axis indices.  Returns the
+// grid index and fills the point as grid_point does.
+__device__ __forceinline__ int32_t work_point(const GridDev &g, int32_t w, float *p, int32_t *i0) {
+    if (!g.permuted) {
+        grid_point(g, w, p, i0);
+        return w;
+    }
+    int32_t idx[kMaxDim];
+    int32_t gi = 0;
+#pragma unroll
+    for (int a = 0; a < kMaxDim; ++a) {
+        idx[a] = (a < g.dim) ? (w / g.wincr[a]) % g.count[a] : 0;
+        gi += (a < g.dim) ? idx[a] * g.incr[a] : 0;
+    }
+#pragma unroll
+    for (int a = 0; a < kMaxDim; ++a)
+        p[a] = (a < g.dim) ? fmaf((float) (idx[a] + g.off[a]), g.res[a], g.start[a]) : 0.0f;
+    *i0 = idx[0];
+    return gi;
 }
 
 enum ParamMode { kCentred = 0, kNonCentred = 1, kRawH = 2, kTranslation = 3 };

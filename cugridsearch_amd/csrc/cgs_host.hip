@@ -3,6 +3,7 @@
 #include "cgs_host.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -114,8 +115,8 @@ cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const c
     } else if (o->param_mode == CGS_PARAM_NORM) {
         return set_error(CGS_ERR_INVALID_ARGUMENT, "CGS_PARAM_NORM belongs to calcMIAlt only");
     }
-    if (o->tile_gpt != 0 && o->tile_gpt != 2 && o->tile_gpt != 4 && o->tile_gpt != 8)
-        return set_error(CGS_ERR_INVALID_ARGUMENT, "tile_gpt must be 0, 2, 4 or 8");
+    if (o->tile_gpt != 0 && o->tile_gpt != 2 && o->tile_gpt != 4 && o->tile_gpt != 6 && o->tile_gpt != 8)
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "tile_gpt must be 0, 2, 4, 6 or 8");
     const int need = o->param_mode == CGS_PARAM_TRANSLATION ? 2 : 8;
     if (grid->dim != need)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "grid dimension does not match the parameter mode");
@@ -135,7 +136,63 @@ void fill_grid_dev(const cgs_grid *grid, GridDev *g) {
         g->res[a] = a < grid->dim ? grid->resolution[a] : 0.0f;
         g->incr[a] = a < grid->dim ? grid->increments[a] : 1;
         g->off[a] = a < grid->dim ? grid->offset[a] : 0;
+        g->wincr[a] = g->incr[a];
     }
+    // samples per axis as the mixed-radix decode sees them (the last axis
+    // takes what is left of n); identity work order
+    for (int a = 0; a < kMaxDim; ++a) {
+        if (a + 1 < g->dim) g->count[a] = g->incr[a + 1] / g->incr[a];
+        else if (a + 1 == g->dim) g->count[a] = (g->n + g->incr[a] - 1) / g->incr[a];
+        else g->count[a] = 1;
+    }
+}
+
+bool work_order(int dim, int param_mode, int *order) {
+    const char *e = std::getenv("CGS_WORK_ORDER");
+    if (e) {   // "grid" (or anything that is not dim digits) keeps the grid order
+        int k = 0;
+        for (const char *c = e; *c && k < dim; ++c)
+            if (*c >= '0' && *c <= '9') order[k++] = *c - '0';
+        return k == dim;
+    }
+    // the centred and non-centred parametrisations: x-translation (axis 4)
+    // fastest, then rotation, then y-translation; workgroups then hold points
+    // whose warps differ by horizontal shifts and share tap cache lines
+    if (dim != kMaxDim || (param_mode != CGS_PARAM_CENTRED && param_mode != CGS_PARAM_NONCENTRED)) return false;
+    static const int kOrder[kMaxDim] = {4, 0, 5, 1, 2, 3, 6, 7};
+    for (int k = 0; k < dim; ++k) order[k] = kOrder[k];
+    return true;
+}
+
+int default_fast_gpt(const GridDev &g) {
+    const char *e = std::getenv("CGS_FAST_GPT");
+    if (e) {
+        const int v = std::atoi(e);
+        if (v == 4 || v == 6 || v == 8) return v;
+    }
+    if (!g.permuted) return 8;
+    int fastest = 0;   // the axis with work stride 1
+    for (int a = 0; a < g.dim; ++a)
+        if (g.wincr[a] == 1 && g.count[a] > 1) fastest = a;
+    return g.count[fastest] % 3 == 0 ? 6 : 8;
+}
+
+bool set_work_order(GridDev *g, const int *order) {
+    int64_t prod = 1;
+    for (int a = 0; a < g->dim; ++a) prod *= g->count[a];
+    if (prod != g->n) return false;   // not a full mixed-radix box: keep the identity order
+    int seen = 0;
+    for (int k = 0; k < g->dim; ++k) {
+        if (order[k] < 0 || order[k] >= g->dim || (seen >> order[k]) & 1) return false;
+        seen |= 1 << order[k];
+    }
+    int32_t stride = 1;
+    for (int k = 0; k < g->dim; ++k) {
+        g->wincr[order[k]] = stride;
+        stride *= g->count[order[k]];
+    }
+    g->permuted = 1;
+    return true;
 }
 
 static uint32_t bitrev9(uint32_t t) {

@@ -68,6 +68,17 @@ cgs_status validate_image(const cgs_image *im, const char *name);
 cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
                            const cgs_search_opts *o);
 void fill_grid_dev(const cgs_grid *grid, GridDev *g);
+// Work-index axis order: order[0] is the fastest axis (a permutation of
+// 0..dim-1).  Returns false (identity order kept) when the grid is not a full
+// mixed-radix box or order is not a permutation.
+bool set_work_order(GridDev *g, const int *order);
+// The work order the tiled kernels use for a parametrisation (false: grid
+// order); CGS_WORK_ORDER ("4,0,5,1,2,3,6,7", or "grid") overrides it.
+bool work_order(int dim, int param_mode, int *order);
+// Grid points per workgroup of the fast kernel when the caller leaves it to
+// the library: 6 when the fastest work axis has a multiple of 3 levels, else
+// 8 (CGS_FAST_GPT = 4, 6 or 8 overrides it).
+int default_fast_gpt(const GridDev &g);
 cgs_argmin decode_key(unsigned long long key, int tie_mode);
 uint32_t reference_rank_inverse(uint32_t rank);
 

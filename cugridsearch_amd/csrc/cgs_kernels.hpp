@@ -121,16 +121,18 @@ __device__ __forceinline__ T *uniform(T *p) {
     return (T *) (((uint64_t) hi << 32) | lo);
 }
 
-// Thread 0..GPT-1 of the workgroup: warp coefficients of grid index base+tid
-// (index 0 for the padding slots past the end of the grid).
+// Thread 0..GPT-1 of the workgroup: warp coefficients of work index base+tid
+// (work index 0 for the padding slots past the end of the grid).  Slot 11
+// holds the grid index as int bits, -1 for padding.
 template <int GPT>
 __device__ __forceinline__ void block_coefficients(const SearchParams &P, int32_t base, float (*s_coef)[12]) {
     const int tid = threadIdx.x;
     if (tid < GPT) {
-        const int32_t gi = base + tid < P.grid.n ? base + tid : 0;
+        const bool live = base + tid < P.grid.n;
         float p[kMaxDim];
         int32_t i0;
-        grid_point(P.grid, gi, p, &i0);
+        const int32_t gi = work_point(P.grid, live ? base + tid : 0, p, &i0);
+        s_coef[tid][11] = __int_as_float(live ? gi : -1);
         float c = 1.0f, s = 0.0f;
         if (P.sincos) {
             const float2 cs = P.sincos[i0];
@@ -156,7 +158,11 @@ __device__ __forceinline__ void block_coefficients(const SearchParams &P, int32_
             wmax = w > wmax ? w : wmax;
         }
         const double scale = fabs((double) co[6]) * P.fw + fabs((double) co[7]) * P.fh + fabs((double) co[8]);
-        const bool posw = wmin > 1e-5 * scale && scale < 1e30;
+        // bounded X and Y coefficients keep every form finite (no NaN), and
+        // W >= 1e-20 keeps fmaf(-lim, W, n) clear of subnormal results
+        bool bounded = true;
+        for (int k = 0; k < 6; ++k) bounded = bounded && fabsf(co[k]) < 1e30f;
+        const bool posw = wmin > 1e-5 * scale && wmin > 1e-20 && scale < 1e30 && bounded;
         s_coef[tid][9] = posw ? 1.0f : 0.0f;
         // W within [1/64, 64] over the image: the refined-reciprocal quotient
         // (div_by) then equals IEEE division (see div_by)
@@ -184,12 +190,12 @@ __device__ __forceinline__ float finish_cost(int cost, const double *t, bool exa
 // Writes the costs of the workgroup's GPT points and folds their minimum
 // into the global argmin key.  Called by thread 0.
 template <int GPT>
-__device__ __forceinline__ void emit(const SearchParams &P, int32_t base, const float *s_cost) {
+__device__ __forceinline__ void emit(const SearchParams &P, const float (*s_coef)[12], const float *s_cost) {
     unsigned long long best = ~0ull;
 #pragma unroll
     for (int g = 0; g < GPT; ++g) {
-        const int32_t gi = base + g;
-        if (gi < P.grid.n) {
+        const int32_t gi = __float_as_int(s_coef[g][11]);
+        if (gi >= 0) {
             if (P.result) P.result[gi] = s_cost[g];
             const unsigned long long k = argmin_key(s_cost[g], (uint32_t) gi, P.tie_mode);
             best = k < best ? k : best;
@@ -347,7 +353,7 @@ __global__ __launch_bounds__(kThreads) void search_exact_kernel(const SearchPara
             for (int j = tid; j < B2; j += kThreads) P.hist_dump[j] = s_hist[j];
     }
     __syncthreads();
-    if (tid == 0) emit<GPT>(P, base, s_cost);
+    if (tid == 0) emit<GPT>(P, s_coef, s_cost);
 }
 
 // ------------------------------------------------------------- fast kernel
@@ -365,7 +371,7 @@ __global__ __launch_bounds__(kThreads) void search_exact_kernel(const SearchPara
 // so the positive-W sweep has no per-lane branch and issues all GPT loads
 // before it consumes any.
 
-constexpr int kFlush = 16;
+constexpr int kFlush = 32;
 constexpr uint32_t kOffsetOut = 0x80000000u;   // beyond any tap table
 
 __device__ __forceinline__ int floor_to_int(float v) {
@@ -399,6 +405,40 @@ __device__ __forceinline__ float4 load_tap(__amdgpu_buffer_rsrc_t rsrc, uint32_t
     return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
 }
 
+// One fixed-image pixel through a buffer resource: offsets past the image
+// read 0.
+template <typename FT> __device__ __forceinline__ float load_fix(__amdgpu_buffer_rsrc_t rsrc, uint32_t offset);
+template <> __device__ __forceinline__ float load_fix<float>(__amdgpu_buffer_rsrc_t rsrc, uint32_t offset) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, offset, 0, 0));
+}
+template <> __device__ __forceinline__ float load_fix<uint8_t>(__amdgpu_buffer_rsrc_t rsrc, uint32_t offset) {
+    return (float) __builtin_amdgcn_raw_buffer_load_b8(rsrc, offset, 0, 0);
+}
+
+// acc += v on the lanes of wave mask m (a subset of the active lanes); the
+// other lanes keep acc.  One VALU add under a narrowed EXEC instead of a
+// select and an add.
+__device__ __forceinline__ void masked_add(float &acc, float v, unsigned long long m) {
+    unsigned long long saved;
+    asm volatile("s_and_saveexec_b64 %1, %3\n\tv_add_f32 %0, %0, %2\n\ts_mov_b64 exec, %1"
+                 : "+v"(acc), "=&s"(saved)
+                 : "v"(v), "s"(m)
+                 : "scc");
+}
+
+// Positive-W in-image test from sign bits.  With Wn > 0, Xn never -0 and no
+// NaN: Xn >= 0 and fmaf(-Wm, Wn, Xn) < 0 (the exact-quotient test) holds
+// exactly when their sign bits differ, since Xn < 0 forces the second
+// negative too (and it is never -0).  Bit 31 of the result is set for lanes
+// inside in both axes: ((Yn ^ b) & (Xn ^ a)), the v_bitop3 table 0x28 of
+// (src0 ^ src1) & src2.
+__device__ __forceinline__ bool inside_posw(float Xn, float Yn, float Wn, float Wm, float Hm) {
+    const uint32_t sx = __float_as_uint(Xn) ^ __float_as_uint(fmaf(-Wm, Wn, Xn));
+    const uint32_t w =
+        __builtin_amdgcn_bitop3_b32(__float_as_uint(Yn), __float_as_uint(fmaf(-Hm, Wn, Yn)), sx, 0x28);
+    return (int32_t) w < 0;
+}
+
 template <int COSTK, typename FT, int GPT, bool POSW>
 __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*s_coef)[12], double (*acc)[3],
                                            uint32_t *cnt) {
@@ -414,6 +454,9 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
     const int base16 = (mw + 1) * 16;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void *) uniform(P.quads), (short) 0, (int) (((size_t) mw * mh + 2 * (size_t) mw + 2) * 16), 0x00020000);
+    const __amdgpu_buffer_rsrc_t frsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *) fix, (short) 0, (int) ((size_t) fw * fh * sizeof(FT)), 0x00020000);
+    const int full = fh / rpi;   // steps in which every lane's row is inside the image
 
     float B[GPT][3];
 #pragma unroll
@@ -426,24 +469,30 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
         const bool valid = (r0 < rpi) && (x < fw);
         const float xf = (float) x;
         float col[GPT][3];
+        // + 0.0f turns a -0 into +0 (and changes nothing else), so that the
+        // forms fmaf(B, y, col) are never -0
 #pragma unroll
         for (int g = 0; g < GPT; ++g)
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                col[g][k] = fmaf(uniform(s_coef[g][3 * k]), xf, uniform(s_coef[g][3 * k + 2]));
+                col[g][k] = fmaf(uniform(s_coef[g][3 * k]), xf, uniform(s_coef[g][3 * k + 2])) + 0.0f;
         float part[GPT][3];
 #pragma unroll
         for (int g = 0; g < GPT; ++g) part[g][0] = part[g][1] = part[g][2] = 0.0f;
         int since = 0;
         // the same trip count on every lane keeps the ballot counts wave-uniform
         const int steps = (fh + rpi - 1) / rpi;
-        for (int it = 0; it < steps; ++it) {
-            const int y = r0 + it * rpi;
-            const bool vy = valid && y < fh;
-            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(vy);
-            const float f = vy ? (float) fix[y * fw + x] : 0.0f;
+        const unsigned long long valid_mask = __builtin_amdgcn_ballot_w64(valid);
+        // byte offset of the lane's pixel; lanes off the image start past its end
+        uint32_t foff = valid ? (uint32_t) ((r0 * fw + x) * (int) sizeof(FT)) : kOffsetOut;
+        const uint32_t fstep = (uint32_t) (rpi * fw * (int) sizeof(FT));
+        // only the step after the full ones (if any) has lanes below the image
+        const unsigned long long tail_mask = __builtin_amdgcn_ballot_w64(valid && r0 + full * rpi < fh);
+        float yf = (float) r0;   // the row, counted in binary32 (exact below 2^24)
+        for (int it = 0; it < steps; ++it, foff += fstep, yf += (float) rpi) {
+            const unsigned long long vmask = it < full ? valid_mask : tail_mask;
+            const float f = load_fix<FT>(frsrc, foff);   // 0 on lanes off the image
             const float f2 = f * f;
-            const float yf = (float) y;
             unsigned long long in[GPT];   // wave mask of lanes whose sample is in the image
             float ax[GPT], ay[GPT];
             float4 q[GPT];
@@ -454,7 +503,7 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
                 const float Wn = fmaf(B[g][2], yf, col[g][2]);
                 unsigned long long m;
                 if constexpr (POSW) {
-                    m = mask_ge0(Xn) & mask_ge0(Yn) & mask_lt0(fmaf(-Wm, Wn, Xn)) & mask_lt0(fmaf(-Hm, Wn, Yn));
+                    m = __builtin_amdgcn_ballot_w64(inside_posw(Xn, Yn, Wn, Wm, Hm));
                 } else {
                     const bool inside = quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm);
                     m = __builtin_amdgcn_ballot_w64(inside);
@@ -480,12 +529,12 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
                 float v = fmaf(ay[g], fmaf(ax[g], q[g].w, q[g].z), fmaf(ax[g], q[g].y, q[g].x));
                 if constexpr (!POSW) v = __uint_as_float(mask_select(in[g], 0u, __float_as_uint(v)));
                 if constexpr (COSTK == 0) {
-                    part[g][0] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint(fabsf(v - f))));
+                    masked_add(part[g][0], fabsf(v - f), in[g]);
                 } else {
-                    part[g][0] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint(f2)));
+                    masked_add(part[g][0], f2, in[g]);
                     part[g][1] = fmaf(v, v, part[g][1]);
                     if constexpr (COSTK == 1) part[g][2] = fmaf(f, v, part[g][2]);
-                    else part[g][2] += __uint_as_float(mask_select(in[g], 0u, __float_as_uint((v - f) * (v - f))));
+                    else masked_add(part[g][2], (v - f) * (v - f), in[g]);
                 }
             }
             if (++since == kFlush) {
@@ -556,7 +605,7 @@ __device__ __forceinline__ void fast_block(const SearchParams &P, int32_t base, 
         else S.cost[tid] = finish_cost<kFamNCC>(P.cost, t, false);
     }
     __syncthreads();
-    if (tid == 0) emit<GPT>(P, base, S.cost);
+    if (tid == 0) emit<GPT>(P, S.coef, S.cost);
     __syncthreads();
 }
 
@@ -833,7 +882,7 @@ __device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, Bl
     if (P.hist_dump && base == 0)
         for (int j = tid; j < B2; j += kThreads) P.hist_dump[j] = hist[j];
     __syncthreads();
-    if (tid == 0) emit<GPT>(P, base, S.cost);
+    if (tid == 0) emit<GPT>(P, S.coef, S.cost);
     __syncthreads();
 }
 

@@ -127,16 +127,6 @@ static hipError_t dispatch_exact(const SearchParams &P, int sampler, int mpix, i
     return exact_gpt<kFamMI, CGS_SAMPLER_BILINEAR>(P, mpix, fpix, stream);
 }
 
-// Grid points per workgroup of the fast kernel: 4 or 8 (CGS_FAST_GPT in the
-// environment overrides the default 4 for tuning sweeps).
-static int fast_gpt() {
-    static int g = [] {
-        const char *e = std::getenv("CGS_FAST_GPT");
-        const int v = e ? std::atoi(e) : 4;
-        return v == 8 ? 8 : 4;
-    }();
-    return g;
-}
 
 template <int COSTK, int G>
 static hipError_t fast_launch(const SearchParams &P, int fpix, hipStream_t stream) {
@@ -150,8 +140,9 @@ static hipError_t fast_launch(const SearchParams &P, int fpix, hipStream_t strea
 
 template <int COSTK>
 static hipError_t fast_pix(const SearchParams &P, int fpix, int gpt, hipStream_t stream) {
-    if (gpt == 0) gpt = fast_gpt();
-    return gpt == 8 ? fast_launch<COSTK, 8>(P, fpix, stream) : fast_launch<COSTK, 4>(P, fpix, stream);
+    if (gpt == 8) return fast_launch<COSTK, 8>(P, fpix, stream);
+    if (gpt == 6) return fast_launch<COSTK, 6>(P, fpix, stream);
+    return fast_launch<COSTK, 4>(P, fpix, stream);
 }
 
 static hipError_t dispatch_fast(const SearchParams &P, int fpix, int gpt, hipStream_t stream) {
@@ -266,6 +257,11 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
 
     SearchParams P{};
     fill_grid_dev(grid, &P.grid);
+    if (kind != kKernelExact) {
+        int order[kMaxDim];
+        if (work_order(grid->dim, o->param_mode, order)) set_work_order(&P.grid, order);
+    }
+    const int gpt = o->tile_gpt ? o->tile_gpt : default_fast_gpt(P.grid);
     P.sincos = levels0 > 0 ? (const float2 *) ws->sincos.ptr : nullptr;
     P.param_mode = o->param_mode;
     P.cost = o->cost;
@@ -287,7 +283,7 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     P.ceil_taps = moved->pixtype == CGS_PIX_F32;
     const bool timed = (o->flags & CGS_FLAG_TIME_KERNEL) != 0;
     if (timed) HIP_TRY(ws->record_start(stream));
-    if (kind == kKernelFast) HIP_TRY(dispatch_fast(P, fixed->pixtype, o->tile_gpt, stream));
+    if (kind == kKernelFast) HIP_TRY(dispatch_fast(P, fixed->pixtype, gpt, stream));
     else if (kind == kKernelMI) HIP_TRY(dispatch_mi(P, moved->pixtype, fixed->pixtype, o->tile_gpt, stream));
     else HIP_TRY(dispatch_exact(P, o->sampler, moved->pixtype, fixed->pixtype, stream));
     if (timed) HIP_TRY(ws->record_stop(stream));

@@ -118,7 +118,8 @@ typedef struct {
     int32_t tie_mode;    /* cgs_tie_mode */
     int32_t mi_bins;     /* joint-histogram bins per axis for the MI costs: 8..64 (reference: 64) */
     int32_t flags;       /* CGS_FLAG_* bits */
-    int32_t tile_gpt;    /* grid points per workgroup; 0 = library default (autotuner choice) */
+    int32_t tile_gpt;    /* grid points per workgroup: 4, 6 or 8 for SAD/NCC/SQD, 2 or 4 for MI;
+                            0 = library default (6 or 8 for SAD/NCC/SQD, 4 for MI) */
 } cgs_search_opts;
 
 /* opts.flags: time the search kernel with HIP events on the caller's
@@ -238,7 +239,7 @@ typedef struct {
 
 /* Tile shapes of the persistent kernel; 0 fields take the defaults. */
 typedef struct {
-    int32_t fast_gpt;        /* grid points per workgroup for SAD/NCC/SQD: 4 */
+    int32_t fast_gpt;        /* grid points per workgroup for SAD/NCC/SQD: 4 or 6 */
     int32_t mi_gpt;          /* grid points per workgroup for MI: 2 or 4 */
     int32_t wg_per_cu;       /* resident workgroups per compute unit: 1..8 */
     int32_t items_per_claim; /* work items taken per atomic claim: 1..64 */

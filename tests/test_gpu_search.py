@@ -114,6 +114,55 @@ def test_full_workload_sample_and_golden_argmin(gpu, oracle, name):
             assert abs(ov - vo) <= 2 * rtol * abs(vo)
 
 
+def _search_with(functor, m, f, grid, gpt, order):
+    """cgs_search with an explicit tile size and work order (CGS_WORK_ORDER
+    is read by every search call); returns the costs and the argmin."""
+    import ctypes
+    old = os.environ.get("CGS_WORK_ORDER")
+    if order is None:
+        os.environ.pop("CGS_WORK_ORDER", None)
+    else:
+        os.environ["CGS_WORK_ORDER"] = order
+    try:
+        o = A.make_opts(functor)
+        o.tile_gpt = gpt
+        out = L.Argmin()
+        res = A.CudaTensor(grid.getAxisSampleCounts())
+        L.check(L.lib().cgs_search(ctypes.byref(grid._g), ctypes.byref(m._img), ctypes.byref(f._img), ctypes.byref(o),
+                                   ctypes.c_void_p(res.data.data_ptr()), ctypes.byref(out), None))
+        return res.data.cpu().numpy(), (out.value, out.index)
+    finally:
+        if old is None:
+            os.environ.pop("CGS_WORK_ORDER", None)
+        else:
+            os.environ["CGS_WORK_ORDER"] = old
+
+
+@pytest.mark.parametrize("counts", [(3, 2, 2, 2, 6, 2, 2, 2), (3, 1, 1, 1, 5, 1, 1, 3)], ids=["box", "ragged45"])
+def test_work_order_and_tiles_bit_identical(gpu, oracle, counts):
+    """Which grid points share a workgroup (the work order, tx-first by
+    default for the centred parametrisation) and how many (tile_gpt) change
+    no cost bit and no argmin: the grid order with every tile size, the
+    default order and a reversed order all agree, and match the oracle."""
+    m, f, _ = synth.make_pair(23, 64, sigma=3.0, perturb=3.0, dtype=np.float32)
+    s, e, _ = synth.grid_ranges(4, 4.0)
+    grid = A.CudaGrid(8)
+    grid.setStartPoint(s)
+    grid.setEndPoint(e)
+    grid.setNumSamples(np.array(counts, np.float32))
+    mi, fi = A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f)
+    for fn in (A.calcNCC, A.calcSQD, A.centredAbsoluteDifference, A.calcMIstream.with_(bins=32)):
+        want, wam = _search_with(fn, mi, fi, grid, 4, "grid")
+        for gpt, order in [(0, None), (6, None), (8, None), (6, "grid"), (8, "grid"), (4, "7,6,5,4,3,2,1,0"),
+                           (2, None)]:
+            got, am = _search_with(fn, mi, fi, grid, gpt, order)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (fn.name, gpt, order)
+            assert am == wam, (fn.name, gpt, order)
+        ref = oracle_search(oracle, fn, m, f, s, e, np.array(counts, np.float32))
+        ok, bad = close(want, ref, MI_RTOL if fn.cost == L.COST_MI else NCC_RTOL)
+        assert ok, (fn.name, bad)
+
+
 def test_mi32_full_and_histograms(gpu, oracle):
     m, f, _ = synth.make_pair(31, 80, sigma=3.0, perturb=4.0, dtype=np.uint8, gamma=True, noise=4.0)
     s, e, c = synth.grid_ranges(3, 4.0)
