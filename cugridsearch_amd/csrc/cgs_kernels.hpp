@@ -396,9 +396,6 @@ __device__ __forceinline__ uint32_t mask_select(unsigned long long m, uint32_t a
     return r;
 }
 
-// Wave masks of ordered float comparisons (llvm.amdgcn.fcmp predicates).
-__device__ __forceinline__ unsigned long long mask_ge0(float v) { return __builtin_amdgcn_fcmpf(v, 0.0f, 3); }
-__device__ __forceinline__ unsigned long long mask_lt0(float v) { return __builtin_amdgcn_fcmpf(v, 0.0f, 4); }
 
 __device__ __forceinline__ float4 load_tap(__amdgpu_buffer_rsrc_t rsrc, uint32_t offset) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, offset, 0, 0);
@@ -609,10 +606,21 @@ __device__ __forceinline__ void fast_block(const SearchParams &P, int32_t base, 
     __syncthreads();
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8).  This bijection hands each XCD a contiguous run of tiles instead, so
+// the cost-tensor lines that neighbouring tiles fill meet in one XCD's L2
+// (and write back whole) rather than being written back in pieces by eight.
+__device__ __forceinline__ int32_t xcd_tile(int32_t b, int32_t n) {
+    constexpr int32_t kXcds = 8;
+    const int32_t xcd = b % kXcds, local = b / kXcds;
+    const int32_t per = n / kXcds, rem = n % kXcds;   // XCDs below rem hold per + 1 tiles
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
 template <int COSTK, typename FT, int GPT>
 __global__ __launch_bounds__(kThreads) void search_fast_kernel(const SearchParams P) {
     __shared__ BlockShared<GPT> S;
-    fast_block<COSTK, FT, GPT>(P, (int32_t) blockIdx.x * GPT, S);
+    fast_block<COSTK, FT, GPT>(P, xcd_tile((int32_t) blockIdx.x, (int32_t) gridDim.x) * GPT, S);
 }
 
 // --------------------------------------------------------------- MI kernel
@@ -719,6 +727,9 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
     const int wE = mw * EB, baseE = (mw + 1) * EB;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void *) uniform(P.quads), (short) 0, (int) (((size_t) mw * mh + 2 * (size_t) mw + 2) * EB), 0x00020000);
+    const __amdgpu_buffer_rsrc_t frsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *) fix, (short) 0, (int) ((size_t) fw * fh * sizeof(FT)), 0x00020000);
+    const int full = fh / rpi;   // steps in which every lane's row is inside the image
     const int bins = uniform(P.bins), B2 = bins * bins;
     const bool ceil_taps = uniform(P.ceil_taps) != 0;
     const float fbins = (float) bins, bscale = fbins / 256.0f;
@@ -735,19 +746,23 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
         const bool valid = (r0 < rpi) && (x < fw);
         const float xf = (float) x;
         float col[GPT][3];
+        // + 0.0f: the forms are never -0 (a -0 sample coordinate would only
+        // change the sign of a zero weight, which no bin sees)
 #pragma unroll
         for (int g = 0; g < GPT; ++g)
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                col[g][k] = fmaf(uniform(s_coef[g][3 * k]), xf, uniform(s_coef[g][3 * k + 2]));
+                col[g][k] = fmaf(uniform(s_coef[g][3 * k]), xf, uniform(s_coef[g][3 * k + 2])) + 0.0f;
         const int steps = (fh + rpi - 1) / rpi;
-        for (int it = 0; it < steps; ++it) {
-            const int y = r0 + it * rpi;
-            const bool vy = valid && y < fh;
-            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(vy);
-            const float f = vy ? (float) fix[y * fw + x] : 0.0f;
+        const unsigned long long valid_mask = __builtin_amdgcn_ballot_w64(valid);
+        const unsigned long long tail_mask = __builtin_amdgcn_ballot_w64(valid && r0 + full * rpi < fh);
+        uint32_t foff = valid ? (uint32_t) ((r0 * fw + x) * (int) sizeof(FT)) : kOffsetOut;
+        const uint32_t fstep = (uint32_t) (rpi * fw * (int) sizeof(FT));
+        float yf = (float) r0;
+        for (int it = 0; it < steps; ++it, foff += fstep, yf += (float) rpi) {
+            const unsigned long long vmask = it < full ? valid_mask : tail_mask;
+            const float f = load_fix<FT>(frsrc, foff);   // 0 on lanes off the image
             const uint32_t rowbin = (uint32_t) (mi_bin_fast<POW2>(f, bins, fbins, bscale) * bins);
-            const float yf = (float) y;
             unsigned long long in[GPT];
             float ax[GPT], ay[GPT];
             float4 q[GPT];
@@ -759,7 +774,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 unsigned long long m;
                 float X, Y;
                 if constexpr (FASTDIV) {
-                    m = mask_ge0(Xn) & mask_ge0(Yn) & mask_lt0(fmaf(-Wm, Wn, Xn)) & mask_lt0(fmaf(-Hm, Wn, Yn));
+                    m = __builtin_amdgcn_ballot_w64(inside_posw(Xn, Yn, Wn, Wm, Hm));
                     const Recip R = refine_rcp(Wn);
                     X = div_by(Xn, R);
                     Y = div_by(Yn, R);
@@ -890,7 +905,7 @@ template <typename MT, typename FT, int GPT>
 __global__ __launch_bounds__(kThreads) void search_mi_kernel(const SearchParams P) {
     __shared__ BlockShared<GPT> S;
     extern __shared__ uint32_t s_hist[];   // GPT histograms + 64 dummy counters
-    mi_block<MT, FT, GPT>(P, (int32_t) blockIdx.x * GPT, S, s_hist);
+    mi_block<MT, FT, GPT>(P, xcd_tile((int32_t) blockIdx.x, (int32_t) gridDim.x) * GPT, S, s_hist);
 }
 
 }  // namespace cgs
