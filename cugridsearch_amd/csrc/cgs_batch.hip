@@ -178,7 +178,7 @@ hipError_t launch_batch_any(cgs_batch *b, hipStream_t s) {
 }
 
 cgs_status parse_tiles(const cgs_tile_config *tiles, cgs_tile_config *out) {
-    cgs_tile_config t{4, 4, 4, 8};
+    cgs_tile_config t{6, 4, 4, 8};
     if (tiles) {
         if (tiles->fast_gpt) t.fast_gpt = tiles->fast_gpt;
         if (tiles->mi_gpt) t.mi_gpt = tiles->mi_gpt;
