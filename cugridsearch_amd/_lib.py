@@ -22,7 +22,7 @@ PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION, PARAM_NORM = ra
 TIE_LOWEST, TIE_REFERENCE = 0, 1
 PIX_U8, PIX_F32 = 0, 1
 MAX_DIM = 8
-ABI_VERSION = 5          # CGS_ABI_VERSION of include/cgs.h these layouts follow
+ABI_VERSION = 6          # CGS_ABI_VERSION of include/cgs.h these layouts follow
 FLAG_TIME_KERNEL = 1
 
 
@@ -95,6 +95,7 @@ SIGNATURES = {
     "cgs_image_filter": (ctypes.c_int, [_P(Image), _P(Image), _P(ctypes.c_float), ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_void_p]),
     "cgs_image_quantize": (ctypes.c_int, [_P(Image), ctypes.c_float, ctypes.c_void_p]),
+    "cgs_image_resize": (ctypes.c_int, [_P(Image), _P(Image), ctypes.c_void_p]),
     "cgs_eval_point": (ctypes.c_int, [_P(Image), _P(Image), _P(SearchOpts), ctypes.c_int32, _P(ctypes.c_float),
                                       _P(ctypes.c_float), _P(ctypes.c_uint32), ctypes.c_void_p]),
     "cgs_batch_create": (ctypes.c_int, [_P(ctypes.c_void_p)]),

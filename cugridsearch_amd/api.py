@@ -240,6 +240,17 @@ def filter_image(img, filt, out=None, stream=None):
     return out
 
 
+def resize_image(img, width, height, stream=None):
+    """stbir_resize_uint8 (stb_image_resize v0.97, as the matcher calls it at
+    src/cpu/multispectral_ImageMatcher.cu:531 and :550): a new uint8 CudaImage
+    of width x height, byte for byte the library's resampling."""
+    t = img.tensor
+    shape = (int(height), int(width)) + tuple(t.shape[2:])
+    out = CudaImage(_torch().empty(shape, dtype=t.dtype, device=t.device))
+    L.check(L.lib().cgs_image_resize(ctypes.byref(img._img), ctypes.byref(out._img), _stream_ptr(stream)))
+    return out
+
+
 def quantize_image(img, mod, stream=None):
     """transformPixel (src/cpu/multispectral_ImageMatcher.cu:411-430), in place
     on a uint8 CudaImage."""

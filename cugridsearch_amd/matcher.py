@@ -8,9 +8,10 @@ Pipeline, step for step with the reference:
 
 1. load both images as one-channel uint8 (stb's luma (77 R + 150 G + 29 B) >> 8
    for colour files, include/third_party/stb_image.h:1726-1729);
-2. when one image's area exceeds 1.5x the other's the reference resizes it
-   with stb_image_resize (:518-557); that resampler is not restated here, so
-   such a pair is refused with a message;
+2. when one image's area exceeds 1.5x the other's, resize it to the other's
+   width and height with stb_image_resize's stbir_resize_uint8 (:518-557),
+   byte for byte on the GPU (cgs_image_resize), and scale the grid's scale
+   axes and the ground truth by the size ratios;
 3. filter both images with the 3x3 unsharp kernel (:363-366, :620-621);
 4. quantise the moving image by MOD when MOD > 0 (:626-631; the search uses
    the filtered image, made before the quantisation, so MOD does not change the
@@ -204,16 +205,28 @@ def run(fixed, moved, mod=0.0, gt=None, depth=DEPTH, names=("fixed", "moving"), 
 
     yf, xf = fixed.shape
     ym, xm = moved.shape
-    sfx = sfy = f32(1)
-    if xf * yf > float(MAX_SIZE_DISCREPANCY) * xm * ym or xm * ym > float(MAX_SIZE_DISCREPANCY) * xf * yf:
-        raise SystemExit(f"images of {xf}x{yf} and {xm}x{ym} differ in area by more than 1.5x: the reference "
-                         "resizes one with stb_image_resize, which this build does not restate; resize beforehand")
     dev = torch.device("cuda")
+    img_fix = A.CudaImage.from_numpy(fixed, dev)
+    img_mov = A.CudaImage.from_numpy(moved, dev)
+    # the larger image (by area, 1.5x) goes to the smaller one's size
+    # (:516-557): int area against the binary32 1.5 * x * y
+    sfx = sfy = f32(1)
+    resized = None
+    if f32(xf * yf) > f32(f32(MAX_SIZE_DISCREPANCY * f32(xm)) * f32(ym)):
+        sfx, sfy = f32(f32(xm) / f32(xf)), f32(f32(ym) / f32(yf))
+        log.write(f"Rescaling {names[0]} from ({xf},{yf}) to ({xm},{ym})\n")
+        img_fix = A.resize_image(img_fix, xm, ym)
+        xf, yf = xm, ym
+        resized = ("image_fixed_resized.png", img_fix)
+    elif f32(xm * ym) > f32(f32(MAX_SIZE_DISCREPANCY * f32(xf)) * f32(yf)):
+        sfx, sfy = f32(f32(xf) / f32(xm)), f32(f32(yf) / f32(ym))
+        log.write(f"Rescaling {names[1]} from ({xm},{ym}) to ({xf},{yf})\n")
+        img_mov = A.resize_image(img_mov, xf, yf)
+        xm, ym = xf, yf
+        resized = ("image_moved_resized.png", img_mov)
     props = torch.cuda.get_device_properties(dev)
     out.write(f'GPU Device 0: "{props.name}" with compute capability {props.major}.{props.minor}\n\n')
     out.write(sobel_banner())
-    img_fix = A.CudaImage.from_numpy(fixed, dev)
-    img_mov = A.CudaImage.from_numpy(moved, dev)
     fix_filtered = A.filter_image(img_fix, UNSHARP_3X3)
     mov_filtered = A.filter_image(img_mov, UNSHARP_3X3)
     if mod > 0.0:
@@ -285,7 +298,8 @@ def run(fixed, moved, mod=0.0, gt=None, depth=DEPTH, names=("fixed", "moving"), 
         with open(results_file, "a") as fh:
             fh.write("".join(line) + "\n")
     return dict(best=best, homography=H, parameters=back, covariance=covar.reshape(8, 8), seconds=elapsed,
-                argmin=(v, i), fixed_filtered=fix_filtered, moving_filtered=mov_filtered)
+                argmin=(v, i), fixed_filtered=fix_filtered, moving_filtered=mov_filtered, scale_factor=(sfx, sfy),
+                resized=resized)
 
 
 def ground_truth(path, fixed_name, sfx, sfy, default):
@@ -328,8 +342,12 @@ def main(argv=None):
     sys.stderr.write(f"Output image filename is {args.output}.\n")
     sys.stderr.write(f"Fused Output image filename is {args.fusedoutput}.\n")
     fixed, moved = load_gray(args.i_ref), load_gray(args.i_mov)
-    run(fixed, moved, mod=args.mod, gt=args.gt, depth=args.depth, names=(args.i_ref, args.i_mov),
-        results_file=args.results_file, fused_name=args.fusedoutput)
+    r = run(fixed, moved, mod=args.mod, gt=args.gt, depth=args.depth, names=(args.i_ref, args.i_mov),
+            results_file=args.results_file, fused_name=args.fusedoutput)
+    if r["resized"] is not None:   # the reference writes the resized image (:540, :559)
+        from PIL import Image
+        name, img = r["resized"]
+        Image.fromarray(img.tensor.cpu().numpy()).save(name)
     sys.stderr.write("output images (-o, -f): the image writers are not part of this build; skipped\n")
     return 0
 

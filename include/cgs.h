@@ -24,7 +24,7 @@ extern "C" {
 /* Layout version of the structures below; cgs_abi_version() returns the
  * value the library was built with, so a caller compiled against another
  * header can refuse to run instead of passing mis-sized structures. */
-#define CGS_ABI_VERSION 5
+#define CGS_ABI_VERSION 6
 
 typedef enum {
     CGS_OK = 0,
@@ -154,6 +154,17 @@ cgs_status cgs_image_filter(const cgs_image *in, const cgs_image *out, const flo
  * a uint8 image: each byte read as a signed char, (int)(v / mod), then
  * (int)(v * mod), stored modulo 256.  mod >= 2^-20.  Asynchronous. */
 cgs_status cgs_image_quantize(const cgs_image *img, float mod, void *stream);
+
+/* stbir_resize_uint8 of stb_image_resize v0.97 (vendored by the reference at
+ * include/third_party/stb_image_resize.h:228, 2462-2470; called by the
+ * matcher at src/cpu/multispectral_ImageMatcher.cu:531 and :550 to bring the
+ * larger image to the smaller one's size): resamples `in` to `out`'s width and
+ * height, byte for byte as the library does (linear colour space, clamped
+ * edges, Mitchell filter on a shrinking axis and Catmull-Rom on a growing one,
+ * channels alike).  uint8 images with equal channel counts, device memory, no
+ * aliasing.  Synchronises the stream (the filter tables are staged from the
+ * host). */
+cgs_status cgs_image_resize(const cgs_image *in, const cgs_image *out, void *stream);
 
 /* CGS_ABI_VERSION of the library's build. */
 int32_t cgs_abi_version(void);

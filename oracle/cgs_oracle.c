@@ -916,3 +916,201 @@ int64_t orc_covariance(int D, const float *start, const float *res, const float 
     }
     return mi;
 }
+
+/* ------------------------------------------------------------------ resize
+ * stbir_resize_uint8 of stb_image_resize v0.97 (the reference vendors it at
+ * include/third_party/stb_image_resize.h and calls it at
+ * src/cpu/multispectral_ImageMatcher.cu:531 and :550 to bring the larger image
+ * to the smaller one's size): linear colour space, no alpha, clamped edges,
+ * Mitchell filter on an axis that shrinks and Catmull-Rom on one that grows
+ * (:429-433), channels filtered alike.  Restated from the library's published
+ * algorithm with its binary32 operations in its order:
+ *   decode     p / 255.0f                                        (:1283-1291)
+ *   kernels    (16 + x*x*(21x - 36))/18, (32 + x*(-60 + x*(36 - 7x)))/18;
+ *              1 - x*x*(2.5 - 1.5x), 2 - x*(4 + x*(0.5x - 2.5))  (:797-837)
+ *   ranges     floor((double) bound +- 0.5) of the float bounds  (:1009-1036)
+ *   weights    growing axis: per output pixel, leading exact zeros dropped,
+ *              scaled by 1/sum, trailing zeros dropped             (:1038-1090);
+ *              shrinking axis: per input pixel kernel(x) * scale, trailing zeros
+ *              dropped, then per output pixel scaled by 1/sum over the input
+ *              pixels that reach it, then leading zeros and outputs < 0 dropped
+ *              (:1092-1201)
+ *   sums       each output sample accumulates from 0 in ascending input order
+ *              (:1450-1656 horizontally; :1870-2065 and the ring buffer loops
+ *              :2067-2204 vertically), then
+ *   encode     (uint8) (int) ((double) (sat(v) * 255.0f) + 0.5)  (:1735-1757).
+ * Checked bit for bit against the library itself (oracle/_ref, built from the
+ * reference's header) by tests/test_oracle_resize.py. */
+
+typedef struct { int n0, n1; } orc_rz_contrib;
+
+static float orc_rz_kernel(int grow, float x) {
+    x = fabsf(x);
+    if (grow) {   /* Catmull-Rom */
+        if (x < 1.0f) return 1.0f - x * x * (2.5f - 1.5f * x);
+        if (x < 2.0f) return 2.0f - x * (4.0f + x * (0.5f * x - 2.5f));
+        return 0.0f;
+    }
+    if (x < 1.0f) return (16.0f + x * x * (21.0f * x - 36.0f)) / 18.0f;   /* Mitchell */
+    if (x < 2.0f) return (32.0f + x * (-60.0f + x * (36.0f - 7.0f * x))) / 18.0f;
+    return 0.0f;
+}
+
+/* One axis: fills, for every output index o, the ascending list of input
+ * indices (clamped into [0, in_size)) and weights that accumulate into it.
+ * first[o]..first[o+1] indexes src/wt (capacity: out_size * cap entries). */
+static int orc_rz_axis(int in_size, int out_size, int *first, int *src, float *wt, int cap) {
+    const float scale = (float) out_size / (float) in_size;
+    const int grow = scale > 1.0f;
+    const int width = 4;                                   /* ceil(2 * support) */
+    int n = 0;
+    if (grow) {
+        const float radius = 2.0f * scale;
+        for (int o = 0; o < out_size; o++) {
+            const float centre = (float) o + 0.5f;
+            const float lb = (centre - radius) / scale, ub = (centre + radius) / scale;
+            const float in_centre = centre / scale;
+            int lo = (int) floor((double) lb + 0.5), hi = (int) floor((double) ub - 0.5);
+            float c[16];
+            float total = 0.0f;
+            int n0 = lo;
+            for (int i = 0; i <= hi - lo; i++) {
+                c[i] = orc_rz_kernel(1, in_centre - ((float) (i + lo) + 0.5f));
+                if (i == 0 && c[i] == 0.0f) {
+                    n0 = ++lo;
+                    i--;
+                    continue;
+                }
+                total += c[i];
+            }
+            const float s = 1.0f / total;
+            int n1 = hi;
+            for (int i = 0; i <= hi - lo; i++) c[i] *= s;
+            for (int i = hi - lo; i >= 0; i--) {
+                if (c[i] != 0.0f) break;
+                n1 = n0 + i - 1;
+            }
+            first[o] = n;
+            for (int k = n0; k <= n1; k++) {
+                if (n >= cap) return -1;
+                src[n] = k < 0 ? 0 : (k >= in_size ? in_size - 1 : k);
+                wt[n++] = c[k - n0];
+            }
+        }
+        first[out_size] = n;
+        return n;
+    }
+    /* shrinking: weights per input pixel (with the margin), normalised per output */
+    const int margin = (int) ceil((double) (2.0f * 2.0f / scale)) / 2;
+    const int ncontrib = in_size + 2 * margin;
+    const float radius = 2.0f / scale;
+    orc_rz_contrib *ct = (orc_rz_contrib *) malloc(sizeof(orc_rz_contrib) * (size_t) ncontrib);
+    /* the groups are contiguous, as the library lays them out: a fifth weight of
+     * pixel j (its output range can span 5) lands in group j+1's first slot
+     * and is read back from there; one spare group takes the last pixel's */
+    float *cf = (float *) calloc((size_t) (ncontrib + 1) * width, sizeof(float));
+    for (int j = 0; j < ncontrib; j++) {
+        const float centre = (float) (j - margin) + 0.5f;
+        const float lb = (centre - radius) * scale, ub = (centre + radius) * scale;
+        const float out_centre = centre * scale;
+        const int lo = (int) floor((double) lb + 0.5), hi = (int) floor((double) ub - 0.5);
+        ct[j].n0 = lo;
+        ct[j].n1 = hi;
+        float *c = cf + (size_t) j * width;
+        for (int i = 0; i <= hi - lo; i++) c[i] = orc_rz_kernel(0, ((float) (i + lo) + 0.5f) - out_centre) * scale;
+        for (int i = hi - lo; i >= 0; i--) {
+            if (c[i] != 0.0f) break;
+            ct[j].n1 = ct[j].n0 + i - 1;
+        }
+    }
+    for (int o = 0; o < out_size; o++) {
+        float total = 0.0f;
+        for (int j = 0; j < ncontrib; j++) {
+            if (o >= ct[j].n0 && o <= ct[j].n1) total += cf[(size_t) j * width + (o - ct[j].n0)];
+            else if (o < ct[j].n0) break;
+        }
+        const float s = 1.0f / total;
+        for (int j = 0; j < ncontrib; j++) {
+            if (o >= ct[j].n0 && o <= ct[j].n1) cf[(size_t) j * width + (o - ct[j].n0)] *= s;
+            else if (o < ct[j].n0) break;
+        }
+    }
+    for (int j = 0; j < ncontrib; j++) {
+        float *c = cf + (size_t) j * width;
+        int skip = 0;
+        while (skip < width && c[skip] == 0.0f) skip++;
+        if (skip == width) {   /* no weight at all: the pixel reaches no output */
+            ct[j].n1 = ct[j].n0 - 1;
+            continue;
+        }
+        ct[j].n0 += skip;
+        while (ct[j].n0 < 0) {
+            ct[j].n0++;
+            skip++;
+        }
+        const int range = ct[j].n1 - ct[j].n0 + 1;
+        const int mx = range < width ? range : width;
+        for (int i = 0; i < mx && i + skip < width; i++) c[i] = c[i + skip];
+        if (ct[j].n1 > out_size - 1) ct[j].n1 = out_size - 1;
+    }
+    /* gather per output, inputs ascending */
+    for (int o = 0; o < out_size; o++) {
+        first[o] = n;
+        for (int j = 0; j < ncontrib; j++) {
+            if (o < ct[j].n0 || o > ct[j].n1) continue;
+            if (n >= cap) {
+                free(ct);
+                free(cf);
+                return -1;
+            }
+            const int k = j - margin;
+            src[n] = k < 0 ? 0 : (k >= in_size ? in_size - 1 : k);
+            wt[n++] = cf[(size_t) j * width + (o - ct[j].n0)];
+        }
+    }
+    first[out_size] = n;
+    free(ct);
+    free(cf);
+    return n;
+}
+
+/* in: ih x iw x C uint8; out: oh x ow x C uint8.  Returns 0, or -1 on bad sizes. */
+int orc_image_resize(const uint8_t *in, int iw, int ih, uint8_t *out, int ow, int oh, int C) {
+    if (iw < 1 || ih < 1 || ow < 1 || oh < 1 || C < 1) return -1;
+    /* entries per axis: at most 4 per input pixel and margin pixel when shrinking,
+     * at most 6 per output pixel when growing */
+    const int capx = 8 * ow + 4 * (iw + 2 * (int) (4.0 * iw / ow + 2)) + 64;
+    const int capy = 8 * oh + 4 * (ih + 2 * (int) (4.0 * ih / oh + 2)) + 64;
+    int *fx = (int *) malloc(sizeof(int) * (size_t) (ow + 1)), *fy = (int *) malloc(sizeof(int) * (size_t) (oh + 1));
+    int *sx = (int *) malloc(sizeof(int) * (size_t) capx), *sy = (int *) malloc(sizeof(int) * (size_t) capy);
+    float *wx = (float *) malloc(sizeof(float) * (size_t) capx), *wy = (float *) malloc(sizeof(float) * (size_t) capy);
+    float *hb = (float *) malloc(sizeof(float) * (size_t) ih * ow * C);
+    int rc = 0;
+    if (orc_rz_axis(iw, ow, fx, sx, wx, capx) < 0 || orc_rz_axis(ih, oh, fy, sy, wy, capy) < 0) {
+        rc = -1;
+        goto done;
+    }
+    /* horizontal pass on every input row (rows outside the image are its
+     * clamped edge rows, identical values) */
+    for (int r = 0; r < ih; r++)
+        for (int x = 0; x < ow; x++)
+            for (int c = 0; c < C; c++) {
+                float acc = 0.0f;
+                for (int t = fx[x]; t < fx[x + 1]; t++) {
+                    const float d = (float) in[((size_t) r * iw + sx[t]) * C + c] / 255.0f;
+                    acc += d * wx[t];
+                }
+                hb[((size_t) r * ow + x) * C + c] = acc;
+            }
+    for (int y = 0; y < oh; y++)
+        for (int x = 0; x < ow; x++)
+            for (int c = 0; c < C; c++) {
+                float acc = 0.0f;
+                for (int t = fy[y]; t < fy[y + 1]; t++) acc += hb[((size_t) sy[t] * ow + x) * C + c] * wy[t];
+                const float v = acc < 0.0f ? 0.0f : (acc > 1.0f ? 1.0f : acc);
+                out[((size_t) y * ow + x) * C + c] = (uint8_t) (int) ((double) (v * 255.0f) + 0.5);
+            }
+done:
+    free(fx); free(fy); free(sx); free(sy); free(wx); free(wy); free(hb);
+    return rc;
+}

@@ -78,6 +78,9 @@ def lib():
         L.orc_mi_alt_area.restype = ctypes.c_float
         L.orc_image_filter.argtypes = [ctypes.POINTER(_Image), ctypes.c_void_p, fp, ctypes.c_int, ctypes.c_int]
         L.orc_image_quantize.argtypes = [ctypes.POINTER(ctypes.c_uint8), i64, ctypes.c_float]
+        L.orc_image_resize.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int]
+        L.orc_image_resize.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -192,6 +195,16 @@ def image_quantize(img, mod):
     """transformPixel on a uint8 image (a copy is returned)."""
     out = np.array(img, np.uint8, copy=True, order="C")
     lib().orc_image_quantize(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), out.size, float(mod))
+    return out
+
+
+def image_resize(img, ow, oh):
+    """stbir_resize_uint8 of an (H, W) or (H, W, C) uint8 image to (oh, ow)."""
+    a = np.ascontiguousarray(img, np.uint8)
+    C = 1 if a.ndim == 2 else a.shape[2]
+    out = np.zeros((oh, ow) + (() if a.ndim == 2 else (C,)), np.uint8)
+    if lib().orc_image_resize(a.ctypes.data, a.shape[1], a.shape[0], out.ctypes.data, ow, oh, C) != 0:
+        raise ValueError("bad resize sizes")
     return out
 
 
