@@ -17,6 +17,7 @@ compute runs in libcgs.so.  Functors are tags naming a cost of the C-ABI
 src/gpu/cudaGridSearch.cuh:284-285).
 """
 import ctypes
+import os
 import dataclasses
 
 import numpy as np
@@ -249,6 +250,40 @@ def resize_image(img, width, height, stream=None):
     out = CudaImage(_torch().empty(shape, dtype=t.dtype, device=t.device))
     L.check(L.lib().cgs_image_resize(ctypes.byref(img._img), ctypes.byref(out._img), _stream_ptr(stream)))
     return out
+
+
+def warp_image(img, H, width, height, out=None, stream=None):
+    """transformImage (src/gpu/cudaImageFunctions.cuh:40-62): img under the
+    8-entry homography H into a width x height image (a new zero-filled one
+    when out is None, as writeTransformedImageToDisk starts from)."""
+    if out is None:
+        t = img.tensor
+        out = CudaImage(_torch().zeros((int(height), int(width)) + tuple(t.shape[2:]), dtype=t.dtype, device=t.device))
+    L.check(L.lib().cgs_image_warp(ctypes.byref(img._img), ctypes.byref(out._img), L.floats(np.asarray(H).reshape(-1)[:8]),
+                                   _stream_ptr(stream)))
+    return out
+
+
+def fuse_images(ref, mov, est_H, true_H, channel=0, local=False, out=None, stream=None):
+    """fuseAlignedImages (src/gpu/cudaImageFunctions.cuh:64-114) or, local=True,
+    fuseAlignedImages_local (src/cpu/multispectral_ImageMatcher.cu:115-179):
+    a 3-channel overlay of ref's size (zero-filled first, as the writers do)."""
+    if out is None:
+        out = CudaImage(_torch().zeros((ref.height(), ref.width(), 3), dtype=ref.tensor.dtype,
+                                       device=ref.tensor.device))
+    L.check(L.lib().cgs_image_fuse(ctypes.byref(ref._img), ctypes.byref(mov._img),
+                                   L.floats(np.asarray(est_H).reshape(-1)[:8]),
+                                   L.floats(np.asarray(true_H).reshape(-1)[:8]), int(channel), 1 if local else 0,
+                                   ctypes.byref(out._img), _stream_ptr(stream)))
+    return out
+
+
+def write_png(path, img):
+    """stbi_write_png of a uint8 CudaImage (or (H, W[, C]) uint8 array)."""
+    a = img.tensor.cpu().numpy() if isinstance(img, CudaImage) else np.asarray(img)
+    a = np.ascontiguousarray(a, np.uint8)
+    c = 1 if a.ndim == 2 else a.shape[2]
+    L.check(L.lib().cgs_write_png(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0], c))
 
 
 def quantize_image(img, mod, stream=None):

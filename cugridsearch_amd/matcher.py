@@ -26,9 +26,13 @@ Pipeline, step for step with the reference:
    Homography_to_parameters[...] covar_matrix[...] (:904-989), plus the
    results line appended to imageMatcherResults.txt (:486-490, :964-1062).
 
+8. the output images: the moving image under the answer on the fixed
+   image's grid (-o, transformImage_local) and the three-channel overlay of
+   the filtered fixed image with the moving image under the answer and under
+   the ground truth (-f, fuseAlignedImages_local) (:1072-1076).
+
 The Levenberg-Marquardt refinement is commented out in the reference
-(minParams are printed unchanged) and so is not run; the output image writers
-(-o, -f) are not part of this build and are reported as skipped on stderr.
+(minParams are printed unchanged) and so is not run.
 """
 import argparse
 import math
@@ -196,9 +200,10 @@ def sobel_banner():
 # --------------------------------------------------------------- pipeline
 
 def run(fixed, moved, mod=0.0, gt=None, depth=DEPTH, names=("fixed", "moving"), out=sys.stdout, log=sys.stderr,
-        results_file=None, fused_name="output_image_fused.png"):
+        results_file=None, fused_name="output_image_fused.png", out_name=None):
     """Runs the matcher on two uint8 (H, W) arrays; writes the protocol to
-    out and returns a dict of the results."""
+    out and returns a dict of the results.  With out_name, also writes the
+    output images to out_name and fused_name (:1072-1076)."""
     import torch
     from . import api as A
     from . import _lib as L
@@ -297,9 +302,32 @@ def run(fixed, moved, mod=0.0, gt=None, depth=DEPTH, names=("fixed", "moving"), 
     if results_file:
         with open(results_file, "a") as fh:
             fh.write("".join(line) + "\n")
+    # the output images (:1072-1076): the moving image under H on the fixed
+    # image's grid, and the overlay of the filtered fixed image with the moving
+    # image under H and under the ground truth
+    warped = A.warp_image(img_mov, H, xf, yf)
+    fused = A.fuse_images(fix_filtered, img_mov, H, gth, channel=0, local=True)
+    if out_name is not None:
+        write_image(out_name, warped, out)
+        write_image(fused_name, fused, out)
     return dict(best=best, homography=H, parameters=back, covariance=covar.reshape(8, 8), seconds=elapsed,
                 argmin=(v, i), fixed_filtered=fix_filtered, moving_filtered=mov_filtered, scale_factor=(sfx, sfy),
-                resized=resized)
+                resized=resized, warped=warped, fused=fused)
+
+
+def write_image(path, img, out=sys.stdout):
+    """The drivers' suffix rule (src/gpu/cudaImageFunctions.cuh:140-149): .png
+    through cgs_write_png, .jpg at quality 95 (through PIL), anything else
+    reported on stdout and not written."""
+    from . import api as A
+    low = path.lower()
+    if low.endswith(".png"):
+        A.write_png(path, img)
+    elif low.endswith(".jpg"):
+        from PIL import Image
+        Image.fromarray(img.tensor.cpu().numpy()).save(path, quality=95)
+    else:
+        out.write("Filename suffix has image format not recognized.\n")
 
 
 def ground_truth(path, fixed_name, sfx, sfy, default):
@@ -343,12 +371,11 @@ def main(argv=None):
     sys.stderr.write(f"Fused Output image filename is {args.fusedoutput}.\n")
     fixed, moved = load_gray(args.i_ref), load_gray(args.i_mov)
     r = run(fixed, moved, mod=args.mod, gt=args.gt, depth=args.depth, names=(args.i_ref, args.i_mov),
-            results_file=args.results_file, fused_name=args.fusedoutput)
+            results_file=args.results_file, fused_name=args.fusedoutput, out_name=args.output)
     if r["resized"] is not None:   # the reference writes the resized image (:540, :559)
-        from PIL import Image
+        from . import api as A
         name, img = r["resized"]
-        Image.fromarray(img.tensor.cpu().numpy()).save(name)
-    sys.stderr.write("output images (-o, -f): the image writers are not part of this build; skipped\n")
+        A.write_png(name, img)
     return 0
 
 

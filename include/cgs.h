@@ -166,6 +166,34 @@ cgs_status cgs_image_quantize(const cgs_image *img, float mod, void *stream);
  * host). */
 cgs_status cgs_image_resize(const cgs_image *in, const cgs_image *out, void *stream);
 
+/* transformImage (src/gpu/cudaImageFunctions.cuh:40-62; transformImage_local,
+ * src/cpu/multispectral_ImageMatcher.cu:181-203, has the same body), which
+ * writeTransformedImageToDisk (src/gpu/cudaImageFunctions.cuh:129-152) runs:
+ * every pixel (x, y) of `out` whose inverse-mapped point (the adjugate of H,
+ * h = h11 h12 h13 h21 h22 h23 h31 h32 with h33 = 1) lies in `in` takes
+ * valueAt_bilinear there, per channel (a uint8 out truncates); the other
+ * pixels keep what `out` holds (the reference zero-fills it first).  Equal
+ * channel counts, uint8 or float each, no aliasing.  Asynchronous. */
+cgs_status cgs_image_warp(const cgs_image *in, const cgs_image *out, const float *h, void *stream);
+
+/* fuseAlignedImages (local = 0, src/gpu/cudaImageFunctions.cuh:64-114) and
+ * fuseAlignedImages_local (local = 1, src/cpu/multispectral_ImageMatcher.cu:
+ * 115-179), which writeAlignedAndFusedImageToDisk(_local) run: a 3-channel
+ * overlay, channel 0 `mov` under est_h, 1 `ref` under the identity, 2 `mov`
+ * under true_h, each sampled at `channel`; elements whose point falls outside
+ * their source keep what `fused` holds.  The local form steps normalised
+ * coordinates in binary32 and writes by rounded target index, as the
+ * reference's loop does, including its overwrites.  Asynchronous for
+ * local = 0; local = 1 synchronises the stream (its step tables are staged
+ * from the host). */
+cgs_status cgs_image_fuse(const cgs_image *ref, const cgs_image *mov, const float *est_h, const float *true_h,
+                          int32_t channel, int32_t local, const cgs_image *fused, void *stream);
+
+/* The drivers' stbi_write_png (src/gpu/cudaImageFunctions.cuh:141-143, 167-169):
+ * writes host pixels (row-major, channels interleaved, 8-bit; 1 grey, 2 grey +
+ * alpha, 3 RGB, 4 RGBA) as a PNG file, the image data stored uncompressed. */
+cgs_status cgs_write_png(const char *path, const void *pixels, int32_t width, int32_t height, int32_t channels);
+
 /* CGS_ABI_VERSION of the library's build. */
 int32_t cgs_abi_version(void);
 

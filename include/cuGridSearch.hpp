@@ -8,6 +8,8 @@
 //   CudaTensor<P, D>        include/cudaTensor.cuh:108-361
 //   CudaImage<P, C>         include/cudaTensor.cuh:507-596
 //   CudaGridSearcher<F,G,D> src/gpu/cudaGridSearch.cuh:540-855
+//   write*ImageToDisk(_local) src/gpu/cudaImageFunctions.cuh:129-179,
+//                           src/cpu/multispectral_ImageMatcher.cu:205-255
 //   umbrella include        include/cuGridSearch.cuh:8-11
 // Containers stay shallow views over caller-allocated device memory
 // (hipMalloc(&x.data(), x.bytesSize())), as in the reference.
@@ -24,6 +26,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cctype>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
@@ -146,6 +149,110 @@ struct CudaImage : CudaTensor<precision, 3> {
         return cgs_image{this->_data, width(), height(), channels(), CgsPixType<precision>::value};
     }
 };
+
+// nv_ext::Vec<T, D> (include/nvVectorNd.h): the fixed-size vector the
+// reference's drivers pass homographies and grid points in.
+namespace nv_ext {
+template <typename T, unsigned int D>
+struct Vec {
+    T v[D] = {};
+    Vec() = default;
+    explicit Vec(const T (&a)[D]) {
+        for (unsigned i = 0; i < D; ++i) v[i] = a[i];
+    }
+    T &operator[](unsigned i) { return v[i]; }
+    const T &operator[](unsigned i) const { return v[i]; }
+};
+}  // namespace nv_ext
+
+// --------------------------------------------------------------- image files
+
+inline bool endsWithCaseInsensitive(const std::string &s, const std::string &suffix) {
+    if (s.size() < suffix.size()) return false;
+    for (size_t i = 0; i < suffix.size(); ++i) {
+        const char a = s[s.size() - suffix.size() + i], b = suffix[i];
+        if (std::tolower((unsigned char) a) != std::tolower((unsigned char) b)) return false;
+    }
+    return true;
+}
+
+// The drivers' suffix rule (src/gpu/cudaImageFunctions.cuh:140-149): .png is
+// written (cgs_write_png); .jpg is not (this build has no JPEG encoder) and is
+// reported; anything else prints the reference's message.
+inline void cgs_write_image_file(const std::string &name, const std::vector<uint8_t> &pixels, int w, int h, int c) {
+    if (endsWithCaseInsensitive(name, ".png"))
+        cgs_check(cgs_write_png(name.c_str(), pixels.data(), w, h, c), "stbi_write_png");
+    else if (endsWithCaseInsensitive(name, ".jpg"))
+        std::cout << "JPEG output is not part of this build; write a .png file instead." << std::endl;
+    else
+        std::cout << "Filename suffix has image format not recognized." << std::endl;
+}
+
+template <typename HV>
+inline void cgs_h8(const HV &H, float (&h)[8]) {
+    for (int k = 0; k < 8; ++k) h[k] = (float) H[k];
+}
+
+// writeTransformedImageToDisk (src/gpu/cudaImageFunctions.cuh:129-152): image
+// under H on a rowsf x colsf grid, zero where its point falls outside, as an
+// 8-bit file.  The _local twin (src/cpu/multispectral_ImageMatcher.cu:205-229)
+// runs the same kernel.
+template <typename pixType, uint32_t CHANNELS, typename HV>
+void writeTransformedImageToDisk(CudaImage<pixType, CHANNELS> image, int rowsf, int colsf, const HV &H,
+                                 std::string img_out_filename) {
+    CudaImage<uint8_t, CHANNELS> out((uint32_t) rowsf, (uint32_t) colsf);
+    cgs_check_hip(hipMalloc(&out.data(), out.bytesSize()), "hipMalloc");
+    float h[8];
+    cgs_h8(H, h);
+    const cgs_image iv = image.view(), ov = out.view();
+    cgs_status st = CGS_OK;
+    std::vector<uint8_t> host;
+    if (hipMemset(out.data(), 0, out.bytesSize()) == hipSuccess && (st = cgs_image_warp(&iv, &ov, h, nullptr)) == CGS_OK)
+        host = out.values();
+    (void) hipFree(out.data());
+    cgs_check(st, "writeTransformedImageToDisk");
+    cgs_write_image_file(img_out_filename, host, colsf, rowsf, (int) CHANNELS);
+}
+
+template <typename pixType, uint32_t CHANNELS, typename HV>
+void writeTransformedImageToDisk_local(CudaImage<pixType, CHANNELS> image, int rowsf, int colsf, const HV &H,
+                                       std::string img_out_filename) {
+    writeTransformedImageToDisk<pixType, CHANNELS>(image, rowsf, colsf, H, img_out_filename);
+}
+
+// writeAlignedAndFusedImageToDisk (src/gpu/cudaImageFunctions.cuh:154-179) and
+// its _local twin (src/cpu/multispectral_ImageMatcher.cu:231-255): the
+// 3-channel overlay of image_fix's size (fuseAlignedImages / _local).
+template <typename pixType, uint32_t CHANNELS, typename HV>
+void cgs_write_fused(CudaImage<pixType, CHANNELS> image_fix, CudaImage<pixType, CHANNELS> image_mov, const HV &est,
+                     const HV &truth, const std::string &name, int local) {
+    CudaImage<uint8_t, 3> fused((uint32_t) image_fix.height(), (uint32_t) image_fix.width());
+    cgs_check_hip(hipMalloc(&fused.data(), fused.bytesSize()), "hipMalloc");
+    float e[8], t[8];
+    cgs_h8(est, e);
+    cgs_h8(truth, t);
+    const cgs_image rv = image_fix.view(), mv = image_mov.view(), fv = fused.view();
+    cgs_status st = CGS_OK;
+    std::vector<uint8_t> host;
+    if (hipMemset(fused.data(), 0, fused.bytesSize()) == hipSuccess &&
+        (st = cgs_image_fuse(&rv, &mv, e, t, 0, local, &fv, nullptr)) == CGS_OK)
+        host = fused.values();
+    (void) hipFree(fused.data());
+    cgs_check(st, "writeAlignedAndFusedImageToDisk");
+    cgs_write_image_file(name, host, image_fix.width(), image_fix.height(), 3);
+}
+
+template <typename pixType, uint32_t CHANNELS, typename HV>
+void writeAlignedAndFusedImageToDisk(CudaImage<pixType, CHANNELS> image_fix, CudaImage<pixType, CHANNELS> image_mov,
+                                     const HV &estimatedH, const HV &trueH, std::string img_fused_filename) {
+    cgs_write_fused<pixType, CHANNELS>(image_fix, image_mov, estimatedH, trueH, img_fused_filename, 0);
+}
+
+template <typename pixType, uint32_t CHANNELS, typename HV>
+void writeAlignedAndFusedImageToDisk_local(CudaImage<pixType, CHANNELS> image_fix, CudaImage<pixType, CHANNELS> image_mov,
+                                           const HV &estimatedH, const HV &trueH, std::string img_fused_filename) {
+    cgs_write_fused<pixType, CHANNELS>(image_fix, image_mov, estimatedH, trueH, img_fused_filename, 1);
+}
 
 // --------------------------------------------------------------------- grid
 

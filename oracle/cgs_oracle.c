@@ -1114,3 +1114,88 @@ done:
     free(fx); free(fy); free(sx); free(sy); free(wx); free(wy); free(hb);
     return rc;
 }
+
+/* ----------------------------------------------------------- image writers
+ * The output images of the reference's drivers, restated literally (one pass
+ * in the reference's loop order, so a later write overwrites an earlier one
+ * exactly as in its single-thread kernels):
+ *   transformImage    src/gpu/cudaImageFunctions.cuh:40-62 (transformImage_local,
+ *                     src/cpu/multispectral_ImageMatcher.cu:181-203, has the same
+ *                     body)
+ *   fuseAlignedImages src/gpu/cudaImageFunctions.cuh:64-114
+ *   fuseAlignedImages_local  src/cpu/multispectral_ImageMatcher.cu:115-179
+ * The inverse map is the reference's adjugate expression of H, every product
+ * and sum rounded on its own (no contraction), the quotient an IEEE division;
+ * the in-image test is inImage on that quotient; samples are valueAt_bilinear
+ * (the declared lerps of orc_sample_bilinear); a uint8 output takes the
+ * sample truncated toward zero (the float-to-uint8 store of at<float>),
+ * saturated to [0, 255]. */
+
+static void orc_inverse_point(const float *H, float x, float y, float *nx, float *ny) {
+    const float h11 = H[0], h12 = H[1], h13 = H[2], h21 = H[3], h22 = H[4], h23 = H[5], h31 = H[6], h32 = H[7];
+    const float w = (h21 * h32 - h22 * h31) * x + (h12 * h31 - h11 * h32) * y + h11 * h22 - h12 * h21;
+    *nx = ((h22 - h23 * h32) * x + (h13 * h32 - h12) * y + h12 * h23 - h13 * h22) / w;
+    *ny = ((h23 * h31 - h21) * x + (h11 - h13 * h31) * y + h13 * h21 - h11 * h23) / w;
+}
+
+static void orc_store(const orc_image *out, int64_t lin, float v) {
+    if (out->pixtype == ORC_PIX_U8) {
+        const float t = truncf(v);
+        ((uint8_t *) out->data)[lin] = (uint8_t) (!(t >= 0.0f) ? 0.0f : (t > 255.0f ? 255.0f : t));
+    } else {
+        ((float *) out->data)[lin] = v;
+    }
+}
+
+/* transformImage: out (rows x cols) pixels whose sample point lies in `in`
+ * get its bilinear value; the others keep what out holds. */
+void orc_image_warp(const orc_image *in, const orc_image *out, const float *H) {
+    const int C = in->channels;
+    for (int x = 0; x < out->width; x++)
+        for (int y = 0; y < out->height; y++) {
+            float nx, ny;
+            orc_inverse_point(H, (float) x, (float) y, &nx, &ny);
+            if (!in_image(in, ny, nx)) continue;
+            for (int c = 0; c < C; c++)
+                orc_store(out, ((int64_t) y * out->width + x) * C + c, bilinear_in(in, ny, nx, c));
+        }
+}
+
+/* fuseAlignedImages (local = 0) / fuseAlignedImages_local (local = 1) into a
+ * 3-channel image: channel 0 the moving image under est, 1 the reference under
+ * the identity, 2 the moving image under truth, each sampled at `channel`. */
+void orc_image_fuse(const orc_image *ref, const orc_image *mov, const float *est, const float *truth, int channel,
+                    int local, const orc_image *fused) {
+    const float ident[8] = {1, 0, 0, 0, 1, 0, 0, 0};
+    const int columns = fused->width, rows = fused->height;
+    const int64_t total = (int64_t) columns * rows * 3;
+    for (int c = 0; c < 3; c++) {
+        const float *H = c == 0 ? est : (c == 1 ? ident : truth);
+        const orc_image *src = c == 1 ? ref : mov;
+        if (!local) {
+            for (int x = 0; x < columns; x++)
+                for (int y = 0; y < rows; y++) {
+                    float nx, ny;
+                    orc_inverse_point(H, (float) x, (float) y, &nx, &ny);
+                    if (in_image(src, ny, nx))
+                        orc_store(fused, ((int64_t) y * columns + x) * 3 + c, bilinear_in(src, ny, nx, channel));
+                }
+            continue;
+        }
+        /* normalised coordinates stepped in binary32 from -0.5 by 1/size; the
+         * sample point scaled back by the moving image's size in binary64; the
+         * target pixel round(size * (t + 0.5)), its linear index possibly the
+         * next row's first pixel (past the end: not written) */
+        for (float x = -0.5f; x < 0.5f; x += 1.0f / (float) columns)
+            for (float y = -0.5f; y < 0.5f; y += 1.0f / (float) rows) {
+                float nx, ny;
+                orc_inverse_point(H, x, y, &nx, &ny);
+                nx = (float) (((double) nx + 0.5) * (double) mov->width);
+                ny = (float) (((double) ny + 0.5) * (double) mov->height);
+                const int tx = (int) round((double) columns * ((double) x + 0.5));
+                const int ty = (int) round((double) rows * ((double) y + 0.5));
+                const int64_t lin = ((int64_t) ty * columns + tx) * 3 + c;
+                if (lin < total && in_image(src, ny, nx)) orc_store(fused, lin, bilinear_in(src, ny, nx, channel));
+            }
+    }
+}

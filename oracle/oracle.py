@@ -81,6 +81,9 @@ def lib():
         L.orc_image_resize.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int]
         L.orc_image_resize.restype = ctypes.c_int
+        L.orc_image_warp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        L.orc_image_fuse.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                     ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -206,6 +209,21 @@ def image_resize(img, ow, oh):
     if lib().orc_image_resize(a.ctypes.data, a.shape[1], a.shape[0], out.ctypes.data, ow, oh, C) != 0:
         raise ValueError("bad resize sizes")
     return out
+
+
+def image_warp(img, H, out):
+    """transformImage of img into out (a numpy array it writes in place)."""
+    im, om = image(img), image(out)
+    lib().orc_image_warp(ctypes.byref(im), ctypes.byref(om), _f32p(np.ascontiguousarray(H, np.float32)))
+    return out
+
+
+def image_fuse(ref, mov, est, truth, channel, local, fused):
+    """fuseAlignedImages(_local) into fused, an (H, W, 3) uint8 array written in place."""
+    rm, mm, fm = image(ref), image(mov), image(fused)
+    lib().orc_image_fuse(ctypes.byref(rm), ctypes.byref(mm), _f32p(np.ascontiguousarray(est, np.float32)),
+                         _f32p(np.ascontiguousarray(truth, np.float32)), int(channel), int(local), ctypes.byref(fm))
+    return fused
 
 
 def sample(img, y, x, sampler, c=0):

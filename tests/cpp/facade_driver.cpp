@@ -2,7 +2,9 @@
 // the way the reference's drivers are written against its headers
 // (src/gpu/cudaGridSearch_main_mi.cu:300-345, src/gpu/cudaGridSearch_example1.cu:97-143).
 //
-// usage: facade_driver <spec.txt> <moving.raw> <fixed.raw> <costs_out.bin>
+// usage: facade_driver <spec.txt> <moving.raw> <fixed.raw> <costs_out.bin> [png_prefix]
+// With png_prefix it also writes the output images <prefix>_out.png,
+// <prefix>_fused.png and <prefix>_fused_local.png.
 // spec.txt holds whitespace-separated fields:
 //   functor sampler pixtype width height channels tie bins start[8] end[8] count[8]
 // functor: ncc | mi | sad | sad_sum | sqd ; sampler: nearest | bilinear ; pixtype: u8 | f32
@@ -24,7 +26,7 @@ static CgsFunctor functor_named(const std::string &n) {
 
 template <typename pixel_precision>
 static int run(std::ifstream &spec, const std::string &fn, const std::string &samp, int W, int H, int tie, int bins,
-               const char *mov_path, const char *fix_path, const char *out_path) {
+               const char *mov_path, const char *fix_path, const char *out_path, const char *png_prefix) {
     constexpr unsigned grid_dimension = 8;
     typedef float grid_precision;
     typedef float func_precision;
@@ -74,6 +76,18 @@ static int run(std::ifstream &spec, const std::string &fn, const std::string &sa
     double covar_trace = 0.0;
     for (unsigned d = 0; d < grid_dimension; ++d) covar_trace += covar[d * grid_dimension + d];
 
+    if (png_prefix) {
+        // the drivers' output images (src/gpu/cudaGridSearch_main.cu:312-316), here
+        // under a fixed shift by (2, 1) and the identity
+        const float shift_vals[8] = {1, 0, 2, 0, 1, 1, 0, 0}, ident_vals[8] = {1, 0, 0, 0, 1, 0, 0, 0};
+        nv_ext::Vec<float, 8> shift(shift_vals), ident(ident_vals);
+        const std::string pre(png_prefix);
+        writeTransformedImageToDisk<pixel_precision, 1>(image_mov, H, W, shift, pre + "_out.png");
+        writeAlignedAndFusedImageToDisk<pixel_precision, 1>(image_fix, image_mov, shift, ident, pre + "_fused.png");
+        writeAlignedAndFusedImageToDisk_local<pixel_precision, 1>(image_fix, image_mov, ident, ident,
+                                                                  pre + "_fused_local.png");
+    }
+
     const std::vector<func_precision> costs = func_values.values();
     std::ofstream(out_path, std::ios::binary).write((const char *) costs.data(), costs.size() * sizeof(float));
     std::printf("{\"fused_index\": %d, \"fused_value\": %.9g, \"extrema_index\": %d, \"extrema_value\": %.9g, "
@@ -88,7 +102,7 @@ static int run(std::ifstream &spec, const std::string &fn, const std::string &sa
 }
 
 int main(int argc, char **argv) {
-    if (argc != 5) {
+    if (argc != 5 && argc != 6) {
         std::cerr << "usage: facade_driver <spec.txt> <moving.raw> <fixed.raw> <costs_out.bin>\n";
         return 2;
     }
@@ -97,8 +111,9 @@ int main(int argc, char **argv) {
     int W, H, C, tie, bins;
     spec >> fn >> samp >> pix >> W >> H >> C >> tie >> bins;
     try {
-        if (pix == "u8") return run<uint8_t>(spec, fn, samp, W, H, tie, bins, argv[2], argv[3], argv[4]);
-        return run<float>(spec, fn, samp, W, H, tie, bins, argv[2], argv[3], argv[4]);
+        const char *png = argc == 6 ? argv[5] : nullptr;
+        if (pix == "u8") return run<uint8_t>(spec, fn, samp, W, H, tie, bins, argv[2], argv[3], argv[4], png);
+        return run<float>(spec, fn, samp, W, H, tie, bins, argv[2], argv[3], argv[4], png);
     } catch (const cgs_error &e) {
         std::cerr << "cgs_error: " << e.what() << "\n";
         return 1;
