@@ -21,6 +21,8 @@ SAMPLER_NEAREST, SAMPLER_BILINEAR = 0, 1
 PARAM_CENTRED, PARAM_NONCENTRED, PARAM_RAW_H, PARAM_TRANSLATION, PARAM_NORM = range(5)
 TIE_LOWEST, TIE_REFERENCE = 0, 1
 PIX_U8, PIX_F32 = 0, 1
+DT_F32, DT_F64, DT_I32, DT_U8 = 0, 1, 2, 3
+OP_ADD, OP_SUB, OP_MUL = 0, 1, 2
 MAX_DIM = 8
 ABI_VERSION = 6          # CGS_ABI_VERSION of include/cgs.h these layouts follow
 FLAG_TIME_KERNEL = 1
@@ -96,6 +98,11 @@ SIGNATURES = {
                                         ctypes.c_void_p]),
     "cgs_image_quantize": (ctypes.c_int, [_P(Image), ctypes.c_float, ctypes.c_void_p]),
     "cgs_image_resize": (ctypes.c_int, [_P(Image), _P(Image), ctypes.c_void_p]),
+    "cgs_tensor_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p]),
+    "cgs_tensor_binary": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                         ctypes.c_void_p]),
+    "cgs_matrix_multiply": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]),
     "cgs_image_warp": (ctypes.c_int, [_P(Image), _P(Image), _P(ctypes.c_float), ctypes.c_void_p]),
     "cgs_image_fuse": (ctypes.c_int, [_P(Image), _P(Image), _P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_int32,
                                       ctypes.c_int32, _P(Image), ctypes.c_void_p]),

@@ -237,6 +237,33 @@ cgs_status cgs_search(const cgs_grid *grid, const cgs_image *moved, const cgs_im
 cgs_status cgs_find_extrema(const float *values, int64_t n, int32_t tie_mode, cgs_argmin *out,
                             void *stream);
 
+/* Element types of the tensor utilities below. */
+typedef enum { CGS_DT_F32 = 0, CGS_DT_F64 = 1, CGS_DT_I32 = 2, CGS_DT_U8 = 3 } cgs_dtype;
+
+/* Element-wise operations of CudaTensor::operator+= / -= / *= (transform with
+ * the reference's lambdas, include/cudaTensor.cuh:322-360, kernel
+ * transformProcess include/cudaTensorKernels.cuh:341-351). */
+typedef enum { CGS_OP_ADD = 0, CGS_OP_SUB = 1, CGS_OP_MUL = 2 } cgs_tensor_op;
+
+/* CudaTensor::fill (include/cudaTensor.cuh:197-206, kernel fillProcess
+ * include/cudaTensorKernels.cuh:230-237): n elements of `data` (device) set to
+ * value converted to the element type.  Asynchronous. */
+cgs_status cgs_tensor_fill(void *data, int64_t n, int32_t dtype, double value, void *stream);
+
+/* a[i] = a[i] op b[i] for i < n in the element type's arithmetic (a uint8
+ * result wraps modulo 256, as the reference's store does).  Device arrays,
+ * asynchronous. */
+cgs_status cgs_tensor_binary(int32_t op, void *a, const void *b, int64_t n, int32_t dtype, void *stream);
+
+/* CudaMatrix::multiply (include/cudaTensor.cuh:474-496, kernel multiplyProcess
+ * include/cudaTensorKernels.cuh:396-440): C (m x n) = A (m x k) * B (k x n),
+ * row-major device matrices of CGS_DT_F32 or CGS_DT_F64.  Each element is the
+ * reference's dot product in its order, k ascending from 0, with each step a
+ * fused multiply-add (nvcc contracts the reference's Csub += a * b).
+ * Asynchronous. */
+cgs_status cgs_matrix_multiply(const void *A, const void *B, void *C, int32_t m, int32_t k, int32_t n, int32_t dtype,
+                               void *stream);
+
 /* Evaluates a single grid point and returns its cost and, for the MI costs,
  * the bins x bins joint histogram (row = fixed-image bin, column = moving
  * bin) in hist_out (host, may be NULL).  Counts are exact integers.

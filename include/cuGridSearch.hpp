@@ -26,12 +26,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cctype>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <typeinfo>
 #include <vector>
 
 #include "cgs.h"
@@ -91,6 +93,13 @@ template <typename T> struct CgsPixType;
 template <> struct CgsPixType<uint8_t> { static constexpr cgs_pixtype value = CGS_PIX_U8; };
 template <> struct CgsPixType<float> { static constexpr cgs_pixtype value = CGS_PIX_F32; };
 
+// Element type of a tensor for the C-ABI's utilities (-1: host-side fallback).
+template <typename T> struct CgsDType { static constexpr int value = -1; };
+template <> struct CgsDType<float> { static constexpr int value = CGS_DT_F32; };
+template <> struct CgsDType<double> { static constexpr int value = CGS_DT_F64; };
+template <> struct CgsDType<int32_t> { static constexpr int value = CGS_DT_I32; };
+template <> struct CgsDType<uint8_t> { static constexpr int value = CGS_DT_U8; };
+
 // CudaTensor<P, D> (include/cudaTensor.cuh:108-361): a shallow view over
 // caller-allocated device memory; axis 0 varies fastest.
 template <typename precision, unsigned int D>
@@ -125,7 +134,39 @@ struct CudaTensor {
         return h;
     }
 
-    void fill(precision value) { setValuesFromVector(std::vector<precision>(size(), value)); }
+    // include/cudaTensor.cuh:197-206 (fillProcess)
+    void fill(precision value) {
+        if constexpr (CgsDType<precision>::value >= 0)
+            cgs_check(cgs_tensor_fill(_data, _total_size, CgsDType<precision>::value, (double) value, nullptr), "fill");
+        else
+            setValuesFromVector(std::vector<precision>(size(), value));
+    }
+
+    // operator+= / -= / *= (include/cudaTensor.cuh:322-360: transform with the
+    // element-wise lambdas); a transform with an arbitrary device lambda has
+    // no C-ABI form and is not offered.
+    CudaTensor &operator+=(const CudaTensor &m) { return elementwise(CGS_OP_ADD, m); }
+    CudaTensor &operator-=(const CudaTensor &m) { return elementwise(CGS_OP_SUB, m); }
+    CudaTensor &operator*=(const CudaTensor &m) { return elementwise(CGS_OP_MUL, m); }
+
+    // operator= (include/cudaTensor.cuh:334-348): takes m's shape and copies its
+    // values device to device into this tensor's memory.
+    CudaTensor &copyFrom(const CudaTensor &m) {
+        for (unsigned d = 0; d < D; ++d) _dims[d] = m._dims[d];
+        _total_size = m._total_size;
+        cgs_check_hip(hipMemcpy(_data, m._data, m.bytesSize(), hipMemcpyDeviceToDevice), "operator=");
+        return *this;
+    }
+
+  private:
+    CudaTensor &elementwise(int op, const CudaTensor &m) {
+        static_assert(CgsDType<precision>::value >= 0, "element-wise operations take float, double, int32 or uint8");
+        if (m._total_size != _total_size) throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "tensor sizes differ");
+        cgs_check(cgs_tensor_binary(op, _data, m._data, _total_size, CgsDType<precision>::value, nullptr), "transform");
+        return *this;
+    }
+
+  public:
 
     // include/cudaTensor.cuh:215-265: global minimum and its int32 index (-1
     // when no value is below +inf); prints the reference's line.
@@ -135,6 +176,71 @@ struct CudaTensor {
         extrema_value = a.value;
         extrema_index1d = a.index;
         std::printf("\n Grid MIN value has idx: %d  value: %f\n", extrema_index1d, (double) extrema_value);
+    }
+};
+
+// CudaMatrix<P>(rows, columns) (include/cudaTensor.cuh:363-505): dims
+// {columns, rows}, row-major.
+template <typename precision>
+struct CudaMatrix : CudaTensor<precision, 2> {
+    CudaMatrix(uint32_t rows, uint32_t columns) : CudaTensor<precision, 2>({columns, rows}) {}
+    int32_t width() const { return this->_dims[0]; }
+    int32_t height() const { return this->_dims[1]; }
+
+    // include/cudaTensor.cuh:386-409: "Matrix name R x C elements of <type>" then one
+    // "{ a, b, ... }" line per row
+    void display(const std::string &name = "") const {
+        const std::vector<precision> h = this->values();
+        std::cout << "Matrix " << name << " " << height() << " x " << width() << " elements of "
+                  << typeid(precision).name() << "\n\n";
+        for (int r = 0; r < height(); ++r) {
+            std::cout << "{ ";
+            for (int c = 0; c < width() - 1; ++c) std::cout << h[(size_t) r * width() + c] << ", ";
+            std::cout << h[(size_t) r * width() + width() - 1] << " }\n";
+        }
+        std::cout << std::endl;
+    }
+
+    // include/cudaTensor.cuh:419-423: the first height() diagonal elements
+    void setDiagonal(const precision value) {
+        const std::vector<precision> v((size_t) height(), value);
+        cgs_check_hip(hipMemcpy2D(this->_data, (width() + 1) * sizeof(precision), v.data(), sizeof(precision),
+                                  sizeof(precision), (size_t) std::min(height(), width()), hipMemcpyHostToDevice),
+                      "setDiagonal");
+    }
+
+    // include/cudaTensor.cuh:433-441
+    void setColumnFromVector(int column_index, const std::vector<precision> &values) {
+        if (column_index < 0 || column_index >= width() || (int) values.size() != height())
+            throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "setColumnFromVector: bad column or length");
+        cgs_check_hip(hipMemcpy2D(this->_data + column_index, width() * sizeof(precision), values.data(),
+                                  sizeof(precision), sizeof(precision), (size_t) height(), hipMemcpyHostToDevice),
+                      "setColumnFromVector");
+    }
+
+    // include/cudaTensor.cuh:451-456
+    void setRowFromVector(int row_index, const std::vector<precision> &values) {
+        if (row_index < 0 || row_index >= height() || (int) values.size() != width())
+            throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "setRowFromVector: bad row or length");
+        cgs_check_hip(hipMemcpy(this->_data + (size_t) row_index * width(), values.data(),
+                                width() * sizeof(precision), hipMemcpyHostToDevice),
+                      "setRowFromVector");
+    }
+
+    // include/cudaTensor.cuh:464-470: zeros with ones at every (width + 1)-th element
+    void setIdentity() {
+        std::vector<precision> v(this->size(), (precision) 0);
+        for (size_t d = 0; d < v.size(); d += (size_t) width() + 1) v[d] = (precision) 1;
+        this->setValuesFromVector(v);
+    }
+
+    // include/cudaTensor.cuh:474-496: C = A * B
+    static void multiply(const CudaMatrix &A, const CudaMatrix &B, CudaMatrix &C) {
+        if (A.width() != B.height() || C.height() != A.height() || C.width() != B.width())
+            throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "multiply: shapes do not chain");
+        cgs_check(cgs_matrix_multiply(A._data, B._data, C._data, A.height(), A.width(), B.width(),
+                                      CgsDType<precision>::value, nullptr),
+                  "multiply");
     }
 };
 

@@ -1199,3 +1199,27 @@ void orc_image_fuse(const orc_image *ref, const orc_image *mov, const float *est
             }
     }
 }
+
+/* ------------------------------------------------------- matrix product
+ * CudaMatrix::multiply (include/cudaTensor.cuh:474-496, kernel multiplyProcess
+ * include/cudaTensorKernels.cuh:396-440): the thread of C[r][c] accumulates
+ * Csub += As[ty][k] * Bs[k][tx] over k ascending in 4-wide tiles from 0, nvcc
+ * contracting each step into one fma; the zero-padded tile tails add exact
+ * zeros, so the value is the plain fma chain over k. */
+void orc_matrix_multiply_f32(const float *A, const float *B, float *C, int m, int k, int n) {
+    for (int r = 0; r < m; r++)
+        for (int c = 0; c < n; c++) {
+            float acc = 0.0f;
+            for (int j = 0; j < k; j++) acc = fmaf(A[(int64_t) r * k + j], B[(int64_t) j * n + c], acc);
+            C[(int64_t) r * n + c] = acc;
+        }
+}
+
+void orc_matrix_multiply_f64(const double *A, const double *B, double *C, int m, int k, int n) {
+    for (int r = 0; r < m; r++)
+        for (int c = 0; c < n; c++) {
+            double acc = 0.0;
+            for (int j = 0; j < k; j++) acc = fma(A[(int64_t) r * k + j], B[(int64_t) j * n + c], acc);
+            C[(int64_t) r * n + c] = acc;
+        }
+}

@@ -82,6 +82,8 @@ def lib():
                                        ctypes.c_int, ctypes.c_int]
         L.orc_image_resize.restype = ctypes.c_int
         L.orc_image_warp.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        for nm in ("orc_matrix_multiply_f32", "orc_matrix_multiply_f64"):
+            getattr(L, nm).argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 3
         L.orc_image_fuse.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
                                      ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         _lib = L
@@ -224,6 +226,16 @@ def image_fuse(ref, mov, est, truth, channel, local, fused):
     lib().orc_image_fuse(ctypes.byref(rm), ctypes.byref(mm), _f32p(np.ascontiguousarray(est, np.float32)),
                          _f32p(np.ascontiguousarray(truth, np.float32)), int(channel), int(local), ctypes.byref(fm))
     return fused
+
+
+def matrix_multiply(A, B):
+    """CudaMatrix::multiply of float32 or float64 row-major matrices."""
+    A = np.ascontiguousarray(A)
+    B = np.ascontiguousarray(B, A.dtype)
+    C = np.zeros((A.shape[0], B.shape[1]), A.dtype)
+    f = lib().orc_matrix_multiply_f32 if A.dtype == np.float32 else lib().orc_matrix_multiply_f64
+    f(A.ctypes.data, B.ctypes.data, C.ctypes.data, A.shape[0], A.shape[1], B.shape[1])
+    return C
 
 
 def sample(img, y, x, sampler, c=0):

@@ -3,6 +3,7 @@
 // (src/gpu/cudaGridSearch_main_mi.cu:300-345, src/gpu/cudaGridSearch_example1.cu:97-143).
 //
 // usage: facade_driver <spec.txt> <moving.raw> <fixed.raw> <costs_out.bin> [png_prefix]
+//        facade_driver --matrix   (the CudaMatrix / CudaTensor utilities)
 // With png_prefix it also writes the output images <prefix>_out.png,
 // <prefix>_fused.png and <prefix>_fused_local.png.
 // spec.txt holds whitespace-separated fields:
@@ -101,7 +102,46 @@ static int run(std::ifstream &spec, const std::string &fn, const std::string &sa
     return 0;
 }
 
+// --matrix: the CudaMatrix / CudaTensor utilities on fixed small operands;
+// prints one JSON line with the results.
+static int matrix_selftest() {
+    CudaMatrix<float> A(3, 4), B(4, 2), C(3, 2);
+    cgs_check_hip(hipMalloc(&A.data(), A.bytesSize()), "hipMalloc");
+    cgs_check_hip(hipMalloc(&B.data(), B.bytesSize()), "hipMalloc");
+    cgs_check_hip(hipMalloc(&C.data(), C.bytesSize()), "hipMalloc");
+    A.setIdentity();
+    A.setDiagonal(2.0f);
+    A.setRowFromVector(2, {1.0f, -2.0f, 3.0f, 0.5f});
+    A.setColumnFromVector(3, {4.0f, 5.0f, 6.0f});
+    B.setValuesFromVector({1, 2, 3, 4, 5, 6, 7, 8});
+    CudaMatrix<float>::multiply(A, B, C);
+    C.display("C");
+    CudaTensor<float, 2> T({2u, 3u});
+    cgs_check_hip(hipMalloc(&T.data(), T.bytesSize()), "hipMalloc");
+    T.copyFrom(C);
+    T += C;   // 2C
+    T *= C;   // 2C^2
+    T -= C;   // 2C^2 - C
+    CudaTensor<uint8_t, 1> u({5u});
+    cgs_check_hip(hipMalloc(&u.data(), u.bytesSize()), "hipMalloc");
+    u.fill(200);
+    u += u;   // 400 mod 256
+    const std::vector<float> a = A.values(), c = C.values(), t = T.values();
+    const std::vector<uint8_t> uv = u.values();
+    auto list = [](const auto &v) {
+        std::string r = "[";
+        for (size_t i = 0; i < v.size(); ++i) r += (i ? ", " : "") + std::to_string((double) v[i]);
+        return r + "]";
+    };
+    std::printf("{\"A\": %s, \"C\": %s, \"T\": %s, \"u\": %s}\n", list(a).c_str(), list(c).c_str(),
+                list(t).c_str(), list(uv).c_str());
+    for (void *p : {(void *) A.data(), (void *) B.data(), (void *) C.data(), (void *) T.data(), (void *) u.data()})
+        (void) hipFree(p);
+    return 0;
+}
+
 int main(int argc, char **argv) {
+    if (argc == 2 && std::string(argv[1]) == "--matrix") return matrix_selftest();
     if (argc != 5 && argc != 6) {
         std::cerr << "usage: facade_driver <spec.txt> <moving.raw> <fixed.raw> <costs_out.bin>\n";
         return 2;
