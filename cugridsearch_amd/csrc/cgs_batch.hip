@@ -153,7 +153,7 @@ int batch_kind(int cost) {
 
 template <int KIND, int G>
 hipError_t launch_kind(cgs_batch *b, hipStream_t s) {
-    const size_t lds = KIND == kBatchMI ? ((size_t) G * b->max_bins * b->max_bins + 64) * sizeof(uint32_t) : 0;
+    const size_t lds = KIND == kBatchMI ? mi_hist_words(G, b->max_bins) * sizeof(uint32_t) : 0;
     const int blocks = std::max(1, b->cus * b->tiles.wg_per_cu);
     hipLaunchKernelGGL((batch_kernel<KIND, G>), dim3(blocks), dim3(kThreads), lds, s,
                        (const BatchDesc *) b->descs.ptr + b->kind_begin[KIND], b->kind_count[KIND],

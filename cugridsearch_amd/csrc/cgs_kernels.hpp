@@ -46,6 +46,14 @@ struct SearchParams {
 
 constexpr int kThreads = 256;
 
+// LDS words of the MI kernel's joint histograms: GPT histograms of bins rows,
+// each row padded to bins + 1 counters (a column of the histogram then spreads
+// over the LDS banks instead of hitting one), then 64 per-lane dummy counters
+// for samples outside the image.
+__host__ __device__ inline size_t mi_hist_words(int gpt, int bins) {
+    return (size_t) gpt * bins * (bins + 1) + 64;
+}
+
 // Elements of zero padding after the moving image: covers every tap the
 // samplers can address for a sample point in [0, W] x [0, H].
 __host__ __device__ inline size_t mov_pad_elems(int W, int C) { return (size_t) C * (2 * (size_t) W + 4); }
@@ -730,10 +738,10 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
     const __amdgpu_buffer_rsrc_t frsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *) fix, (short) 0, (int) ((size_t) fw * fh * sizeof(FT)), 0x00020000);
     const int full = fh / rpi;   // steps in which every lane's row is inside the image
-    const int bins = uniform(P.bins), B2 = bins * bins;
+    const int bins = uniform(P.bins), RS = bins + 1, HS = bins * RS;   // rows padded to bins + 1 counters
     const bool ceil_taps = uniform(P.ceil_taps) != 0;
     const float fbins = (float) bins, bscale = fbins / 256.0f;
-    const uint32_t dummy = (uint32_t) (GPT * B2 + lane);
+    const uint32_t dummy = (uint32_t) (GPT * HS + lane);
 
     float B[GPT][3];
 #pragma unroll
@@ -762,7 +770,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
         for (int it = 0; it < steps; ++it, foff += fstep, yf += (float) rpi) {
             const unsigned long long vmask = it < full ? valid_mask : tail_mask;
             const float f = load_fix<FT>(frsrc, foff);   // 0 on lanes off the image
-            const uint32_t rowbin = (uint32_t) (mi_bin_fast<POW2>(f, bins, fbins, bscale) * bins);
+            const uint32_t rowbin = (uint32_t) (mi_bin_fast<POW2>(f, bins, fbins, bscale) * RS);
             unsigned long long in[GPT];
             float ax[GPT], ay[GPT];
             float4 q[GPT];
@@ -815,7 +823,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 const float top = fmaf(ax[g], tr, (1.0f - ax[g]) * q[g].x);
                 const float bottom = fmaf(ax[g], br, (1.0f - ax[g]) * bl);
                 const float v = fmaf(ay[g], bottom, (1.0f - ay[g]) * top);
-                const uint32_t h = (uint32_t) (g * B2) + rowbin + (uint32_t) mi_bin_fast<POW2>(v, bins, fbins, bscale);
+                const uint32_t h = (uint32_t) (g * HS) + rowbin + (uint32_t) mi_bin_fast<POW2>(v, bins, fbins, bscale);
                 atomicAdd(&hist[mask_select(in[g], dummy, h)], 1u);
             }
         }
@@ -825,18 +833,19 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
 // calcMIstream's entropy step (src/gpu/cudaErrorFunction_miStreams.cuh:306-365)
 // in binary64 for grid point g of the workgroup; n = in-image sample count.
 // Every thread calls it; thread 0 returns the cost.
-__device__ inline double mi_entropy_cost(const uint32_t *h, int bins, double n, int cost, uint32_t *s_px, uint32_t *s_py,
-                                  double *s_lpx, double *s_lpy, double *s_scr) {
+// h: bins rows of rs >= bins counters (row = fixed-image bin).
+__device__ inline double mi_entropy_cost(const uint32_t *h, int bins, int rs, double n, int cost, uint32_t *s_px,
+                                  uint32_t *s_py, double *s_lpx, double *s_lpy, double *s_scr) {
     const int tid = threadIdx.x;
     const int B2 = bins * bins;
     if (tid < bins) {
         uint32_t r = 0;
-        for (int j = 0; j < bins; ++j) r += h[tid * bins + j];
+        for (int j = 0; j < bins; ++j) r += h[tid * rs + j];
         s_px[tid] = r;
     } else if (tid < 2 * bins) {
         uint32_t r = 0;
         const int col = tid - bins;
-        for (int i = 0; i < bins; ++i) r += h[i * bins + col];
+        for (int i = 0; i < bins; ++i) r += h[i * rs + col];
         s_py[col] = r;
     }
     __syncthreads();
@@ -854,7 +863,7 @@ __device__ inline double mi_entropy_cost(const uint32_t *h, int bins, double n, 
     double mi = 0.0;
     for (int j = tid; j < B2; j += kThreads) {
         const int i = j / bins, k = j - i * bins;
-        const double pxy = h[j] / n, pxi = s_px[i] / n, pyj = s_py[k] / n;
+        const double pxy = h[i * rs + k] / n, pxi = s_px[i] / n, pyj = s_py[k] / n;
         if (pxy > 0 && pxi > 0 && pyj > 0) mi += pxy * (log2(pxy) - (s_lpx[i] + s_lpy[k]));
     }
     const double hx = block_sum(tid < bins ? hxy : 0.0, s_scr);
@@ -866,14 +875,14 @@ __device__ inline double mi_entropy_cost(const uint32_t *h, int bins, double n, 
     return r;
 }
 
-// One workgroup's GPT grid points with the MI sweep; hist: GPT * bins^2 + 64
-// words of LDS.
+// One workgroup's GPT grid points with the MI sweep; hist: mi_hist_words(GPT,
+// bins) words of LDS.
 template <typename MT, typename FT, int GPT>
 __device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, BlockShared<GPT> &S, uint32_t *hist) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int B2 = P.bins * P.bins;
+    const int B2 = P.bins * P.bins, RS = P.bins + 1, HS = P.bins * RS;
     block_coefficients<GPT>(P, base, S.coef);
-    for (int i = tid; i < GPT * B2 + 64; i += kThreads) hist[i] = 0u;
+    for (int i = tid; i < (int) mi_hist_words(GPT, P.bins); i += kThreads) hist[i] = 0u;
     __syncthreads();
     bool fastdiv = true;
 #pragma unroll
@@ -891,11 +900,11 @@ __device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, Bl
     __syncthreads();
     for (int g = 0; g < GPT; ++g) {
         const double n = S.red[0][g][3] + S.red[1][g][3] + S.red[2][g][3] + S.red[3][g][3];
-        const double r = mi_entropy_cost(hist + g * B2, P.bins, n, P.cost, S.px, S.py, S.lpx, S.lpy, S.scr);
+        const double r = mi_entropy_cost(hist + g * HS, P.bins, RS, n, P.cost, S.px, S.py, S.lpx, S.lpy, S.scr);
         if (tid == 0) S.cost[g] = (float) r;
     }
     if (P.hist_dump && base == 0)
-        for (int j = tid; j < B2; j += kThreads) P.hist_dump[j] = hist[j];
+        for (int j = tid; j < B2; j += kThreads) P.hist_dump[j] = hist[(j / P.bins) * RS + j % P.bins];
     __syncthreads();
     if (tid == 0) emit<GPT>(P, S.coef, S.cost);
     __syncthreads();

@@ -154,7 +154,7 @@ static hipError_t dispatch_fast(const SearchParams &P, int fpix, int gpt, hipStr
 template <typename MT, typename FT, int G>
 static hipError_t mi_launch(const SearchParams &P, hipStream_t stream) {
     const int blocks = (P.grid.n + G - 1) / G;
-    const size_t lds = ((size_t) G * P.bins * P.bins + 64) * sizeof(uint32_t);
+    const size_t lds = mi_hist_words(G, P.bins) * sizeof(uint32_t);
     hipLaunchKernelGGL((search_mi_kernel<MT, FT, G>), dim3(blocks), dim3(kThreads), lds, stream, P);
     return hipGetLastError();
 }
