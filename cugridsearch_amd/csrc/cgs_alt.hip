@@ -184,18 +184,19 @@ __device__ __forceinline__ float row_value(const T *img, float X, int row_off, i
 // MI of a wavefront's joint histogram h (row = fixed bin) over n samples,
 // binary64: mi = sum pxy (log2 pxy - (log2 px + log2 py)), hx / hy the
 // marginal entropies.  Every lane gets the totals.  Lane k handles column k.
-__device__ __forceinline__ void wave_mi(const uint32_t *h, int bins, double n, double *mi_out, double *hx_out,
+// h: bins rows of rs >= bins counters (row = fixed-image bin).
+__device__ __forceinline__ void wave_mi(const uint32_t *h, int bins, int rs, double n, double *mi_out, double *hx_out,
                                         double *hy_out) {
     const int lane = threadIdx.x & 63;
     double px = 0.0, py = 0.0;
     if (lane < bins) {
-        uint32_t rs = 0, cs = 0;
+        uint32_t rsum = 0, csum = 0;
         for (int j = 0; j < bins; ++j) {
-            rs += h[lane * bins + j];
-            cs += h[j * bins + lane];
+            rsum += h[lane * rs + j];
+            csum += h[j * rs + lane];
         }
-        px = rs / n;
-        py = cs / n;
+        px = rsum / n;
+        py = csum / n;
     }
     const double lpx = px > 0 ? log2(px) : 0.0, lpy = py > 0 ? log2(py) : 0.0;
     double hx = px > 0 ? -(px * lpx) : 0.0, hy = py > 0 ? -(py * lpy) : 0.0;
@@ -203,7 +204,7 @@ __device__ __forceinline__ void wave_mi(const uint32_t *h, int bins, double n, d
     for (int i = 0; i < bins; ++i) {
         const double lpx_i = __shfl(lpx, i), px_i = __shfl(px, i);
         if (lane < bins) {
-            const double pxy = h[i * bins + lane] / n;
+            const double pxy = h[i * rs + lane] / n;
             if (pxy > 0 && px_i > 0 && py > 0) mi += pxy * (log2(pxy) - (lpx_i + lpy));
         }
     }
@@ -410,6 +411,11 @@ __global__ __launch_bounds__(kThreads) void alt_step_kernel(const AltParams P) {
 // of 64 rows in turn (wavefront w: blocks w, w + 4, ...) into one LDS joint
 // histogram.  LDS: bins^2 counters, then one binary64 moved-area sum per row
 // block (summed in block order).
+// LDS words of the joint histogram: bins rows padded to bins + 1 counters (a
+// column spreads over the LDS banks), rounded up to keep what follows 8-byte
+// aligned.
+__host__ __device__ inline int alt_hist_words(int bins) { return (bins * (bins + 1) + 1) & ~1; }
+
 __device__ __forceinline__ uint32_t *alt_lds_hist() {
     extern __shared__ uint32_t s_dyn[];
     return s_dyn;
@@ -422,10 +428,10 @@ __global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P
     __shared__ float s_cost;
     uint32_t *hist = alt_lds_hist();
     const int B2 = P.bins * P.bins;
-    double *s_mov = (double *) (hist + ((B2 + 1) & ~1));
+    double *s_mov = (double *) (hist + alt_hist_words(P.bins));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int32_t gi = (int32_t) blockIdx.x;
-    for (int i = threadIdx.x; i < B2; i += kThreads) hist[i] = 0u;
+    for (int i = threadIdx.x; i < alt_hist_words(P.bins); i += kThreads) hist[i] = 0u;
     __syncthreads();
     const auto mov = make_sampler<MT, ONE>(P.mq, P.mov, P.C, P.mw, P.mh);
     const auto fix = make_sampler<FT, ONE>(P.fq, P.fix, P.C, P.fw, P.fh);
@@ -451,7 +457,7 @@ __global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P
                     for (int ch = 0; ch < C; ++ch) {
                         const float mv = mov.at(nx, ny, ch);
                         const float fv = fix.at_row(lam, row, ch);
-                        atomicAdd(&hist[mi_bin(fv, bins) * bins + mi_bin(mv, bins)], 1u);
+                        atomicAdd(&hist[mi_bin(fv, bins) * (bins + 1) + mi_bin(mv, bins)], 1u);
                     }
                 });
             if (lane == j && nrow > 0) term = (double) nrow * (double) magr;
@@ -469,11 +475,11 @@ __global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P
         float cost = 123123.0f;
         if (alt_overlap((float) mov_area, (float) total, P, p[2])) {
             double mi, hx, hy;
-            wave_mi(hist, P.bins, (double) total * (double) C, &mi, &hx, &hy);
+            wave_mi(hist, P.bins, P.bins + 1, (double) total * (double) C, &mi, &hx, &hy);
             cost = (float) (-2.0 * mi / (hx + hy));
         }
         if (P.hist_dump && gi == 0)
-            for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[i];
+            for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[(i / P.bins) * (P.bins + 1) + i % P.bins];
         if (lane == 0) s_cost = cost;
     }
     __syncthreads();
@@ -491,7 +497,7 @@ __global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
     const int B2 = P.bins * P.bins;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int32_t gi = (int32_t) blockIdx.x;
-    for (int i = threadIdx.x; i < B2; i += kThreads) hist[i] = 0u;
+    for (int i = threadIdx.x; i < alt_hist_words(P.bins); i += kThreads) hist[i] = 0u;
     __syncthreads();
     const auto mov = make_sampler<MT, ONE>(P.mq, P.mov, P.C, P.mw, P.mh);
     const auto fix = make_sampler<FT, ONE>(P.fq, P.fix, P.C, P.fw, P.fh);
@@ -539,7 +545,7 @@ __global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
                     for (int ch = 0; ch < C; ++ch) {
                         const float fv = fix.at(li, ri, ch);
                         const float mv = mov.at(nx, ny, ch);
-                        atomicAdd(&hist[mi_bin(fv, bins) * bins + mi_bin(mv, bins)], 1u);
+                        atomicAdd(&hist[mi_bin(fv, bins) * (bins + 1) + mi_bin(mv, bins)], 1u);
                     }
                 }
                 const int nok = __popcll(m);
@@ -555,9 +561,9 @@ __global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
         const int32_t total = s_steps[0] + s_steps[1] + s_steps[2] + s_steps[3];
         const float fa = P.area[total < P.narea ? total : P.narea - 1];
         double mi = 0.0, hx, hy;
-        if (total > 0) wave_mi(hist, P.bins, (double) total * (double) C, &mi, &hx, &hy);
+        if (total > 0) wave_mi(hist, P.bins, P.bins + 1, (double) total * (double) C, &mi, &hx, &hy);
         if (P.hist_dump && gi == 0)
-            for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[i];
+            for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[(i / P.bins) * (P.bins + 1) + i % P.bins];
         if (lane == 0) s_cost = (float) (-mi * (double) fa);
     }
     __syncthreads();
@@ -601,8 +607,7 @@ static hipError_t tap_table(const cgs_image *im, void *dst, hipStream_t s) {
 
 template <typename MT, typename FT, bool ONE>
 static hipError_t alt_launch(const AltParams &P, hipStream_t s) {
-    const int B2 = P.bins * P.bins;
-    const size_t lds = sizeof(uint32_t) * (size_t) ((B2 + 1) & ~1) + sizeof(double) * (size_t) ((P.fh + 63) / 64);
+    const size_t lds = sizeof(uint32_t) * (size_t) alt_hist_words(P.bins) + sizeof(double) * (size_t) ((P.fh + 63) / 64);
     if (P.cost == CGS_COST_MI_ALT)
         hipLaunchKernelGGL((alt_norm_kernel<MT, FT, ONE>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
     else if (P.cost == CGS_COST_MI_STREAM_ALT)
@@ -669,7 +674,7 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
             : ((st = ws->mov.ensure(nm * esz_m)) != CGS_OK || (st = ws->fixp.ensure(nf * esz_f)) != CGS_OK))
         return st;
     if (hist_dump_host && mi && (st = ws->hist.ensure(B2 * sizeof(uint32_t))) != CGS_OK) return st;
-    if (mi && sizeof(uint32_t) * B2 + sizeof(double) * ((size_t) (fixed->height + 63) / 64) > 64 * 1024)
+    if (mi && sizeof(uint32_t) * alt_hist_words(o->mi_bins) + sizeof(double) * ((size_t) (fixed->height + 63) / 64) > 64 * 1024)
         return set_error(CGS_ERR_TOO_LARGE, "line-traversal MI: the histogram and row-block sums exceed 64 KiB of LDS");
     unsigned long long *key = (unsigned long long *) ws->key.ptr;
     hipLaunchKernelGGL(alt_key_kernel, dim3(1), dim3(64), 0, stream, key);
