@@ -165,7 +165,7 @@ def run_proposal(args):
     """Learned proposal + local refinement on held-out 256x256 pairs (6^8 NCC
     grid): pairs/s of the whole pipeline (input preparation, network,
     3^8 boxes, full-grid fallbacks) at its agreement with the plain
-    full-grid argmin, which is searched first as the judge and as the plain
+    full-grid argmin, which is searched first as the arbiter and as the plain
     pairs/s; plus the ablation without the network."""
     import torch
     from cugridsearch_amd import api as A
@@ -177,15 +177,15 @@ def run_proposal(args):
     net = cnn.ProposalNet.load(PROPOSAL_WEIGHTS, max_batch=256)
     g = P.grid()
     _, images = P.upload_pairs(range(n), heldout=True, keep_images=True)
-    judge, t_full = P.run_full_grid(images, g)
-    rep, _, _ = P.evaluate(net, images, g, threshold, judge=judge, modes=modes)
+    reference_argmin, t_full = P.run_full_grid(images, g)
+    rep, _, _ = P.evaluate(net, images, g, threshold, reference_argmin=reference_argmin, modes=modes)
     for _ in range(max(0, args.warmup - 1)):
         P.run_learned(net, images, g, threshold, modes=modes)
     times, agree, pts = [], [], []
     for _ in range(args.steps):
         res, t, _ = P.run_learned(net, images, g, threshold, modes=modes)
         times.append(t)
-        agree.append(float((res["index"] == judge).mean()))
+        agree.append(float((res["index"] == reference_argmin).mean()))
         pts.append(int(res["points"].sum()))
     dt = float(np.mean(times))
     flop = np.mean(pts) * 256 * 256 * FLOP_PER_PE[("bilinear", "ncc")]

@@ -2,7 +2,7 @@
 proposal -> grid-index conversion and the propose / refine / fall back
 pipeline.  No reference counterpart (the reference always searches the full
 grid, src/gpu/cudaGridSearch.cuh:736-765); the full grid search stays the
-judge of every result.
+arbiter of every result.
 
 Workload (256x256 FP32 pairs, NCC, centred parametrisation):
   pairs   synth.make_pair(seed, 256, sigma=4, perturb=32, float32,
@@ -415,7 +415,7 @@ def run_without_network(images, g, threshold, max_steps=8, starts=1):
 
 
 def run_full_grid(images, g):
-    """The judge: plain full-grid argmins, one search per pair."""
+    """The arbiter: plain full-grid argmins, one search per pair."""
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -466,13 +466,13 @@ def main(argv=None):
     return 0
 
 
-def evaluate(net, images, g, threshold, max_steps=8, judge=None, log=None, modes=("nearest",)):
+def evaluate(net, images, g, threshold, max_steps=8, reference_argmin=None, log=None, modes=("nearest",)):
     """Agreement with the full-grid argmin and pairs/s of: the plain full
-    grid (the judge), the learned pipeline, the same refinement from as many
+    grid (the arbiter), the learned pipeline, the same refinement from as many
     fixed starts (no network), and the network's nearest grid point alone."""
     n = len(images)
-    if judge is None:
-        judge, t_full = run_full_grid(images, g)
+    if reference_argmin is None:
+        reference_argmin, t_full = run_full_grid(images, g)
     else:
         t_full = None
     run_learned(net, images[:min(8, n)], g, threshold, max_steps, modes=modes)   # first-call set-up untimed
@@ -484,21 +484,21 @@ def evaluate(net, images, g, threshold, max_steps=8, judge=None, log=None, modes
     full_pts = g.numElements()
 
     def summary(r, t):
-        return dict(agree=float((r["index"] == judge).mean()), seconds=t, pairs_per_s=n / t,
+        return dict(agree=float((r["index"] == reference_argmin).mean()), seconds=t, pairs_per_s=n / t,
                     local=float((r["outcome"] == 0).mean()), fallback_boundary=float((r["outcome"] == 1).mean()),
                     fallback_threshold=float((r["outcome"] == 2).mean()), mean_steps=float(r["steps"].mean()),
                     points_per_pair=float(r["points"].mean()), points_vs_full=float(r["points"].mean() / full_pts))
 
     rep = dict(pairs=n, grid_points=full_pts, threshold=float(threshold), max_steps=max_steps, starts=list(modes),
                learned=summary(res, t_learn), without_network=summary(abl, t_abl),
-               network_alone_agree=float((cnn_idx == judge).mean()))
+               network_alone_agree=float((cnn_idx == reference_argmin).mean()))
     if t_full is not None:
         rep["full_grid"] = dict(seconds=t_full, pairs_per_s=n / t_full)
         rep["speedup_learned_vs_full"] = t_full / t_learn
         rep["speedup_without_network_vs_full"] = t_full / t_abl
     if log:
         log("[evaluate] " + str(rep))
-    return rep, res, judge
+    return rep, res, reference_argmin
 
 
 def saved_settings(path):
