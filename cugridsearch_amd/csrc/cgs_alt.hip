@@ -253,16 +253,21 @@ __device__ __forceinline__ void alt_emit(const AltParams &P, int32_t base, const
 // sample (entry L + W + 1 holds p[L], p[L+1], p[L+W], p[L+W+1], 0 outside);
 // the ceil taps are picked by selects.  Out-of-image lanes read past the
 // table, where the buffer's range check returns zeros.
+// The tap index fy * W + fx of an in-image sample is an integer below the
+// image's element count (< 2^24, checked when the image is validated), so
+// one binary32 fma forms it exactly.
 template <typename T>
 struct TabSampler {
     __amdgpu_buffer_rsrc_t rsrc;
     int w;
+    uint32_t base;   // byte offset of entry 0 of the image: (w + 1) entries
     float W, H;
     __device__ TabSampler(const void *table, int width, int height) {
         rsrc = __builtin_amdgcn_make_buffer_rsrc(
             (void *) uniform(table), (short) 0,
             (int) (((size_t) width * height + 2 * (size_t) width + 2) * TapTable<T>::kBytes), 0x00020000);
         w = width;
+        base = (uint32_t) ((width + 1) * TapTable<T>::kBytes);
         W = (float) width;
         H = (float) height;
     }
@@ -270,7 +275,8 @@ struct TabSampler {
         const bool in = X >= 0.0f && Y >= 0.0f && X < W && Y < H;
         const float fx = floorf(X), fy = floorf(Y);
         const float ax = X - fx, ay = Y - fy;
-        const uint32_t off = in ? (uint32_t) (((int) fy * w + (int) fx + w + 1) * TapTable<T>::kBytes) : kOffsetOut;
+        const uint32_t idx = (uint32_t) (int) fmaf(fy, W, fx);
+        const uint32_t off = in ? idx * TapTable<T>::kBytes + base : kOffsetOut;
         const float4 q = TapTable<T>::load(rsrc, off);
         const float tr = ax != 0.0f ? q.y : q.x;
         const float bl = ay != 0.0f ? q.z : q.x;
@@ -323,15 +329,16 @@ __device__ __forceinline__ int32_t walk_unit_row(float ls, float le, float FW, f
     float x0 = ls;
     int32_t nrow = 0;
     while (true) {
-        // x0 + 63 inside x0's binade [1, 2^23): every step is the exact x0 + j
-        const bool inside = x0 >= 1.0f && x0 + 63.0f < u2f((f2u(x0) & 0x7F800000u) + 0x00800000u);
-        const float lam = inside ? x0 + flane : advance_unit(x0, lane);
+        // x0 is the same in every lane: one branch per pass
+        const float lam = unit_pass_exact(x0)                 ? x0 + flane
+                          : x0 >= 32.0f && x0 < 8388608.0f ? unit_pass_cross(x0, lane)
+                                                            : advance_unit(x0, lane);
         const bool ok = lam < le && lam < FW;
-        const unsigned long long m = __ballot(ok);
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
         if (m == 0ull) break;
         if (ok) at(lam, lam * vx + px, lam * vy + py);
         nrow += __popcll(m);
-        if (!(m >> 63)) break;   // valid lanes form a prefix: the row ended in this pass
+        if (m != ~0ull) break;   // valid lanes form a prefix: the row ended in this pass
         x0 = lane_value(lam, 63) + 1.0f;
     }
     return nrow;
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
                 const int run = __builtin_amdgcn_readfirstlane(step_run(x0, sc, &a, &d, &U));
                 const float lam = lane == 0 ? x0 : (float) (a + lane * d) * U;
                 const bool ok = lane < run && lam < le && lam < 1;
-                const unsigned long long m = __ballot(ok);
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
                 if (m == 0ull) break;
                 if (ok) {
                     float nx = lam * vx + px, ny = lam * vy + py;

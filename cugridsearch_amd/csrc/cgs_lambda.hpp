@@ -62,6 +62,29 @@ CGS_HD inline float advance_unit(float x, int r) {
     return x;
 }
 
+// True when 64 steps of lambda = lambda + 1.0f from x0 are the exact x0 + j
+// (j = 0..63): x0 + 63 stays inside x0's binade [1, 2^23), so every add is
+// exact; or x0 is a non-negative integer (sign bit clear, so not -0) below
+// 2^24 - 63, so every partial sum is an integer below 2^24.
+CGS_HD inline bool unit_pass_exact(float x0) {
+    return (x0 >= 1.0f && x0 + 63.0f < u2f((f2u(x0) & 0x7F800000u) + 0x00800000u)) ||
+           (f2u(x0) < 0x4B7FFFC1u && x0 == __builtin_truncf(x0));
+}
+
+// Step j (0..63) of lambda = lambda + 1.0f from x0 when 32 <= x0 < 2^23.
+// Then x0 + 63 < 2B for the top B of x0's binade, so the pass leaves the
+// binade at most once: steps j <= t (t = ceil(B - x0) - 1; B - x0 is exact)
+// are the exact x0 + j, step t + 1 is the one rounded add y = (x0 + t) + 1,
+// and later steps are the exact y + (j - t - 1) inside [B, 2B), whose ulp is
+// at most 1.  t and y are the same for every lane.
+CGS_HD inline float unit_pass_cross(float x0, int j) {
+    const float B = u2f((f2u(x0) & 0x7F800000u) + 0x00800000u);
+    const float t = __builtin_ceilf(B - x0) - 1.0f;
+    const float y = (x0 + t) + 1.0f;
+    const float fj = (float) j;
+    return fj <= t ? x0 + fj : y + (fj - (t + 1.0f));
+}
+
 // A run of the lambda = lambda + s sequence (x >= 0 or -0, s > 0, x and s
 // normal) that a wavefront takes in one pass: returns n in [1, 64] such that
 // step j < n of the run is (float) (a + j * d) * U (step 0 is x itself), all

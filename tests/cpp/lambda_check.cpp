@@ -1,6 +1,9 @@
 // Checks the closed forms of cgs_lambda.hpp against the sequential binary32
-// loops they replace.  Prints "<mismatches> <comparisons>".
+// loops they replace.  Prints "<mismatches> <comparisons> <runs of the
+// 512-wide rows> <starts whose 64-step pass is exact> <starts of the
+// one-crossing closed form>".
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -25,6 +28,54 @@ int main() {
         for (int k = 0; k <= 600; ++k) {
             ++n;
             if (!same(cgs::advance_unit(x0, k), seq)) ++bad;
+            seq = seq + 1.0f;
+        }
+    }
+    // unit_pass_exact(x0): a pass of 64 steps is x0 + j exactly.  Starts:
+    // integers (0 and the top of the range included), binade interiors and
+    // edges, fractions below 1, -0; passes counts the starts it accepts
+    long passes = 0;
+    std::vector<float> pstarts = {0.0f, -0.0f, 1.0f, 63.0f, 64.0f, 0.5f, 1.5f, 1.25f, 16777152.0f, 16777153.0f,
+                                  16777151.0f, 8388607.5f, 8388608.0f, 4194303.5f, 1.0000001f, 2.0000002f};
+    for (int i = 0; i < 20000; ++i) {
+        const int e = (int) (rng() % 25);
+        const float v = u01(rng) * (float) (1 << e);
+        pstarts.push_back(v);
+        pstarts.push_back(__builtin_floorf(v));
+    }
+    for (float x0 : pstarts) {
+        if (!cgs::unit_pass_exact(x0)) continue;
+        ++passes;
+        float seq = x0;
+        for (int j = 0; j < 64; ++j) {
+            ++n;
+            if (!same(x0 + (float) j, seq)) ++bad;
+            seq = seq + 1.0f;
+        }
+    }
+    // unit_pass_cross(x0, j) for 32 <= x0 < 2^23: starts just below and above
+    // powers of two, binade interiors, integers
+    long crosses = 0;
+    std::vector<float> cstarts = {32.0f, 32.5f, 63.9999962f, 64.0f, 127.3f, 127.99999f, 8388607.5f, 8388544.25f,
+                                  4194303.75f, 65535.99f, 200.3f, 1000.7f};
+    for (int e = 5; e < 23; ++e)
+        for (int k = 1; k <= 70; ++k) {
+            const float top = (float) (1 << (e + 1));
+            cstarts.push_back(top - (float) k * 0.9f);
+            cstarts.push_back(std::nextafter(top - (float) k, 0.0f));
+            cstarts.push_back(top - (float) k + 0.5f);
+        }
+    for (int i = 0; i < 20000; ++i) {
+        const int e = 5 + (int) (rng() % 18);
+        cstarts.push_back((1.0f + u01(rng)) * (float) (1 << e));
+    }
+    for (float x0 : cstarts) {
+        if (!(x0 >= 32.0f && x0 < 8388608.0f)) continue;
+        ++crosses;
+        float seq = x0;
+        for (int j = 0; j < 64; ++j) {
+            ++n;
+            if (!same(cgs::unit_pass_cross(x0, j), seq)) ++bad;
             seq = seq + 1.0f;
         }
     }
@@ -63,6 +114,6 @@ int main() {
             }
         }
     }
-    std::printf("%ld %ld %ld\n", bad, n, runs512);
+    std::printf("%ld %ld %ld %ld %ld\n", bad, n, runs512, passes, crosses);
     return 0;
 }

@@ -14,8 +14,12 @@ def test_closed_form_lambda_matches_sequential_loops(tmp_path):
                     "-I", os.path.join(ROOT, "cugridsearch_amd", "csrc"), "-o", exe,
                     os.path.join(HERE, "cpp", "lambda_check.cpp")], check=True)
     out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
-    bad, n, runs512 = int(out[0]), int(out[1]), int(out[2])
+    bad, n, runs512, passes, crosses = (int(v) for v in out[:5])
     assert n > 2_000_000 and bad == 0, (bad, n)
+    # the integer starts of the exact-pass test are most of the 40016 starts
+    assert passes > 20000, passes
+    # the one-crossing closed form saw every start of its range
+    assert crosses > 20000, crosses
     # a 512-wide row (step 1/511, a tie in one binade) from lambda = 0 (twice:
     # +0 and -0) goes in a few runs, not one pass per step
     assert runs512 <= 2 * 16, runs512
