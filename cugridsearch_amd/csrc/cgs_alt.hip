@@ -25,6 +25,10 @@
 //     lambda += 1/(W_f - 1) in runs inside one binade): one grid point per
 //     workgroup, the four wavefronts taking row blocks in turn into one LDS
 //     joint histogram.
+// Every kernel asks for 8 wavefronts per SIMD (at most 64 VGPRs): the
+// loops wait on dependent tap loads, and the extra wavefronts hide that
+// latency; what the bound spills is the grid point's set-up, reloaded once
+// per row block.
 // The moved area is summed per row block in butterfly order and the block
 // sums in block order (the oracle's rule).  Compiled with -ffp-contract=off
 // (see cgs_device.hpp).
@@ -359,7 +363,7 @@ __device__ __forceinline__ AltLine block_line(const float *h, const AltParams &P
 // calcSQDAlt (SQD true) / calcNCCAlt / calcNCCstreamAlt: one grid point per
 // wavefront.  ONE: one-channel images through tap tables.
 template <typename MT, typename FT, bool SQD, bool ONE>
-__global__ __launch_bounds__(kThreads) void alt_step_kernel(const AltParams P) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void alt_step_kernel(const AltParams P) {
     __shared__ float s_cost[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int32_t base = (int32_t) blockIdx.x * 4;
@@ -430,7 +434,7 @@ __device__ __forceinline__ uint32_t *alt_lds_hist() {
 
 // calcMIstreamAlt.
 template <typename MT, typename FT, bool ONE>
-__global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void alt_step_mi_kernel(const AltParams P) {
     __shared__ int32_t s_steps[4];
     __shared__ float s_cost;
     uint32_t *hist = alt_lds_hist();
@@ -497,7 +501,7 @@ __global__ __launch_bounds__(kThreads) void alt_step_mi_kernel(const AltParams P
 // stay inside one binade with one increment (step_run), lane j taking step j
 // of the run; the fixed area after n steps comes from the host's table.
 template <typename MT, typename FT, bool ONE>
-__global__ __launch_bounds__(kThreads) void alt_norm_kernel(const AltParams P) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void alt_norm_kernel(const AltParams P) {
     __shared__ int32_t s_steps[4];
     __shared__ float s_cost;
     uint32_t *hist = alt_lds_hist();
