@@ -185,6 +185,23 @@ __device__ __forceinline__ float row_value(const T *img, float X, int row_off, i
     return fmaf(0.0f, top, top);
 }
 
+// mi_bin of a sample of image type T.  A uint8 image's bilinear value lies
+// in [0, 255] up to a few ulps (each lerp is a rounded convex combination of
+// taps in [0, 255]; 0 outside the image), so t = bins * (v / 256) is in
+// [0, bins): mi_bin's clamps do nothing and the truncating conversion is its
+// floor.
+template <typename T>
+__device__ __forceinline__ int alt_bin(float v, int bins) {
+    if constexpr (std::is_same<T, uint8_t>::value) return (int) ((float) bins * (v / 256.0f));
+    else return mi_bin(v, bins);
+}
+
+// Counter of the joint histogram's cell (fixed bin fb, moving bin mb) in
+// rows of bins + 1 counters; both factors are below 2^24.
+__device__ __forceinline__ uint32_t alt_cell(int fb, int mb, int bins) {
+    return __umul24((uint32_t) fb, (uint32_t) (bins + 1)) + (uint32_t) mb;
+}
+
 // MI of a wavefront's joint histogram h (row = fixed bin) over n samples,
 // binary64: mi = sum pxy (log2 pxy - (log2 px + log2 py)), hx / hy the
 // marginal entropies.  Every lane gets the totals.  Lane k handles column k.
@@ -468,7 +485,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                     for (int ch = 0; ch < C; ++ch) {
                         const float mv = mov.at(nx, ny, ch);
                         const float fv = fix.at_row(lam, row, ch);
-                        atomicAdd(&hist[mi_bin(fv, bins) * (bins + 1) + mi_bin(mv, bins)], 1u);
+                        atomicAdd(&hist[alt_cell(alt_bin<FT>(fv, bins), alt_bin<MT>(mv, bins), bins)], 1u);
                     }
                 });
             if (lane == j && nrow > 0) term = (double) nrow * (double) magr;
@@ -520,6 +537,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     const float sc = 1 / ((float) (P.fw - 1));
     const float wm1 = (float) (P.fw - 1);
     const double dmw = (double) P.mw, dmh = (double) P.mh;
+    const float fmw = (float) P.mw, fmh = (float) P.mh;
+    // power-of-two moving sizes: the binary64 rescale has an exact binary32 form
+    const bool pow2 = (P.mw & (P.mw - 1)) == 0 && (P.mh & (P.mh - 1)) == 0;
     const int max_runs = P.fw + 3;   // lambda < 1 in steps of 1/(W_f - 1): at most W_f steps, >= 1 per run
     int32_t steps = 0;
     for (int blk = wave; blk * 64 < P.nrows; blk += 4) {
@@ -550,13 +570,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                 if (m == 0ull) break;
                 if (ok) {
                     float nx = lam * vx + px, ny = lam * vy + py;
-                    nx = (float) (((double) nx + 0.5) * dmw);
-                    ny = (float) (((double) ny + 0.5) * dmh);
+                    if (pow2) {
+                        nx = rescale_pow2(nx, fmw);
+                        ny = rescale_pow2(ny, fmh);
+                    } else {
+                        nx = (float) (((double) nx + 0.5) * dmw);
+                        ny = (float) (((double) ny + 0.5) * dmh);
+                    }
                     const float li = lam * wm1;
                     for (int ch = 0; ch < C; ++ch) {
                         const float fv = fix.at(li, ri, ch);
                         const float mv = mov.at(nx, ny, ch);
-                        atomicAdd(&hist[mi_bin(fv, bins) * (bins + 1) + mi_bin(mv, bins)], 1u);
+                        atomicAdd(&hist[alt_cell(alt_bin<FT>(fv, bins), alt_bin<MT>(mv, bins), bins)], 1u);
                     }
                 }
                 const int nok = __popcll(m);

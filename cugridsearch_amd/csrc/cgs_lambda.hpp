@@ -1,6 +1,6 @@
 // cgs_lambda.hpp -- closed forms of the binary32 lambda sequences of the
 // line-traversal costs, so that a lane can start at step j of a row without
-// walking steps 0..j-1.
+// walking steps 0..j-1, and a binary32 form of calcMIAlt's binary64 rescale.
 //
 //   unit step   lambda = lambda + 1.0f     (src/gpu/cudaErrorFunctions.cuh:270, 346)
 //   norm step   lambda = lambda + sc, sc = 1 / (float)(W_f - 1)
@@ -84,6 +84,15 @@ CGS_HD inline float unit_pass_cross(float x0, int j) {
     const float fj = (float) j;
     return fj <= t ? x0 + fj : y + (fj - (t + 1.0f));
 }
+
+// calcMIAlt scales a normalised coordinate back to pixels through binary64,
+// (float) (((double) x + 0.5) * W).  For W = 2^k that is (x + 0.5f) * W in
+// binary32: the scaling by 2^k is exact and commutes with both roundings
+// (binary64 cannot overflow; binary32 overflows to the same infinity), and
+// RN32(RN64(x + 0.5)) = RN32(x + 0.5), because x + 0.5 is exact in binary64
+// unless |x| < 2^-31 (both give 0.5, far from a binary32 tie) or
+// |x| >= 2^52 (both give x).
+CGS_HD inline float rescale_pow2(float x, float W) { return (x + 0.5f) * W; }
 
 // A run of the lambda = lambda + s sequence (x >= 0 or -0, s > 0, x and s
 // normal) that a wavefront takes in one pass: returns n in [1, 64] such that

@@ -1,7 +1,7 @@
 // Checks the closed forms of cgs_lambda.hpp against the sequential binary32
 // loops they replace.  Prints "<mismatches> <comparisons> <runs of the
 // 512-wide rows> <starts whose 64-step pass is exact> <starts of the
-// one-crossing closed form>".
+// one-crossing closed form> <rescales compared>".
 #include <cstdio>
 #include <cmath>
 #include <cstring>
@@ -79,6 +79,29 @@ int main() {
             seq = seq + 1.0f;
         }
     }
+    // rescale_pow2(x, 2^k) against the binary64 expression, x over every
+    // magnitude and sign: zeros, subnormals, around 2^-31 and 2^52, huge
+    // values and infinities (NaN left out: only its class matters)
+    long rescales = 0;
+    std::vector<float> xs = {0.0f, -0.0f, 0.5f, -0.5f, 1e-45f, -1e-45f, 1e-40f, 3.4e38f, -3.4e38f,
+                             INFINITY, -INFINITY, 4503599627370496.0f, 9007199254740992.0f, 4.6566e-10f, -4.6566e-10f};
+    for (int i = 0; i < 200000; ++i) {
+        uint32_t b = (uint32_t) rng();
+        if ((b & 0x7F800000u) == 0x7F800000u) continue;   // NaN / inf
+        float x;
+        std::memcpy(&x, &b, 4);
+        xs.push_back(x);
+        xs.push_back(u01(rng) - 0.5f);
+        xs.push_back((u01(rng) - 0.5f) * 1e-8f);
+    }
+    for (int k = 0; k <= 14; ++k) {
+        const float W = (float) (1 << k);
+        for (float x : xs) {
+            ++n;
+            ++rescales;
+            if (!same(cgs::rescale_pow2(x, W), (float) (((double) x + 0.5) * (double) W))) ++bad;
+        }
+    }
     // lambda += 1/(float)(W - 1) from starts in [0, 1), walked in runs (the
     // kernel's passes); runs counts the passes of the starts at 0
     std::vector<int> widths;
@@ -114,6 +137,6 @@ int main() {
             }
         }
     }
-    std::printf("%ld %ld %ld %ld %ld\n", bad, n, runs512, passes, crosses);
+    std::printf("%ld %ld %ld %ld %ld %ld\n", bad, n, runs512, passes, crosses, rescales);
     return 0;
 }

@@ -91,9 +91,13 @@ def test_alt_sentinels_and_sizes(gpu, oracle, fn):
         assert (want < 0).all()
 
 
+@pytest.mark.parametrize("size", [(48, 40), (64, 64), (64, 40)], ids=["48x40", "64x64", "64x40"])
 @pytest.mark.parametrize("fn", [A.calcMIstreamAlt, A.calcMIAlt], ids=["calcMIstreamAlt", "calcMIAlt"])
-def test_alt_mi_histograms_bit_exact(gpu, oracle, fn):
-    m, f, _ = synth.make_pair(9, 48, 40, sigma=2.0, perturb=3.0, dtype=np.uint8, gamma=True, noise=4.0)
+def test_alt_mi_histograms_bit_exact(gpu, oracle, fn, size):
+    """Joint histograms equal the oracle's count for count.  64x64 takes
+    calcMIAlt's binary32 rescale (power-of-two moving sizes); 48x40 and
+    64x40 its binary64 one."""
+    m, f, _ = synth.make_pair(9, size[0], size[1], sigma=2.0, perturb=3.0, dtype=np.uint8, gamma=True, noise=4.0)
     gm, gf = A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f)
     rng = np.random.default_rng(4)
     for _ in range(10):

@@ -1,6 +1,7 @@
 """The closed-form lambda sequences of the line-traversal kernels
-(csrc/cgs_lambda.hpp) against the literal binary32 loops, compiled for the
-host with g++ -ffp-contract=off.  CPU only."""
+(csrc/cgs_lambda.hpp) against the literal binary32 loops, and the binary32
+form of calcMIAlt's power-of-two rescale against its binary64 expression,
+compiled for the host with g++ -ffp-contract=off.  CPU only."""
 import os
 import subprocess
 
@@ -14,12 +15,15 @@ def test_closed_form_lambda_matches_sequential_loops(tmp_path):
                     "-I", os.path.join(ROOT, "cugridsearch_amd", "csrc"), "-o", exe,
                     os.path.join(HERE, "cpp", "lambda_check.cpp")], check=True)
     out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
-    bad, n, runs512, passes, crosses = (int(v) for v in out[:5])
+    bad, n, runs512, passes, crosses, rescales = (int(v) for v in out[:6])
     assert n > 2_000_000 and bad == 0, (bad, n)
     # the integer starts of the exact-pass test are most of the 40016 starts
     assert passes > 20000, passes
     # the one-crossing closed form saw every start of its range
     assert crosses > 20000, crosses
+    # calcMIAlt's binary32 rescale for power-of-two sizes equals the binary64
+    # expression on every compared value
+    assert rescales > 8_000_000, rescales
     # a 512-wide row (step 1/511, a tie in one binade) from lambda = 0 (twice:
     # +0 and -0) goes in a few runs, not one pass per step
     assert runs512 <= 2 * 16, runs512
