@@ -564,7 +564,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                 int32_t a, d;
                 float U;
                 const int run = __builtin_amdgcn_readfirstlane(step_run(x0, sc, &a, &d, &U));
-                const float lam = lane == 0 ? x0 : (float) (a + lane * d) * U;
+                // a + lane * d <= 2^24 - 1 inside the run, d < 2^24 (step_run)
+                const float lam = lane == 0 ? x0 : (float) (int32_t) (__umul24((uint32_t) lane, (uint32_t) d) + (uint32_t) a) * U;
                 const bool ok = lane < run && lam < le && lam < 1;
                 const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
                 if (m == 0ull) break;

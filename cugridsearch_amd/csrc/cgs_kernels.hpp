@@ -713,11 +713,15 @@ __device__ __forceinline__ int clamp_int(int v, int lo, int hi) {
 
 // floor(bins * (v / 256)) clamped to [0, bins-1].  POW2: bins is a power of
 // two, so both scalings are exact and one multiply by bins/256 gives the same
-// value.
-template <bool POW2>
+// value.  U8: v is a bilinear value of a uint8 image, in [0, 255] up to a
+// few ulps (rounded convex combinations of taps in [0, 255]), so t lies in
+// [0, bins) and the clamp does nothing (lanes outside the image, whose v
+// may be anything, have their counter replaced by the caller).
+template <bool POW2, bool U8 = false>
 __device__ __forceinline__ int mi_bin_fast(float v, int bins, float fbins, float scale) {
     const float t = POW2 ? v * scale : fbins * (v / 256.0f);
-    return clamp_int(floor_to_int(t), 0, bins - 1);
+    if constexpr (U8) return floor_to_int(t);
+    else return clamp_int(floor_to_int(t), 0, bins - 1);
 }
 
 template <typename MT, typename FT, int GPT, bool FASTDIV, bool POW2>
@@ -823,7 +827,8 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 const float top = fmaf(ax[g], tr, (1.0f - ax[g]) * q[g].x);
                 const float bottom = fmaf(ax[g], br, (1.0f - ax[g]) * bl);
                 const float v = fmaf(ay[g], bottom, (1.0f - ay[g]) * top);
-                const uint32_t h = (uint32_t) (g * HS) + rowbin + (uint32_t) mi_bin_fast<POW2>(v, bins, fbins, bscale);
+                const uint32_t h = (uint32_t) (g * HS) + rowbin +
+                                   (uint32_t) mi_bin_fast<POW2, std::is_same<MT, uint8_t>::value>(v, bins, fbins, bscale);
                 atomicAdd(&hist[mask_select(in[g], dummy, h)], 1u);
             }
         }
