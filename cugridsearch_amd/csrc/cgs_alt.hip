@@ -351,9 +351,7 @@ __device__ __forceinline__ int32_t walk_unit_row(float ls, float le, float FW, f
     int32_t nrow = 0;
     while (true) {
         // x0 is the same in every lane: one branch per pass
-        const float lam = unit_pass_exact(x0)                 ? x0 + flane
-                          : x0 >= 32.0f && x0 < 8388608.0f ? unit_pass_cross(x0, lane)
-                                                            : advance_unit(x0, lane);
+        const float lam = unit_pass_direct(x0) ? x0 + flane : unit_pass_chain(x0, lane);
         const bool ok = lam < le && lam < FW;
         const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
         if (m == 0ull) break;

@@ -8,11 +8,12 @@
 //
 // Both give exactly the values of the sequential loop; the host build (g++,
 // -ffp-contract=off) is checked against that loop by
-// tests/test_lambda_steps.py.  Within a binade [2^e, 2^(e+1)) with ulp U,
-// x + s for x on the U grid rounds to x + delta with one delta per binade
-// (delta = s rounded to the U grid), so runs of steps inside a binade are
-// taken at once in integer units of U, and the step that leaves the binade
-// is taken as the literal rounded add.
+// tests/test_lambda_steps.py.  Unit steps: see unit_pass_direct.  Norm
+// steps: within a binade [2^e, 2^(e+1)) with ulp U, x + s for x on the U
+// grid rounds to x + delta with one delta per binade (delta = s rounded to
+// the U grid), so runs of steps inside a binade are taken at once in integer
+// units of U, and the step that leaves the binade is taken as the literal
+// rounded add.
 #pragma once
 
 #include <stdint.h>
@@ -38,51 +39,56 @@ CGS_HD inline float u2f(uint32_t u) {
     return f;
 }
 
-// lambda after r steps of lambda = lambda + 1.0f from x (x >= 0 or -0,
-// x < 2^23).  Every +1 inside a binade of x >= 1 is exact; below 1 every step
-// leaves the binade.
-CGS_HD inline float advance_unit(float x, int r) {
-    while (r > 0) {
-        if (!(x >= 1.0f)) {
-            x = x + 1.0f;
-            r -= 1;
-            continue;
+// Unit steps.  From x >= 1 a step that leaves a binade rounds the sum to a
+// grid twice as coarse, dropping one low bit by ties-to-even.  Below 2^23
+// every grid has spacing 1/2 or finer, so adding an integer (an even
+// multiple of the grid) keeps a tie a tie and its even neighbour even: the
+// rounding commutes with it.  Dropping one bit and then the next by
+// ties-to-even gives what dropping both at once gives (the four values of
+// the two bits), so below 2^23 a run of +1 steps from x that crosses at most
+// two binade boundaries ends at the one rounded sum x + k.  Three can differ
+// (...0|101 rounds down bit by bit and up at once), and so can a run into
+// [2^23, 2^24), where +1 is an odd multiple of the grid:
+// 8388607.5 -> 8388608 (a tie, to even) -> 8388609, but 8388607.5 + 2 ->
+// 8388610.
+
+// True when x0 + j, one rounded add, is step j (0..63) of lambda = lambda +
+// 1.0f from x0: 16 <= x0 and x0 + 63 < 2^23 (the pass crosses at most two
+// binade boundaries, all below 2^23), or x0 is a non-negative integer (sign
+// bit clear, so not -0) below 2^24 - 63 (every sum is an exact integer).
+CGS_HD inline bool unit_pass_direct(float x0) {
+    return (x0 >= 16.0f && x0 < 8388545.0f) || (f2u(x0) < 0x4B7FFFC1u && x0 == __builtin_truncf(x0));
+}
+
+// Step j (0..63) of lambda = lambda + 1.0f from any x0 >= 0 or -0 below
+// 2^23, the pass cut into the same segments for every lane.  A segment
+// starts at lane jb with value base and holds the one rounded base + (j - jb)
+// for t steps: up to t = ceil(4B - base) - 1, B the top of base's binade, so
+// that at most two boundaries are crossed inside it, while 4B <= 2^23; from
+// the binade below 2^23 up, to the binade's end (t = ceil(B - base) - 1; in
+// [2^23, 2^24) every value is an integer and t runs past the pass); below 1
+// a segment is base alone, because that first add drops many bits at once.
+// The next segment starts at the literal add (base + t) + 1.  The segments
+// depend on x0 only (at most 5 of them), each lane keeps the value of the
+// last segment that starts at or before it, and lane 0 keeps x0 itself
+// (-0 + 0 is +0).
+CGS_HD inline float unit_pass_chain(float x0, int j) {
+    float base = x0, lam = x0;
+    int jb = 0;
+    while (true) {
+        float t = 0.0f;
+        if (base >= 1.0f) {
+            const float B = u2f((f2u(base) & 0x7F800000u) + 0x00800000u);
+            t = __builtin_ceilf(B - base) - 1.0f;   // B - base is exact
+            if (B <= 2097152.0f) t += 3.0f * B;     // 4B <= 2^23
         }
-        const float B = u2f((f2u(x) & 0x7F800000u) + 0x00800000u);   // top of x's binade
-        const float t = __builtin_ceilf(B - x) - 1.0f;                // steps staying below B; B - x exact
-        if ((float) r <= t) {
-            x = x + (float) r;
-            r = 0;
-        } else {
-            x = x + t;
-            x = x + 1.0f;
-            r -= (int) t + 1;
-        }
+        if (j >= jb) lam = base + (float) (j - jb);
+        const float next = (float) jb + t + 1.0f;
+        if (next > 63.0f) break;
+        jb = (int) next;
+        base = (base + t) + 1.0f;
     }
-    return x;
-}
-
-// True when 64 steps of lambda = lambda + 1.0f from x0 are the exact x0 + j
-// (j = 0..63): x0 + 63 stays inside x0's binade [1, 2^23), so every add is
-// exact; or x0 is a non-negative integer (sign bit clear, so not -0) below
-// 2^24 - 63, so every partial sum is an integer below 2^24.
-CGS_HD inline bool unit_pass_exact(float x0) {
-    return (x0 >= 1.0f && x0 + 63.0f < u2f((f2u(x0) & 0x7F800000u) + 0x00800000u)) ||
-           (f2u(x0) < 0x4B7FFFC1u && x0 == __builtin_truncf(x0));
-}
-
-// Step j (0..63) of lambda = lambda + 1.0f from x0 when 32 <= x0 < 2^23.
-// Then x0 + 63 < 2B for the top B of x0's binade, so the pass leaves the
-// binade at most once: steps j <= t (t = ceil(B - x0) - 1; B - x0 is exact)
-// are the exact x0 + j, step t + 1 is the one rounded add y = (x0 + t) + 1,
-// and later steps are the exact y + (j - t - 1) inside [B, 2B), whose ulp is
-// at most 1.  t and y are the same for every lane.
-CGS_HD inline float unit_pass_cross(float x0, int j) {
-    const float B = u2f((f2u(x0) & 0x7F800000u) + 0x00800000u);
-    const float t = __builtin_ceilf(B - x0) - 1.0f;
-    const float y = (x0 + t) + 1.0f;
-    const float fj = (float) j;
-    return fj <= t ? x0 + fj : y + (fj - (t + 1.0f));
+    return j == 0 ? x0 : lam;
 }
 
 // calcMIAlt scales a normalised coordinate back to pixels through binary64,

@@ -1,7 +1,7 @@
 // Checks the closed forms of cgs_lambda.hpp against the sequential binary32
 // loops they replace.  Prints "<mismatches> <comparisons> <runs of the
-// 512-wide rows> <starts whose 64-step pass is exact> <starts of the
-// one-crossing closed form> <rescales compared>".
+// 512-wide rows> <starts unit_pass_direct accepts> <starts through
+// unit_pass_chain> <rescales compared>".
 #include <cstdio>
 #include <cmath>
 #include <cstring>
@@ -16,67 +16,48 @@ int main() {
     std::mt19937 rng(12345);
     std::uniform_real_distribution<float> u01(0.0f, 1.0f);
     long bad = 0, n = 0, runs512 = 0;
-    // lambda += 1.0f from starts in [0, 65536), including 0, -0, integers and
-    // values just below powers of two
-    std::vector<float> starts = {0.0f, -0.0f, 1.0f, 0.5f, 0.999999940f, 63.9999962f, 255.5f, 1023.99994f};
-    for (int i = 0; i < 4000; ++i) {
-        const int e = (int) (rng() % 17);
-        starts.push_back(u01(rng) * (float) (1 << e));
+    // unit steps, 64 per pass: unit_pass_direct(x0) accepts a start only
+    // when every x0 + j is the loop's value; unit_pass_chain must match the
+    // loop from every start.  Starts: zeros, integers (the top of the range
+    // included), fractions below 1, binade interiors and edges from 2^-6 to
+    // 2^23, starts just below powers of two (a pass crossing one, two or
+    // three boundaries), fractional starts just below 2^22 and 2^23 (where
+    // the +1 steps become odd multiples of the grid)
+    long direct = 0, chains = 0;
+    std::vector<float> starts = {0.0f, -0.0f, 1e-30f, 1e-7f, 0.5f, 0.999999940f, 1.0f, 1.25f, 1.5f, 3.75f, 7.3f,
+                                 15.9999990f, 16.0f, 16.5f, 31.7f, 31.9999981f, 32.0f, 32.5f, 63.0f, 64.0f, 100.25f,
+                                 127.99999f, 255.5f, 1023.99994f, 8388544.5f, 8388607.5f, 8388608.0f, 4194303.75f,
+                                 16777151.0f, 16777152.0f, 16777153.0f, 1.0000001f, 2.0000002f};
+    for (int e = 0; e < 23; ++e)
+        for (int k = 1; k <= 40; ++k) {
+            const float top = (float) (1 << (e + 1));
+            starts.push_back(top - (float) k * 0.9f);
+            starts.push_back(std::nextafter(top - (float) k, 0.0f));
+            starts.push_back(top - (float) k + 0.5f);
+        }
+    for (int i = 0; i < 2000; ++i) {   // fractional starts within 200 of 2^22 and 2^23
+        starts.push_back(8388608.0f - 200.0f * u01(rng));
+        starts.push_back(4194304.0f - 200.0f * u01(rng));
+    }
+    for (int i = 0; i < 40000; ++i) {
+        const int e = (int) (rng() % 30) - 6;
+        const float v = u01(rng) * std::ldexp(1.0f, e);
+        starts.push_back(v);
+        starts.push_back(std::floor(v));
     }
     for (float x0 : starts) {
-        float seq = x0;
-        for (int k = 0; k <= 600; ++k) {
-            ++n;
-            if (!same(cgs::advance_unit(x0, k), seq)) ++bad;
-            seq = seq + 1.0f;
+        std::vector<float> seq(64);
+        seq[0] = x0;
+        for (int j = 1; j < 64; ++j) seq[j] = seq[j - 1] + 1.0f;
+        if (cgs::unit_pass_direct(x0)) {
+            ++direct;
+            for (int j = 0; j < 64; ++j, ++n)
+                if (!same(x0 + (float) j, seq[j])) ++bad;
         }
-    }
-    // unit_pass_exact(x0): a pass of 64 steps is x0 + j exactly.  Starts:
-    // integers (0 and the top of the range included), binade interiors and
-    // edges, fractions below 1, -0; passes counts the starts it accepts
-    long passes = 0;
-    std::vector<float> pstarts = {0.0f, -0.0f, 1.0f, 63.0f, 64.0f, 0.5f, 1.5f, 1.25f, 16777152.0f, 16777153.0f,
-                                  16777151.0f, 8388607.5f, 8388608.0f, 4194303.5f, 1.0000001f, 2.0000002f};
-    for (int i = 0; i < 20000; ++i) {
-        const int e = (int) (rng() % 25);
-        const float v = u01(rng) * (float) (1 << e);
-        pstarts.push_back(v);
-        pstarts.push_back(__builtin_floorf(v));
-    }
-    for (float x0 : pstarts) {
-        if (!cgs::unit_pass_exact(x0)) continue;
-        ++passes;
-        float seq = x0;
-        for (int j = 0; j < 64; ++j) {
-            ++n;
-            if (!same(x0 + (float) j, seq)) ++bad;
-            seq = seq + 1.0f;
-        }
-    }
-    // unit_pass_cross(x0, j) for 32 <= x0 < 2^23: starts just below and above
-    // powers of two, binade interiors, integers
-    long crosses = 0;
-    std::vector<float> cstarts = {32.0f, 32.5f, 63.9999962f, 64.0f, 127.3f, 127.99999f, 8388607.5f, 8388544.25f,
-                                  4194303.75f, 65535.99f, 200.3f, 1000.7f};
-    for (int e = 5; e < 23; ++e)
-        for (int k = 1; k <= 70; ++k) {
-            const float top = (float) (1 << (e + 1));
-            cstarts.push_back(top - (float) k * 0.9f);
-            cstarts.push_back(std::nextafter(top - (float) k, 0.0f));
-            cstarts.push_back(top - (float) k + 0.5f);
-        }
-    for (int i = 0; i < 20000; ++i) {
-        const int e = 5 + (int) (rng() % 18);
-        cstarts.push_back((1.0f + u01(rng)) * (float) (1 << e));
-    }
-    for (float x0 : cstarts) {
-        if (!(x0 >= 32.0f && x0 < 8388608.0f)) continue;
-        ++crosses;
-        float seq = x0;
-        for (int j = 0; j < 64; ++j) {
-            ++n;
-            if (!same(cgs::unit_pass_cross(x0, j), seq)) ++bad;
-            seq = seq + 1.0f;
+        if (x0 < 8388608.0f) {
+            ++chains;
+            for (int j = 0; j < 64; ++j, ++n)
+                if (!same(cgs::unit_pass_chain(x0, j), seq[j])) ++bad;
         }
     }
     // rescale_pow2(x, 2^k) against the binary64 expression, x over every
@@ -137,6 +118,6 @@ int main() {
             }
         }
     }
-    std::printf("%ld %ld %ld %ld %ld %ld\n", bad, n, runs512, passes, crosses, rescales);
+    std::printf("%ld %ld %ld %ld %ld %ld\n", bad, n, runs512, direct, chains, rescales);
     return 0;
 }
