@@ -126,7 +126,7 @@ CGS_HD inline int step_run(float x, float s, int32_t *a, int32_t *d, float *U) {
     const int32_t am = (int32_t) ((bits & 0x007FFFFFu) | 0x00800000u);  // x / U
     if ((err == 0.5f * u || err == -0.5f * u) && (am & 1)) return 1;
     *a = am;
-    *d = (int32_t) (delta / u);
+    *d = (int32_t) __builtin_ldexpf(delta, 150 - (int32_t) (expo >> 23));   // delta / u, exact: u = 2^(E - 150)
     *U = u;
     const int32_t room = (0x00FFFFFF - *a) / *d;
     return room + 1 < 64 ? room + 1 : 64;
