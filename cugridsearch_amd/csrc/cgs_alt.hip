@@ -56,6 +56,7 @@ struct AltParams {
     const void *fix;        // fixed image + zero pad (several channels)
     const void *mq;         // moving image's exact tap table (one channel)
     const void *fq;         // fixed image's exact tap table (one channel)
+    const void *fraw;       // fixed image itself (one channel): rows at integral lambda
     int mw, mh, fw, fh, C;
     int bins;
     const float *rows;      // calcMIAlt: the row values
@@ -332,6 +333,27 @@ struct RawSampler {
 template <typename T, bool ONE>
 using Sampler = typename std::conditional<ONE, TabSampler<T>, RawSampler<T>>::type;
 
+// A one-channel fixed row at an integral lambda (0 <= lambda < W, as every
+// walked step is): the lerp weight is 0, so at_row's value is
+// fmaf(0, top, top) with top = fmaf(0, p, 1 * p) for the pixel p itself (p
+// when finite, NaN otherwise), from one 4-byte load of a float image instead
+// of a 16-byte tap entry.
+template <typename T>
+struct IntRow {
+    __amdgpu_buffer_rsrc_t rsrc;
+    int w;
+    __device__ IntRow(const void *img, int width, int height) {
+        rsrc = __builtin_amdgcn_make_buffer_rsrc((void *) uniform(img), (short) 0,
+                                                 (int) ((size_t) width * height * sizeof(T)), 0x00020000);
+        w = width;
+    }
+    __device__ __forceinline__ float at(float lam, int row) const {
+        const float p = load_fix<T>(rsrc, (uint32_t) ((row * w + (int) lam) * (int) sizeof(T)));
+        const float top = fmaf(0.0f, p, p);
+        return fmaf(0.0f, top, top);
+    }
+};
+
 template <typename T, bool ONE>
 __device__ __forceinline__ Sampler<T, ONE> make_sampler(const void *tab, const void *raw, int C, int w, int h) {
     if constexpr (ONE) return TabSampler<T>(tab, w, h);
@@ -339,9 +361,10 @@ __device__ __forceinline__ Sampler<T, ONE> make_sampler(const void *tab, const v
 }
 
 // Walks one row's unit-step line: 64 consecutive lambda steps per pass, lane
-// j taking step j; calls at(lam, nx, ny) for the lane's step while it is
-// below the line's end and the row's width.  Returns the row's step count
-// (wavefront-uniform).
+// j taking step j; calls at(lam, nx, ny, ipass) for the lane's step while it
+// is below the line's end and the row's width, ipass telling (the same in
+// every lane) that the pass's lambdas are integers.  Returns the row's step
+// count (wavefront-uniform).
 template <typename F>
 __device__ __forceinline__ int32_t walk_unit_row(float ls, float le, float FW, float vx, float vy, float px,
                                                  float py, F &&at) {
@@ -352,10 +375,11 @@ __device__ __forceinline__ int32_t walk_unit_row(float ls, float le, float FW, f
     while (true) {
         // x0 is the same in every lane: one branch per pass
         const float lam = unit_pass_direct(x0) ? x0 + flane : unit_pass_chain(x0, lane);
+        const bool ipass = f2u(x0) < 0x4B7FFFC1u && x0 == truncf(x0);   // lam = x0 + j, all integers
         const bool ok = lam < le && lam < FW;
         const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
         if (m == 0ull) break;
-        if (ok) at(lam, lam * vx + px, lam * vy + py);
+        if (ok) at(lam, lam * vx + px, lam * vy + py, ipass);
         nrow += __popcll(m);
         if (m != ~0ull) break;   // valid lanes form a prefix: the row ended in this pass
         x0 = lane_value(lam, 63) + 1.0f;
@@ -385,6 +409,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     const int32_t gi = base + wave;
     const auto mov = make_sampler<MT, ONE>(P.mq, P.mov, P.C, P.mw, P.mh);
     const auto fix = make_sampler<FT, ONE>(P.fq, P.fix, P.C, P.fw, P.fh);
+    // integral-lambda rows from the image itself for one-channel float images
+    // (whose tap entries are 16 bytes; a uint8 entry is one word already)
+    constexpr bool IROW = ONE && std::is_same<FT, float>::value;
+    const IntRow<FT> irow(IROW ? P.fraw : P.fq, P.fw, P.fh);
     const int C = ONE ? 1 : P.C;
     float cost = 0.0f;
     if (gi < P.grid.n) {
@@ -407,10 +435,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                 const int32_t nrow = walk_unit_row(
                     lane_value(Lr.ls, j), lane_value(Lr.le, j), (float) P.fw, lane_value(Lr.vx, j),
                     lane_value(Lr.vy, j), lane_value(Lr.px, j), lane_value(Lr.py, j),
-                    [&](float lam, float nx, float ny) {
+                    [&](float lam, float nx, float ny, bool ipass) {
                         for (int ch = 0; ch < C; ++ch) {
                             const double md = (double) mov.at(nx, ny, ch);
-                            const double fd = (double) fix.at_row(lam, row, ch);
+                            float fv;
+                            if constexpr (IROW) fv = ipass ? irow.at(lam, row) : fix.at_row(lam, row, ch);
+                            else fv = fix.at_row(lam, row, ch);
+                            const double fd = (double) fv;
                             if constexpr (SQD) s0 = fma(md - fd, md - fd, s0);
                             else s0 = fma(md, fd, s0);
                             s1 = fma(fd, fd, s1);
@@ -461,6 +492,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     __syncthreads();
     const auto mov = make_sampler<MT, ONE>(P.mq, P.mov, P.C, P.mw, P.mh);
     const auto fix = make_sampler<FT, ONE>(P.fq, P.fix, P.C, P.fw, P.fh);
+    // integral-lambda rows from the image itself for one-channel float images
+    // (whose tap entries are 16 bytes; a uint8 entry is one word already)
+    constexpr bool IROW = ONE && std::is_same<FT, float>::value;
+    const IntRow<FT> irow(IROW ? P.fraw : P.fq, P.fw, P.fh);
     const int C = ONE ? 1 : P.C;
     const int bins = P.bins;
     float p[kMaxDim], c, s, h[8];
@@ -479,10 +514,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
             const int row = rb + j;
             const int32_t nrow = walk_unit_row(
                 lane_value(Lr.ls, j), lane_value(Lr.le, j), (float) P.fw, lane_value(Lr.vx, j), lane_value(Lr.vy, j),
-                lane_value(Lr.px, j), lane_value(Lr.py, j), [&](float lam, float nx, float ny) {
+                lane_value(Lr.px, j), lane_value(Lr.py, j), [&](float lam, float nx, float ny, bool ipass) {
                     for (int ch = 0; ch < C; ++ch) {
                         const float mv = mov.at(nx, ny, ch);
-                        const float fv = fix.at_row(lam, row, ch);
+                        float fv;
+                        if constexpr (IROW) fv = ipass ? irow.at(lam, row) : fix.at_row(lam, row, ch);
+                        else fv = fix.at_row(lam, row, ch);
                         atomicAdd(&hist[alt_cell(alt_bin<FT>(fv, bins), alt_bin<MT>(mv, bins), bins)], 1u);
                     }
                 });
@@ -762,6 +799,7 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
     P.fix = one ? nullptr : ws->fixp.ptr;
     P.mq = one ? ws->quads.ptr : nullptr;
     P.fq = one ? ws->fixq.ptr : nullptr;
+    P.fraw = one ? fixed->data : nullptr;
     P.mw = moved->width;
     P.mh = moved->height;
     P.fw = fixed->width;
