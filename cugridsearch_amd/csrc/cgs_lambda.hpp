@@ -56,9 +56,24 @@ CGS_HD inline float u2f(uint32_t u) {
 // 1.0f from x0: 16 <= x0 and x0 + 63 < 2^23 (the pass crosses at most two
 // binade boundaries, all below 2^23), or x0 is a non-negative integer (sign
 // bit clear, so not -0) below 2^24 - 63 (every sum is an exact integer).
-CGS_HD inline bool unit_pass_direct(float x0) {
-    return (x0 >= 16.0f && x0 < 8388545.0f) || (f2u(x0) < 0x4B7FFFC1u && x0 == __builtin_truncf(x0));
+// Both tests read the bit pattern, whose order is the value's order for
+// non-negative floats, so that a wave-uniform x0 is tested with scalar
+// instructions.
+CGS_HD inline bool unit_int_start(float x0) {   // x0 a non-negative integer below 2^24 - 63
+    const uint32_t b = f2u(x0);
+    if (b == 0u) return true;                                  // +0
+    if (b < 0x3F800000u || b >= 0x4B7FFFC1u) return false;     // (0, 1), >= 2^24 - 63, or sign bit set
+    return (b & ((1u << (150u - (b >> 23))) - 1u)) == 0u;      // no fraction bits below the point
 }
+CGS_HD inline bool unit_pass_direct(float x0) {
+    const uint32_t b = f2u(x0);
+    return (b >= 0x41800000u && b < 0x4AFFFF82u) || unit_int_start(x0);   // 16 <= x0 < 2^23 - 63
+}
+
+// The same two tests in value form (vector compares).  A kernel whose scalar
+// unit is busy runs faster with these; the host check holds both forms equal.
+CGS_HD inline bool unit_int_start_v(float x0) { return f2u(x0) < 0x4B7FFFC1u && x0 == __builtin_truncf(x0); }
+CGS_HD inline bool unit_pass_direct_v(float x0) { return (x0 >= 16.0f && x0 < 8388545.0f) || unit_int_start_v(x0); }
 
 // Step j (0..63) of lambda = lambda + 1.0f from any x0 >= 0 or -0 below
 // 2^23, the pass cut into the same segments for every lane.  A segment

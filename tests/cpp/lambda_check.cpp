@@ -49,6 +49,12 @@ int main() {
         std::vector<float> seq(64);
         seq[0] = x0;
         for (int j = 1; j < 64; ++j) seq[j] = seq[j - 1] + 1.0f;
+        // the bit-pattern predicates and the header's value forms against the
+        // definition
+        const bool int_start = !std::signbit(x0) && x0 == std::trunc(x0) && x0 < 16777153.0f;
+        const bool direct_def = int_start || (x0 >= 16.0f && x0 < 8388545.0f);
+        if (cgs::unit_int_start(x0) != int_start || cgs::unit_int_start_v(x0) != int_start) ++bad;
+        if (cgs::unit_pass_direct(x0) != direct_def || cgs::unit_pass_direct_v(x0) != direct_def) ++bad;
         if (cgs::unit_pass_direct(x0)) {
             ++direct;
             for (int j = 0; j < 64; ++j, ++n)
