@@ -84,7 +84,8 @@ CGS_HD inline bool unit_pass_direct_v(float x0) { return (x0 >= 16.0f && x0 < 83
 // [2^23, 2^24) every value is an integer and t runs past the pass); below 1
 // a segment is base alone, because that first add drops many bits at once.
 // The next segment starts at the literal add (base + t) + 1.  The segments
-// depend on x0 only (at most 5 of them), each lane keeps the value of the
+// depend on x0 only (at most 3: below 1, from [1, 2) to 8, from [8, 16) to
+// 64; or up to 2^23 and past it), each lane keeps the value of the
 // last segment that starts at or before it, and lane 0 keeps x0 itself
 // (-0 + 0 is +0).
 CGS_HD inline float unit_pass_chain(float x0, int j) {
