@@ -144,6 +144,8 @@ CGS_HD inline int step_run(float x, float s, int32_t *a, int32_t *d, float *U) {
     *a = am;
     *d = (int32_t) __builtin_ldexpf(delta, 150 - (int32_t) (expo >> 23));   // delta / u, exact: u = 2^(E - 150)
     *U = u;
+    // a + 63 d < 2^31 (a < 2^24, d < 2^24); the division only near the binade's top
+    if ((uint32_t) *a + 63u * (uint32_t) *d <= 0x00FFFFFFu) return 64;
     const int32_t room = (0x00FFFFFF - *a) / *d;
     return room + 1 < 64 ? room + 1 : 64;
 }
