@@ -380,10 +380,13 @@ __device__ __forceinline__ int32_t walk_unit_row(float ls, float le, float FW, f
         const bool direct = SCALAR_TESTS ? unit_pass_direct(x0) : unit_pass_direct_v(x0);
         const float lam = direct ? x0 + flane : unit_pass_chain(x0, lane);
         const bool ipass = SCALAR_TESTS ? unit_int_start(x0) : unit_int_start_v(x0);   // lam = x0 + j, integers
-        const bool ok = lam < le && lam < FW;
-        const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
+        // one ballot per compare: each is the compare's own lane mask, and
+        // their AND is scalar (a ballot of the combined bool costs a
+        // select and a compare per pass)
+        const unsigned long long m =
+            __builtin_amdgcn_ballot_w64(lam < le) & __builtin_amdgcn_ballot_w64(lam < FW);
         if (m == 0ull) break;
-        if (ok) at(lam, lam * vx + px, lam * vy + py, ipass);
+        if (lam < le && lam < FW) at(lam, lam * vx + px, lam * vy + py, ipass);   // the same compares
         nrow += __popcll(m);
         if (m != ~0ull) break;   // valid lanes form a prefix: the row ended in this pass
         x0 = lane_value(lam, 63) + 1.0f;
@@ -605,10 +608,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                 const int run = __builtin_amdgcn_readfirstlane(step_run(x0, sc, &a, &d, &U));
                 // a + lane * d <= 2^24 - 1 inside the run, d < 2^24 (step_run)
                 const float lam = lane == 0 ? x0 : (float) (int32_t) (__umul24((uint32_t) lane, (uint32_t) d) + (uint32_t) a) * U;
-                const bool ok = lane < run && lam < le && lam < 1;
-                const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
+                // one ballot per compare (see walk_unit_row)
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(lane < run) &
+                                             __builtin_amdgcn_ballot_w64(lam < le) &
+                                             __builtin_amdgcn_ballot_w64(lam < 1.0f);
                 if (m == 0ull) break;
-                if (ok) {
+                if (lane < run && lam < le && lam < 1.0f) {   // the same compares
                     float nx = lam * vx + px, ny = lam * vy + py;
                     if (pow2) {
                         nx = rescale_pow2(nx, fmw);
