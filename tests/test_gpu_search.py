@@ -104,14 +104,48 @@ def test_full_workload_sample_and_golden_argmin(gpu, oracle, name):
         want, _ = oracle.cost_point(m, f, fn.cost, fn.sampler, grid.getGridPoint(k), bins=fn.bins)
         ok, _ = close([got[k]], [want], rtol)
         assert ok, (k, got[k], want)
-    if os.path.exists(GOLDEN):
-        gold = json.load(open(GOLDEN)).get(name)
-        if gold:
-            vo = gold["argmin_value"]
-            assert abs(v - vo) <= rtol * abs(vo), (v, vo)
-            assert abs(got[gold["argmin_index"]] - vo) <= rtol * abs(vo)
-            ov, _ = oracle.cost_point(m, f, fn.cost, fn.sampler, grid.getGridPoint(i), bins=fn.bins)
-            assert abs(ov - vo) <= 2 * rtol * abs(vo)
+    gold = json.load(open(GOLDEN))[name]
+    assert gold["num_elements"] == n
+    vo = gold["argmin_value"]
+    assert abs(v - vo) <= rtol * abs(vo), (v, vo)
+    assert abs(got[gold["argmin_index"]] - vo) <= rtol * abs(vo)
+    ov, _ = oracle.cost_point(m, f, fn.cost, fn.sampler, grid.getGridPoint(i), bins=fn.bins)
+    assert abs(ov - vo) <= 2 * rtol * abs(vo)
+    # every committed full-grid oracle sample (2048 seeded indices)
+    si = np.array(gold["sample_index"])
+    ok, bad = close(got[si], np.array(gold["sample_value"], np.float64), rtol)
+    assert ok, (si[bad], got[si[bad]], np.array(gold["sample_value"])[bad])
+    # the oracle's 16 lowest costs: the GPU's value at each, and the GPU's own
+    # 16 lowest values equal them (the same set up to near-ties)
+    bi = np.array(gold["best16_index"])
+    bv = np.array(gold["best16_value"], np.float64)
+    ok, bad = close(got[bi], bv, rtol)
+    assert ok, (bi[bad], got[bi[bad]], bv[bad])
+    mine = np.sort(got[np.isfinite(got)])[:16].astype(np.float64)
+    ok, bad = close(mine, bv, 2 * rtol)
+    assert ok, (mine[bad], bv[bad])
+    assert int(np.isnan(got).sum()) == gold["nan_count"]
+
+
+def test_mi32_512_histograms_bit_exact(gpu, oracle):
+    """The C3 workload (512x512 uint8, 32 bins): joint histograms of 8 grid
+    points, the golden argmin among them, bit for bit against the oracle."""
+    m, f, s, e, c, spec = synth.workload("bilinear_mi32_512")
+    gold = json.load(open(GOLDEN))["bilinear_mi32_512"]
+    grid = A.CudaGrid(8)
+    grid.setStartPoint(s)
+    grid.setEndPoint(e)
+    grid.setNumSamples(c)
+    gm, gf = A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f)
+    fn = A.calcMIstream.with_(bins=spec["bins"])
+    rng = np.random.default_rng(11)
+    for k in [gold["argmin_index"]] + [int(x) for x in rng.choice(grid.numElements(), 7, replace=False)]:
+        p = grid.getGridPoint(k)
+        cost, hist = A.eval_point(fn, gm, gf, p)
+        oc, _, oh = oracle.cost_point(m, f, fn.cost, fn.sampler, p, bins=fn.bins, want_hist=True)
+        assert hist.sum() > 0
+        assert np.array_equal(hist, oh), k
+        assert close([cost], [oc], MI_RTOL)[0], (k, cost, oc)
 
 
 def _search_with(functor, m, f, grid, gpt, order):

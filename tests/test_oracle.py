@@ -116,6 +116,104 @@ def test_integer_translation_is_exact(oracle):
     assert np.array_equal(oracle.warp_point(coef, 4, 7)[0], np.array([6.5, 6.0], np.float32))
 
 
+def _noncentred_fp64(p):
+    """averageAbsoluteDifference's H built as matrices: the 2x2 block is
+    rotation . shear . scale, the translation column is scale . t, the last
+    row is (kx, ky, 1) (src/gpu/cudaErrorFunctions.cuh:50-57)."""
+    th, sx, sy, sh, tx, ty, kx, ky = [float(v) for v in p]
+    R = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+    A2 = R @ np.array([[1, sh], [0, 1]]) @ np.diag([sx, sy])
+    H = np.eye(3)
+    H[:2, :2] = A2
+    H[:2, 2] = np.diag([sx, sy]) @ np.array([tx, ty])
+    H[2, :2] = [kx, ky]
+    return H
+
+
+def _random_params(rng, t=8.0, k=1e-3):
+    return np.array([rng.uniform(-0.3, 0.3), rng.uniform(0.8, 1.2), rng.uniform(0.8, 1.2), rng.uniform(-0.1, 0.1),
+                     rng.uniform(-t, t), rng.uniform(-t, t), rng.uniform(-k, k), rng.uniform(-k, k)], np.float32)
+
+
+def test_noncentred_homography_matches_matrix_product(oracle):
+    """The non-centred H against its matrix construction: a dropped scale on
+    the translation (h13 = tx), a flipped sine or a swapped keystone each
+    miss by far more than the tolerance at these parameter ranges."""
+    rng = np.random.default_rng(17)
+    for _ in range(100):
+        p = _random_params(rng)
+        h = oracle.homography(p, 99.0, 77.0, mode=oracle.PARAM_NONCENTRED)   # the centre plays no part
+        want = _noncentred_fp64(p).reshape(-1)[:8]
+        assert np.allclose(h, want, rtol=1e-6, atol=1e-6), (p, h, want)
+
+
+def test_noncentred_inverse_warp_matches_fp64_inverse(oracle):
+    rng = np.random.default_rng(18)
+    for _ in range(100):
+        p = _random_params(rng, t=4.0)
+        Hi = np.linalg.inv(_noncentred_fp64(p))
+        coef = oracle.warp_coeffs(p, 0.0, 0.0, mode=oracle.PARAM_NONCENTRED)
+        for x, y in [(0, 0), (63, 0), (0, 63), (31, 17), (63, 63)]:
+            q = Hi @ np.array([x, y, 1.0])
+            got, _ = oracle.warp_point(coef, x, y)
+            assert np.allclose(got, q[:2] / q[2], atol=2e-3), (p, x, y, got, q[:2] / q[2])
+
+
+def test_noncentred_power_of_two_scale_translation_exact(oracle):
+    """Rotation and shear 0, scales 2 and 1/2, integer translation: the
+    non-centred H is [[2, 0, 2tx], [0, 1/2, ty/2], [0, 0, 1]], so its inverse
+    samples (x/2 - tx, 2y - ty), every operation exact in binary32."""
+    for tx, ty in [(3, -2), (-4, 4), (0, 1), (5, 0)]:
+        p = [0, 2.0, 0.5, 0, tx, ty, 0, 0]
+        assert np.array_equal(oracle.homography(p, 0, 0, mode=oracle.PARAM_NONCENTRED),
+                              np.array([2, 0, 2 * tx, 0, 0.5, 0.5 * ty, 0, 0], np.float32))
+        coef = oracle.warp_coeffs(p, 0, 0, mode=oracle.PARAM_NONCENTRED)
+        for x, y in [(0, 0), (5, 9), (63, 63), (17, 40)]:
+            got, inside = oracle.warp_point(coef, x, y, 64, 64)
+            assert got[0] == x / 2 - tx and got[1] == 2 * y - ty, (tx, ty, x, y, got)
+            assert inside == (0 <= x / 2 - tx < 64 and 0 <= 2 * y - ty < 64)
+
+
+def test_raw_h_forward_exact_cases(oracle):
+    """averageAbsoluteDifferenceH maps forward (src/gpu/cudaErrorFunctions.cuh:205-207):
+    the identity samples (x, y), a translation H samples (x + tx, y + ty) (the
+    inverse would give x - tx), and a shear H [[1, 1/2, 0], [0, 1, 0], ...]
+    samples (x + y/2, y): each entry lands in its own slot."""
+    I = [1, 0, 0, 0, 1, 0, 0, 0]
+    coef = oracle.warp_coeffs(I, 0, 0, mode=oracle.PARAM_RAW_H)
+    for x, y in [(0, 0), (5, 9), (63, 63)]:
+        assert np.array_equal(oracle.warp_point(coef, x, y)[0], np.array([x, y], np.float32))
+    coef = oracle.warp_coeffs([1, 0, 3, 0, 1, -2, 0, 0], 0, 0, mode=oracle.PARAM_RAW_H)
+    for x, y in [(0, 0), (5, 9), (63, 63)]:
+        assert np.array_equal(oracle.warp_point(coef, x, y)[0], np.array([x + 3, y - 2], np.float32))
+    coef = oracle.warp_coeffs([1, 0.5, 0, 0, 1, 0, 0, 0], 0, 0, mode=oracle.PARAM_RAW_H)
+    assert np.array_equal(oracle.warp_point(coef, 6, 8)[0], np.array([10, 8], np.float32))
+    coef = oracle.warp_coeffs([1, 0, 0, 0.25, 1, 0, 0, 0], 0, 0, mode=oracle.PARAM_RAW_H)
+    assert np.array_equal(oracle.warp_point(coef, 8, 6)[0], np.array([8, 8], np.float32))
+    # perspective row: (h7 x + h8 y + 1) divides both coordinates
+    coef = oracle.warp_coeffs([1, 0, 0, 0, 1, 0, 0.25, 0], 0, 0, mode=oracle.PARAM_RAW_H)
+    assert np.array_equal(oracle.warp_point(coef, 4, 6)[0], np.array([2, 3], np.float32))
+    coef = oracle.warp_coeffs([1, 0, 0, 0, 1, 0, 0, 0.5], 0, 0, mode=oracle.PARAM_RAW_H)
+    assert np.array_equal(oracle.warp_point(coef, 4, 6)[0], np.array([1, 1.5], np.float32))
+
+
+def test_raw_h_forward_matches_fp64_product(oracle):
+    rng = np.random.default_rng(19)
+    for _ in range(100):
+        h = np.array([rng.uniform(0.8, 1.2), rng.uniform(-0.2, 0.2), rng.uniform(-8, 8),
+                      rng.uniform(-0.2, 0.2), rng.uniform(0.8, 1.2), rng.uniform(-8, 8),
+                      rng.uniform(-1e-3, 1e-3), rng.uniform(-1e-3, 1e-3)], np.float32)
+        H = np.append(h.astype(np.float64), 1.0).reshape(3, 3)
+        coef = oracle.warp_coeffs(h, 0.0, 0.0, mode=oracle.PARAM_RAW_H)
+        for x, y in [(0, 0), (63, 0), (0, 63), (31, 17), (63, 63), (255, 200)]:
+            q = H @ np.array([x, y, 1.0])
+            got, _ = oracle.warp_point(coef, x, y)
+            want = q[:2] / q[2]
+            # two fmaf forms and a correctly rounded quotient: a few binary32 ulps
+            assert np.allclose(got, want, rtol=0, atol=4 * np.spacing(np.float32(np.abs(want).max() + 1))), \
+                (h, x, y, got, want)
+
+
 def test_in_image_on_exact_quotient(oracle):
     """The declared inImage (exact quotient of the rounded forms) agrees with
     the reference's inImage of the rounded quotient away from a 2^-24-wide
@@ -435,3 +533,24 @@ def test_covariance_closed_forms(oracle):
     # no negative cost: the scan keeps (index 0, value 0), log 0 = -inf, every slope < 0 -> zero matrix
     C, mi, mv = oracle.covariance(start, r, c2, np.array([3.0, 1.0, 2.0, 5.0, 4.0], np.float32))
     assert mi == 0 and mv == 0.0 and not C.any()
+
+
+def test_golden_workloads_match_generator_and_oracle(oracle):
+    """The committed full-grid fixtures still describe the generator's
+    inputs (image hashes), and the 64x64 NN-SAD workload's fixture equals a
+    fresh oracle run bit for bit (its full grid takes well under a second)."""
+    import hashlib
+    gold = json.load(open(os.path.join(HERE, "golden", "workloads.json")))
+    for name, g in gold.items():
+        m, f, s, e, c, spec = synth.workload(name)
+        assert g["generator_version"] == synth.GENERATOR_VERSION
+        assert hashlib.sha256(np.ascontiguousarray(f).tobytes()).hexdigest() == g["fixed_sha256"], name
+        assert hashlib.sha256(np.ascontiguousarray(m).tobytes()).hexdigest() == g["moving_sha256"], name
+    g = gold["nn_sad_64"]
+    m, f, s, e, c, spec = synth.workload("nn_sad_64")
+    c2, r = oracle.grid_update(s, e, c)
+    out = oracle.search(m, f, oracle.COST_SAD, oracle.NEAREST, s, r, c2)
+    assert oracle.argmin(out) == (np.float32(g["argmin_value"]), g["argmin_index"])
+    assert oracle.argmin(out, oracle.TIE_REFERENCE) == (np.float32(g["argmin_reference_value"]),
+                                                        g["argmin_reference_index"])
+    assert np.array_equal(out[g["sample_index"]], np.array(g["sample_value"], np.float32))
