@@ -64,6 +64,30 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
                           const cgs_search_opts *o, float *result, cgs_argmin *out, hipStream_t stream,
                           uint32_t *hist_dump_host);
 
+// Which kernel runs a plain search, and the scratch it stages: the moving
+// image's tap table (fast and MI kernels) or zero-padded copy (exact kernel),
+// the per-level (cos, sin) table of axis 0, the histogram dump.
+enum KernelKind { kKernelExact = 0, kKernelFast = 1, kKernelMI = 2 };
+struct SearchSizes {
+    KernelKind kind = kKernelExact;
+    size_t nmov = 0, ntot = 0;   // moving-image elements, and with the zero pad
+    size_t tap_bytes = 0;        // tap table (fast / MI kernels)
+    size_t mov_bytes = 0;        // padded copy (exact kernel)
+    int levels0 = 0;             // (cos, sin) entries
+    size_t hist_bytes = 0;       // one joint histogram
+};
+SearchSizes search_sizes(const cgs_grid *grid, const cgs_image *moved, const cgs_search_opts *o);
+
+// Axis-0 levels the allocation-free search stages (cos, sin) pairs for.
+constexpr int kNoAllocMaxLevels0 = 1024;
+
+// One plain (not line-traversal) search with every device scratch buffer
+// carved from `scratch` (scratch_bytes long, 256-byte aligned) and nothing
+// allocated; synchronises the stream.  The body of cgs_search_noalloc.
+cgs_status run_search_in(void *scratch, size_t scratch_bytes, const cgs_grid *grid, const cgs_image *moved,
+                         const cgs_image *fixed, const cgs_search_opts *o, float *result, cgs_argmin *out,
+                         hipStream_t stream);
+
 cgs_status validate_image(const cgs_image *im, const char *name);
 cgs_status validate_search(const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
                            const cgs_search_opts *o);

@@ -177,8 +177,6 @@ static hipError_t dispatch_mi(const SearchParams &P, int mpix, int fpix, int gpt
 // Which kernel runs a search: the fast kernel for bilinear SAD/NCC/SQD on one
 // channel, the MI kernel for bilinear MI on one channel, the exact kernel for
 // everything else (nearest neighbour, several channels).
-enum KernelKind { kKernelExact = 0, kKernelFast = 1, kKernelMI = 2 };
-
 static KernelKind kernel_kind(const cgs_image *moved, const cgs_search_opts *o) {
     if (o->sampler != CGS_SAMPLER_BILINEAR || moved->channels != 1 || moved->width > 65536 ||
         moved->height > 65536)
@@ -186,8 +184,119 @@ static KernelKind kernel_kind(const cgs_image *moved, const cgs_search_opts *o) 
     return family_of(o->cost) == kFamMI ? kKernelMI : kKernelFast;
 }
 
-// Runs one plain search: stages the moving image (padded copy or tap table),
-// uploads the (cos, sin) table, clears the key, launches, reads the key back.
+// Device and host buffers one plain search stages its scratch in: the
+// argmin key, the moving image's tap table (fast and MI kernels) or padded
+// copy (exact kernel), the (cos, sin) table and, for cgs_eval_point, the
+// histogram dump.  Host buffers hold the (cos, sin) upload and the key
+// read-back until the stream is synchronised.
+struct SearchBuffers {
+    unsigned long long *key = nullptr;
+    void *taps = nullptr;
+    void *mov = nullptr;
+    float2 *sincos = nullptr;
+    uint32_t *hist = nullptr;
+    float *sincos_host = nullptr;
+    unsigned long long *key_host = nullptr;
+};
+
+SearchSizes search_sizes(const cgs_grid *grid, const cgs_image *moved, const cgs_search_opts *o) {
+    SearchSizes z;
+    z.kind = kernel_kind(moved, o);
+    const size_t tap_bytes = z.kind == kKernelFast ? sizeof(float4) : (moved->pixtype == CGS_PIX_U8 ? 4 : 16);
+    const size_t esz = moved->pixtype == CGS_PIX_U8 ? 1 : 4;
+    const size_t nmov = (size_t) moved->width * moved->height * moved->channels;
+    z.nmov = nmov;
+    z.ntot = nmov + mov_pad_elems(moved->width, moved->channels);
+    const size_t nquad = (size_t) moved->width * moved->height + 2 * (size_t) moved->width + 2;
+    z.tap_bytes = z.kind != kKernelExact ? nquad * tap_bytes : 0;
+    z.mov_bytes = z.kind == kKernelExact ? z.ntot * esz : 0;
+    z.levels0 = (o->param_mode == CGS_PARAM_CENTRED || o->param_mode == CGS_PARAM_NONCENTRED)
+                    ? (int) grid->count[0] : 0;
+    z.hist_bytes = (size_t) o->mi_bins * o->mi_bins * sizeof(uint32_t);
+    return z;
+}
+
+// Enqueues one plain search on the stream with its scratch in B: stages the
+// moving image, uploads the (cos, sin) table, clears the key, launches and
+// copies the key (and the histogram dump) back.  ws, when not null, times the
+// search kernel with its events.  The caller synchronises the stream.
+static cgs_status enqueue_search(const SearchBuffers &B, const SearchSizes &z, const cgs_grid *grid,
+                                 const cgs_image *moved, const cgs_image *fixed, const cgs_search_opts *o,
+                                 float *result, hipStream_t stream, uint32_t *hist_dump_host, Workspace *ws) {
+    const unsigned prep_blocks = 1024;
+    if (z.kind == kKernelMI) {
+        if (moved->pixtype == CGS_PIX_U8)
+            hipLaunchKernelGGL(tap_u8_kernel, dim3(prep_blocks), dim3(256), 0, stream, (const uint8_t *) moved->data,
+                               (uint32_t *) B.taps, moved->width, moved->height);
+        else
+            hipLaunchKernelGGL(tap_f32_kernel, dim3(prep_blocks), dim3(256), 0, stream, (const float *) moved->data,
+                               (float4 *) B.taps, moved->width, moved->height);
+    } else if (z.kind == kKernelFast) {
+        if (moved->pixtype == CGS_PIX_U8)
+            hipLaunchKernelGGL(quad_kernel<uint8_t>, dim3(prep_blocks), dim3(256), 0, stream,
+                               (const uint8_t *) moved->data, (float4 *) B.taps, moved->width, moved->height);
+        else
+            hipLaunchKernelGGL(quad_kernel<float>, dim3(prep_blocks), dim3(256), 0, stream,
+                               (const float *) moved->data, (float4 *) B.taps, moved->width, moved->height);
+    } else if (moved->pixtype == CGS_PIX_U8) {
+        hipLaunchKernelGGL(pad_copy_kernel<uint8_t>, dim3(prep_blocks), dim3(256), 0, stream,
+                           (const uint8_t *) moved->data, (uint8_t *) B.mov, z.nmov, z.ntot);
+    } else {
+        hipLaunchKernelGGL(pad_copy_kernel<float>, dim3(prep_blocks), dim3(256), 0, stream,
+                           (const float *) moved->data, (float *) B.mov, z.nmov, z.ntot);
+    }
+    HIP_TRY(hipGetLastError());
+    if (z.levels0 > 0) {
+        float *h = B.sincos_host;
+        for (int i = 0; i < z.levels0; ++i) {
+            const float theta = fmaf((float) (grid->offset[0] + i), grid->resolution[0], grid->start[0]);
+            h[2 * i] = (float) std::cos((double) theta);
+            h[2 * i + 1] = (float) std::sin((double) theta);
+        }
+        HIP_TRY(hipMemcpyAsync(B.sincos, h, sizeof(float) * 2 * z.levels0, hipMemcpyHostToDevice, stream));
+    }
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, stream, B.key, 1, ~0ull);
+
+    SearchParams P{};
+    fill_grid_dev(grid, &P.grid);
+    if (z.kind != kKernelExact) {
+        int order[kMaxDim];
+        if (work_order(grid->dim, o->param_mode, order)) set_work_order(&P.grid, order);
+    }
+    const int gpt = o->tile_gpt ? o->tile_gpt : default_fast_gpt(P.grid);
+    P.sincos = z.levels0 > 0 ? B.sincos : nullptr;
+    P.param_mode = o->param_mode;
+    P.cost = o->cost;
+    P.mov = B.mov;
+    P.quads = z.kind != kKernelExact ? (const float4 *) B.taps : nullptr;   // entry 0: tap L = -(W+1)
+    P.mw = moved->width;
+    P.mh = moved->height;
+    P.mc = moved->channels;
+    P.fix = fixed->data;
+    P.fw = fixed->width;
+    P.fh = fixed->height;
+    P.cx = (float) moved->width / 2;
+    P.cy = (float) moved->height / 2;
+    P.result = result;
+    P.key = B.key;
+    P.tie_mode = o->tie_mode;
+    P.bins = o->mi_bins;
+    P.hist_dump = hist_dump_host ? B.hist : nullptr;
+    P.ceil_taps = moved->pixtype == CGS_PIX_F32;
+    const bool timed = ws && (o->flags & CGS_FLAG_TIME_KERNEL) != 0;
+    if (timed) HIP_TRY(ws->record_start(stream));
+    if (z.kind == kKernelFast) HIP_TRY(dispatch_fast(P, fixed->pixtype, gpt, stream));
+    else if (z.kind == kKernelMI) HIP_TRY(dispatch_mi(P, moved->pixtype, fixed->pixtype, o->tile_gpt, stream));
+    else HIP_TRY(dispatch_exact(P, o->sampler, moved->pixtype, fixed->pixtype, stream));
+    if (timed) HIP_TRY(ws->record_stop(stream));
+    HIP_TRY(hipMemcpyAsync(B.key_host, B.key, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    if (hist_dump_host)
+        HIP_TRY(hipMemcpyAsync(hist_dump_host, B.hist, z.hist_bytes, hipMemcpyDeviceToHost, stream));
+    return CGS_OK;
+}
+
+// Runs one plain search with its scratch in the per-device workspace (grown
+// on demand), synchronises the stream and decodes the key.
 static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
                              const cgs_search_opts *o, float *result, cgs_argmin *out, hipStream_t stream,
                              uint32_t *hist_dump_host) {
@@ -199,100 +308,68 @@ static cgs_status run_search(const cgs_grid *grid, const cgs_image *moved, const
     std::lock_guard<std::mutex> lock(ws->mu);
     if (is_alt_cost(o->cost)) return run_alt_search(ws, grid, moved, fixed, o, result, out, stream, hist_dump_host);
 
-    const KernelKind kind = kernel_kind(moved, o);
-    const bool fast = kind == kKernelFast;
-    const size_t tap_bytes = kind == kKernelFast ? sizeof(float4) : (moved->pixtype == CGS_PIX_U8 ? 4 : 16);
-    const size_t esz = moved->pixtype == CGS_PIX_U8 ? 1 : 4;
-    const size_t nmov = (size_t) moved->width * moved->height * moved->channels;
-    const size_t ntot = nmov + mov_pad_elems(moved->width, moved->channels);
-    const size_t nquad = (size_t) moved->width * moved->height + 2 * (size_t) moved->width + 2;
-    const int levels0 = (o->param_mode == CGS_PARAM_CENTRED || o->param_mode == CGS_PARAM_NONCENTRED)
-                            ? (int) grid->count[0] : 0;
-    const size_t B2 = (size_t) o->mi_bins * o->mi_bins;
+    const SearchSizes z = search_sizes(grid, moved, o);
     if ((st = ws->key.ensure(64)) != CGS_OK || (st = ws->key_host.ensure(64, true)) != CGS_OK) return st;
-    if (kind != kKernelExact) {
-        if ((st = ws->quads.ensure(nquad * tap_bytes)) != CGS_OK) return st;
-    } else if ((st = ws->mov.ensure(ntot * esz)) != CGS_OK) {
+    if (z.tap_bytes && (st = ws->quads.ensure(z.tap_bytes)) != CGS_OK) return st;
+    if (z.mov_bytes && (st = ws->mov.ensure(z.mov_bytes)) != CGS_OK) return st;
+    if (z.levels0 > 0 && ((st = ws->sincos.ensure(8 * (size_t) z.levels0)) != CGS_OK ||
+                          (st = ws->sincos_host.ensure(8 * (size_t) z.levels0, true)) != CGS_OK))
         return st;
-    }
-    if (levels0 > 0 && ((st = ws->sincos.ensure(8 * (size_t) levels0)) != CGS_OK ||
-                        (st = ws->sincos_host.ensure(8 * (size_t) levels0, true)) != CGS_OK))
-        return st;
-    if (hist_dump_host && (st = ws->hist.ensure(B2 * sizeof(uint32_t))) != CGS_OK) return st;
-
-    const unsigned prep_blocks = 1024;
-    if (kind == kKernelMI) {
-        if (moved->pixtype == CGS_PIX_U8)
-            hipLaunchKernelGGL(tap_u8_kernel, dim3(prep_blocks), dim3(256), 0, stream, (const uint8_t *) moved->data,
-                               (uint32_t *) ws->quads.ptr, moved->width, moved->height);
-        else
-            hipLaunchKernelGGL(tap_f32_kernel, dim3(prep_blocks), dim3(256), 0, stream, (const float *) moved->data,
-                               (float4 *) ws->quads.ptr, moved->width, moved->height);
-    } else if (fast) {
-        if (moved->pixtype == CGS_PIX_U8)
-            hipLaunchKernelGGL(quad_kernel<uint8_t>, dim3(prep_blocks), dim3(256), 0, stream,
-                               (const uint8_t *) moved->data, (float4 *) ws->quads.ptr, moved->width, moved->height);
-        else
-            hipLaunchKernelGGL(quad_kernel<float>, dim3(prep_blocks), dim3(256), 0, stream,
-                               (const float *) moved->data, (float4 *) ws->quads.ptr, moved->width, moved->height);
-    } else if (moved->pixtype == CGS_PIX_U8) {
-        hipLaunchKernelGGL(pad_copy_kernel<uint8_t>, dim3(prep_blocks), dim3(256), 0, stream,
-                           (const uint8_t *) moved->data, (uint8_t *) ws->mov.ptr, nmov, ntot);
-    } else {
-        hipLaunchKernelGGL(pad_copy_kernel<float>, dim3(prep_blocks), dim3(256), 0, stream,
-                           (const float *) moved->data, (float *) ws->mov.ptr, nmov, ntot);
-    }
-    HIP_TRY(hipGetLastError());
-    if (levels0 > 0) {
-        float *h = (float *) ws->sincos_host.ptr;
-        for (int i = 0; i < levels0; ++i) {
-            const float theta = fmaf((float) (grid->offset[0] + i), grid->resolution[0], grid->start[0]);
-            h[2 * i] = (float) std::cos((double) theta);
-            h[2 * i + 1] = (float) std::sin((double) theta);
-        }
-        HIP_TRY(hipMemcpyAsync(ws->sincos.ptr, h, sizeof(float) * 2 * levels0, hipMemcpyHostToDevice, stream));
-    }
-    unsigned long long *key = (unsigned long long *) ws->key.ptr;
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, stream, key, 1, ~0ull);
-
-    SearchParams P{};
-    fill_grid_dev(grid, &P.grid);
-    if (kind != kKernelExact) {
-        int order[kMaxDim];
-        if (work_order(grid->dim, o->param_mode, order)) set_work_order(&P.grid, order);
-    }
-    const int gpt = o->tile_gpt ? o->tile_gpt : default_fast_gpt(P.grid);
-    P.sincos = levels0 > 0 ? (const float2 *) ws->sincos.ptr : nullptr;
-    P.param_mode = o->param_mode;
-    P.cost = o->cost;
-    P.mov = ws->mov.ptr;
-    P.quads = kind != kKernelExact ? (const float4 *) ws->quads.ptr : nullptr;   // entry 0: tap L = -(W+1)
-    P.mw = moved->width;
-    P.mh = moved->height;
-    P.mc = moved->channels;
-    P.fix = fixed->data;
-    P.fw = fixed->width;
-    P.fh = fixed->height;
-    P.cx = (float) moved->width / 2;
-    P.cy = (float) moved->height / 2;
-    P.result = result;
-    P.key = key;
-    P.tie_mode = o->tie_mode;
-    P.bins = o->mi_bins;
-    P.hist_dump = hist_dump_host ? (uint32_t *) ws->hist.ptr : nullptr;
-    P.ceil_taps = moved->pixtype == CGS_PIX_F32;
-    const bool timed = (o->flags & CGS_FLAG_TIME_KERNEL) != 0;
-    if (timed) HIP_TRY(ws->record_start(stream));
-    if (kind == kKernelFast) HIP_TRY(dispatch_fast(P, fixed->pixtype, gpt, stream));
-    else if (kind == kKernelMI) HIP_TRY(dispatch_mi(P, moved->pixtype, fixed->pixtype, o->tile_gpt, stream));
-    else HIP_TRY(dispatch_exact(P, o->sampler, moved->pixtype, fixed->pixtype, stream));
-    if (timed) HIP_TRY(ws->record_stop(stream));
-    HIP_TRY(hipMemcpyAsync(ws->key_host.ptr, key, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    if (hist_dump_host)
-        HIP_TRY(hipMemcpyAsync(hist_dump_host, ws->hist.ptr, B2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    if (hist_dump_host && (st = ws->hist.ensure(z.hist_bytes)) != CGS_OK) return st;
+    SearchBuffers B;
+    B.key = (unsigned long long *) ws->key.ptr;
+    B.taps = ws->quads.ptr;
+    B.mov = ws->mov.ptr;
+    B.sincos = (float2 *) ws->sincos.ptr;
+    B.hist = (uint32_t *) ws->hist.ptr;
+    B.sincos_host = (float *) ws->sincos_host.ptr;
+    B.key_host = (unsigned long long *) ws->key_host.ptr;
+    if ((st = enqueue_search(B, z, grid, moved, fixed, o, result, stream, hist_dump_host, ws)) != CGS_OK) return st;
     HIP_TRY(hipStreamSynchronize(stream));
-    if (timed) HIP_TRY(ws->elapsed_ms());
-    if (out) *out = decode_key(*(unsigned long long *) ws->key_host.ptr, o->tie_mode);
+    if (o->flags & CGS_FLAG_TIME_KERNEL) HIP_TRY(ws->elapsed_ms());
+    if (out) *out = decode_key(*B.key_host, o->tie_mode);
+    return CGS_OK;
+}
+
+// The search of cgs_search_noalloc: the same kernels with their scratch
+// carved from `scratch` (device, scratch_bytes long, 256-byte aligned), host
+// staging on this call's stack, nothing allocated.
+cgs_status run_search_in(void *scratch, size_t scratch_bytes, const cgs_grid *grid, const cgs_image *moved,
+                         const cgs_image *fixed, const cgs_search_opts *o, float *result, cgs_argmin *out,
+                         hipStream_t stream) {
+    cgs_status st = validate_search(grid, moved, fixed, o);
+    if (st != CGS_OK) return st;
+    if (is_alt_cost(o->cost))
+        return set_error(CGS_ERR_UNSUPPORTED, "search_noalloc: the line-traversal costs are not supported");
+    if (o->flags & CGS_FLAG_TIME_KERNEL)
+        return set_error(CGS_ERR_UNSUPPORTED, "search_noalloc: CGS_FLAG_TIME_KERNEL is not supported");
+    const SearchSizes z = search_sizes(grid, moved, o);
+    constexpr size_t kKeyBytes = 256, kSinCosBytes = 8 * (size_t) kNoAllocMaxLevels0;
+    const size_t stage = z.tap_bytes + z.mov_bytes;
+    if (z.levels0 > kNoAllocMaxLevels0)
+        return set_error(CGS_ERR_TOO_LARGE, "search_noalloc: more than " + std::to_string(kNoAllocMaxLevels0) +
+                                                " levels on axis 0");
+    if (kKeyBytes + kSinCosBytes + stage > scratch_bytes)
+        return set_error(CGS_ERR_TOO_LARGE, "search_noalloc: the moving image needs " + std::to_string(stage) +
+                                                " bytes of staging, beyond the " +
+                                                std::to_string(scratch_bytes - kKeyBytes - kSinCosBytes) +
+                                                " the static scratch holds");
+    char *base = (char *) scratch;
+    float sincos_host[2 * kNoAllocMaxLevels0];
+    unsigned long long key_host = ~0ull;
+    SearchBuffers B;
+    B.key = (unsigned long long *) base;
+    B.sincos = (float2 *) (base + kKeyBytes);
+    B.taps = z.tap_bytes ? base + kKeyBytes + kSinCosBytes : nullptr;
+    B.mov = z.mov_bytes ? base + kKeyBytes + kSinCosBytes : nullptr;
+    B.sincos_host = sincos_host;
+    B.key_host = &key_host;
+    // the stack buffers stay alive until the stream is synchronised below
+    st = enqueue_search(B, z, grid, moved, fixed, o, result, stream, nullptr, nullptr);
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (st != CGS_OK) return st;
+    if (e != hipSuccess) return set_error(CGS_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    if (out) *out = decode_key(key_host, o->tie_mode);
     return CGS_OK;
 }
 

@@ -232,6 +232,28 @@ cgs_status cgs_search(const cgs_grid *grid, const cgs_image *moved, const cgs_im
                       const cgs_search_opts *opts, float *result, cgs_argmin *argmin_out,
                       void *stream);
 
+/* Bytes of the static device scratch cgs_search_noalloc stages in (8 MiB per
+ * GPU the library runs on, held until the process ends). */
+#define CGS_NOALLOC_SCRATCH_BYTES (8u << 20)
+
+/* cgs_search without any allocation.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and stream synchronisation as cgs_search, but the
+ * argmin key, the moving image's tap table or padded copy and the (cos, sin)
+ * table live in one static device array of CGS_NOALLOC_SCRATCH_BYTES per GPU,
+ * and no hipMalloc, hipFree or host allocation is made.  Limits: plain costs
+ * only (the line-traversal costs return CGS_ERR_UNSUPPORTED), no
+ * CGS_FLAG_TIME_KERNEL, at most 1024 levels on axis 0, and a moving image
+ * whose staging fits in the scratch (a one-channel bilinear SAD/NCC/SQD
+ * search stages 16 bytes per pixel: up to about 720 x 720 pixels); larger
+ * inputs return CGS_ERR_TOO_LARGE.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other threads, other streams) race
+ * and must be serialised by the caller. */
+cgs_status cgs_search_noalloc(const cgs_grid *grid, const cgs_image *moved, const cgs_image *fixed,
+                              const cgs_search_opts *opts, float *result, cgs_argmin *argmin_out,
+                              void *stream);
+
 /* CudaTensor::find_extrema (include/cudaTensor.cuh:215-265) over a device
  * float array of n values. */
 cgs_status cgs_find_extrema(const float *values, int64_t n, int32_t tie_mode, cgs_argmin *out,
