@@ -423,21 +423,23 @@ cgs_status cgs_image_resize(const cgs_image *in, const cgs_image *out, void *str
     const ResizeAxis X = resize_axis(in->width, out->width), Y = resize_axis(in->height, out->height);
     // one staging block: decode table, both axes' lists, then the row buffer
     std::vector<char> host;
-    auto put = [&](const void *p, size_t bytes) {
+    // copies `bytes` from p and leaves `pad` zero bytes after them (a list
+    // may be empty; the kernels never read its padding)
+    auto put = [&](const void *p, size_t bytes, size_t pad) {
         const size_t at = (host.size() + 255) / 256 * 256;
-        host.resize(at + bytes);
-        std::memcpy(host.data() + at, p, bytes);
+        host.resize(at + bytes + pad, 0);
+        if (bytes) std::memcpy(host.data() + at, p, bytes);
         return at;
     };
     float decode[256];
     for (int p = 0; p < 256; ++p) decode[p] = (float) p / 255.0f;
-    const size_t o_dec = put(decode, sizeof(decode));
-    const size_t o_xf = put(X.first.data(), X.first.size() * sizeof(int));
-    const size_t o_xs = put(X.src.data(), X.src.size() * sizeof(int) + 4);
-    const size_t o_xw = put(X.wt.data(), X.wt.size() * sizeof(float) + 4);
-    const size_t o_yf = put(Y.first.data(), Y.first.size() * sizeof(int));
-    const size_t o_ys = put(Y.src.data(), Y.src.size() * sizeof(int) + 4);
-    const size_t o_yw = put(Y.wt.data(), Y.wt.size() * sizeof(float) + 4);
+    const size_t o_dec = put(decode, sizeof(decode), 0);
+    const size_t o_xf = put(X.first.data(), X.first.size() * sizeof(int), 0);
+    const size_t o_xs = put(X.src.data(), X.src.size() * sizeof(int), 4);
+    const size_t o_xw = put(X.wt.data(), X.wt.size() * sizeof(float), 4);
+    const size_t o_yf = put(Y.first.data(), Y.first.size() * sizeof(int), 0);
+    const size_t o_ys = put(Y.src.data(), Y.src.size() * sizeof(int), 4);
+    const size_t o_yw = put(Y.wt.data(), Y.wt.size() * sizeof(float), 4);
     const size_t o_hb = (host.size() + 255) / 256 * 256;
     const size_t hb_bytes = (size_t) in->height * out->width * in->channels * sizeof(float);
     char *dev = nullptr;
@@ -529,8 +531,8 @@ cgs_status cgs_image_fuse(const cgs_image *ref, const cgs_image *mov, const floa
     std::vector<char> host;
     auto put = [&](const void *p, size_t bytes) {
         const size_t at = (host.size() + 255) / 256 * 256;
-        host.resize(at + bytes + 4);
-        std::memcpy(host.data() + at, p, bytes);
+        host.resize(at + bytes + 4, 0);
+        if (bytes) std::memcpy(host.data() + at, p, bytes);
         return at;
     };
     const size_t o_xs = put(xs.data(), xs.size() * sizeof(float)), o_ys = put(ys.data(), ys.size() * sizeof(float));

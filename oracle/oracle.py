@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_SO = os.path.join(_HERE, "_build", "libcgs_oracle.so")
+# CGS_ORACLE_SO names another build of the same source (the sanitizer build
+# of oracle/Makefile's `sanitize` target).
+_SO = os.environ.get("CGS_ORACLE_SO") or os.path.join(_HERE, "_build", "libcgs_oracle.so")
 
 COST_SAD, COST_NCC, COST_MI, COST_SQD, COST_SAD_SUM, COST_MI_PLAIN = range(6)
 COST_NCC_STREAM, COST_SQD_STREAM, COST_SAD_STREAM = 6, 7, 8
