@@ -235,10 +235,34 @@ __device__ __forceinline__ void wave_mi(const uint32_t *h, int bins, int rs, dou
     *hy_out = wave_sum(hy);
 }
 
-// The overlap test of the unit-step functors (src/gpu/cudaErrorFunctions.cuh:296, 372).
-__device__ __forceinline__ bool alt_overlap(float mov_area, float fix_area, const AltParams &P, float scale_y) {
-    return (double) (mov_area / ((float) (P.mw * P.mh) * scale_y)) > 0.25 &&
-           (double) (fix_area / (float) (P.fw * P.fh)) > 0.35;
+// The overlap test of the unit-step functors (src/gpu/cudaErrorFunctions.cuh:296, 372;
+// src/gpu/cudaErrorFunctionsStreams.cuh:272; src/gpu/cudaErrorFunction_miStreams.cuh:199),
+// its two halves: moved area / (W_m H_m scaleY) > 0.25, fixed area / (W_f H_f) > 0.35.
+__device__ __forceinline__ float alt_mov_denom(const AltParams &P, float scale_y) {
+    return (float) (P.mw * P.mh) * scale_y;
+}
+__device__ __forceinline__ bool alt_mov_test(float mov_area, float denom) { return (double) (mov_area / denom) > 0.25; }
+__device__ __forceinline__ bool alt_fix_test(float fix_area, const AltParams &P) {
+    return (double) (fix_area / (float) (P.fw * P.fh)) > 0.35;
+}
+
+// The moved-area test on the reference's binary32 sum M, decided from S, the
+// same steps * |v| terms summed in binary64 (exact to 2^-40 relative), where
+// that is certain.  M is nadds rounded adds of non-negative terms, so every
+// partial sum is at most Mmax = S (1 + nadds 2^-23) (for nadds <= 2^24) and
+// |M - S| <= nadds * ulp(Mmax) / 2.  The test is monotone in M: equal at both
+// ends of that interval, it is decided.  Returns 1 or 0, or -1 when the
+// interval straddles the threshold (then the caller forms M the reference's
+// way).
+__device__ inline int alt_mov_test_band(double S, double nadds, float denom) {
+    if (!(S >= 0.0) || !(S < 1e30) || nadds > 16777216.0) return -1;
+    const double Mmax = S * (1.0 + nadds * 0x1p-23) + 0x1p-126;
+    int e;
+    (void) frexp(Mmax, &e);   // Mmax in [2^(e-1), 2^e): binary32 half-ulp 2^(e-25)
+    const double E = nadds * ldexp(1.0, e - 25) + S * 0x1p-40;
+    const double lo = S - E > 0.0 ? S - E : 0.0, hi = S + E;
+    const bool tlo = alt_mov_test(__double2float_rd(lo), denom), thi = alt_mov_test(__double2float_ru(hi), denom);
+    return tlo == thi ? (int) tlo : -1;
 }
 
 __device__ __forceinline__ void alt_grid_point(const AltParams &P, int32_t gi, float *p, float *c, float *s) {
@@ -406,6 +430,36 @@ __device__ __forceinline__ AltLine block_line(const float *h, const AltParams &P
     return L;
 }
 
+// The reference's binary32 moved area of one grid point (centred H h), the
+// rows walked again for their step counts only: PER_ROW false adds |v| once
+// per step into one sum over every row (calcSQDAlt, calcNCCAlt,
+// src/gpu/cudaErrorFunctions.cuh:289, 365); true sums each row on its own,
+// `adds` times per step (1 for calcNCCstreamAlt, the channel count for
+// calcMIstreamAlt: src/gpu/cudaErrorFunctionsStreams.cuh:245,
+// src/gpu/cudaErrorFunction_miStreams.cuh:110), and adds the row sums in row
+// order.  Repeated adds go through add_repeat.  Wavefront-uniform; the rare
+// path of the overlap test.
+template <bool SCALAR_TESTS>
+__device__ __forceinline__ float alt_exact_mov_area(const float *h, const AltParams &P, bool per_row, int adds) {
+    float total = 0.0f;
+    for (int rb = 0; rb < P.fh; rb += 64) {
+        float magr;
+        const AltLine Lr = block_line(h, P, rb, &magr);
+        unsigned long long rows_in = __ballot(Lr.in);
+        while (rows_in) {
+            const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
+            rows_in &= rows_in - 1;
+            const int32_t k = walk_unit_row<SCALAR_TESTS>(
+                lane_value(Lr.ls, j), lane_value(Lr.le, j), (float) P.fw, lane_value(Lr.vx, j), lane_value(Lr.vy, j),
+                lane_value(Lr.px, j), lane_value(Lr.py, j), [](float, float, float, bool) {});
+            const float mag = lane_value(magr, j);
+            if (per_row) total = total + add_repeat(0.0f, mag, (int64_t) k * adds);
+            else total = add_repeat(total, mag, k);
+        }
+    }
+    return total;
+}
+
 // calcSQDAlt (SQD true) / calcNCCAlt / calcNCCstreamAlt: one grid point per
 // wavefront.  ONE: one-channel images through tap tables.
 template <typename MT, typename FT, bool SQD, bool ONE>
@@ -460,7 +514,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
             }
             mov_area += wave_sum(term);
         }
-        const bool pass = alt_overlap((float) mov_area, (float) steps, P, p[2]);
+        // the fixed area counts steps in binary32, exactly (fewer than 2^24)
+        const float denom = alt_mov_denom(P, p[2]);
+        const bool stream = P.cost == CGS_COST_NCC_STREAM_ALT;
+        int mt = alt_mov_test_band(mov_area, (double) steps + (double) P.fh, denom);
+        if (mt < 0) mt = alt_mov_test(alt_exact_mov_area<false>(h, P, stream, 1), denom);
+        const bool pass = mt && alt_fix_test((float) steps, P);
         const double S0 = wave_sum(s0), S1 = wave_sum(s1), S2 = wave_sum(s2);
         if (!pass) cost = P.cost == CGS_COST_NCC_STREAM_ALT ? 0.0f : 123123.0f;
         else if (SQD) cost = (float) (S0 / sqrt(S1 * S2));
@@ -543,7 +602,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
         for (int blk = 0; blk * 64 < P.fh; ++blk) mov_area += s_mov[blk];   // block order
         const int32_t total = s_steps[0] + s_steps[1] + s_steps[2] + s_steps[3];
         float cost = 123123.0f;
-        if (alt_overlap((float) mov_area, (float) total, P, p[2])) {
+        // both areas grow once per step and channel; the fixed one in binary32
+        // exactly (fewer than 2^24)
+        const float denom = alt_mov_denom(P, p[2]);
+        int mt = alt_mov_test_band((double) C * mov_area, (double) total * C + (double) P.fh, denom);
+        if (mt < 0) mt = alt_mov_test(alt_exact_mov_area<true>(h, P, true, C), denom);
+        if (mt && alt_fix_test((float) (total * C), P)) {
             double mi, hx, hy;
             wave_mi(hist, P.bins, P.bins + 1, (double) total * (double) C, &mi, &hx, &hy);
             cost = (float) (-2.0 * mi / (hx + hy));

@@ -150,4 +150,48 @@ CGS_HD inline int step_run(float x, float s, int32_t *a, int32_t *d, float *U) {
     return room + 1 < 64 ? room + 1 : 64;
 }
 
+// s + d added n times in binary32, the loop `area += d` of the
+// line-traversal functors' moved area (src/gpu/cudaErrorFunctions.cuh:289,
+// 365; src/gpu/cudaErrorFunctionsStreams.cuh:245;
+// src/gpu/cudaErrorFunction_miStreams.cuh:110), for s >= 0 and d >= 0 (or
+// NaN).  Inside a binade [2^e, 2^(e+1)) with grid U an add moves s by d
+// rounded to the U grid, the same increment at every step (a tie goes to the
+// even neighbour: from an even multiple of U that is the same increment
+// again, so only a tie from an odd multiple is taken alone), so a run of adds
+// that stays inside the binade is one integer multiply-add in units of U.
+// The add that leaves the binade, the first add from 0 and adds to sums
+// below 2^-103 are taken literally; an add that leaves s unchanged leaves it
+// unchanged for good.  Returns exactly what the loop returns.
+CGS_HD inline float add_repeat(float s, float d, int64_t n) {
+    if (n <= 0) return s;
+    if (!(s < __builtin_inff()) || !(d < __builtin_inff())) return s + d;   // inf and NaN stay
+    while (n > 0) {
+        const float y = s + d;
+        if (y == s) return s;
+        const uint32_t bits = f2u(s);
+        const uint32_t expo = bits & 0x7F800000u;
+        const float B = u2f(expo + 0x00800000u);
+        if (expo <= (24u << 23) || !(y < B)) {   // from 0 or tiny sums, or leaving the binade
+            s = y;
+            --n;
+            continue;
+        }
+        const float u = u2f(expo - (23u << 23));
+        const float delta = y - s;                 // exact: both on the U grid
+        const float err = d - delta;               // exact, |err| <= U / 2
+        const int32_t a = (int32_t) ((bits & 0x007FFFFFu) | 0x00800000u);   // s / U
+        if ((err == 0.5f * u || err == -0.5f * u) && (a & 1)) {
+            s = y;
+            --n;
+            continue;
+        }
+        const int32_t dd = (int32_t) __builtin_ldexpf(delta, 150 - (int32_t) (expo >> 23));   // delta / U
+        const int32_t room = (0x00FFFFFF - a) / dd;   // >= 1: a + dd < 2^24
+        const int64_t k = n < (int64_t) room ? n : (int64_t) room;
+        s = (float) (a + (int32_t) k * dd) * u;
+        n -= k;
+    }
+    return s;
+}
+
 }  // namespace cgs

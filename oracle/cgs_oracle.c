@@ -589,11 +589,17 @@ static int alt_overlap(float mov_area, float fix_area, int mw, int mh, int fw, i
  * a row goes to lane k mod 64; each lane adds, in row and step order,
  * fma(m, f) (or fma(m - f, m - f) for SQD), fma(m, m), fma(f, f) in binary64
  * (the 1/255 scaling of the reference cancels in every one of these costs);
- * the lanes combine in lane_tree order.  Moved area: the binary64 terms
- * steps * |v| (binary32 |v|) of each block of 64 rows are summed in
- * lane_tree order (row r of the block as lane r), the block sums added in
- * block order, the total rounded to binary32; fixed area: the step count.
- * MI: joint counts of every sample. */
+ * the lanes combine in lane_tree order.  Moved and fixed areas are the
+ * reference's binary32 loops: calcSQDAlt / calcNCCAlt add |v| and 1 to one
+ * sum at every step of every row (cudaErrorFunctions.cuh:289-290, 365-366);
+ * calcNCCstreamAlt adds them per row, once per step
+ * (cudaErrorFunctionsStreams.cuh:245-246), and calcMIstreamAlt per row once
+ * per step and channel (cudaErrorFunction_miStreams.cuh:110-111), and both
+ * then add the row sums in row order (cudaErrorFunctionsStreams.cuh:259-264,
+ * cudaErrorFunction_miStreams.cuh:129-135; rows without a line add 0, the MI
+ * functor's uninitialised row sums read as 0).  aux[9] is the moved area's
+ * exact value, the same adds of |v| summed in binary64.  MI: joint counts of every
+ * sample. */
 static float cost_alt_step(const orc_image *moved, const orc_image *fixed, int cost, const float *params,
                            int bins, uint32_t *hist, double *aux) {
     const int mw = moved->width, mh = moved->height, fw = fixed->width, fh = fixed->height, C = fixed->channels;
@@ -601,17 +607,29 @@ static float cost_alt_step(const orc_image *moved, const orc_image *fixed, int c
     homography_centred(params, (float) mw / 2, (float) mh / 2, h);
     double s0[64], s1[64], s2[64];
     for (int i = 0; i < 64; i++) s0[i] = s1[i] = s2[i] = 0.0;
-    double mov = 0.0, mov_block[64];
+    double mov = 0.0;
     int64_t steps = 0, count = 0;
     const int is_mi = cost == ORC_COST_MI_STREAM_ALT;
+    const int per_row = is_mi || cost == ORC_COST_NCC_STREAM_ALT;   /* the stream functors' row sums */
+    const int adds = is_mi ? C : 1;                                 /* area adds per step */
+    float mov_area = 0.0f, fix_area = 0.0f;
     for (int row = 0; row < fh; row++) {
-        if (row % 64 == 0) memset(mov_block, 0, sizeof(mov_block));
         orc_line L;
         line_centred(h, mw, mh, fw, fh, row, &L);
+        float mov_row = 0.0f, fix_row = 0.0f;
         if (L.in) {
             const float mag = sqrtf(L.vx * L.vx + L.vy * L.vy);
             int64_t k = 0;
             for (float lam = L.ls; lam < L.le && lam < (float) fw; lam += 1.0f, k++) {
+                for (int a = 0; a < adds; a++) {
+                    if (per_row) {
+                        mov_row += mag;
+                        fix_row += 1.0f;
+                    } else {
+                        mov_area += mag;
+                        fix_area += 1.0f;
+                    }
+                }
                 const float nx = lam * L.vx + L.px, ny = lam * L.vy + L.py;
                 const int lane = (int) (k & 63);
                 for (int c = 0; c < C; c++) {
@@ -629,12 +647,14 @@ static float cost_alt_step(const orc_image *moved, const orc_image *fixed, int c
                 }
                 count += C;
             }
-            if (k > 0) mov_block[row % 64] = (double) k * (double) mag;
+            mov += (double) (k * adds) * (double) mag;
             steps += k;
         }
-        if (row % 64 == 63 || row == fh - 1) mov += lane_tree(mov_block);
+        if (per_row) {
+            mov_area += mov_row;
+            fix_area += fix_row;
+        }
     }
-    const float mov_area = (float) mov, fix_area = (float) steps;
     const int pass = alt_overlap(mov_area, fix_area, mw, mh, fw, fh, params[2]);
     const double S0 = lane_tree(s0), S1 = lane_tree(s1), S2 = lane_tree(s2);
     double mi = 0, hx = 0, hy = 0;
@@ -649,7 +669,7 @@ static float cost_alt_step(const orc_image *moved, const orc_image *fixed, int c
     }
     if (aux) {
         aux[0] = (double) steps; aux[1] = S1; aux[2] = S2; aux[3] = S0; aux[4] = (double) count;
-        aux[5] = mi; aux[6] = (double) mov_area; aux[7] = (double) fix_area; aux[8] = pass;
+        aux[5] = mi; aux[6] = (double) mov_area; aux[7] = (double) fix_area; aux[8] = pass; aux[9] = mov;
     }
     return out;
 }

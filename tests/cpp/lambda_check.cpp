@@ -124,6 +124,50 @@ int main() {
             }
         }
     }
-    std::printf("%ld %ld %ld %ld %ld %ld\n", bad, n, runs512, direct, chains, rescales);
+    // add_repeat(s, d, n) against n literal adds, the moved area's
+    // recurrence: increments |v| around 1 (the line lengths of near-identity
+    // warps), exact halves and quarters of a grid (ties from odd and even
+    // multiples), tiny and huge increments, starts at 0 and along the way,
+    // and every intermediate count of each walk
+    long areas = 0;
+    std::vector<float> ds = {1.0f, 0.5f, 1.5f, 0.75f, 1.0000001f, 0.99999994f, 1e-3f, 1e-8f, 3.0f, 123.456f, 0.0f,
+                             1.0f + 0x1p-10f, 1.0f + 0x1p-11f + 0x1p-12f, 0x1p-20f, 0x1p-130f, 1e30f};
+    for (int i = 0; i < 300; ++i) ds.push_back(0.7f + 0.8f * u01(rng));
+    for (int i = 0; i < 60; ++i) ds.push_back(std::ldexp(u01(rng) + 0.5f, (int) (rng() % 40) - 20));
+    for (int i = 0; i < 60; ++i)   // (m + 1/2) * 2^-k: a tie on the grid of sums in [2^(24-k-1), 2^(24-k))
+        ds.push_back(std::ldexp((float) (rng() % 64) + 0.5f, -(int) (rng() % 14)));
+    for (float d : ds) {
+        const std::vector<float> s0s = {0.0f, 1.0f, 3.0f, 1000.5f, 4096.0f, 65535.75f, 16777215.0f, 0x1p-110f,
+                                        u01(rng) * 1000.0f};
+        for (float s0 : s0s) {
+            const int64_t len = d < 1e-6f ? 5000 : 70000;
+            float seq = s0;
+            int64_t next = 1;   // compare at n = 1, 2, 3, 5, 8, ... and at the end
+            for (int64_t k = 1; k <= len; ++k) {
+                seq = seq + d;
+                if (k == next || k == len) {
+                    ++n;
+                    ++areas;
+                    if (!same(cgs::add_repeat(s0, d, k), seq)) ++bad;
+                    next = next + next / 2 + 1;
+                }
+            }
+        }
+    }
+    // summing per-row partial sums as the stream functors do: the chained
+    // closed form equals one long literal loop
+    for (int t = 0; t < 200; ++t) {
+        float lit = 0.0f, fast = 0.0f;
+        for (int row = 0; row < 300; ++row) {
+            const float d = 0.8f + 0.5f * u01(rng);
+            const int k = (int) (rng() % 400);
+            for (int j = 0; j < k; ++j) lit = lit + d;
+            fast = cgs::add_repeat(fast, d, k);
+        }
+        ++n;
+        ++areas;
+        if (!same(lit, fast)) ++bad;
+    }
+    std::printf("%ld %ld %ld %ld %ld %ld %ld\n", bad, n, runs512, direct, chains, rescales, areas);
     return 0;
 }

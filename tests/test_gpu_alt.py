@@ -157,3 +157,75 @@ def test_alt_nonfinite_fixed_image(gpu, oracle, fn):
         assert np.array_equal(got[fin].view(np.uint32), want[fin].view(np.uint32))
     else:
         assert np.all((got[fin] == want[fin]) | (np.abs(got[fin] - want[fin]) <= MI_RTOL * np.abs(want[fin])))
+
+
+CASES_256 = [(fn, np.float32, 256, 256, 256, 256) for fn in FUNCTORS] + \
+    [(fn, np.uint8, 256, 256, 256, 256) for fn in FUNCTORS] + [(A.calcMIAlt, np.uint8, 200, 168, 200, 168)]
+
+
+@pytest.mark.parametrize("fn,dtype,mw,mh,fw,fh", CASES_256,
+                         ids=[f"{c[0].name}-{np.dtype(c[1]).name}-{c[2]}x{c[3]}" for c in CASES_256])
+def test_alt_256_sample(gpu, oracle, fn, dtype, mw, mh, fw, fh):
+    """Rows of 256 samples take four 64-step passes (the second and later
+    passes start from the previous pass's last lambda), 256 rows make four
+    64-row blocks over the four wavefronts (MI: one shared LDS histogram), and
+    200 x 168 is a moving size whose calcMIAlt rescale takes the binary64
+    form.  A 3^8 grid; 24 sampled points and the argmin against the oracle:
+    SQD/NCC costs bit for bit, MI joint histograms bit for bit and costs within
+    1e-6."""
+    m, _, _ = synth.make_pair(70 + fn.cost, mw, mh, sigma=4.0, perturb=6.0, dtype=dtype, gamma=dtype == np.uint8,
+                              noise=3.0 if dtype == np.uint8 else 0.0)
+    _, f, _ = synth.make_pair(70 + fn.cost, fw, fh, sigma=4.0, perturb=6.0, dtype=dtype)
+    s, e, c = grid_for(fn, levels=3, translate=6.0)
+    got, (v, i), grid = run(fn, m, f, s, e, c)
+    gm, gf = A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f)
+    rng = np.random.default_rng(fn.cost + mw)
+    nonempty = 0
+    for k in list(rng.choice(grid.numElements(), 24, replace=False)) + [i]:
+        p = grid.getGridPoint(k)
+        if fn.name in EXACT:
+            w, _ = oracle.cost_point(m, f, fn.cost, L.SAMPLER_BILINEAR, p, mode=fn.param_mode)
+            assert np.float32(w).view(np.uint32) == got[k].view(np.uint32), (k, w, got[k])
+        else:
+            w, _, oh = oracle.cost_point(m, f, fn.cost, L.SAMPLER_BILINEAR, p, mode=fn.param_mode, bins=fn.bins,
+                                         want_hist=True)
+            cost, hist = A.eval_point(fn, gm, gf, p)
+            assert np.array_equal(hist, oh), k
+            nonempty += hist.sum() > 0
+            for g in (got[k], cost):
+                assert g == w or abs(g - w) <= MI_RTOL * abs(w), (k, g, w)
+    assert v == np.nanmin(got) and got[i] == v
+    assert fn.name in EXACT or nonempty >= 12, nonempty
+
+
+@pytest.mark.parametrize("fn", [A.calcSQDAlt, A.calcNCCAlt, A.calcNCCstreamAlt, A.calcMIstreamAlt],
+                         ids=["calcSQDAlt", "calcNCCAlt", "calcNCCstreamAlt", "calcMIstreamAlt"])
+def test_alt_overlap_test_near_threshold(gpu, oracle, fn):
+    """A 36-point x-translation axis across the moved-area threshold of a
+    256 x 256 float pair at scale 1.3.  The kernels decide the overlap test
+    from a binary64 area and its rounding bound, and form the reference's
+    binary32 area where the bound straddles the threshold; costs equal the
+    oracle's (its literal binary32 loops) at every point.  The sweep holds
+    points of that exact path for every functor and, for the per-step sums of
+    calcSQDAlt / calcNCCAlt, points where the binary32 and the exact area
+    decide the test differently."""
+    m, f, _ = synth.make_pair(77, 256, 256, sigma=4.0, perturb=4.0, dtype=np.float32)
+    s = np.float32([0.02, 1.3, 1.3, 0, 186.76, 0.3, 0, 0])
+    e = s.copy()
+    e[4] = 186.795
+    c = np.float32([1, 1, 1, 1, 36, 1, 1, 1])
+    got, (v, i), grid = run(fn, m, f, s, e, c)
+    want = want_of(oracle, fn, m, f, s, e, c)
+    check(fn.name, got, want, v, i, oracle)
+    denom = float(np.float32(np.float32(256 * 256) * np.float32(1.3)))
+    undecided = flips = 0
+    for k in range(grid.numElements()):
+        _, aux = oracle.cost_point(m, f, fn.cost, L.SAMPLER_BILINEAR, grid.getGridPoint(k), bins=fn.bins)
+        S, nadds = aux[9], aux[7] + 256
+        E = nadds * 2.0 ** (np.frexp(S * (1 + nadds * 2.0 ** -23))[1] - 25)
+        undecided += (S - E) / denom <= 0.25 < (S + E) / denom
+        flips += (float(np.float32(np.float32(aux[6]) / np.float32(denom))) > 0.25) != \
+            (float(np.float32(np.float32(S) / np.float32(denom))) > 0.25)
+    assert undecided >= 10, undecided
+    if fn.cost in (L.COST_SQD_ALT, L.COST_NCC_ALT):
+        assert flips >= 10, flips

@@ -15,7 +15,7 @@ def test_closed_form_lambda_matches_sequential_loops(tmp_path):
                     "-I", os.path.join(ROOT, "cugridsearch_amd", "csrc"), "-o", exe,
                     os.path.join(HERE, "cpp", "lambda_check.cpp")], check=True)
     out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
-    bad, n, runs512, direct, chains, rescales = (int(v) for v in out[:6])
+    bad, n, runs512, direct, chains, rescales, areas = (int(v) for v in out[:7])
     assert n > 2_000_000 and bad == 0, (bad, n)
     # the direct form saw the integer starts and the starts from 16 up; the
     # chain every start below 2^23
@@ -26,3 +26,5 @@ def test_closed_form_lambda_matches_sequential_loops(tmp_path):
     # a 512-wide row (step 1/511, a tie in one binade) from lambda = 0 (twice:
     # +0 and -0) goes in a few runs, not one pass per step
     assert runs512 <= 2 * 16, runs512
+    # the moved area's closed-form repeated add equals the literal loop
+    assert areas > 100_000, areas

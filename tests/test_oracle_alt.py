@@ -364,3 +364,92 @@ def test_mi_alt_matches_its_parts(oracle):
     assert vs < v < 0
     v0, aux0 = oracle.cost_point(f, f, oracle.COST_MI_ALT, oracle.BILINEAR, [0, 1, 1, 0, 0, 0, 0, 0], bins=64)
     assert aux0[0] == 0 and v0 == 0.0
+
+
+def _areas_numpy(oracle, moved, fixed, p, cost):
+    """The reference's binary32 area loops in numpy float32 scalars, on the
+    oracle's own chords (orc_alt_line): calcSQDAlt / calcNCCAlt add |v| and 1
+    at every step into one sum; calcNCCstreamAlt per row, once per step;
+    calcMIstreamAlt per row, once per step and channel; the stream functors
+    then add the row sums in row order."""
+    mh, mw = moved.shape[:2]
+    fh, fw = fixed.shape[:2]
+    C = 1 if fixed.ndim == 2 else fixed.shape[2]
+    h = oracle.alt_homography(p, mw, mh, fw, fh)
+    f32 = np.float32
+    per_row = cost in (oracle.COST_NCC_STREAM_ALT, oracle.COST_MI_STREAM_ALT)
+    adds = C if cost == oracle.COST_MI_STREAM_ALT else 1
+    mov, fix = f32(0), f32(0)
+    for row in range(fh):
+        L = oracle.alt_line(h, mw, mh, fw, fh, row)
+        ls, le, vx, vy, inside = f32(L["ls"]), f32(L["le"]), f32(L["v"][0]), f32(L["v"][1]), L["inside"]
+        rm, rf = f32(0), f32(0)
+        if inside:
+            mag = np.sqrt(f32(vx * vx) + f32(vy * vy), dtype=np.float32)
+            lam = ls
+            while lam < le and lam < f32(fw):
+                for _ in range(adds):
+                    if per_row:
+                        rm, rf = f32(rm + mag), f32(rf + f32(1))
+                    else:
+                        mov, fix = f32(mov + mag), f32(fix + f32(1))
+                lam = f32(lam + f32(1))
+        if per_row:
+            mov, fix = f32(mov + rm), f32(fix + rf)
+    return mov, fix
+
+
+@pytest.mark.parametrize("cost_name", ["COST_SQD_ALT", "COST_NCC_ALT", "COST_NCC_STREAM_ALT", "COST_MI_STREAM_ALT"])
+def test_step_alt_area_recurrences(oracle, cost_name):
+    """Moved and fixed areas (aux[6], aux[7]) are the reference's binary32
+    loops bit for bit, on one- and three-channel pairs; the binary64 moved
+    area (aux[9]) is within the loops' rounding bound of them."""
+    cost = getattr(oracle, cost_name)
+    rng = np.random.default_rng(21)
+    for channels in (1, 3):
+        planes = [synth.make_pair(40 + c, 48, 40, sigma=2.0, perturb=3.0, dtype=np.uint8) for c in range(channels)]
+        m = planes[0][0] if channels == 1 else np.stack([q[0] for q in planes], -1)
+        f = planes[0][1] if channels == 1 else np.stack([q[1] for q in planes], -1)
+        for _ in range(6):
+            p = _rand_params(rng, t=6.0, k=2e-3)
+            _, aux = oracle.cost_point(m, f, cost, oracle.BILINEAR, p, bins=32)
+            mov, fix = _areas_numpy(oracle, m, f, p, cost)
+            assert np.float32(aux[6]) == mov and np.float32(aux[7]) == fix, (cost, channels, p, aux[6], mov, aux[7], fix)
+            n = aux[7] + 40
+            assert abs(aux[6] - aux[9]) <= n * np.spacing(np.float32(max(aux[9], 1.0) * 1.01)) / 2 + 1e-9
+
+
+def test_mi_stream_alt_areas_count_channels(oracle):
+    """calcMIstreamAlt adds both areas once per step and channel, so with 3
+    channels an overlap of about 27% of the fixed image passes its 0.35
+    fixed-area test (3 x 0.27) while the one-channel pair fails it."""
+    m1, f1, _ = synth.make_pair(44, 40, 40, sigma=2.0, perturb=2.0, dtype=np.uint8)
+    m3, f3 = np.stack([m1] * 3, -1), np.stack([f1] * 3, -1)
+    p = [0, 1, 1, 0, 29.0, 0, 0, 0]
+    v1, aux1 = oracle.cost_point(m1, f1, oracle.COST_MI_STREAM_ALT, oracle.BILINEAR, p, bins=32)
+    v3, aux3 = oracle.cost_point(m3, f3, oracle.COST_MI_STREAM_ALT, oracle.BILINEAR, p, bins=32)
+    assert aux1[0] == aux3[0] == 11 * 40                     # steps: 11 columns of 40 rows
+    assert aux1[7] == 440 and aux3[7] == 3 * 440             # fixed areas
+    assert v1 == np.float32(123123.0) and v3 != np.float32(123123.0)
+    # calcNCCstreamAlt adds once per step: the 3-channel pair fails as well
+    vn, auxn = oracle.cost_point(m3, f3, oracle.COST_NCC_STREAM_ALT, oracle.BILINEAR, p)
+    assert auxn[7] == 440 and vn == 0.0
+
+
+def test_sequential_area_binary32_decides_the_overlap(oracle):
+    """On a 256 x 256 pair at scale 1.3, x-translations between 186.7673 and
+    186.7873 leave the reference's binary32 moved area (the per-step sum of
+    calcNCCAlt) on one side of the 0.25 threshold and the exact area on the
+    other: the oracle's cost follows the binary32 sum (the sentinel or not)."""
+    m, f, _ = synth.make_pair(77, 256, 256, sigma=4.0, perturb=4.0, dtype=np.float32)
+    s = np.float32(1.3)
+    denom = np.float32(np.float32(256 * 256) * s)
+    flips = 0
+    for tx in np.linspace(186.76, 186.795, 36).astype(np.float32):
+        p = np.float32([0.02, s, s, 0, tx, 0.3, 0, 0])
+        v, aux = oracle.cost_point(m, f, oracle.COST_NCC_ALT, oracle.BILINEAR, p)
+        d32 = float(np.float32(np.float32(aux[6]) / denom)) > 0.25
+        d64 = float(np.float32(np.float32(aux[9]) / denom)) > 0.25
+        assert bool(aux[8]) == d32 and (v == np.float32(123123.0)) == (not d32)
+        flips += d32 != d64
+    assert flips >= 10, flips
