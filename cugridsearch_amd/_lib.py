@@ -98,6 +98,8 @@ SIGNATURES = {
     "cgs_covariance": (ctypes.c_int, [_P(Grid), ctypes.c_void_p, _P(ctypes.c_float), _P(Argmin), ctypes.c_void_p]),
     "cgs_image_filter": (ctypes.c_int, [_P(Image), _P(Image), _P(ctypes.c_float), ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_void_p]),
+    "cgs_image_filter_channels": (ctypes.c_int, [_P(Image), _P(Image), _P(ctypes.c_float), ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p]),
     "cgs_image_quantize": (ctypes.c_int, [_P(Image), ctypes.c_float, ctypes.c_void_p]),
     "cgs_image_resize": (ctypes.c_int, [_P(Image), _P(Image), ctypes.c_void_p]),
     "cgs_tensor_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p]),

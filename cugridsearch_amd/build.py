@@ -25,25 +25,30 @@ def hipcc_command(out=LIB, extra=()):
             os.path.join(HERE, "csrc"), *extra, "-o", out, *srcs]
 
 
-def driver_command(out):
-    """The C++ facade driver used by the GPU tests (tests/cpp/facade_driver.cpp)."""
+def driver_command(out, source="facade_driver.cpp"):
+    """A C++ driver of the facade used by the GPU tests (tests/cpp/<source>)."""
     return ["/opt/rocm/bin/hipcc", "--offload-arch=" + ARCH, "-O2", "-std=c++17", "-Wno-unused-result",
-            "-I", os.path.join(ROOT, "include"), "-o", out, os.path.join(ROOT, "tests", "cpp", "facade_driver.cpp"),
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+            "-I", os.path.join(ROOT, "include"), "-o", out, os.path.join(ROOT, "tests", "cpp", source),
             "-L", HERE, "-lcgs", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../../cugridsearch_amd"]
 
 
 DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "facade_driver")
+FUNCTOR_DRIVER = os.path.join(ROOT, "tests", "cpp", "_build", "functor_driver")
 
 
 def build_driver(verbose=False):
+    """Builds both facade drivers: the tag-functor driver and the one with
+    user-written __device__ functors."""
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
-    cmd = driver_command(DRIVER)
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building the facade driver")
+    for out, src in ((DRIVER, "facade_driver.cpp"), (FUNCTOR_DRIVER, "functor_driver.cpp")):
+        cmd = driver_command(out, src)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc failed building " + src)
     return DRIVER
 
 
@@ -57,6 +62,34 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed building libcgs.so")
     os.replace(LIB + ".tmp", LIB)
     return LIB
+
+
+MEX_DIR = os.path.join(HERE, "mex")
+MEX_CHECKS = {"cgs_mex.c": os.path.join(ROOT, "tests", "mex", "_build", "mex_add_check"),
+              "cgs_search_mex.c": os.path.join(ROOT, "tests", "mex", "_build", "mex_search_check")}
+
+
+def mex_command(gateway, out):
+    """A MEX gateway of cugridsearch_amd/mex/ linked with the MEX API stand-in
+    of tests/mex/ and its check program, as a plain executable (no MATLAB)."""
+    check = os.path.join(ROOT, "tests", "mex", os.path.basename(out) + ".c")
+    return ["gcc", "-O2", "-std=c11", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "tests", "mex"),
+            "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", "-o", out,
+            os.path.join(MEX_DIR, gateway), os.path.join(ROOT, "tests", "mex", "fake_mex.c"), check,
+            "-L", HERE, "-lcgs", "-L", "/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath," + HERE,
+            "-Wl,-rpath,/opt/rocm/lib"]
+
+
+def build_mex(verbose=False):
+    for gateway, out in MEX_CHECKS.items():
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        cmd = mex_command(gateway, out)
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("gcc failed building the MEX check of " + gateway)
 
 
 if __name__ == "__main__":

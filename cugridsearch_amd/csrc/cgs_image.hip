@@ -41,9 +41,12 @@ template <> __device__ __forceinline__ float to_pixel<float>(float v) { return v
 // filterProcess: the output pixel (x, y) correlates the filter with the
 // window whose top-left tap is (x - (fw-1)/2, y - (fh-1)/2); a tap outside
 // the image reads the centre pixel instead.  The running sum is not reset
-// between channels (the reference declares it once per pixel).
+// between channels (the reference declares it once per pixel).  Channel c is
+// filtered when bit c of filter_mask is set (FILTER) and copied otherwise
+// (COPY, which leaves the running sum alone).
 template <typename T>
-__global__ void filter_kernel(const T *in, T *out, int W, int H, int C, const FilterTaps taps, int fw, int fh) {
+__global__ void filter_kernel(const T *in, T *out, int W, int H, int C, const FilterTaps taps, int fw, int fh,
+                              uint32_t filter_mask) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= W || y >= H) return;
@@ -51,6 +54,10 @@ __global__ void filter_kernel(const T *in, T *out, int W, int H, int C, const Fi
     const size_t centre = ((size_t) y * W + x) * C;
     float pix = 0.0f;
     for (int c = 0; c < C; ++c) {
+        if (!((filter_mask >> c) & 1u)) {
+            out[centre + c] = in[centre + c];
+            continue;
+        }
         for (int fr = 0; fr < fh; ++fr) {
             const int yy = y0 + fr;
             for (int fc = 0; fc < fw; ++fc) {
@@ -389,6 +396,11 @@ extern "C" {
 
 cgs_status cgs_image_filter(const cgs_image *in, const cgs_image *out, const float *filter, int32_t fw, int32_t fh,
                             void *stream) {
+    return cgs_image_filter_channels(in, out, filter, fw, fh, 0xFu, stream);
+}
+
+cgs_status cgs_image_filter_channels(const cgs_image *in, const cgs_image *out, const float *filter, int32_t fw,
+                                     int32_t fh, uint32_t filter_mask, void *stream) {
     clear_error();
     cgs_status st;
     if ((st = validate_image(in, "in")) != CGS_OK || (st = validate_image(out, "out")) != CGS_OK) return st;
@@ -404,10 +416,10 @@ cgs_status cgs_image_filter(const cgs_image *in, const cgs_image *out, const flo
     hipStream_t s = (hipStream_t) stream;
     if (in->pixtype == CGS_PIX_U8)
         hipLaunchKernelGGL(filter_kernel<uint8_t>, grid, block, 0, s, (const uint8_t *) in->data,
-                           (uint8_t *) out->data, in->width, in->height, in->channels, taps, fw, fh);
+                           (uint8_t *) out->data, in->width, in->height, in->channels, taps, fw, fh, filter_mask);
     else
         hipLaunchKernelGGL(filter_kernel<float>, grid, block, 0, s, (const float *) in->data, (float *) out->data,
-                           in->width, in->height, in->channels, taps, fw, fh);
+                           in->width, in->height, in->channels, taps, fw, fh, filter_mask);
     HIP_TRY(hipGetLastError());
     return CGS_OK;
 }

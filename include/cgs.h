@@ -150,6 +150,16 @@ const char *cgs_version(void);
 cgs_status cgs_image_filter(const cgs_image *in, const cgs_image *out, const float *filter, int32_t fw, int32_t fh,
                             void *stream);
 
+/* CudaImage::filter with per-channel actions (include/cudaTensor.cuh:577-596,
+ * CHANNEL_ACTION :64-67): channel c is filtered as above when bit c of
+ * filter_mask is set (FILTER) and copied from `in` otherwise (COPY, which
+ * leaves the running sum untouched).  cgs_image_filter is filter_mask 0xF.
+ * The reference writes the output through at<pixType>(row, col), which for
+ * uint8 images truncates row and column to 8 bits (images over 256 pixels
+ * wrap); here every pixel is written at its own position.  Asynchronous. */
+cgs_status cgs_image_filter_channels(const cgs_image *in, const cgs_image *out, const float *filter, int32_t fw,
+                                     int32_t fh, uint32_t filter_mask, void *stream);
+
 /* transformPixel (src/cpu/multispectral_ImageMatcher.cu:411-430) in place on
  * a uint8 image: each byte read as a signed char, (int)(v / mod), then
  * (int)(v * mod), stored modulo 256.  mod >= 2^-20.  Asynchronous. */

@@ -14,11 +14,16 @@
 // Containers stay shallow views over caller-allocated device memory
 // (hipMalloc(&x.data(), x.bytesSize())), as in the reference.
 //
+// Functors come two ways:
+//   * tags naming a cost of the C-ABI (calcNCC, calcMIstream,
+//     averageAbsoluteDifference, ...): the library's kernels, specialised per
+//     cost, with the fused argmin;
+//   * a user's own __device__ functor, bound as in the reference (a
+//     __device__ func_byvalue_t / func_byreference_t variable read back with
+//     hipMemcpyFromSymbol, src/gpu/cudaGridSearch_main.cu:69-72, 270-271) and
+//     evaluated once per grid point by the kernels defined below, reading its
+//     images through CudaImage's device accessors;
 // Differences, by design:
-//   * functors are tags naming a cost of the C-ABI (calcNCC, calcMIstream,
-//     averageAbsoluteDifference, ...) instead of __device__ function pointers
-//     read back with cudaMemcpyFromSymbol; the kernels are specialised per
-//     cost;
 //   * errors throw cgs_error (the reference prints and exits the process);
 //   * the grid's start / end / resolution / count rows are kept on the host
 //     and mirrored into the caller's 4 x D device matrix when one is set.
@@ -33,6 +38,7 @@
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <typeinfo>
 #include <vector>
 
@@ -106,22 +112,76 @@ template <typename precision, unsigned int D>
 struct CudaTensor {
     precision *_data = nullptr;
     int32_t _dims[D] = {};
+    int32_t _dimensional_increments[D] = {};
     int32_t _total_size = 0;
 
     CudaTensor() = default;
 
     template <typename T>
-    explicit CudaTensor(const T (&dims)[D]) {
+    __host__ __device__ explicit CudaTensor(const T (&dims)[D]) {
         _total_size = 1;
         for (unsigned d = 0; d < D; ++d) {
             _dims[d] = (int32_t) dims[d];
+            _dimensional_increments[d] = d == 0 ? 1 : _dims[d - 1] * _dimensional_increments[d - 1];
             _total_size *= _dims[d];
         }
     }
 
-    precision *&data() { return _data; }
-    uint32_t size(int dim = -1) const { return dim == -1 ? (uint32_t) _total_size : (uint32_t) _dims[dim]; }
-    uint32_t bytesSize() const { return size() * sizeof(precision); }
+    __host__ __device__ precision *&data() { return _data; }
+    __host__ __device__ uint32_t size(int dim = -1) const {
+        return dim == -1 ? (uint32_t) _total_size : (dim < (int) D ? (uint32_t) _dims[dim] : 0u);
+    }
+    __host__ __device__ uint32_t bytesSize() const { return size() * sizeof(precision); }
+
+    // toIndex1d (include/cudaTensor.cuh:171-180): axis 0 fastest.  With
+    // floating-point indices the reference adds each float term to the running
+    // int index and truncates; here each step is one fmaf (the library's
+    // declared "truncated float linear index"), so device code indexes exactly
+    // as the library's nearest-neighbour sampler.
+    template <typename T>
+    __host__ __device__ int toIndex1d(const T (&axis_index)[D]) const {
+        int idx = (int) axis_index[0];
+        for (unsigned a = 1; a < D; ++a) {
+            if constexpr (std::is_floating_point<T>::value)
+                idx = (int) fmaf((float) _dimensional_increments[a], (float) axis_index[a], (float) idx);
+            else
+                idx += _dimensional_increments[a] * (int) axis_index[a];
+        }
+        return idx;
+    }
+
+    // toIndexNd (include/cudaTensor.cuh:182-188)
+    __host__ __device__ void toIndexNd(uint32_t index1d, uint32_t (&indexNd)[D]) const {
+        for (int a = (int) D - 1; a >= 0; --a) {
+            indexNd[a] = index1d / (uint32_t) _dimensional_increments[a];
+            index1d -= indexNd[a] * (uint32_t) _dimensional_increments[a];
+        }
+    }
+
+    // element access on the device (include/cudaTensor.cuh:147-150, 162-164)
+    __device__ precision &operator[](int index) const { return _data[index]; }
+    __device__ precision &at(const uint32_t (&pt)[D]) const { return _data[toIndex1d(pt)]; }
+
+    // display (include/cudaTensor.cuh:273-300): "Tensor <name> d0 x d1 ... elements
+    // of <type>", then every value in one brace group, a line break after each
+    // row_stride values.
+    void display(const std::string &name = "", uint32_t row_stride = 0xFFFFFFFFu) const {
+        const std::vector<precision> h = values();
+        std::cout << "Tensor " << name << " ";
+        for (unsigned d = 0; d < D; ++d) std::cout << _dims[d] << (d + 1 < D ? " x " : "");
+        std::cout << " elements of " << typeid(precision).name() << "\n\n{ ";
+        for (int32_t i = 0; i < _total_size; ++i) {
+            std::cout << h[i] << (i + 1 < _total_size ? ", " : " ");
+            if ((uint32_t) (i + 1) % row_stride == 0) std::cout << std::endl;
+        }
+        std::cout << "} " << std::endl;
+    }
+
+    // transform (include/cudaTensor.cuh:322-332, kernel transformProcess
+    // include/cudaTensorKernels.cuh:341-351): *this[i] = fn(*this[i], A[i]) with a
+    // __device__ callable fn, one thread per element.
+    template <typename T>
+    CudaTensor transform(const CudaTensor &A, T fn);
 
     void setValuesFromVector(const std::vector<precision> &vals) const {
         cgs_check_hip(hipMemcpy(_data, vals.data(), vals.size() * sizeof(precision), hipMemcpyHostToDevice),
@@ -244,15 +304,85 @@ struct CudaMatrix : CudaTensor<precision, 2> {
     }
 };
 
+// Per-channel action of CudaImage::filter (include/cudaTensor.cuh:64-67).
+enum CHANNEL_ACTION { COPY = 0, FILTER = 1 };
+
 // CudaImage<P, C>(H, W) (include/cudaTensor.cuh:507-596): dims {C, W, H}.
 template <typename precision, uint32_t CHANNELS = 1>
 struct CudaImage : CudaTensor<precision, 3> {
-    CudaImage(uint32_t height, uint32_t width) : CudaTensor<precision, 3>({CHANNELS, width, height}) {}
-    int32_t channels() const { return this->_dims[0]; }
-    int32_t width() const { return this->_dims[1]; }
-    int32_t height() const { return this->_dims[2]; }
+    __host__ __device__ CudaImage(uint32_t height, uint32_t width) : CudaTensor<precision, 3>({CHANNELS, width, height}) {}
+    __host__ __device__ int32_t channels() const { return this->_dims[0]; }
+    __host__ __device__ int32_t width() const { return this->_dims[1]; }
+    __host__ __device__ int32_t height() const { return this->_dims[2]; }
     cgs_image view() const {
         return cgs_image{this->_data, width(), height(), channels(), CgsPixType<precision>::value};
+    }
+
+    // ---- device accessors (include/cudaTensor.cuh:531-575), for user functors.
+    // They follow the library's declared arithmetic, so a functor that samples
+    // through them sees what the library's kernels see: the truncated float
+    // linear index with one fmaf per axis, bilinear taps at floor and ceil with
+    // each lerp as fmaf(a, q, (1 - a) * p), and 0 for a tap past the image (the
+    // reference reads past its buffer there).
+
+    // toIndex (include/cudaTensor.cuh:531-534): element (y, x, c)
+    template <typename T>
+    __host__ __device__ int toIndex(T y, T x, T c = 0) const {
+        const T ix[3] = {c, x, y};
+        return this->toIndex1d(ix);
+    }
+
+    // at (include/cudaTensor.cuh:536-539)
+    template <typename T>
+    __device__ precision &at(T row_index, T column_index, T channel_index = 0) const {
+        return this->_data[toIndex(row_index, column_index, channel_index)];
+    }
+
+    // inImage (include/cudaTensor.cuh:548-550)
+    __host__ __device__ bool inImage(float y, float x) const {
+        return x >= 0 && y >= 0 && x < (float) width() && y < (float) height();
+    }
+
+    // valueAt (include/cudaTensor.cuh:552-555): bilinear
+    __device__ float valueAt(float y, float x, float c = 0) const { return valueAt_bilinear(y, x, c); }
+
+    // valueAt_nearest_neighbor (include/cudaTensor.cuh:557-559)
+    __device__ precision valueAt_nearest_neighbor(float y, float x, float c = 0) const {
+        return inImage(y, x) ? (precision) tap(toIndex(y, x, c)) : (precision) 0;
+    }
+
+    // valueAt_bilinear (include/cudaTensor.cuh:561-575)
+    __device__ float valueAt_bilinear(float y, float x, float c = 0) const {
+#pragma clang fp contract(off)
+        if (!inImage(y, x)) return 0.0f;
+        const float fx = floorf(x), fy = floorf(y);
+        const float ax = x - fx, ay = y - fy;
+        const int C = channels(), W = width();
+        const int b = ((int) fy * W + (int) fx) * C + (int) c;
+        const int dx = ax != 0.0f ? C : 0, dy = ay != 0.0f ? C * W : 0;   // ceil == floor at an integer
+        const float tl = tap(b), tr = tap(b + dx), bl = tap(b + dy), br = tap(b + dy + dx);
+        const float top = fmaf(ax, tr, (1.0f - ax) * tl);
+        const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
+        return fmaf(ay, bottom, (1.0f - ay) * top);
+    }
+
+    // filter (include/cudaTensor.cuh:577-596): filterProcess with a per-channel
+    // COPY / FILTER action (CHANNEL_ACTION), through cgs_image_filter_channels;
+    // the filter image's values are read back to the host first.
+    CudaImage filter(const CudaImage<float, 1> &filt, CudaImage &out, const CHANNEL_ACTION actions[CHANNELS]) {
+        const std::vector<float> taps = filt.values();
+        uint32_t mask = 0;
+        for (uint32_t c = 0; c < CHANNELS; ++c) mask |= (actions[c] == COPY ? 0u : 1u) << c;
+        const cgs_image in = view(), ov = out.view();
+        cgs_check(cgs_image_filter_channels(&in, &ov, taps.data(), filt.width(), filt.height(), mask, nullptr),
+                  "CudaImage::filter");
+        cgs_check_hip(hipDeviceSynchronize(), "CudaImage::filter");
+        return out;
+    }
+
+  private:
+    __device__ float tap(int i) const {
+        return (i >= 0 && i < this->_total_size) ? (float) this->_data[i] : 0.0f;
     }
 };
 
@@ -263,11 +393,15 @@ template <typename T, unsigned int D>
 struct Vec {
     T v[D] = {};
     Vec() = default;
-    explicit Vec(const T (&a)[D]) {
+    __host__ __device__ explicit Vec(const T (&a)[D]) {
         for (unsigned i = 0; i < D; ++i) v[i] = a[i];
     }
-    T &operator[](unsigned i) { return v[i]; }
-    const T &operator[](unsigned i) const { return v[i]; }
+    __host__ __device__ explicit Vec(T fill) {
+        for (unsigned i = 0; i < D; ++i) v[i] = fill;
+    }
+    __host__ __device__ T &operator[](unsigned i) { return v[i]; }
+    __host__ __device__ const T &operator[](unsigned i) const { return v[i]; }
+    __host__ __device__ constexpr unsigned size() const { return D; }
 };
 }  // namespace nv_ext
 
@@ -358,6 +492,107 @@ template <typename pixType, uint32_t CHANNELS, typename HV>
 void writeAlignedAndFusedImageToDisk_local(CudaImage<pixType, CHANNELS> image_fix, CudaImage<pixType, CHANNELS> image_mov,
                                            const HV &estimatedH, const HV &trueH, std::string img_fused_filename) {
     cgs_write_fused<pixType, CHANNELS>(image_fix, image_mov, estimatedH, trueH, img_fused_filename, 1);
+}
+
+// ------------------------------------------------- device callables, functors
+
+// func_byvalue_t / func_byreference_t (src/gpu/cudaGridSearch.cuh:284-285,
+// 484-485): the signatures of a user's cost functor.  Bind one as the
+// reference's drivers do:
+//   __device__ func_byvalue_t<float, float, 8, CudaImage<uint8_t>, CudaImage<uint8_t>> dev_ptr = myCost<...>;
+//   hipMemcpyFromSymbol(&host_ptr, HIP_SYMBOL(dev_ptr), sizeof(host_ptr));
+//   searcher.search_by_value(host_ptr, image_mov, image_fix);
+template <typename func_precision, typename grid_precision, unsigned int D, typename... Types>
+using func_byvalue_t = func_precision (*)(nv_ext::Vec<grid_precision, D> &gridpt, Types... arg_vals);
+template <typename func_precision, typename grid_precision, unsigned int D, typename... Types>
+using func_byreference_t = func_precision (*)(nv_ext::Vec<grid_precision, D> &gridpt, Types *... arg_ptrs);
+
+namespace cgs_facade {
+
+// indexToGridPoint (src/gpu/cudaGridSearch.cuh:160-187) as the library's
+// kernels compute it: mixed-radix decode, axis 0 fastest, each coordinate
+// fmaf(offset + i, resolution, start).
+template <typename G, unsigned D>
+__device__ inline void grid_point(const cgs_grid &g, int32_t index, nv_ext::Vec<G, D> &pt) {
+    int32_t rem = index;
+    for (int a = (int) D - 1; a >= 0; --a) {
+        const int32_t i = rem / g.increments[a];
+        rem -= i * g.increments[a];
+        pt[a] = (G) fmaf((float) (i + g.offset[a]), g.resolution[a], g.start[a]);
+    }
+}
+
+// evaluationKernel_by_value / _by_reference (src/gpu/cudaGridSearch.cuh:287-314,
+// 487-511): one thread per grid point.
+template <typename F, typename G, unsigned D, typename... Types>
+__global__ void by_value_kernel(cgs_grid g, F *result, int32_t n, func_byvalue_t<F, G, D, Types...> op,
+                                Types... args) {
+    const int32_t i = (int32_t) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    nv_ext::Vec<G, D> pt;
+    grid_point(g, i, pt);
+    result[i] = (*op)(pt, args...);
+}
+
+template <typename F, typename G, unsigned D, typename... Types>
+__global__ void by_reference_kernel(cgs_grid g, F *result, int32_t n, func_byreference_t<F, G, D, Types...> op,
+                                    Types *... args) {
+    const int32_t i = (int32_t) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    nv_ext::Vec<G, D> pt;
+    grid_point(g, i, pt);
+    result[i] = (*op)(pt, args...);
+}
+
+// evaluationKernel_by_value_block / _stream and _by_reference_stream
+// (src/gpu/cudaGridSearch.cuh:421-482, 513-538): one workgroup per grid point,
+// every thread calls the functor (the cooperative "stream" functors take
+// threadIdx.x as their row and reduce through shared memory), and thread 0's
+// value is stored.  The reference lets every thread store its own return
+// value, racing with thread 0's; thread 0's is the one its functors compute.
+template <typename F, typename G, unsigned D, typename... Types>
+__global__ void by_value_block_kernel(cgs_grid g, F *result, int32_t n, func_byvalue_t<F, G, D, Types...> op,
+                                      Types... args) {
+    const int32_t i = (int32_t) blockIdx.x;
+    if (i >= n) return;
+    nv_ext::Vec<G, D> pt;
+    grid_point(g, i, pt);
+    const F v = (*op)(pt, args...);
+    if (threadIdx.x == 0) result[i] = v;
+}
+
+template <typename F, typename G, unsigned D, typename... Types>
+__global__ void by_reference_block_kernel(cgs_grid g, F *result, int32_t n,
+                                          func_byreference_t<F, G, D, Types...> op, Types *... args) {
+    const int32_t i = (int32_t) blockIdx.x;
+    if (i >= n) return;
+    nv_ext::Vec<G, D> pt;
+    grid_point(g, i, pt);
+    const F v = (*op)(pt, args...);
+    if (threadIdx.x == 0) result[i] = v;
+}
+
+// transformProcess (include/cudaTensorKernels.cuh:341-351)
+template <typename P, unsigned D, typename T>
+__global__ void transform_kernel(CudaTensor<P, D> A, CudaTensor<P, D> B, T fn) {
+    const int32_t i = (int32_t) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= A._total_size) return;
+    A._data[i] = fn(A._data[i], B._data[i]);
+}
+
+}  // namespace cgs_facade
+
+template <typename precision, unsigned int D>
+template <typename T>
+CudaTensor<precision, D> CudaTensor<precision, D>::transform(const CudaTensor &A, T fn) {
+    if (A._total_size != _total_size) throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "transform: tensor sizes differ");
+    if (_total_size > 0) {
+        hipLaunchKernelGGL((cgs_facade::transform_kernel<precision, D, T>), dim3((_total_size + 255) / 256), dim3(256),
+                           0, nullptr, *this, A, fn);
+        cgs_check_hip(hipGetLastError(), "transform");
+        cgs_check_hip(hipDeviceSynchronize(), "transform");
+    }
+    return *this;
 }
 
 // --------------------------------------------------------------------- grid
@@ -483,6 +718,90 @@ struct CudaGridSearcher {
         for (unsigned i = 0; i < D * D; ++i) covar_matrix[i] += c[i];
     }
 
+    // ---- a user's own __device__ functor (src/gpu/cudaGridSearch.cuh:540-765):
+    // the costs land in the result tensor in grid order and last_argmin holds
+    // find_extrema's minimum of them (tie_mode).  Synchronises the stream.
+    template <typename... Types>
+    void search(func_byvalue_t<func_precision, grid_precision, D, Types...> fn, Types &... args) {
+        search_by_value(fn, args...);
+    }
+
+    // evaluationKernel_by_value: one thread per grid point
+    template <typename... Types>
+    void search_by_value(func_byvalue_t<func_precision, grid_precision, D, Types...> fn, Types &... args) {
+        const int32_t n = begin_user_search();
+        hipLaunchKernelGGL((cgs_facade::by_value_kernel<func_precision, grid_precision, D, Types...>),
+                           dim3((n + 255) / 256), dim3(256), 0, stream, _grid->_g, _result->data(), n, fn, args...);
+        finish_user_search("search_by_value");
+    }
+
+    // evaluationKernel_by_value_stream: one workgroup of numCores threads per
+    // grid point (at most 1024); STREAM_BLOCK_DIM, the reference's stream
+    // count, is accepted and unused (one launch covers every point)
+    template <typename... Types>
+    void search_by_value_stream(func_byvalue_t<func_precision, grid_precision, D, Types...> fn, int STREAM_BLOCK_DIM,
+                                int numCores, Types &... args) {
+        (void) STREAM_BLOCK_DIM;
+        search_by_value_block(fn, numCores, args...);
+    }
+
+    // evaluationKernel_by_value_block: one workgroup of numThreads threads per point
+    template <typename... Types>
+    void search_by_value_block(func_byvalue_t<func_precision, grid_precision, D, Types...> fn, int numThreads,
+                               Types &... args) {
+        const int32_t n = begin_user_search();
+        hipLaunchKernelGGL((cgs_facade::by_value_block_kernel<func_precision, grid_precision, D, Types...>), dim3(n),
+                           dim3(block_threads(numThreads)), 0, stream, _grid->_g, _result->data(), n, fn, args...);
+        finish_user_search("search_by_value_block");
+    }
+
+    // evaluationKernel_by_reference: the arguments are device pointers
+    template <typename... Types>
+    void search_by_reference(func_byreference_t<func_precision, grid_precision, D, Types...> fn, Types *... args) {
+        const int32_t n = begin_user_search();
+        hipLaunchKernelGGL((cgs_facade::by_reference_kernel<func_precision, grid_precision, D, Types...>),
+                           dim3((n + 255) / 256), dim3(256), 0, stream, _grid->_g, _result->data(), n, fn, args...);
+        finish_user_search("search_by_reference");
+    }
+
+    template <typename... Types>
+    void search_by_reference_stream(func_byreference_t<func_precision, grid_precision, D, Types...> fn,
+                                    int STREAM_BLOCK_DIM, int numCores, Types *... args) {
+        (void) STREAM_BLOCK_DIM;
+        const int32_t n = begin_user_search();
+        hipLaunchKernelGGL((cgs_facade::by_reference_block_kernel<func_precision, grid_precision, D, Types...>),
+                           dim3(n), dim3(block_threads(numCores)), 0, stream, _grid->_g, _result->data(), n, fn,
+                           args...);
+        finish_user_search("search_by_reference_stream");
+    }
+
+  private:
+    static int block_threads(int t) { return t < 1 ? 1 : (t > 1024 ? 1024 : t); }
+
+    int32_t begin_user_search() {
+        if (_result->size() != _grid->numElements())
+            throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "result tensor size differs from the grid's point count");
+        if (!_result->data()) throw cgs_error(CGS_ERR_INVALID_ARGUMENT, "result tensor has no device memory");
+        std::cout << "CudaGrid has " << _grid->numElements() << " search samples." << std::endl;
+        return (int32_t) _grid->numElements();
+    }
+
+    void finish_user_search(const char *what) {
+        cgs_check_hip(hipGetLastError(), what);
+        cgs_check_hip(hipStreamSynchronize(stream), what);
+        if constexpr (std::is_same<func_precision, float>::value) {
+            cgs_check(cgs_find_extrema((const float *) _result->data(), _result->size(), tie_mode, &last_argmin,
+                                       stream),
+                      what);
+        } else {   // the C-ABI's argmin reads floats: the lowest-index minimum on the host
+            const std::vector<func_precision> v = _result->values();
+            last_argmin = cgs_argmin{INFINITY, -1};
+            for (size_t i = 0; i < v.size(); ++i)
+                if ((float) v[i] < last_argmin.value) last_argmin = cgs_argmin{(float) v[i], (int32_t) i};
+        }
+    }
+
+  public:
     template <typename Img> void search_by_value_stream(const CgsFunctor &f, int, int, Img &m, Img &x) {
         search_by_value(f, m, x);
     }

@@ -254,7 +254,7 @@ def cost_point(moved, fixed, cost, sampler, params, mode=PARAM_CENTRED, bins=32,
     m, f = image(moved), image(fixed)
     p = np.zeros(8, np.float32)
     p[:len(params)] = params
-    aux = np.zeros(9, np.float64)
+    aux = np.zeros(16, np.float64)
     hist = np.zeros(bins * bins, np.uint32) if want_hist else None
     hp = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if want_hist else None
     v = lib().orc_cost_point(ctypes.byref(m), ctypes.byref(f), cost, sampler, mode, _f32p(p), bins, hp,
