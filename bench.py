@@ -9,7 +9,10 @@ Prints one JSON line.  value = Gpixel-evals/s; pairs_per_s and whether the
 argmin matches the oracle's full-grid argmin (committed fixture) ride along.
 roofline: algorithmic FLOP per launch / the search kernel's average duration
 measured with HIP events on its stream.  cpu_baseline: the C oracle on a
-bounded contiguous slice of the same grid, OpenMP over the host cores.
+bounded contiguous slice of the same grid, OpenMP over the host's CPU share
+(and one thread); for the default workload the CPU configuration (64x64
+NN-SAD, 4^8 grid) runs in full on both sides.  accuracy: the run's costs
+against the committed full-grid oracle samples (binary64 sums).
 """
 import argparse
 import json
@@ -56,22 +59,113 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(m, f, fn, start, end, count, pe_per_point, target_s):
+def host_cpu():
+    """The host's CPU: the model name, the logical CPU count, the CPUs this
+    process may run on, and the thread count the baseline uses (the
+    OMP_NUM_THREADS the box sets for its CPU share, else every allowed CPU)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or allowed)
+    return {"model": model, "nproc": os.cpu_count(), "allowed_cpus": allowed, "threads": max(1, min(threads, allowed))}
+
+
+def _oracle_rate(m, f, fn, start, end, count, pe_per_point, target_s, threads):
+    """The C oracle on the first n grid points of the workload, n sized for
+    about target_s seconds: (Gpixel-evals/s, n, seconds)."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
     c2, r = O.grid_update(start, end, count)
     kw = dict(mode=fn.param_mode, bins=fn.bins, nthreads=threads)
-    n = 256
+    total = O.grid_num_elements(c2)
+    n, dt = 64, 0.0
+    while n < total:   # calibrate on a sample long enough to time (>= 1 s, or a tenth of the target)
+        t = time.perf_counter()
+        O.search(m, f, fn.cost, fn.sampler, start, r, c2, i0=0, i1=n, **kw)
+        dt = time.perf_counter() - t
+        if dt >= min(1.0, target_s / 10):
+            break
+        n = min(total, n * 4)
+    n = int(min(total, max(64, n * target_s / max(dt, 1e-6))))
     t = time.perf_counter()
     O.search(m, f, fn.cost, fn.sampler, start, r, c2, i0=0, i1=n, **kw)
     dt = time.perf_counter() - t
-    n = int(min(O.grid_num_elements(c2), max(256, n * target_s / max(dt, 1e-6))))
-    t = time.perf_counter()
-    O.search(m, f, fn.cost, fn.sampler, start, r, c2, i0=0, i1=n, **kw)
-    dt = time.perf_counter() - t
-    return {"value": n * pe_per_point / dt / 1e9, "unit": "Gpixel-evals/s", "cores": threads, "kind": "port",
+    return n * pe_per_point / dt / 1e9, n, dt
+
+
+def cpu_baseline(m, f, fn, start, end, count, pe_per_point, target_s):
+    """The C oracle (the CPU restatement of the reference's functor loops) on
+    a bounded sample of the same workload: all of the host's threads, then one
+    thread on a smaller sample."""
+    cpu = host_cpu()
+    v, n, dt = _oracle_rate(m, f, fn, start, end, count, pe_per_point, target_s, cpu["threads"])
+    v1, n1, dt1 = _oracle_rate(m, f, fn, start, end, count, pe_per_point, max(1.0, target_s / 4), 1)
+    return {"value": v, "unit": "Gpixel-evals/s", "cores": cpu["threads"], "kind": "port",
             "sample": f"grid indices [0, {n}) of the same workload ({n * pe_per_point:.3e} pixel-evals, "
-                      f"{dt:.1f} s), C oracle -O3, OpenMP {threads} threads"}
+                      f"{dt:.1f} s), C oracle -O3, OpenMP {cpu['threads']} threads; one thread: [0, {n1}) in "
+                      f"{dt1:.1f} s",
+            "one_thread_value": v1, "cpu_model": cpu["model"], "nproc": cpu["nproc"],
+            "allowed_cpus": cpu["allowed_cpus"]}
+
+
+def c1_cpu_and_gpu():
+    """The CPU configuration (64x64 uint8, nearest-neighbour SAD, 4^8 grid) in
+    full on the host (all threads and one) and on the GPU: rates, and whether
+    the argmins agree bit for bit."""
+    import ctypes
+    import torch
+    from cugridsearch_amd import _lib as L
+    from cugridsearch_amd import api as A
+    from cugridsearch_amd import synth
+    from oracle import oracle as O
+    m, f, s, e, c, _ = synth.workload("nn_sad_64")
+    fn = A.centredAbsoluteDifference.with_(sampler=L.SAMPLER_NEAREST)
+    c2, r = O.grid_update(s, e, c)
+    n = O.grid_num_elements(c2)
+    pe = n * f.size
+    cpu = host_cpu()
+    out = {"workload": "nn_sad_64 (64x64 uint8, nearest, SAD, 4^8 grid)", "pixel_evals": pe, "threads": cpu["threads"]}
+    for key, th in (("cpu_gpixel_evals_per_s", cpu["threads"]), ("cpu_one_thread_gpixel_evals_per_s", 1)):
+        t = time.perf_counter()
+        want = O.search(m, f, O.COST_SAD, O.NEAREST, s, r, c2, nthreads=th)
+        out[key] = pe / (time.perf_counter() - t) / 1e9
+    g = A.CudaGrid(8)
+    g.setStartPoint(s)
+    g.setEndPoint(e)
+    g.setNumSamples(c)
+    res = A.CudaTensor(g.getAxisSampleCounts())
+    gm, gf = A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f)
+    opts = A.make_opts(fn)
+    opts.flags = L.FLAG_TIME_KERNEL
+    am = L.Argmin()
+    kms = []
+    for _ in range(6):
+        L.check(L.lib().cgs_search(ctypes.byref(g._g), ctypes.byref(gm._img), ctypes.byref(gf._img),
+                                   ctypes.byref(opts), ctypes.c_void_p(res.data.data_ptr()), ctypes.byref(am), None))
+        kms.append(L.lib().cgs_last_kernel_ms())
+    torch.cuda.synchronize()
+    got = res.data.cpu().numpy()
+    out["gpu_kernel_ms"] = float(np.median(kms[1:]))
+    out["gpu_gpixel_evals_per_s"] = pe / (out["gpu_kernel_ms"] / 1000.0) / 1e9
+    out["argmin"] = {"index": am.index, "value": am.value}
+    out["bit_exact_vs_cpu"] = bool(np.array_equal(got.view(np.uint32), want.view(np.uint32))
+                                   and (am.value, am.index) == O.argmin(want))
+    return out
+
+
+def fma_peak():
+    """The device's measured non-packed FP32 FMA throughput (TFLOP/s)."""
+    import ctypes
+    from cugridsearch_amd import _lib as L
+    v = ctypes.c_float()
+    L.check(L.lib().cgs_fma_peak(ctypes.byref(v), None))
+    return float(v.value)
 
 
 def run_batch(args):
@@ -392,6 +486,7 @@ def main():
     kernel_ms = float(np.mean(kms))
 
     matched = None
+    accuracy = None
     gold_path = os.path.join(ROOT, "tests", "golden", "workloads.json")
     if os.path.exists(gold_path) and not args.functor:
         gold = json.load(open(gold_path)).get(args.workload)
@@ -399,10 +494,41 @@ def main():
             rtol = 0.0 if spec["sampler"] == "nearest" else 1e-5
             matched = bool(out.index == gold["argmin_index"] or
                            abs(out.value - gold["argmin_value"]) <= rtol * abs(gold["argmin_value"]))
+            # every committed full-grid oracle sample (binary64 accumulation, rounded
+            # to binary32) against this run's cost tensor, outside the timed region
+            got = res.data.cpu().numpy().astype(np.float64)
+            si = np.array(gold["sample_index"])
+            sv = np.array(gold["sample_value"], np.float64)
+            fin = np.isfinite(sv) & (sv != 0)
+            rel = np.abs(got[si][fin] - sv[fin]) / np.abs(sv[fin])
+            accuracy = {"max_rel_err_vs_fp64_oracle": float(rel.max()), "mean_rel_err_vs_fp64_oracle": float(rel.mean()),
+                        "samples": int(fin.sum()),
+                        "argmin_rel_err_vs_fp64_oracle": abs(out.value - gold["argmin_value"]) /
+                        abs(gold["argmin_value"]),
+                        "oracle": "tests/golden/workloads.json (full-grid C oracle, binary64 sums)"}
 
     flop_pe = FLOP_PER_PE_ALT[fn.name] if args.functor else FLOP_PER_PE[(spec["sampler"], spec["cost"])]
     flop = pe * flop_pe
+    steps_note = None
+    if args.functor:
+        # the line-traversal kernels sample only the steps inside the overlap:
+        # count them on 32 sampled grid points (the oracle's step count) and
+        # price the FLOP on that estimate; check the argmin's cost as well
+        from oracle import oracle as O
+        rng = np.random.default_rng(0)
+        ks = rng.choice(n, 32, replace=False)
+        steps = [O.cost_point(m, f, fn.cost, fn.sampler, grid.getGridPoint(int(k)), mode=fn.param_mode,
+                              bins=fn.bins)[1][0] for k in ks]
+        per_point = float(np.mean(steps))
+        flop = per_point * n * flop_pe
+        steps_note = {"samples_per_point_est": per_point, "fixed_pixels_per_point": pe_per_point,
+                      "estimated_from": "oracle step counts of 32 seeded grid points"}
+        ov, _ = O.cost_point(m, f, fn.cost, fn.sampler, grid.getGridPoint(out.index), mode=fn.param_mode, bins=fn.bins)
+        exact = fn.cost in (L.COST_SQD_ALT, L.COST_NCC_ALT, L.COST_NCC_STREAM_ALT)
+        matched = bool(np.float32(ov) == np.float32(out.value) if exact else
+                       abs(ov - out.value) <= 1e-6 * abs(ov))
     achieved = flop / (kernel_ms / 1000.0) / 1e12
+    measured_peak = fma_peak()
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"search_{args.workload}_traffic.json")
     if os.path.exists(tpath) and not args.functor:
@@ -425,10 +551,18 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_VALU_TFLOPS, "traffic": traffic,
                      "frac_of_nonpacked_fp32": achieved / NONPACKED_FP32_TFLOPS,
+                     "measured_fma_peak": measured_peak, "frac_of_measured_fma_peak": achieved / measured_peak,
                      "flop_per_pixel_eval": flop_pe},
+        "lib": L.lib().cgs_version().decode(),
     }
+    if accuracy:
+        line["accuracy"] = accuracy
+    if steps_note:
+        line["roofline"]["steps"] = steps_note
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(m, f, fn, start, end, count, pe_per_point, args.cpu_seconds)
+        if args.workload == "bilinear_ncc_256" and not args.functor:
+            line["c1"] = c1_cpu_and_gpu()
     print(json.dumps(line), flush=True)
 
 

@@ -7,6 +7,7 @@ Flags that carry semantics:
   -target-feature -packed-fp32-ops   no packed FP32 instructions are emitted.
 """
 import glob
+import hashlib
 import os
 import subprocess
 import sys
@@ -17,12 +18,29 @@ LIB = os.path.join(HERE, "libcgs.so")
 ARCH = os.environ.get("CGS_OFFLOAD_ARCH", "gfx950")
 
 
+def source_files():
+    """The files libcgs.so is built from: csrc/* and include/*."""
+    return sorted(glob.glob(os.path.join(HERE, "csrc", "*"))) + sorted(glob.glob(os.path.join(ROOT, "include", "*")))
+
+
+def source_hash():
+    """First 16 hex digits of the SHA-256 of the library's source files (path
+    relative to the repository, then content, in sorted order).  build()
+    embeds it in libcgs.so, where cgs_version() reports it."""
+    h = hashlib.sha256()
+    for p in source_files():
+        h.update(os.path.relpath(p, ROOT).encode())
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def hipcc_command(out=LIB, extra=()):
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
     return ["/opt/rocm/bin/hipcc", "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include"), "-I",
-            os.path.join(HERE, "csrc"), *extra, "-o", out, *srcs]
+            "-Wno-unused-command-line-argument", f'-DCGS_SOURCE_HASH="{source_hash()}"',
+            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"), *extra, "-o", out, *srcs]
 
 
 def driver_command(out, source="facade_driver.cpp"):

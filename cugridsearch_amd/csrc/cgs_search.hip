@@ -379,7 +379,14 @@ using namespace cgs;
 
 extern "C" {
 
-const char *cgs_version(void) { return "cgs 0.3.0 gfx950"; }
+// CGS_SOURCE_HASH: the build's hash of the library's sources
+// (cugridsearch_amd/build.py source_hash), so that a caller can tell which
+// sources a libcgs.so was built from.
+#ifndef CGS_SOURCE_HASH
+#define CGS_SOURCE_HASH "unknown"
+#endif
+
+const char *cgs_version(void) { return "cgs 0.4.0 gfx950 src " CGS_SOURCE_HASH; }
 
 int32_t cgs_abi_version(void) { return CGS_ABI_VERSION; }
 

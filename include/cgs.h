@@ -133,7 +133,8 @@ typedef struct {
     int32_t index;
 } cgs_argmin;
 
-/* Library version string. */
+/* Library version string: "cgs <version> gfx950 src <hash>", the hash being
+ * the build's digest of the library's source files. */
 const char *cgs_version(void);
 
 /* ---------------------------------------------------------- preprocessing
@@ -467,6 +468,13 @@ cgs_status cgs_cnn_get_grads(const cgs_cnn *net, float *grads);
 /* One Adam step (bias-corrected) on the master weights with the last
  * gradient, then the bf16 working copies are refreshed.  Asynchronous. */
 cgs_status cgs_cnn_adam_step(cgs_cnn *net, float lr, float beta1, float beta2, float eps, void *stream);
+
+/* The current device's measured non-packed FP32 FMA throughput in TFLOP/s
+ * (2 FLOP per fmaf): a kernel of independent fmaf chains, timed with HIP
+ * events, best of three.  The VALU roofline the search kernels are measured
+ * against beside the published peak, which counts packed FP32.  No
+ * reference counterpart.  Allocates a small output buffer; synchronous. */
+cgs_status cgs_fma_peak(float *tflops_out, void *stream);
 
 /* Duration in ms of the calling thread's last timed search kernel (flags &
  * CGS_FLAG_TIME_KERNEL), measured with HIP events around that launch; -1

@@ -33,8 +33,12 @@ def test_library_exports_every_declared_symbol():
     assert set(L.SIGNATURES) <= syms
 
 
-def test_version():
-    assert L.lib().cgs_version().decode().startswith("cgs")
+def test_version_names_this_trees_sources():
+    """The library reports the hash of the sources it was built from, and it
+    is this tree's (build() after every source change)."""
+    from cugridsearch_amd import build as B
+    v = L.lib().cgs_version().decode()
+    assert v.startswith("cgs") and v.endswith(" src " + B.source_hash()), (v, B.source_hash())
 
 
 def _grid(dim, start, end, count):
