@@ -37,6 +37,9 @@ FLOP_PER_PE = {("nearest", "sad"): 17 + 3, ("bilinear", "ncc"): 17 + 13 + 6, ("b
 FLOP_PER_PE_ALT = {"calcSQDAlt": 4 + 26 + 6, "calcNCCAlt": 4 + 26 + 6, "calcNCCstreamAlt": 4 + 26 + 6,
                    "calcMIstreamAlt": 4 + 26 + 2, "calcMIAlt": 8 + 26 + 2}
 BATCH_WORKLOAD = "batch_ragged_256"
+# many small pairs (sides U{48..96}, 3^8 grids): the per-pair loop is bound by
+# its launches and synchronisations there, the batch by the device
+BATCH_SMALL_WORKLOAD = "batch_small_1024"
 PROPOSAL_WORKLOAD = "proposal_1024"
 CNN_WORKLOAD = "cnn_train_step"
 MATCHER_WORKLOADS = {"matcher_256": 256, "matcher_512": 512}
@@ -176,7 +179,12 @@ def run_batch(args):
     from cugridsearch_amd import api as A
     from cugridsearch_amd import synth, tuner
 
-    specs = synth.batch_workload(n=args.pairs)
+    small = args.workload == BATCH_SMALL_WORKLOAD
+    if small:
+        specs = synth.batch_workload(n=1024 if args.pairs == 256 else args.pairs, seed=0xC6507, sides=(48, 96),
+                                     levels=(3,))
+    else:
+        specs = synth.batch_workload(n=args.pairs)
     fns = {"sad": A.centredAbsoluteDifference, "ncc": A.calcNCC, "mi": A.calcMIstream.with_(bins=32)}
     pairs = []
     for sp in specs:
@@ -212,11 +220,11 @@ def run_batch(args):
     kernel_ms = ev0.elapsed_time(ev1) / args.steps
     # per-pair plain launch loop over the same pairs
     loop_am = []
+    searchers = [A.CudaGridSearcher(g, A.CudaTensor(g.getAxisSampleCounts())) for _, g, _, _, _ in pairs]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for fn, g, m, f, _ in pairs:
-        res = A.CudaTensor(g.getAxisSampleCounts())
-        loop_am.append(A.CudaGridSearcher(g, res).search_by_value(fn, m, f, stream=stream))
+    for (fn, g, m, f, _), srch in zip(pairs, searchers):
+        loop_am.append(srch.search_by_value(fn, m, f, stream=stream))
     torch.cuda.synchronize()
     loop_dt = time.perf_counter() - t1
     matched = sum(1 for a, c in zip(batch_am, loop_am) if a == c)
@@ -230,8 +238,9 @@ def run_batch(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1000.0,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": BATCH_WORKLOAD, "pairs": len(pairs), "sides": "U{64..512} x U{64..512}",
-                   "grids": "4^8..6^8", "costs": "SAD:NCC:MI 1:1:1", "pixel_evals_per_step": info["pixel_evals"],
+        "config": {"workload": args.workload, "pairs": len(pairs),
+                   "sides": "U{48..96} x U{48..96}" if small else "U{64..512} x U{64..512}",
+                   "grids": "3^8" if small else "4^8..6^8", "costs": "SAD:NCC:MI 1:1:1", "pixel_evals_per_step": info["pixel_evals"],
                    "grid_points_per_step": info["points"], "work_items": info["items"], "hipgraph": use_graph,
                    "tiles": tiles},
         "pairs_per_s": len(pairs) / dt,
@@ -422,7 +431,7 @@ def main():
               "(the search shards by grid-index ranges across GPUs by design, but only the one-GPU path is built)",
               file=sys.stderr)
         sys.exit(2)
-    if args.workload == BATCH_WORKLOAD:
+    if args.workload in (BATCH_WORKLOAD, BATCH_SMALL_WORKLOAD):
         print(json.dumps(run_batch(args)), flush=True)
         return
     if args.workload == CNN_WORKLOAD:

@@ -1,16 +1,18 @@
 // cgs_batch.hip -- batched ragged grid search: one persistent kernel over
-// (pair, grid-tile) work items of uneven size, optionally replayed from a
-// captured hipGraph.
+// (pair, grid-tile) work items of uneven size and of every cost kind,
+// optionally replayed from a captured hipGraph.
 //
-// Every work item is one workgroup's tile of GPT grid points of one pair,
-// run by the same per-workgroup bodies as the single-pair kernels
-// (fast_block / mi_block in cgs_kernels.hpp) on per-pair descriptors kept in
-// device memory, so each pair's costs and argmin are identical to cgs_search
-// on that pair.  Workgroups claim items_per_claim items at a time from one
-// device counter and exit when the counter passes the last item, so the
-// ragged mix of image sizes, grid sizes and costs load-balances itself.
-// Float tap tables and float fixed images are staged per pair, so one code
-// path per cost serves uint8 and float pairs alike (uint8 -> float is exact).
+// Every work item is one workgroup's tile of grid points of one pair, run by
+// the same per-workgroup bodies as the single-pair kernels (fast_block /
+// mi_block in cgs_kernels.hpp) on per-pair descriptors kept in device memory,
+// so each pair's costs and argmin are identical to cgs_search on that pair.
+// Workgroups claim items_per_claim items at a time from one device counter
+// and exit when the counter passes the last item, so the ragged mix of image
+// sizes, grid sizes and costs load-balances itself; the MI items, the most
+// expensive, are numbered first so that the tail is made of cheap ones.  The
+// moving image is staged per pair in its kernel's tap table (the fast
+// kernel's float differences, the MI kernel's raw float taps, or for uint8 MI
+// pairs the 4-byte packed taps), the fixed image as float (exact for uint8).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,7 +28,13 @@
 
 namespace cgs {
 
-enum BatchKind : int32_t { kBatchSAD = 0, kBatchNCC = 1, kBatchSQD = 2, kBatchMI = 3 };
+// The per-workgroup body a descriptor runs: the fast sweep with the SAD / NCC /
+// SQD sums (fast_block's cost template argument is the kind itself), or the MI
+// sweep on float taps or on uint8 packed taps.
+enum BatchKind : int32_t { kBatchSAD = 0, kBatchNCC = 1, kBatchSQD = 2, kBatchMI = 3, kBatchMIU8 = 4 };
+constexpr int kBatchKinds = 5;
+// Item numbering order of the kinds: MI first (the most work per item).
+constexpr int kKindOrder[kBatchKinds] = {kBatchMI, kBatchMIU8, kBatchNCC, kBatchSQD, kBatchSAD};
 
 struct BatchDesc {
     SearchParams P;         // as for a single-pair search, pointing at the batch's staging
@@ -55,22 +63,24 @@ __global__ void batch_taps_kernel(const T *src, float4 *dst, int W, int H) {
     }
 }
 
-constexpr int kBatchKinds = 4;
-
-__global__ void batch_clear_kernel(unsigned long long *keys, int n, uint32_t *counters) {
+__global__ void batch_clear_kernel(unsigned long long *keys, int n, uint32_t *counter) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) keys[i] = ~0ull;
-    if (blockIdx.x == 0 && threadIdx.x < kBatchKinds) counters[threadIdx.x * 16] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
 }
 
-// One persistent kernel per cost kind (only that kind's sweep is compiled in,
-// so it keeps the single-pair kernel's register budget and occupancy).
-// descs: the kind's pairs, item_begin relative to the kind's first item.
-template <int KIND, int G>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void batch_kernel(const BatchDesc *descs, int npairs, uint32_t total,
-                                                         uint32_t *counter, int per_claim) {
-    __shared__ BlockShared<G> S;
+// The one persistent kernel: GF grid points per fast item, GM per MI item.
+// descs: every descriptor, sorted by item_begin (items numbered over all
+// descriptors in kKindOrder).  The item's kind is wave-uniform, so a
+// workgroup runs one body per item with no divergence.
+template <int GF, int GM>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void batch_kernel(
+    const BatchDesc *descs, int npairs, uint32_t total, uint32_t *counter, int per_claim) {
+    __shared__ union {
+        BlockShared<GF> f;
+        BlockShared<GM> m;
+    } S;
     __shared__ uint32_t s_claim;
-    extern __shared__ uint32_t s_hist[];   // MI: G histograms of the batch's largest bin count + 64 dummies
+    extern __shared__ uint32_t s_hist[];   // MI: GM histograms of the batch's largest bin count + 64 dummies
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s_claim = atomicAdd(counter, (uint32_t) per_claim);
@@ -91,8 +101,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
             }
             const SearchParams &P = descs[lo].P;
             const int32_t tile = (int32_t) (item - descs[lo].item_begin);
-            if constexpr (KIND == kBatchMI) mi_block<float, float, G>(P, tile * G, S, s_hist);
-            else fast_block<KIND, float, G>(P, tile * G, S);
+            switch (__builtin_amdgcn_readfirstlane(descs[lo].kind)) {
+            case kBatchMIU8: mi_block<uint8_t, float, GM>(P, tile * GM, S.m, s_hist); break;
+            case kBatchMI: mi_block<float, float, GM>(P, tile * GM, S.m, s_hist); break;
+            case kBatchNCC: fast_block<kBatchNCC, float, GF>(P, tile * GF, S.f); break;
+            case kBatchSQD: fast_block<kBatchSQD, float, GF>(P, tile * GF, S.f); break;
+            default: fast_block<kBatchSAD, float, GF>(P, tile * GF, S.f); break;
+            }
         }
     }
 }
@@ -107,8 +122,7 @@ struct cgs_batch_s {
     int cus = 0;
     int max_bins = 8;        // largest MI bin count of the batch (sizes the LDS histograms)
     uint32_t total_items = 0;
-    int kind_begin[kBatchKinds] = {}, kind_count[kBatchKinds] = {};   // descriptor ranges per kind
-    uint32_t kind_items[kBatchKinds] = {};
+    bool has_mi = false;     // some descriptor of the current launch runs the MI sweep
     int nslots = 0;          // descriptors (= keys) of the current launch
     std::vector<int> slot_pair;                                       // pair of each key slot
     int64_t total_points = 0, total_pe = 0;
@@ -142,39 +156,32 @@ struct cgs_batch_s {
 
 namespace {
 
-int batch_kind(int cost) {
-    switch (cost) {
+bool is_mi_kind(int kind) { return kind == kBatchMI || kind == kBatchMIU8; }
+
+int batch_kind(const cgs_batch_pair &p) {
+    switch (p.opts.cost) {
     case CGS_COST_NCC: case CGS_COST_NCC_STREAM: return kBatchNCC;
     case CGS_COST_SQD: case CGS_COST_SQD_STREAM: return kBatchSQD;
-    case CGS_COST_MI: case CGS_COST_MI_PLAIN: return kBatchMI;
+    case CGS_COST_MI: case CGS_COST_MI_PLAIN: return p.moved.pixtype == CGS_PIX_U8 ? kBatchMIU8 : kBatchMI;
     default: return kBatchSAD;
     }
 }
 
-template <int KIND, int G>
-hipError_t launch_kind(cgs_batch *b, hipStream_t s) {
-    const size_t lds = KIND == kBatchMI ? mi_hist_words(G, b->max_bins) * sizeof(uint32_t) : 0;
+template <int GF, int GM>
+hipError_t launch_batch(cgs_batch *b, hipStream_t s) {
+    const size_t lds = b->has_mi ? mi_hist_words(GM, b->max_bins) * sizeof(uint32_t) : 0;
     const int blocks = std::max(1, b->cus * b->tiles.wg_per_cu);
-    hipLaunchKernelGGL((batch_kernel<KIND, G>), dim3(blocks), dim3(kThreads), lds, s,
-                       (const BatchDesc *) b->descs.ptr + b->kind_begin[KIND], b->kind_count[KIND],
-                       b->kind_items[KIND], (uint32_t *) b->counter.ptr + 16 * KIND, b->tiles.items_per_claim);
+    hipLaunchKernelGGL((batch_kernel<GF, GM>), dim3(blocks), dim3(kThreads), lds, s, (const BatchDesc *) b->descs.ptr,
+                       (int) b->host_descs.size(), b->total_items, (uint32_t *) b->counter.ptr,
+                       b->tiles.items_per_claim);
     return hipGetLastError();
 }
 
-template <int KIND>
-hipError_t launch_fast(cgs_batch *b, hipStream_t s) {
-    return b->tiles.fast_gpt == 6 ? launch_kind<KIND, 6>(b, s) : launch_kind<KIND, 4>(b, s);
-}
-
 hipError_t launch_batch_any(cgs_batch *b, hipStream_t s) {
-    const bool m4 = b->tiles.mi_gpt == 4;
-    hipError_t e = hipSuccess;
-    if (b->kind_count[kBatchSAD] && e == hipSuccess) e = launch_fast<kBatchSAD>(b, s);
-    if (b->kind_count[kBatchNCC] && e == hipSuccess) e = launch_fast<kBatchNCC>(b, s);
-    if (b->kind_count[kBatchSQD] && e == hipSuccess) e = launch_fast<kBatchSQD>(b, s);
-    if (b->kind_count[kBatchMI] && e == hipSuccess)
-        e = m4 ? launch_kind<kBatchMI, 4>(b, s) : launch_kind<kBatchMI, 2>(b, s);
-    return e;
+    if (b->host_descs.empty() || b->total_items == 0) return hipSuccess;
+    const bool f6 = b->tiles.fast_gpt == 6, m4 = b->tiles.mi_gpt == 4;
+    if (f6) return m4 ? launch_batch<6, 4>(b, s) : launch_batch<6, 2>(b, s);
+    return m4 ? launch_batch<4, 4>(b, s) : launch_batch<4, 2>(b, s);
 }
 
 cgs_status parse_tiles(const cgs_tile_config *tiles, cgs_tile_config *out) {
@@ -209,7 +216,7 @@ cgs_status stage_alloc(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, con
         if (p.opts.sampler != CGS_SAMPLER_BILINEAR || p.moved.channels != 1 || p.moved.width > 65536 ||
             p.moved.height > 65536)
             return set_error(CGS_ERR_UNSUPPORTED, "batch pairs must be one-channel bilinear searches");
-        if (batch_kind(p.opts.cost) == kBatchMI && p.opts.mi_bins > 32 && t.mi_gpt > 2)
+        if (is_mi_kind(batch_kind(p)) && p.opts.mi_bins > 32 && t.mi_gpt > 2)
             return set_error(CGS_ERR_UNSUPPORTED, "MI with more than 32 bins needs mi_gpt 2");
     }
     b->drop_graph();
@@ -222,16 +229,17 @@ cgs_status stage_alloc(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, con
     b->sincos.resize(n);
     if ((st = b->keys.ensure(sizeof(unsigned long long) * n)) != CGS_OK ||
         (st = b->keys_host.ensure(sizeof(unsigned long long) * n, true)) != CGS_OK ||
-        (st = b->counter.ensure(64 * kBatchKinds)) != CGS_OK ||
+        (st = b->counter.ensure(64)) != CGS_OK ||
         (st = b->descs.ensure(sizeof(BatchDesc) * n)) != CGS_OK)
         return st;
     std::vector<float> sc;
     for (int32_t i = 0; i < n; ++i) {
         const cgs_batch_pair &p = pairs[i];
-        if (batch_kind(p.opts.cost) == kBatchMI) b->max_bins = std::max(b->max_bins, (int) p.opts.mi_bins);
+        if (is_mi_kind(batch_kind(p))) b->max_bins = std::max(b->max_bins, (int) p.opts.mi_bins);
         const int W = p.moved.width, H = p.moved.height;
         const size_t ntaps = (size_t) W * H + 2 * (size_t) W + 2;
-        if ((st = b->taps[i].ensure(ntaps * sizeof(float4))) != CGS_OK) return st;
+        const size_t tap_bytes = batch_kind(p) == kBatchMIU8 ? 4 : sizeof(float4);
+        if ((st = b->taps[i].ensure(ntaps * tap_bytes)) != CGS_OK) return st;
         if ((st = b->fixf[i].ensure((size_t) p.fixed.width * p.fixed.height * sizeof(float))) != CGS_OK) return st;
         const bool has_theta = p.opts.param_mode == CGS_PARAM_CENTRED || p.opts.param_mode == CGS_PARAM_NONCENTRED;
         const int levels0 = has_theta ? (int) p.grid.count[0] : 0;
@@ -254,14 +262,16 @@ cgs_status enqueue_staging(cgs_batch *b, hipStream_t s) {
     const unsigned pb = 256;
     for (size_t i = 0; i < b->pairs.size(); ++i) {
         const cgs_batch_pair &p = b->pairs[i];
-        const bool mi = batch_kind(p.opts.cost) == kBatchMI;
+        const int kind = batch_kind(p);
+        const bool mi = is_mi_kind(kind);
         const int W = p.moved.width, H = p.moved.height;
         float4 *taps = (float4 *) b->taps[i].ptr;
-        if (p.moved.pixtype == CGS_PIX_U8) {
-            if (mi) hipLaunchKernelGGL((batch_taps_kernel<uint8_t, false>), dim3(pb), dim3(256), 0, s,
-                                       (const uint8_t *) p.moved.data, taps, W, H);
-            else hipLaunchKernelGGL((batch_taps_kernel<uint8_t, true>), dim3(pb), dim3(256), 0, s,
-                                    (const uint8_t *) p.moved.data, taps, W, H);
+        if (kind == kBatchMIU8) {   // the plain MI kernel's packed uint8 taps
+            hipLaunchKernelGGL(tap_u8_kernel, dim3(pb), dim3(256), 0, s, (const uint8_t *) p.moved.data,
+                               (uint32_t *) taps, W, H);
+        } else if (p.moved.pixtype == CGS_PIX_U8) {
+            hipLaunchKernelGGL((batch_taps_kernel<uint8_t, true>), dim3(pb), dim3(256), 0, s,
+                               (const uint8_t *) p.moved.data, taps, W, H);
         } else {
             if (mi) hipLaunchKernelGGL((batch_taps_kernel<float, false>), dim3(pb), dim3(256), 0, s,
                                        (const float *) p.moved.data, taps, W, H);
@@ -307,12 +317,12 @@ BatchDesc describe(const cgs_batch *b, int i, const cgs_grid &g, float *result) 
     D.P.tie_mode = p.opts.tie_mode;
     D.P.bins = p.opts.mi_bins;
     D.P.ceil_taps = p.moved.pixtype == CGS_PIX_F32;
-    D.kind = batch_kind(p.opts.cost);
+    D.kind = batch_kind(p);
     return D;
 }
 
-// Groups the slot-ordered descriptors by kind (slot order kept inside a
-// kind), numbers their work items and uploads them.
+// Orders the slot-ordered descriptors by kind (kKindOrder; slot order kept
+// inside a kind), numbers their work items over all of them and uploads them.
 cgs_status set_descs(cgs_batch *b, std::vector<BatchDesc> &by_slot, const std::vector<int> &slot_pair) {
     // every slot has its own descriptor and key
     const size_t slots = std::max<size_t>(1, by_slot.size());
@@ -321,27 +331,25 @@ cgs_status set_descs(cgs_batch *b, std::vector<BatchDesc> &by_slot, const std::v
         (st = b->keys.ensure(sizeof(unsigned long long) * slots)) != CGS_OK ||
         (st = b->keys_host.ensure(sizeof(unsigned long long) * slots, true)) != CGS_OK)
         return st;
-    uint64_t kitems[kBatchKinds] = {};
     int64_t points = 0, pe = 0;
     for (size_t k = 0; k < by_slot.size(); ++k) {
-        BatchDesc &D = by_slot[k];
-        D.P.key = (unsigned long long *) b->keys.ptr + k;
-        const int gpt = D.kind == kBatchMI ? b->tiles.mi_gpt : b->tiles.fast_gpt;
-        D.item_begin = (uint32_t) kitems[D.kind];
-        kitems[D.kind] += (uint64_t) ((D.P.grid.n + gpt - 1) / gpt);
-        points += D.P.grid.n;
-        pe += (int64_t) D.P.grid.n * D.P.fw * D.P.fh;
-        if (kitems[D.kind] >= 0xFFFFFFF0ull) return set_error(CGS_ERR_TOO_LARGE, "batch has 2^32 or more work items");
+        by_slot[k].P.key = (unsigned long long *) b->keys.ptr + k;
+        points += by_slot[k].P.grid.n;
+        pe += (int64_t) by_slot[k].P.grid.n * by_slot[k].P.fw * by_slot[k].P.fh;
     }
     uint64_t items = 0;
     b->host_descs.clear();
-    for (int k = 0; k < kBatchKinds; ++k) {
-        b->kind_begin[k] = (int) b->host_descs.size();
-        for (const BatchDesc &D : by_slot)
-            if (D.kind == k) b->host_descs.push_back(D);
-        b->kind_count[k] = (int) b->host_descs.size() - b->kind_begin[k];
-        b->kind_items[k] = (uint32_t) kitems[k];
-        items += kitems[k];
+    b->has_mi = false;
+    for (int kind : kKindOrder) {
+        for (BatchDesc D : by_slot) {
+            if (D.kind != kind) continue;
+            const int gpt = is_mi_kind(kind) ? b->tiles.mi_gpt : b->tiles.fast_gpt;
+            D.item_begin = (uint32_t) items;
+            items += (uint64_t) ((D.P.grid.n + gpt - 1) / gpt);
+            if (items >= 0xFFFFFFF0ull) return set_error(CGS_ERR_TOO_LARGE, "batch has 2^32 or more work items");
+            b->has_mi = b->has_mi || is_mi_kind(kind);
+            b->host_descs.push_back(D);
+        }
     }
     if (!b->host_descs.empty())
         HIP_TRY(hipMemcpy(b->descs.ptr, b->host_descs.data(), sizeof(BatchDesc) * b->host_descs.size(),

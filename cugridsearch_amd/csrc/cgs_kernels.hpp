@@ -444,14 +444,44 @@ __device__ __forceinline__ bool inside_posw(float Xn, float Yn, float Wn, float 
     return (int32_t) w < 0;
 }
 
+// Column chunks of the fixed image for the sweeps.  A lane owns one column of
+// a chunk and walks its rows; a chunk of cw columns runs kThreads / cw row
+// phases at once.  The image is cut into full chunks of kThreads columns (one
+// phase), then the remaining columns into k equal chunks of ceil(rem / k),
+// k = 1..8 chosen for the fewest row steps (a 300-wide image: 256 columns, then
+// 44 columns in 5 phases, instead of a second pass with 44 of 256 lanes busy;
+// an 86-wide one: 2 chunks of 43 columns in 5 phases instead of 86 in 2).
+// Every value is workgroup-uniform.
+struct ColChunks {
+    int nfull, ksplit, cwsplit;
+    __device__ ColChunks(int fw, int fh) {
+        nfull = fw / kThreads;
+        const int rem = fw - nfull * kThreads;
+        ksplit = rem ? 1 : 0;
+        cwsplit = rem;
+        int best = rem ? (fh + kThreads / rem - 1) / (kThreads / rem) : 0;
+        for (int k = 2; k <= 8 && k <= rem; ++k) {
+            const int cw = (rem + k - 1) / k, rpi = kThreads / cw;
+            const int chunks = (rem + cw - 1) / cw;   // no empty chunk at the end
+            const int steps = chunks * ((fh + rpi - 1) / rpi);
+            if (steps < best) {
+                best = steps;
+                ksplit = chunks;
+                cwsplit = cw;
+            }
+        }
+    }
+    __device__ int count() const { return nfull + ksplit; }
+    __device__ int x0(int c) const { return c < nfull ? c * kThreads : nfull * kThreads + (c - nfull) * cwsplit; }
+    __device__ int width(int c) const { return c < nfull ? kThreads : cwsplit; }
+};
+
 template <int COSTK, typename FT, int GPT, bool POSW>
 __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*s_coef)[12], double (*acc)[3],
                                            uint32_t *cnt) {
     const int tid = threadIdx.x;
     const int fw = uniform(P.fw), fh = uniform(P.fh);
-    const int cols = fw < kThreads ? fw : kThreads;
-    const int rpi = kThreads / cols;
-    const int r0 = tid / cols, xc = tid - r0 * cols;
+    const ColChunks chunks(fw, fh);
     const int mw = uniform(P.mw), mh = uniform(P.mh);
     const float Wm = (float) mw, Hm = (float) mh;
     const FT *fix = uniform((const FT *) P.fix);
@@ -461,7 +491,6 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
         (void *) uniform(P.quads), (short) 0, (int) (((size_t) mw * mh + 2 * (size_t) mw + 2) * 16), 0x00020000);
     const __amdgpu_buffer_rsrc_t frsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *) fix, (short) 0, (int) ((size_t) fw * fh * sizeof(FT)), 0x00020000);
-    const int full = fh / rpi;   // steps in which every lane's row is inside the image
 
     float B[GPT][3];
 #pragma unroll
@@ -469,7 +498,11 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
 #pragma unroll
         for (int k = 0; k < 3; ++k) B[g][k] = uniform(s_coef[g][3 * k + 1]);
 
-    for (int x0 = 0; x0 < fw; x0 += kThreads) {
+    for (int c = 0; c < chunks.count(); ++c) {
+        const int x0 = chunks.x0(c), cols = chunks.width(c);
+        const int rpi = kThreads / cols;
+        const int r0 = tid / cols, xc = tid - r0 * cols;
+        const int full = fh / rpi;   // steps in which every lane's row is inside the image
         const int x = x0 + xc;
         const bool valid = (r0 < rpi) && (x < fw);
         const float xf = (float) x;
@@ -729,9 +762,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                                          uint32_t *cnt) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int fw = uniform(P.fw), fh = uniform(P.fh);
-    const int cols = fw < kThreads ? fw : kThreads;
-    const int rpi = kThreads / cols;
-    const int r0 = tid / cols, xc = tid - r0 * cols;
+    const ColChunks chunks(fw, fh);
     const int mw = uniform(P.mw), mh = uniform(P.mh);
     const float Wm = (float) mw, Hm = (float) mh;
     const FT *fix = uniform((const FT *) P.fix);
@@ -741,7 +772,6 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
         (void *) uniform(P.quads), (short) 0, (int) (((size_t) mw * mh + 2 * (size_t) mw + 2) * EB), 0x00020000);
     const __amdgpu_buffer_rsrc_t frsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *) fix, (short) 0, (int) ((size_t) fw * fh * sizeof(FT)), 0x00020000);
-    const int full = fh / rpi;   // steps in which every lane's row is inside the image
     const int bins = uniform(P.bins), RS = bins + 1, HS = bins * RS;   // rows padded to bins + 1 counters
     const bool ceil_taps = uniform(P.ceil_taps) != 0;
     const float fbins = (float) bins, bscale = fbins / 256.0f;
@@ -753,7 +783,11 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
 #pragma unroll
         for (int k = 0; k < 3; ++k) B[g][k] = uniform(s_coef[g][3 * k + 1]);
 
-    for (int x0 = 0; x0 < fw; x0 += kThreads) {
+    for (int c = 0; c < chunks.count(); ++c) {
+        const int x0 = chunks.x0(c), cols = chunks.width(c);
+        const int rpi = kThreads / cols;
+        const int r0 = tid / cols, xc = tid - r0 * cols;
+        const int full = fh / rpi;   // steps in which every lane's row is inside the image
         const int x = x0 + xc;
         const bool valid = (r0 < rpi) && (x < fw);
         const float xf = (float) x;
