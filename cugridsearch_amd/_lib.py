@@ -112,6 +112,8 @@ SIGNATURES = {
     "cgs_image_fuse": (ctypes.c_int, [_P(Image), _P(Image), _P(ctypes.c_float), _P(ctypes.c_float), ctypes.c_int32,
                                       ctypes.c_int32, _P(Image), ctypes.c_void_p]),
     "cgs_write_png": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "cgs_read_png": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                                    _P(ctypes.c_int32), _P(ctypes.c_int32), _P(ctypes.c_int32)]),
     "cgs_eval_point": (ctypes.c_int, [_P(Image), _P(Image), _P(SearchOpts), ctypes.c_int32, _P(ctypes.c_float),
                                       _P(ctypes.c_float), _P(ctypes.c_uint32), ctypes.c_void_p]),
     "cgs_batch_create": (ctypes.c_int, [_P(ctypes.c_void_p)]),

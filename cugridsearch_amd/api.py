@@ -286,6 +286,19 @@ def write_png(path, img):
     L.check(L.lib().cgs_write_png(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0], c))
 
 
+def read_png(path, req_comp=0):
+    """stbi_load for a PNG file (cgs_read_png): a uint8 (H, W) array for one
+    channel, (H, W, C) otherwise; req_comp 0 keeps the file's channels.
+    Raises L.CgsError when the file cannot be read or decoded."""
+    w, h, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    name = os.fsencode(path)
+    L.check(L.lib().cgs_read_png(name, int(req_comp), None, 0, ctypes.byref(w), ctypes.byref(h), ctypes.byref(n)))
+    c = int(req_comp) or n.value
+    a = np.empty((h.value, w.value, c), np.uint8)
+    L.check(L.lib().cgs_read_png(name, int(req_comp), a.ctypes.data, a.nbytes, None, None, None))
+    return a[:, :, 0].copy() if c == 1 else a
+
+
 def quantize_image(img, mod, stream=None):
     """transformPixel (src/cpu/multispectral_ImageMatcher.cu:411-430), in place
     on a uint8 CudaImage."""

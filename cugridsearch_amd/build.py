@@ -40,7 +40,7 @@ def hipcc_command(out=LIB, extra=()):
     return ["/opt/rocm/bin/hipcc", "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument", f'-DCGS_SOURCE_HASH="{source_hash()}"',
-            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"), *extra, "-o", out, *srcs]
+            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"), *extra, "-o", out, *srcs, "-lz"]
 
 
 def driver_command(out, source="facade_driver.cpp"):
@@ -108,6 +108,32 @@ def build_mex(verbose=False):
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError("gcc failed building the MEX check of " + gateway)
+
+
+CLI = os.path.join(HERE, "_build", "cgs_matcher")
+
+
+def cli_command(out=CLI):
+    """The matcher CLI (cli/cgs_matcher.cpp): host C++ over the C-ABI, linked
+    against libcgs.so (found next to it through the rpath) and the HIP
+    runtime.  -ffp-contract=off keeps its binary32 expressions unfused."""
+    return ["g++", "-O2", "-std=c++17", "-Wall", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+            "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", "-o", out,
+            os.path.join(HERE, "cli", "cgs_matcher.cpp"), "-L", HERE, "-lcgs", "-L", "/opt/rocm/lib", "-lamdhip64",
+            "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+
+
+def build_cli(verbose=False):
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    cmd = cli_command(CLI + ".tmp")
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building cgs_matcher")
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -55,17 +55,11 @@ f32 = np.float32
 # ------------------------------------------------------------------ images
 
 def load_gray(path):
-    """stbi_load(path, ..., req_comp = 1): a one-channel uint8 (H, W) array."""
-    from PIL import Image
-    with Image.open(path) as im:
-        im.load()
-        if im.mode in ("L", "LA"):
-            return np.asarray(im.getchannel(0), np.uint8).copy()
-        if im.mode in ("I;16", "I;16B", "I"):
-            return (np.asarray(im, np.uint32) >> 8).astype(np.uint8)   # stbi__convert_16_to_8
-        rgb = np.asarray(im.convert("RGB"), np.int32)
-    y = (rgb[..., 0] * 77 + rgb[..., 1] * 150 + rgb[..., 2] * 29) >> 8
-    return y.astype(np.uint8)
+    """stbi_load(path, ..., req_comp = 1) of a PNG file through cgs_read_png: a
+    one-channel uint8 (H, W) array (grey from colour as stb's luma
+    (77 R + 150 G + 29 B) >> 8, include/third_party/stb_image.h:1726-1729)."""
+    from . import api as A
+    return A.read_png(path, 1)
 
 
 # ------------------------------------------------------------- grid set-up
@@ -148,8 +142,12 @@ def homography_to_parameters(h):
 # ----------------------------------------------------------------- output
 
 def g(v):
-    """std::ostream's default rendering of a float (precision 6, %g)."""
-    return "%g" % float(v)
+    """std::ostream's default rendering of a float (precision 6, %g), a NaN
+    with its sign bit set included ("-nan", as glibc prints it)."""
+    x = float(v)
+    if math.isnan(x):
+        return "-nan" if math.copysign(1.0, x) < 0 else "nan"
+    return "%g" % x
 
 
 def eigen_column(vals):
@@ -294,7 +292,10 @@ def run(fixed, moved, mod=0.0, gt=None, depth=DEPTH, names=("fixed", "moving"), 
 
     gth = list(H)
     if gt:
-        gth = ground_truth(gt, names[0], sfx, sfy, default=gth)
+        try:
+            gth = ground_truth(gt, names[0], sfx, sfy, default=gth)
+        except ValueError as e:   # std::stof throws there and the reference terminates
+            raise GroundTruthError(f"Ground truth file {gt}: {e}") from e
     line.append("Homography," + "".join(g(v) + "," for v in H))
     line.append("GT Homography," + "".join(g(v) + "," for v in gth))
     with np.errstate(divide="ignore", invalid="ignore"):
@@ -330,26 +331,85 @@ def write_image(path, img, out=sys.stdout):
         out.write("Filename suffix has image format not recognized.\n")
 
 
+class GroundTruthError(ValueError):
+    """A ground-truth row whose values do not parse."""
+
+
+def image_number(fixed_name):
+    """The image number the reference reads from the fixed image's file name
+    (:993-1000): the characters after the last one outside "0123456789.png",
+    up to the next '.', through atoi (the reference copies them into a
+    5-byte buffer, which more than four characters overrun; here all of them
+    are read)."""
+    keep = set("0123456789.png")
+    i = len(fixed_name) - 1
+    while i >= 0 and fixed_name[i] in keep:
+        i -= 1
+    j = fixed_name.find(".", i + 1)
+    return atoi(fixed_name[i + 1:] if j < 0 else fixed_name[i + 1:j])
+
+
+def atoi(text):
+    """C's atoi: leading white space, an optional sign, then decimal digits
+    (0 when there are none)."""
+    t = text.lstrip(" \t\n\v\f\r")
+    sign, k = 1, 0
+    if t[:1] in ("+", "-"):
+        sign, k = (-1 if t[0] == "-" else 1), 1
+    n = k
+    while n < len(t) and t[n].isdigit() and t[n].isascii():
+        n += 1
+    return sign * int(t[k:n]) if n > k else 0
+
+
+def stof(text):
+    """std::stof: strtof on the text (the decimal rounded once, straight to
+    binary32); ValueError when it holds no number."""
+    import ctypes
+    import errno
+    libc = ctypes.CDLL(None, use_errno=True)
+    libc.strtof.restype = ctypes.c_float
+    libc.strtof.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    buf = ctypes.create_string_buffer(text.encode("latin-1"))
+    start = ctypes.addressof(buf)
+    end = ctypes.c_void_p()
+    ctypes.set_errno(0)
+    v = libc.strtof(start, ctypes.byref(end))
+    if end.value == start:
+        raise ValueError("stof: no conversion: %r" % text)
+    if ctypes.get_errno() == errno.ERANGE:
+        raise ValueError("stof: out of range: %r" % text)
+    return f32(v)
+
+
 def ground_truth(path, fixed_name, sfx, sfy, default):
-    """The GT CSV lookup (:990-1061): the row whose first field is the fixed
-    image's number, eight homography entries after it, the first three scaled
-    by the x factor and the next three by the y factor."""
-    base = os.path.basename(fixed_name)
-    digits = "".join(ch for ch in os.path.splitext(base)[0] if ch.isdigit())
-    if not os.path.exists(path) or not digits:
-        return default
-    num = int(digits)
-    gth = list(default)
-    with open(path) as fh:
-        for row in fh:
-            parts = row.strip().split(",")
-            try:
-                if int(parts[0]) != num:
-                    continue
-                gth = [f32(float(x)) for x in parts[1:9]]
-            except (ValueError, IndexError):
-                continue
-            break
+    """The GT CSV lookup (:988-1031).  When the file opens: the first row whose
+    leading field (atoi) is the fixed image's number gives eight values
+    (stof), read field after field; past the row's last field the reader
+    wraps to its start (the reference's std::string::npos + 1 = 0).  The first
+    three are then scaled by the x factor and the next three by the y factor,
+    whether or not a row matched.  A file that does not open leaves the
+    default unscaled."""
+    try:
+        fh = open(path, newline="", encoding="latin-1")
+    except OSError:
+        return list(default)
+    num = image_number(fixed_name)
+    gth = [f32(v) for v in default]
+    with fh:
+        lines = fh.read().split("\n")
+    if lines and lines[-1] == "":   # std::getline yields no line after a final newline
+        lines.pop()
+    for line in lines:
+        comma = line.find(",")
+        if atoi(line if comma < 0 else line[:comma]) != num:
+            continue
+        at = comma + 1 if comma >= 0 else 0
+        for i in range(8):
+            nxt = line.find(",", at)
+            gth[i] = stof(line[at:] if nxt < 0 else line[at:nxt])
+            at = 0 if nxt < 0 else nxt + 1
+        break
     return [f32(v * sfx) for v in gth[:3]] + [f32(v * sfy) for v in gth[3:6]] + gth[6:8]
 
 
@@ -369,9 +429,20 @@ def main(argv=None):
     args = ap.parse_args(argv)
     sys.stderr.write(f"Output image filename is {args.output}.\n")
     sys.stderr.write(f"Fused Output image filename is {args.fusedoutput}.\n")
-    fixed, moved = load_gray(args.i_ref), load_gray(args.i_mov)
-    r = run(fixed, moved, mod=args.mod, gt=args.gt, depth=args.depth, names=(args.i_ref, args.i_mov),
-            results_file=args.results_file, fused_name=args.fusedoutput, out_name=args.output)
+    images = []
+    for name, role in ((args.i_ref, "Reference"), (args.i_mov, "Moving")):
+        try:
+            images.append(load_gray(name))
+        except Exception as e:   # the reference's "failed to load!" exit (:505-514)
+            sys.stderr.write(f"{role} image {name} failed to load! ({e})\n")
+            return 1
+    fixed, moved = images
+    try:
+        r = run(fixed, moved, mod=args.mod, gt=args.gt, depth=args.depth, names=(args.i_ref, args.i_mov),
+                results_file=args.results_file, fused_name=args.fusedoutput, out_name=args.output)
+    except GroundTruthError as e:
+        sys.stderr.write(f"{e}\n")
+        return 1
     if r["resized"] is not None:   # the reference writes the resized image (:540, :559)
         from . import api as A
         name, img = r["resized"]

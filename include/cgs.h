@@ -205,6 +205,19 @@ cgs_status cgs_image_fuse(const cgs_image *ref, const cgs_image *mov, const floa
  * alpha, 3 RGB, 4 RGBA) as a PNG file, the image data stored uncompressed. */
 cgs_status cgs_write_png(const char *path, const void *pixels, int32_t width, int32_t height, int32_t channels);
 
+/* stbi_load for PNG files (stb_image v2.27, include/third_party/stb_image.h:
+ * 5023-5229), which the matcher loads its inputs with
+ * (src/cpu/multispectral_ImageMatcher.cu:503-514): 8-bit samples, row-major,
+ * channels interleaved, req_comp channels per pixel (1 grey, 2 grey + alpha,
+ * 3 RGB, 4 RGBA; 0 = the file's own), converted as stb converts (grey from
+ * colour as (77 R + 150 G + 29 B) >> 8, 16-bit samples to their high byte,
+ * sub-8-bit grey scaled to 0..255, palettes expanded, tRNS keys to alpha).
+ * width, height and channels_in_file (stb's *n) are written when not NULL.
+ * pixels NULL: only the header facts are read.  Otherwise pixels is host
+ * memory of capacity bytes, at least width * height * channels.  Host-only. */
+cgs_status cgs_read_png(const char *path, int32_t req_comp, void *pixels, int64_t capacity, int32_t *width,
+                        int32_t *height, int32_t *channels_in_file);
+
 /* CGS_ABI_VERSION of the library's build. */
 int32_t cgs_abi_version(void);
 

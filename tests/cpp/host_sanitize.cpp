@@ -2,11 +2,16 @@
 // UndefinedBehaviorSanitizer on a machine without a GPU: grid bookkeeping,
 // sub-boxes, argument validation, and the filter tables that cgs_image_resize
 // and cgs_image_fuse(local = 1) build before their first device allocation
-// (which then fails for want of a device).  Built host-only (no device code)
-// by tests/test_host_sanitize.py; a sanitizer report fails that test.
+// (which then fails for want of a device), and the PNG reader on good,
+// truncated, corrupted and oversized files.  Built by
+// tests/test_host_sanitize.py with the host code instrumented; a sanitizer
+// report fails that test.
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "cgs.h"
@@ -108,10 +113,101 @@ static void tables() {
         }
 }
 
+static std::vector<unsigned char> slurp(const char *path) {
+    std::vector<unsigned char> b;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return b;
+    int c;
+    while ((c = std::fgetc(f)) != EOF) b.push_back((unsigned char) c);
+    std::fclose(f);
+    return b;
+}
+
+static void spill(const char *path, const std::vector<unsigned char> &b, size_t n) {
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return;
+    std::fwrite(b.data(), 1, n, f);
+    std::fclose(f);
+}
+
+static uint32_t crc32_png(const unsigned char *p, size_t n) {
+    uint32_t c = 0xFFFFFFFFu;
+    for (size_t i = 0; i < n; ++i) {
+        c ^= p[i];
+        for (int k = 0; k < 8; ++k) c = (c >> 1) ^ (0xEDB88320u & (0u - (c & 1u)));
+    }
+    return c ^ 0xFFFFFFFFu;
+}
+
+// PNG reading: round trips through cgs_write_png at every channel count and
+// req_comp, then every truncation, byte corruptions, and headers whose sizes
+// are far larger than the data.
+static void png() {
+    char path[] = "/tmp/cgs_host_sanitize_XXXXXX";
+    const int fd = mkstemp(path);
+    EXPECT(fd >= 0);
+    if (fd < 0) return;
+    close(fd);
+    for (int c = 1; c <= 4; ++c) {
+        const int w = 37, h = 5;
+        std::vector<unsigned char> px((size_t) w * h * c);
+        for (size_t i = 0; i < px.size(); ++i) px[i] = (unsigned char) (i * 131 + 7);
+        EXPECT(cgs_write_png(path, px.data(), w, h, c) == CGS_OK);
+        for (int req = 0; req <= 4; ++req) {
+            int32_t W = 0, H = 0, N = 0;
+            EXPECT(cgs_read_png(path, req, nullptr, 0, &W, &H, &N) == CGS_OK && W == w && H == h && N == c);
+            std::vector<unsigned char> out((size_t) w * h * (req ? req : c));
+            EXPECT(cgs_read_png(path, req, out.data(), (int64_t) out.size(), nullptr, nullptr, nullptr) == CGS_OK);
+            if (req == 0) EXPECT(out == px);
+            EXPECT(cgs_read_png(path, req, out.data(), (int64_t) out.size() - 1, nullptr, nullptr, nullptr) != CGS_OK);
+        }
+    }
+    const std::vector<unsigned char> good = slurp(path);
+    std::vector<unsigned char> out(37 * 5 * 4);
+    for (size_t n = 0; n < good.size(); ++n) {   // every truncation
+        spill(path, good, n);
+        (void) cgs_read_png(path, 0, out.data(), (int64_t) out.size(), nullptr, nullptr, nullptr);
+    }
+    uint32_t r = 12345;
+    for (int t = 0; t < 2000; ++t) {   // corrupted bytes past the signature
+        std::vector<unsigned char> b = good;
+        for (int k = 0; k < 1 + t % 4; ++k) {
+            r = r * 1664525u + 1013904223u;
+            b[8 + (r >> 8) % (b.size() - 8)] ^= (unsigned char) (1u + (r >> 24) % 255u);
+        }
+        spill(path, b, b.size());
+        int32_t W = 0, H = 0, N = 0;
+        if (cgs_read_png(path, 0, nullptr, 0, &W, &H, &N) == CGS_OK && (int64_t) W * H * N <= (1 << 22)) {
+            std::vector<unsigned char> o((size_t) W * H * N);
+            (void) cgs_read_png(path, 0, o.data(), (int64_t) o.size(), nullptr, nullptr, nullptr);
+        }
+    }
+    // headers far larger than their data (1 x 2^24 and 4096 x 4096, both
+    // within the size rules): the decode runs out of data and fails cleanly
+    for (uint32_t side : {1u << 24, 4096u}) {
+        std::vector<unsigned char> b = good;
+        const uint32_t w = side == 4096u ? 4096u : 1u, h = side;
+        for (int k = 0; k < 4; ++k) {
+            b[16 + k] = (unsigned char) (w >> (24 - 8 * k));
+            b[20 + k] = (unsigned char) (h >> (24 - 8 * k));
+        }
+        const uint32_t crc = crc32_png(&b[12], 17);
+        for (int k = 0; k < 4; ++k) b[29 + k] = (unsigned char) (crc >> (24 - 8 * k));
+        spill(path, b, b.size());
+        int32_t W = 0, H = 0, N = 0;
+        if (cgs_read_png(path, 1, nullptr, 0, &W, &H, &N) == CGS_OK) {
+            std::vector<unsigned char> o((size_t) W * H);
+            EXPECT(cgs_read_png(path, 1, o.data(), (int64_t) o.size(), nullptr, nullptr, nullptr) != CGS_OK);
+        }
+    }
+    std::remove(path);
+}
+
 int main() {
     grids();
     validation();
     tables();
+    png();
     std::printf("host_sanitize %s\n", g_fail ? "FAILED" : "ok");
     return g_fail;
 }

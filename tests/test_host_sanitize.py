@@ -4,8 +4,8 @@
   oracle/cgs_oracle.c (oracle/Makefile `sanitize`, loaded through
   CGS_ORACLE_SO with the sanitizer runtimes preloaded).
 * libcgs: tests/cpp/host_sanitize.cpp drives the library's host paths (grid
-  bookkeeping, sub-boxes, validation, the resize and local-fuse filter tables)
-  in a build whose host code is instrumented (-Xarch_host -fsanitize=...,
+  bookkeeping, sub-boxes, validation, the resize and local-fuse filter tables,
+  the PNG reader on good, truncated, corrupted and oversized files) in a build whose host code is instrumented (-Xarch_host -fsanitize=...,
   libstdc++'s vector annotations on); without a GPU each device path stops
   at its first allocation.  The build takes over a minute, so it is kept
   under tests/cpp/_build and redone only when a source changes.
@@ -81,7 +81,7 @@ def _build_driver():
            "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include"),
            "-I", os.path.join(ROOT, "cugridsearch_amd", "csrc"), "-o", exe + ".tmp",
            os.path.join(ROOT, "tests", "cpp", "host_sanitize.cpp"),
-           *sorted(glob.glob(os.path.join(ROOT, "cugridsearch_amd", "csrc", "*.hip")))]
+           *sorted(glob.glob(os.path.join(ROOT, "cugridsearch_amd", "csrc", "*.hip"))), "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, (r.stdout + r.stderr)[-4000:]
     os.replace(exe + ".tmp", exe)

@@ -118,3 +118,71 @@ def test_protocol_formatting():
     assert M.eigen_column([1.5, -0.25, 10]) == "  1.5\n-0.25\n   10"
     banner = M.sobel_banner().splitlines()
     assert len(banner) == 25 and banner[2] == "1, 0, 0, 0, 0, "
+
+
+def test_nan_renders_with_its_sign():
+    """glibc prints a NaN whose sign bit is set as "-nan" (what the
+    reference's % Error column shows for 0/0 on x86), and others as "nan"."""
+    z = np.float32(0)
+    with np.errstate(invalid="ignore"):
+        q = z / z
+    assert M.g(q) == ("-nan" if math.copysign(1.0, float(q)) < 0 else "nan")
+    assert M.g(float("nan")) == "nan" and M.g(-float("nan")) == "-nan"
+    assert M.g(float("inf")) == "inf" and M.g(-float("inf")) == "-inf"
+
+
+def test_image_number_from_the_reference_rule():
+    """src/cpu/multispectral_ImageMatcher.cu:991-999: the characters after the
+    last one outside "0123456789.png", up to the next '.', through atoi.
+    The set holds 'p', 'n' and 'g', so a name whose digits follow one of those
+    letters reads 0 ("img12.png" -> "g12"), as does a ".jpg" name ("pg")."""
+    cases = {"fixed_0007.png": 7, "/data/set_2/im_12.png": 12, "/data/set_2/img12.png": 0, "png_3.png": 3,
+             "x/7.png": 7, "7.png": 7, "a_ng5.png": 0, "scan-0042.jpg": 0, "name_.png": 0, "noext_15": 15,
+             "data/img_2_0009.png": 9}
+    for name, want in cases.items():
+        assert M.image_number(name) == want, name
+    assert M.atoi("  -12abc") == -12 and M.atoi("+7") == 7 and M.atoi("x1") == 0 and M.atoi("") == 0
+
+
+def test_stof_is_strtof():
+    """std::stof: one rounding, decimal straight to binary32 (not through
+    binary64), leading white space skipped, trailing text ignored, and an
+    error for no number or a result out of range."""
+    assert M.stof("1.5") == np.float32(1.5) and M.stof("  -2.25,3") == np.float32(-2.25)
+    assert M.stof("1e3x") == np.float32(1000) and M.stof("0x1p-3") == np.float32(0.125)
+    # a decimal between two binary32 values whose binary64 rounding lands on
+    # their midpoint: rounding twice goes the wrong way, strtof does not
+    tie = "1.000000059604644775390625000001"     # just above 1 + 2^-24
+    assert float(np.float32(float(tie))) == 1.0
+    assert M.stof(tie) == np.float32(1.0000001192092896)
+    for bad in ("", ",1", "abc", "1e-50", "1e50"):
+        with pytest.raises(ValueError):
+            M.stof(bad)
+
+
+def test_ground_truth_lookup(tmp_path):
+    """src/cpu/multispectral_ImageMatcher.cu:988-1031: the matching row's
+    eight values, the scale factors on the first six, the wrap past a short
+    row's end, and no scaling when the file does not open."""
+    H = [np.float32(v) for v in (1, 2, 3, 4, 5, 6, 7, 8)]
+    sx, sy = np.float32(0.5), np.float32(0.25)
+    p = tmp_path / "gt.csv"
+    p.write_text("id,h11,h12\n3,9,9,9,9,9,9,9,9\n7,1.5,0.1,-3,4e-2,5,6,0.001,-0.002\n7,0,0,0,0,0,0,0,0\n")
+    got = M.ground_truth(str(p), "dir/fixed_0007.png", sx, sy, H)
+    vals = [np.float32(v) for v in ("1.5", "0.1", "-3", "4e-2", "5", "6", "0.001", "-0.002")]
+    want = [np.float32(v * sx) for v in vals[:3]] + [np.float32(v * sy) for v in vals[3:6]] + vals[6:]
+    assert [np.float32(v) for v in got] == want
+    # a short row wraps to its own start: "7,1.5,2" reads 1.5, 2, 7, 1.5, 2, 7, ...
+    p.write_text("7,1.5,2\n")
+    got = M.ground_truth(str(p), "fixed_7.png", np.float32(1), np.float32(1), H)
+    assert [float(v) for v in got] == [1.5, 2, 7, 1.5, 2, 7, 1.5, 2]
+    # no row matches: the estimate, scaled; no file: the estimate as it is
+    p.write_text("3,1,1,1,1,1,1,1,1\n")
+    got = M.ground_truth(str(p), "fixed_0007.png", sx, sy, H)
+    assert [np.float32(v) for v in got] == [np.float32(v * sx) for v in H[:3]] + \
+        [np.float32(v * sy) for v in H[3:6]] + H[6:]
+    assert M.ground_truth(str(tmp_path / "none.csv"), "fixed_0007.png", sx, sy, H) == H
+    # a field that is no number: std::stof throws in the reference
+    p.write_text("7,1,2,x,4,5,6,7,8\n")
+    with pytest.raises(ValueError):
+        M.ground_truth(str(p), "fixed_0007.png", sx, sy, H)
