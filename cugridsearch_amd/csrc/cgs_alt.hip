@@ -22,7 +22,8 @@
 //     so the costs are the oracle's bit for bit.
 //   alt_step_mi_kernel  calcMIstreamAlt, and
 //   alt_norm_kernel     calcMIAlt (normalised H, rows from a host table,
-//     lambda += 1/(W_f - 1) in runs inside one binade): one grid point per
+//     lambda += 1/(W_f - 1) from segments inside one binade each, 64
+//     consecutive steps per pass): one grid point per
 //     workgroup, the four wavefronts taking row blocks in turn into one LDS
 //     joint histogram.
 // Every kernel asks for 8 wavefronts per SIMD (at most 64 VGPRs): the
@@ -190,17 +191,32 @@ __device__ __forceinline__ float row_value(const T *img, float X, int row_off, i
 // in [0, 255] up to a few ulps (each lerp is a rounded convex combination of
 // taps in [0, 255]; 0 outside the image), so t = bins * (v / 256) is in
 // [0, bins): mi_bin's clamps do nothing and the truncating conversion is its
-// floor.
+// floor.  With kb = bins / 256 (exact, bins < 2^24), v * kb rounds the same
+// real number bins * v / 256 as bins * (v / 256) does (v / 256 is exact for a
+// normal v, and a v small enough for it to round gives 0 either way): one
+// multiply.
 template <typename T>
-__device__ __forceinline__ int alt_bin(float v, int bins) {
-    if constexpr (std::is_same<T, uint8_t>::value) return (int) ((float) bins * (v / 256.0f));
+__device__ __forceinline__ int alt_bin(float v, int bins, float kb) {
+    if constexpr (std::is_same<T, uint8_t>::value) return (int) (v * kb);
     else return mi_bin(v, bins);
 }
 
-// Counter of the joint histogram's cell (fixed bin fb, moving bin mb) in
-// rows of bins + 1 counters; both factors are below 2^24.
-__device__ __forceinline__ uint32_t alt_cell(int fb, int mb, int bins) {
-    return __umul24((uint32_t) fb, (uint32_t) (bins + 1)) + (uint32_t) mb;
+// LDS address of the joint histogram's cell (fixed bin fb, moving bin mb):
+// base + 4 mb + fb rowb, rows of bins + 1 counters (rowb = 4 (bins + 1)
+// bytes), a shift-add and a 24-bit multiply-add (both factors below 2^24).
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ __forceinline__ uint32_t alt_lds_base(uint32_t *hist) {
+    return (uint32_t) (uintptr_t) (lds_u32 *) hist;
+}
+__device__ __forceinline__ uint32_t alt_cell(int fb, int mb, uint32_t rowb, uint32_t base) {
+    uint32_t t, r;
+    asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(t) : "v"(mb), "s"(base));
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(fb), "s"(rowb), "v"(t));
+    return r;
+}
+
+__device__ __forceinline__ void alt_count(uint32_t addr) {
+    atomicAdd((uint32_t *) (lds_u32 *) (uintptr_t) addr, 1u);
 }
 
 // MI of a wavefront's joint histogram h (row = fixed bin) over n samples,
@@ -297,13 +313,20 @@ __device__ __forceinline__ void alt_emit(const AltParams &P, int32_t base, const
 //
 // TabSampler: one-channel images through their exact tap table, one load per
 // sample (entry L + W + 1 holds p[L], p[L+1], p[L+W], p[L+W+1], 0 outside);
-// the ceil taps are picked by selects.  Out-of-image lanes read past the
-// table, where the buffer's range check returns zeros.
+// for a float image the ceil taps are picked by selects (see kFinite).
+// Out-of-image lanes read past the table, where the buffer's range check
+// returns zeros.
 // The tap index fy * W + fx of an in-image sample is an integer below the
 // image's element count (< 2^24, checked when the image is validated), so
 // one binary32 fma forms it exactly.
 template <typename T>
 struct TabSampler {
+    // A uint8 image's taps are finite, so the ceil taps need no selects: where
+    // ceil equals floor the lerp weight is 0, and fmaf(0, t, (1 - 0) p) = p for
+    // the neighbouring tap t the entry holds (0 * finite adds nothing), as for
+    // the bottom row when the row weight is 0.  A float image may hold inf or
+    // NaN there, which 0 * t would turn into NaN: it keeps the selects.
+    static constexpr bool kFinite = std::is_same<T, uint8_t>::value;
     __amdgpu_buffer_rsrc_t rsrc;
     int w;
     uint32_t base;   // byte offset of entry 0 of the image: (w + 1) entries
@@ -324,18 +347,36 @@ struct TabSampler {
         const uint32_t idx = (uint32_t) (int) fmaf(fy, W, fx);
         const uint32_t off = in ? idx * TapTable<T>::kBytes + base : kOffsetOut;
         const float4 q = TapTable<T>::load(rsrc, off);
-        const float tr = ax != 0.0f ? q.y : q.x;
-        const float bl = ay != 0.0f ? q.z : q.x;
-        const float br = ay != 0.0f ? (ax != 0.0f ? q.w : q.z) : tr;
+        float tr = q.y, bl = q.z, br = q.w;
+        if constexpr (!kFinite) {
+            tr = ax != 0.0f ? q.y : q.x;
+            bl = ay != 0.0f ? q.z : q.x;
+            br = ay != 0.0f ? (ax != 0.0f ? q.w : q.z) : tr;
+        }
         const float top = fmaf(ax, tr, (1.0f - ax) * q.x);
         const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
         const float v = fmaf(ay, bottom, (1.0f - ay) * top);
         return in ? v : 0.0f;
     }
+    // at(X, Y) for a point the caller knows to be inside the image.
+    __device__ __forceinline__ float at_inside(float X, float Y) const {
+        const float fx = floorf(X), fy = floorf(Y);
+        const float ax = X - fx, ay = Y - fy;
+        const float4 q = TapTable<T>::load(rsrc, (uint32_t) (int) fmaf(fy, W, fx) * TapTable<T>::kBytes + base);
+        float tr = q.y, bl = q.z, br = q.w;
+        if constexpr (!kFinite) {
+            tr = ax != 0.0f ? q.y : q.x;
+            bl = ay != 0.0f ? q.z : q.x;
+            br = ay != 0.0f ? (ax != 0.0f ? q.w : q.z) : tr;
+        }
+        const float top = fmaf(ax, tr, (1.0f - ax) * q.x);
+        const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
+        return fmaf(ay, bottom, (1.0f - ay) * top);
+    }
     __device__ __forceinline__ float at_row(float X, int row, int) const {
         const float fx = floorf(X), ax = X - fx;
         const float4 q = TapTable<T>::load(rsrc, (uint32_t) ((row * w + (int) fx + w + 1) * TapTable<T>::kBytes));
-        const float tr = ax != 0.0f ? q.y : q.x;
+        const float tr = kFinite || ax != 0.0f ? q.y : q.x;
         const float top = fmaf(ax, tr, (1.0f - ax) * q.x);
         return fmaf(0.0f, top, top);
     }
@@ -532,24 +573,40 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
 
 // MI costs: one grid point per workgroup, the four wavefronts taking blocks
 // of 64 rows in turn (wavefront w: blocks w, w + 4, ...) into one LDS joint
-// histogram.  LDS: bins^2 counters, then one binary64 moved-area sum per row
-// block (summed in block order).
+// histogram.  LDS, all of it dynamic so that the histogram starts at LDS
+// address 0 and a cell's byte offset is its address: the bins^2 counters, one
+// binary64 moved-area sum per row block (summed in block order), then the
+// wavefronts' step counts and the cost (AltTail).
 // LDS words of the joint histogram: bins rows padded to bins + 1 counters (a
 // column spreads over the LDS banks), rounded up to keep what follows 8-byte
 // aligned.
 __host__ __device__ inline int alt_hist_words(int bins) { return (bins * (bins + 1) + 1) & ~1; }
+
+struct AltTail {
+    int32_t steps[4];
+    float cost;
+    int32_t pad;
+};
+
+__host__ __device__ inline size_t alt_lds_bytes(int bins, int fh) {
+    return sizeof(uint32_t) * (size_t) alt_hist_words(bins) + sizeof(double) * (size_t) ((fh + 63) / 64) +
+           sizeof(AltTail);
+}
 
 __device__ __forceinline__ uint32_t *alt_lds_hist() {
     extern __shared__ uint32_t s_dyn[];
     return s_dyn;
 }
 
+__device__ __forceinline__ AltTail *alt_tail(uint32_t *hist, int bins, int fh) {
+    return (AltTail *) ((char *) hist + alt_lds_bytes(bins, fh) - sizeof(AltTail));
+}
+
 // calcMIstreamAlt.
 template <typename MT, typename FT, bool ONE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void alt_step_mi_kernel(const AltParams P) {
-    __shared__ int32_t s_steps[4];
-    __shared__ float s_cost;
     uint32_t *hist = alt_lds_hist();
+    AltTail *tail = alt_tail(hist, P.bins, P.fh);
     const int B2 = P.bins * P.bins;
     double *s_mov = (double *) (hist + alt_hist_words(P.bins));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -564,6 +621,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     const IntRow<FT> irow(IROW ? P.fraw : P.fq, P.fw, P.fh);
     const int C = ONE ? 1 : P.C;
     const int bins = P.bins;
+    const float kb = (float) bins / 256.0f;
+    const uint32_t rowb = 4u * (uint32_t) (bins + 1);
+    const uint32_t hbase = __builtin_amdgcn_readfirstlane(alt_lds_base(hist));
     float p[kMaxDim], c, s, h[8];
     alt_grid_point(P, gi, p, &c, &s);
     homography_centred(p, c, s, (float) P.mw / 2, (float) P.mh / 2, h);
@@ -586,7 +646,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                         float fv;
                         if constexpr (IROW) fv = ipass ? irow.at(lam, row) : fix.at_row(lam, row, ch);
                         else fv = fix.at_row(lam, row, ch);
-                        atomicAdd(&hist[alt_cell(alt_bin<FT>(fv, bins), alt_bin<MT>(mv, bins), bins)], 1u);
+                        alt_count(alt_cell(alt_bin<FT>(fv, bins, kb), alt_bin<MT>(mv, bins, kb), rowb, hbase));
                     }
                 });
             if (lane == j && nrow > 0) term = (double) nrow * (double) magr;
@@ -595,12 +655,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
         const double bsum = wave_sum(term);
         if (lane == 0) s_mov[blk] = bsum;
     }
-    if (lane == 0) s_steps[wave] = steps;
+    if (lane == 0) tail->steps[wave] = steps;
     __syncthreads();
     if (wave == 0) {
         double mov_area = 0.0;
         for (int blk = 0; blk * 64 < P.fh; ++blk) mov_area += s_mov[blk];   // block order
-        const int32_t total = s_steps[0] + s_steps[1] + s_steps[2] + s_steps[3];
+        const int32_t total = tail->steps[0] + tail->steps[1] + tail->steps[2] + tail->steps[3];
         float cost = 123123.0f;
         // both areas grow once per step and channel; the fixed one in binary32
         // exactly (fewer than 2^24)
@@ -614,20 +674,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
         }
         if (P.hist_dump && gi == 0)
             for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[(i / P.bins) * (P.bins + 1) + i % P.bins];
-        if (lane == 0) s_cost = cost;
+        if (lane == 0) tail->cost = cost;
     }
     __syncthreads();
-    if (threadIdx.x == 0) alt_emit(P, gi, &s_cost, 1);
+    if (threadIdx.x == 0) alt_emit(P, gi, &tail->cost, 1);
 }
 
-// calcMIAlt: rows from the host's table, lambda += 1/(W_f - 1) in runs that
-// stay inside one binade with one increment (step_run), lane j taking step j
-// of the run; the fixed area after n steps comes from the host's table.
+// A segment computed from wavefront-uniform values, held in scalar registers.
+__device__ __forceinline__ LamSeg uniform_seg(LamSeg g) {
+    g.k0 = __builtin_amdgcn_readfirstlane(g.k0);
+    g.a = __builtin_amdgcn_readfirstlane(g.a);
+    g.d = __builtin_amdgcn_readfirstlane(g.d);
+    g.lim = __builtin_amdgcn_readfirstlane(g.lim);
+    g.U = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, g.U)));
+    return g;
+}
+
+// calcMIAlt: rows from the host's table, lambda += 1/(W_f - 1) taken 64
+// consecutive steps per pass from the sequence's binade segments
+// (lam_segment), so that the short segments near lambda = 0 share a pass; the
+// fixed area after n steps comes from the host's table.
 template <typename MT, typename FT, bool ONE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void alt_norm_kernel(const AltParams P) {
-    __shared__ int32_t s_steps[4];
-    __shared__ float s_cost;
     uint32_t *hist = alt_lds_hist();
+    AltTail *tail = alt_tail(hist, P.bins, P.fh);
     const int B2 = P.bins * P.bins;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int32_t gi = (int32_t) blockIdx.x;
@@ -646,7 +716,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     const float fmw = (float) P.mw, fmh = (float) P.mh;
     // power-of-two moving sizes: the binary64 rescale has an exact binary32 form
     const bool pow2 = (P.mw & (P.mw - 1)) == 0 && (P.mh & (P.mh - 1)) == 0;
-    const int max_runs = P.fw + 3;   // lambda < 1 in steps of 1/(W_f - 1): at most W_f steps, >= 1 per run
+    // lambda < 1 in steps of 1/(W_f - 1): at most W_f + 1 steps per row
+    const int max_passes = P.fw / 64 + 3;
+    const float kb = (float) bins / 256.0f;
+    const uint32_t rowb = 4u * (uint32_t) (bins + 1);
+    const uint32_t hbase = __builtin_amdgcn_readfirstlane(alt_lds_base(hist));
     int32_t steps = 0;
     for (int blk = wave; blk * 64 < P.nrows; blk += 4) {
         const int k = blk * 64 + lane;
@@ -665,19 +739,38 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
             const float vx = lane_value(Lr.vx, j), vy = lane_value(Lr.vy, j);
             const float px = lane_value(Lr.px, j), py = lane_value(Lr.py, j);
             const float ri = lane_value(rir, j);
-            float x0 = lane_value(Lr.ls, j);
-            for (int run_i = 0; run_i < max_runs; ++run_i) {
-                int32_t a, d;
-                float U;
-                const int run = __builtin_amdgcn_readfirstlane(step_run(x0, sc, &a, &d, &U));
-                // a + lane * d <= 2^24 - 1 inside the run, d < 2^24 (step_run)
-                const float lam = lane == 0 ? x0 : (float) (int32_t) (__umul24((uint32_t) lane, (uint32_t) d) + (uint32_t) a) * U;
-                // one ballot per compare (see walk_unit_row)
-                const unsigned long long m = __builtin_amdgcn_ballot_w64(lane < run) &
-                                             __builtin_amdgcn_ballot_w64(lam < le) &
-                                             __builtin_amdgcn_ballot_w64(lam < 1.0f);
+            // passes of 64 consecutive steps (lane i: step k0 + i), filled from
+            // as many segments as reach into the pass: each lane takes the last
+            // segment that starts at or before its step and still holds it
+            LamSeg g = uniform_seg(lam_segment(lane_value(Lr.ls, j), sc, 0));
+            float carry = 0.0f;   // the step before the pass
+            int32_t k0 = 0;
+            for (int pass = 0; pass < max_passes; ++pass) {
+                const int32_t kl = k0 + lane;
+                float lam = 2.0f;   // lanes past the row's last step
+                unsigned long long covered = 0ull;
+                for (int fill = 0; fill <= 64; ++fill) {
+                    // t * d + a < 2^32 on every lane that can hold the segment
+                    // (t <= its length + 63, its length * d <= 2^24 + d)
+                    const uint32_t w = __umul24((uint32_t) (kl - g.k0), g.d) + g.a;
+                    const unsigned long long m =
+                        __builtin_amdgcn_ballot_w64(kl >= g.k0) & __builtin_amdgcn_ballot_w64(w <= g.lim);
+                    lam = __uint_as_float(mask_select(m, __float_as_uint(lam), __float_as_uint((float) w * g.U)));
+                    covered |= m;
+                    if (covered >> 63) break;
+                    // the next segment starts at the literal add after the last
+                    // covered step (the step before the pass when none is)
+                    const int last = covered ? 63 - __builtin_clzll(covered) : -1;
+                    const float x = (last < 0 ? carry : lane_value(lam, last)) + sc;
+                    if (!(x < le) || !(x < 1.0f)) break;   // the row ends before it
+                    g = uniform_seg(lam_segment(x, sc, k0 + last + 1));
+                }
+                // one ballot per compare (see walk_unit_row); the steps grow
+                // with the lane, so the valid lanes form a prefix
+                const unsigned long long m =
+                    __builtin_amdgcn_ballot_w64(lam < le) & __builtin_amdgcn_ballot_w64(lam < 1.0f);
                 if (m == 0ull) break;
-                if (lane < run && lam < le && lam < 1.0f) {   // the same compares
+                if (lam < le && lam < 1.0f) {   // the same compares
                     float nx = lam * vx + px, ny = lam * vy + py;
                     if (pow2) {
                         nx = rescale_pow2(nx, fmw);
@@ -686,33 +779,39 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                         nx = (float) (((double) nx + 0.5) * dmw);
                         ny = (float) (((double) ny + 0.5) * dmh);
                     }
+                    // (li, ri) lies inside the fixed image: 0 <= lambda < 1 gives
+                    // 0 <= li <= (1 - 2^-24) (W_f - 1) < W_f, and the table's rows,
+                    // -0.5 <= row < 0.5, give 0 <= ri < H_f ((row + 0.5) (H_f - 1)
+                    // < H_f - 1 rounds to at most H_f - 1)
                     const float li = lam * wm1;
                     for (int ch = 0; ch < C; ++ch) {
-                        const float fv = fix.at(li, ri, ch);
+                        float fv;
+                        if constexpr (ONE) fv = fix.at_inside(li, ri);
+                        else fv = fix.at(li, ri, ch);
                         const float mv = mov.at(nx, ny, ch);
-                        atomicAdd(&hist[alt_cell(alt_bin<FT>(fv, bins), alt_bin<MT>(mv, bins), bins)], 1u);
+                        alt_count(alt_cell(alt_bin<FT>(fv, bins, kb), alt_bin<MT>(mv, bins, kb), rowb, hbase));
                     }
                 }
-                const int nok = __popcll(m);
-                steps += nok;
-                if (nok < run) break;   // valid lanes form a prefix: the row ended in this run
-                x0 = lane_value(lam, run - 1) + sc;
+                steps += __popcll(m);
+                if (m != ~0ull) break;   // the row ended in this pass
+                carry = lane_value(lam, 63);
+                k0 += 64;
             }
         }
     }
-    if (lane == 0) s_steps[wave] = steps;
+    if (lane == 0) tail->steps[wave] = steps;
     __syncthreads();
     if (wave == 0) {
-        const int32_t total = s_steps[0] + s_steps[1] + s_steps[2] + s_steps[3];
+        const int32_t total = tail->steps[0] + tail->steps[1] + tail->steps[2] + tail->steps[3];
         const float fa = P.area[total < P.narea ? total : P.narea - 1];
         double mi = 0.0, hx, hy;
         if (total > 0) wave_mi(hist, P.bins, P.bins + 1, (double) total * (double) C, &mi, &hx, &hy);
         if (P.hist_dump && gi == 0)
             for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[(i / P.bins) * (P.bins + 1) + i % P.bins];
-        if (lane == 0) s_cost = (float) (-mi * (double) fa);
+        if (lane == 0) tail->cost = (float) (-mi * (double) fa);
     }
     __syncthreads();
-    if (threadIdx.x == 0) alt_emit(P, gi, &s_cost, 1);
+    if (threadIdx.x == 0) alt_emit(P, gi, &tail->cost, 1);
 }
 
 // ------------------------------------------------------------------ host
@@ -752,7 +851,7 @@ static hipError_t tap_table(const cgs_image *im, void *dst, hipStream_t s) {
 
 template <typename MT, typename FT, bool ONE>
 static hipError_t alt_launch(const AltParams &P, hipStream_t s) {
-    const size_t lds = sizeof(uint32_t) * (size_t) alt_hist_words(P.bins) + sizeof(double) * (size_t) ((P.fh + 63) / 64);
+    const size_t lds = alt_lds_bytes(P.bins, P.fh);
     if (P.cost == CGS_COST_MI_ALT)
         hipLaunchKernelGGL((alt_norm_kernel<MT, FT, ONE>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
     else if (P.cost == CGS_COST_MI_STREAM_ALT)
@@ -819,7 +918,7 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
             : ((st = ws->mov.ensure(nm * esz_m)) != CGS_OK || (st = ws->fixp.ensure(nf * esz_f)) != CGS_OK))
         return st;
     if (hist_dump_host && mi && (st = ws->hist.ensure(B2 * sizeof(uint32_t))) != CGS_OK) return st;
-    if (mi && sizeof(uint32_t) * alt_hist_words(o->mi_bins) + sizeof(double) * ((size_t) (fixed->height + 63) / 64) > 64 * 1024)
+    if (mi && alt_lds_bytes(o->mi_bins, fixed->height) > 64 * 1024)
         return set_error(CGS_ERR_TOO_LARGE, "line-traversal MI: the histogram and row-block sums exceed 64 KiB of LDS");
     unsigned long long *key = (unsigned long long *) ws->key.ptr;
     hipLaunchKernelGGL(alt_key_kernel, dim3(1), dim3(64), 0, stream, key);

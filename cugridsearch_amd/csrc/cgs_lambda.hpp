@@ -116,38 +116,43 @@ CGS_HD inline float unit_pass_chain(float x0, int j) {
 // |x| >= 2^52 (both give x).
 CGS_HD inline float rescale_pow2(float x, float W) { return (x + 0.5f) * W; }
 
-// A run of the lambda = lambda + s sequence (x >= 0 or -0, s > 0, x and s
-// normal) that a wavefront takes in one pass: returns n in [1, 64] such that
-// step j < n of the run is (float) (a + j * d) * U (step 0 is x itself), all
-// inside x's binade with one increment d * U; the step after the run is the
-// literal add of s to step n - 1.  When s is an odd multiple of U / 2 every
-// sum is a tie and ties-to-even picks the even one of the two neighbouring
-// grid points: from an even a the increment is the even one of
-// floor(s / U) and floor(s / U) + 1 at every step, so the run goes on; from an
-// odd a the first step is taken alone.  A run of 1 also covers x == 0 and a
-// step that leaves the binade.
-CGS_HD inline int step_run(float x, float s, int32_t *a, int32_t *d, float *U) {
-    *a = 0;
-    *d = 0;
-    *U = 1.0f;
-    if (x == 0.0f) return 1;
+// A segment of the lambda = lambda + s sequence (s > 0 normal) from x: the
+// steps k >= k0 with a + (k - k0) d <= lim (unsigned) are (float) (a + (k -
+// k0) d) * U.  Inside x's binade, with ulp U, every add moves lambda by the
+// same delta = s rounded to the U grid, so the steps up to the binade's top
+// are a + j d in units of U (a = x / U, d = delta / U, lim = 2^24 - 1).  When
+// s is an odd multiple of U / 2 every sum is a tie, which ties-to-even sends
+// to the even neighbour: from an even a the increment is the even one of
+// floor(s / U) and floor(s / U) + 1 at every step, so the segment goes on;
+// from an odd a the first step is taken alone.  A step taken alone (x == 0, an
+// add that leaves the binade or changes nothing, a tie from an odd a, a
+// subnormal or non-finite x) is the one-step segment a = d = lim = 1, U = x,
+// whose step is (float) 1 * x = x itself.  The step after a segment is the
+// literal add of s to its last step.
+struct LamSeg {
+    int32_t k0;
+    uint32_t a, d, lim;
+    float U;
+};
+
+CGS_HD inline LamSeg lam_segment(float x, float s, int32_t k0) {
+    LamSeg g{k0, 1u, 1u, 1u, x};
+    if (x == 0.0f) return g;
     const uint32_t bits = f2u(x);
     const uint32_t expo = bits & 0x7F800000u;
-    const float u = u2f(expo - (23u << 23));
     const float B = u2f(expo + 0x00800000u);
     const float y = x + s;
     const float delta = y - x;
-    if (!(y < B) || delta == 0.0f) return 1;
-    const float err = s - delta;                                   // exact, |err| <= U / 2
-    const int32_t am = (int32_t) ((bits & 0x007FFFFFu) | 0x00800000u);  // x / U
-    if ((err == 0.5f * u || err == -0.5f * u) && (am & 1)) return 1;
-    *a = am;
-    *d = (int32_t) __builtin_ldexpf(delta, 150 - (int32_t) (expo >> 23));   // delta / u, exact: u = 2^(E - 150)
-    *U = u;
-    // a + 63 d < 2^31 (a < 2^24, d < 2^24); the division only near the binade's top
-    if ((uint32_t) *a + 63u * (uint32_t) *d <= 0x00FFFFFFu) return 64;
-    const int32_t room = (0x00FFFFFF - *a) / *d;
-    return room + 1 < 64 ? room + 1 : 64;
+    if (!(y < B) || delta == 0.0f) return g;
+    const float u = u2f(expo - (23u << 23));
+    const float err = s - delta;                                        // exact, |err| <= U / 2
+    const uint32_t am = (bits & 0x007FFFFFu) | 0x00800000u;             // x / U
+    if ((err == 0.5f * u || err == -0.5f * u) && (am & 1u)) return g;
+    g.a = am;
+    g.d = (uint32_t) (int32_t) __builtin_ldexpf(delta, 150 - (int32_t) (expo >> 23));   // delta / U, exact
+    g.lim = 0x00FFFFFFu;
+    g.U = u;
+    return g;
 }
 
 // s + d added n times in binary32, the loop `area += d` of the

@@ -1,7 +1,7 @@
 // Checks the closed forms of cgs_lambda.hpp against the sequential binary32
-// loops they replace.  Prints "<mismatches> <comparisons> <runs of the
-// 512-wide rows> <starts unit_pass_direct accepts> <starts through
-// unit_pass_chain> <rescales compared>".
+// loops they replace.  Prints "<mismatches> <comparisons> <passes of the
+// 512-wide rows from 0> <starts unit_pass_direct accepts> <starts through
+// unit_pass_chain> <rescales compared> <areas compared>".
 #include <cstdio>
 #include <cmath>
 #include <cstring>
@@ -11,6 +11,56 @@
 #include "cgs_lambda.hpp"
 
 static bool same(float a, float b) { return std::memcmp(&a, &b, 4) == 0; }
+
+// alt_norm_kernel's walk of one calcMIAlt row, lane by lane: passes of 64
+// consecutive steps; in each, segments (lam_segment) are laid over the lanes
+// in order, lane i of the pass taking the last segment that starts at or
+// before step k0 + i and holds it (a + t d <= lim, with the 24-bit multiply
+// of the kernel); the next segment starts at the literal add after the last
+// covered step.  Unfilled lanes hold 2; the valid lanes (lambda < le and
+// lambda < 1) must form a prefix.  Returns the row's steps.
+static std::vector<float> packed_walk(float ls, float le, float s, int w, int *passes, long *segs, long *bad) {
+    std::vector<float> out;
+    cgs::LamSeg g = cgs::lam_segment(ls, s, 0);
+    float carry = 0.0f;
+    int k0 = 0;
+    *passes = 0;
+    for (int pass = 0; pass < w / 64 + 3; ++pass) {
+        float lam[64];
+        for (float &v : lam) v = 2.0f;
+        uint64_t covered = 0;
+        for (int fill = 0; fill <= 64; ++fill) {
+            uint64_t m = 0;
+            for (int i = 0; i < 64; ++i) {
+                const int kl = k0 + i;
+                const uint32_t t = (uint32_t) (kl - g.k0);
+                const uint32_t wv = (uint32_t) ((uint64_t) (t & 0xFFFFFFu) * (uint64_t) (g.d & 0xFFFFFFu)) + g.a;
+                if (kl >= g.k0 && wv <= g.lim) {
+                    m |= 1ull << i;
+                    lam[i] = (float) wv * g.U;
+                }
+            }
+            covered |= m;
+            if (covered >> 63) break;
+            const int last = covered ? 63 - __builtin_clzll(covered) : -1;
+            const float x = (last < 0 ? carry : lam[last]) + s;
+            if (!(x < le) || !(x < 1.0f)) break;
+            g = cgs::lam_segment(x, s, k0 + last + 1);
+            ++*segs;
+        }
+        uint64_t m = 0;
+        for (int i = 0; i < 64; ++i)
+            if (lam[i] < le && lam[i] < 1.0f) m |= 1ull << i;
+        if (m == 0) break;
+        ++*passes;
+        if (m & (m + 1)) ++*bad;   // not a prefix
+        for (int i = 0; i < 64 && (m >> i & 1); ++i) out.push_back(lam[i]);
+        if (m != ~0ull) break;
+        carry = lam[63];
+        k0 += 64;
+    }
+    return out;
+}
 
 int main() {
     std::mt19937 rng(12345);
@@ -89,38 +139,34 @@ int main() {
             if (!same(cgs::rescale_pow2(x, W), (float) (((double) x + 0.5) * (double) W))) ++bad;
         }
     }
-    // lambda += 1/(float)(W - 1) from starts in [0, 1), walked in runs (the
-    // kernel's passes); runs counts the passes of the starts at 0
+    // lambda += 1/(float)(W - 1) from starts in [0, 1), walked as the calcMIAlt
+    // kernel walks a row (packed_walk): passes of 64 steps filled from
+    // lam_segment's segments, against the literal loop with the row's end at
+    // 1, inside (0, 1), at the start and below it, and NaN; passes512 counts
+    // the passes of the 512-wide rows from 0 (+0 and -0) with the end at 1
     std::vector<int> widths;
     for (int w = 2; w <= 300; ++w) widths.push_back(w);
     for (int i = 0; i < 300; ++i) widths.push_back(300 + (int) (rng() % 8000));
     for (int w : {256, 257, 384, 385, 511, 512, 513, 640, 1000, 1024, 1025, 4096, 4097, 6001}) widths.push_back(w);
+    long segs = 0;
     for (int w : widths) {
         const float s = 1 / ((float) (w - 1));
-        std::vector<float> xs = {0.0f, -0.0f, s, 0.5f * s, 0.25f, 0.5f, 0.499999970f};
+        std::vector<float> xs = {0.0f, -0.0f, s, 0.5f * s, 0.25f, 0.5f, 0.499999970f, 1e-40f, 0.999999940f};
         for (int i = 0; i < 6; ++i) xs.push_back(u01(rng));
         for (int i = 0; i < 3; ++i) xs.push_back(u01(rng) * 1e-3f);
+        for (int i = 0; i < 3; ++i) xs.push_back(u01(rng) * 0.07f);
         for (float x0 : xs) {
-            float seq = x0;
-            std::vector<float> want;
-            for (int k = 0; seq < 1.0f && k < w + 4; ++k) {
-                want.push_back(seq);
-                seq = seq + s;
-            }
-            float base = x0;
-            size_t k = 0;
-            while (k < want.size()) {
-                if (x0 == 0.0f && w == 512) ++runs512;
-                int32_t a, d;
-                float U;
-                const int run = cgs::step_run(base, s, &a, &d, &U);
-                float last = base;
-                for (int j = 0; j < run && k < want.size(); ++j, ++k) {
-                    last = j == 0 ? base : (float) (a + j * d) * U;
-                    ++n;
-                    if (!same(last, want[k])) ++bad;
-                }
-                base = last + s;
+            const std::vector<float> les = {1.0f, x0 + 0.5f * u01(rng), x0 + 0.03f * u01(rng), x0, x0 * 0.5f, NAN};
+            for (float le : les) {
+                std::vector<float> want;
+                for (float seq = x0; seq < le && seq < 1.0f && (int) want.size() < w + 4; seq = seq + s)
+                    want.push_back(seq);
+                int passes = 0;
+                const std::vector<float> got = packed_walk(x0, le, s, w, &passes, &segs, &bad);
+                if (got.size() != want.size()) ++bad;
+                for (size_t k = 0; k < want.size() && k < got.size(); ++k, ++n)
+                    if (!same(got[k], want[k])) ++bad;
+                if (w == 512 && x0 == 0.0f && le == 1.0f) runs512 += passes;
             }
         }
     }

@@ -160,7 +160,9 @@ def test_alt_nonfinite_fixed_image(gpu, oracle, fn):
 
 
 CASES_256 = [(fn, np.float32, 256, 256, 256, 256) for fn in FUNCTORS] + \
-    [(fn, np.uint8, 256, 256, 256, 256) for fn in FUNCTORS] + [(A.calcMIAlt, np.uint8, 200, 168, 200, 168)]
+    [(fn, np.uint8, 256, 256, 256, 256) for fn in FUNCTORS] + [(A.calcMIAlt, np.uint8, 200, 168, 200, 168)] + \
+    [(A.calcMIAlt, np.uint8, 512, 512, 512, 512), (A.calcMIAlt, np.uint8, 300, 150, 600, 300),
+     (A.calcMIAlt, np.float32, 512, 512, 512, 512)]
 
 
 @pytest.mark.parametrize("fn,dtype,mw,mh,fw,fh", CASES_256,
@@ -170,7 +172,9 @@ def test_alt_256_sample(gpu, oracle, fn, dtype, mw, mh, fw, fh):
     passes start from the previous pass's last lambda), 256 rows make four
     64-row blocks over the four wavefronts (MI: one shared LDS histogram), and
     200 x 168 is a moving size whose calcMIAlt rescale takes the binary64
-    form.  A 3^8 grid; 24 sampled points and the argmin against the oracle:
+    form.  calcMIAlt at 512 x 512 (the benchmark's size: rows of 511 steps of
+    1/511 in 8 passes, the short segments near lambda = 0 sharing the first)
+    and with a 600-wide fixed image twice the moving one's width.  A 3^8 grid; 24 sampled points and the argmin against the oracle:
     SQD/NCC costs bit for bit, MI joint histograms bit for bit and costs within
     1e-6."""
     m, _, _ = synth.make_pair(70 + fn.cost, mw, mh, sigma=4.0, perturb=6.0, dtype=dtype, gamma=dtype == np.uint8,

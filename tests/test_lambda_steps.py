@@ -24,7 +24,8 @@ def test_closed_form_lambda_matches_sequential_loops(tmp_path):
     # expression on every compared value
     assert rescales > 8_000_000, rescales
     # a 512-wide row (step 1/511, a tie in one binade) from lambda = 0 (twice:
-    # +0 and -0) goes in a few runs, not one pass per step
-    assert runs512 <= 2 * 16, runs512
+    # +0 and -0) takes its 512 steps in 8 full passes of 64, its short
+    # segments near 0 sharing the first
+    assert runs512 == 2 * 8, runs512
     # the moved area's closed-form repeated add equals the literal loop
     assert areas > 100_000, areas
