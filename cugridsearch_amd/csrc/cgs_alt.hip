@@ -64,6 +64,9 @@ struct AltParams {
     int nrows;
     const float *area;      // calcMIAlt: fixed area after n steps
     int narea;
+    const float *lam0;      // calcMIAlt: the lambda sequence from 0 (below 1)
+    int n0;
+    const uint8_t *fb0;     // calcMIAlt, one channel: fixed-image bin of row k at lam0[j], entry k * n0 + j
     float *result;
     unsigned long long *key;
     int tie_mode;
@@ -718,6 +721,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     const bool pow2 = (P.mw & (P.mw - 1)) == 0 && (P.mh & (P.mh - 1)) == 0;
     // lambda < 1 in steps of 1/(W_f - 1): at most W_f + 1 steps per row
     const int max_passes = P.fw / 64 + 3;
+    const int n0 = P.n0;
+    const __amdgpu_buffer_rsrc_t lam0_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *) P.lam0, (short) 0, n0 * (int) sizeof(float), 0x00020000);
+    const __amdgpu_buffer_rsrc_t fb0_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) P.fb0, (short) 0, ONE ? P.nrows * n0 : 0, 0x00020000);
     const float kb = (float) bins / 256.0f;
     const uint32_t rowb = 4u * (uint32_t) (bins + 1);
     const uint32_t hbase = __builtin_amdgcn_readfirstlane(alt_lds_base(hist));
@@ -739,17 +747,29 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
             const float vx = lane_value(Lr.vx, j), vy = lane_value(Lr.vy, j);
             const float px = lane_value(Lr.px, j), py = lane_value(Lr.py, j);
             const float ri = lane_value(rir, j);
+            const float ls = lane_value(Lr.ls, j);
+            // a one-channel row from lambda = 0 (or -0, whose later steps are the
+            // same and whose samples are those of +0) takes its steps and its
+            // fixed bins from the search's tables
+            const bool from0 = ONE && ls == 0.0f;
+            const uint32_t fb0_row = (uint32_t) (blk * 64 + j) * (uint32_t) P.n0;
             // passes of 64 consecutive steps (lane i: step k0 + i), filled from
             // as many segments as reach into the pass: each lane takes the last
             // segment that starts at or before its step and still holds it
-            LamSeg g = uniform_seg(lam_segment(lane_value(Lr.ls, j), sc, 0));
+            LamSeg g = uniform_seg(lam_segment(ls, sc, 0));
             float carry = 0.0f;   // the step before the pass
             int32_t k0 = 0;
             for (int pass = 0; pass < max_passes; ++pass) {
                 const int32_t kl = k0 + lane;
                 float lam = 2.0f;   // lanes past the row's last step
                 unsigned long long covered = 0ull;
-                for (int fill = 0; fill <= 64; ++fill) {
+                if (from0) {
+                    // offsets past the table read 0, which the select replaces
+                    const float t = __builtin_bit_cast(
+                        float, __builtin_amdgcn_raw_buffer_load_b32(lam0_rsrc, (uint32_t) kl * 4u, 0, 0));
+                    lam = kl < n0 ? t : 2.0f;
+                }
+                for (int fill = 0; fill <= 64 && !from0; ++fill) {
                     // t * d + a < 2^32 on every lane that can hold the segment
                     // (t <= its length + 63, its length * d <= 2^24 + d)
                     const uint32_t w = __umul24((uint32_t) (kl - g.k0), g.d) + g.a;
@@ -785,11 +805,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                     // < H_f - 1 rounds to at most H_f - 1)
                     const float li = lam * wm1;
                     for (int ch = 0; ch < C; ++ch) {
-                        float fv;
-                        if constexpr (ONE) fv = fix.at_inside(li, ri);
-                        else fv = fix.at(li, ri, ch);
+                        int fb;
+                        if constexpr (ONE) {
+                            if (from0)
+                                fb = (int) __builtin_amdgcn_raw_buffer_load_b8(fb0_rsrc, fb0_row + (uint32_t) kl, 0, 0);
+                            else
+                                fb = alt_bin<FT>(fix.at_inside(li, ri), bins, kb);
+                        } else {
+                            fb = alt_bin<FT>(fix.at(li, ri, ch), bins, kb);
+                        }
                         const float mv = mov.at(nx, ny, ch);
-                        alt_count(alt_cell(alt_bin<FT>(fv, bins, kb), alt_bin<MT>(mv, bins, kb), rowb, hbase));
+                        alt_count(alt_cell(fb, alt_bin<MT>(mv, bins, kb), rowb, hbase));
                     }
                 }
                 steps += __popcll(m);
@@ -812,6 +838,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
     }
     __syncthreads();
     if (threadIdx.x == 0) alt_emit(P, gi, &tail->cost, 1);
+}
+
+// calcMIAlt's fixed-image bin of row k (the row table's entry) at lambda =
+// lam0[j], for the rows that start at lambda = 0: the arithmetic of
+// alt_norm_kernel's fixed sample, entry by entry.
+template <typename FT>
+__global__ __launch_bounds__(256) void alt_fbin0_kernel(const AltParams P, uint8_t *fb0) {
+    const TabSampler<FT> fix(P.fq, P.fw, P.fh);
+    const float wm1 = (float) (P.fw - 1);
+    const float kb = (float) P.bins / 256.0f;
+    const size_t n = (size_t) P.nrows * (size_t) P.n0;
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x) {
+        const int k = (int) (i / (size_t) P.n0), j = (int) (i % (size_t) P.n0);
+        const float ri = (float) (((double) P.rows[k] + 0.5) * (double) (P.fh - 1));
+        const float li = P.lam0[j] * wm1;
+        fb0[i] = (uint8_t) alt_bin<FT>(fix.at_inside(li, ri), P.bins, kb);
+    }
 }
 
 // ------------------------------------------------------------------ host
@@ -880,10 +923,15 @@ static hipError_t alt_dispatch(const AltParams &P, int mpix, int fpix, hipStream
 // src/gpu/cudaErrorFunction_mi.cuh:162) and the fixed area after n steps
 // (fix_area += 1.0 / (W_f H_f) through binary64, rounded to binary32, :198)
 // for n up to rows * (W_f + 2).
-static void mi_alt_tables(int fw, int fh, std::vector<float> &rows, std::vector<float> &area) {
+static void mi_alt_tables(int fw, int fh, std::vector<float> &rows, std::vector<float> &area,
+                          std::vector<float> &lam0) {
     rows.clear();
     const float step = 1 / ((float) (fh - 1));
     for (float row = -0.5f; row < 0.5; row += step) rows.push_back(row);
+    // a row that starts at lambda = 0 walks this sequence (src/gpu/cudaErrorFunction_mi.cuh:177)
+    lam0.clear();
+    const float sc = 1 / ((float) (fw - 1));
+    for (float lam = 0.0f; lam < 1.0f && lam0.size() <= (size_t) fw + 1; lam += sc) lam0.push_back(lam);
     const size_t n = rows.size() * ((size_t) fw + 2) + 1;
     area.resize(n);
     const double d = 1.0 / (double) (fw * fh);
@@ -935,9 +983,10 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
         if (fixed->width < 2 || fixed->height < 2)
             return set_error(CGS_ERR_INVALID_ARGUMENT, "calcMIAlt needs a fixed image of at least 2 x 2");
         std::vector<float> rows, area;
-        mi_alt_tables(fixed->width, fixed->height, rows, area);
+        std::vector<float> lam0;
+        mi_alt_tables(fixed->width, fixed->height, rows, area, lam0);
         if (area.size() > (size_t) INT32_MAX) return set_error(CGS_ERR_TOO_LARGE, "calcMIAlt: fixed image too large");
-        const size_t bytes = sizeof(float) * (rows.size() + area.size());
+        const size_t bytes = sizeof(float) * (rows.size() + area.size() + lam0.size());
         if (ws->alt_tab_fw != fixed->width || ws->alt_tab_fh != fixed->height) {
             if ((st = ws->alt_tab.ensure(bytes)) != CGS_OK || (st = ws->alt_tab_host.ensure(bytes, true)) != CGS_OK)
                 return st;
@@ -945,6 +994,7 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
             float *h = (float *) ws->alt_tab_host.ptr;
             std::copy(rows.begin(), rows.end(), h);
             std::copy(area.begin(), area.end(), h + rows.size());
+            std::copy(lam0.begin(), lam0.end(), h + rows.size() + area.size());
             HIP_TRY(hipMemcpyAsync(ws->alt_tab.ptr, h, bytes, hipMemcpyHostToDevice, stream));
             ws->alt_tab_fw = fixed->width;
             ws->alt_tab_fh = fixed->height;
@@ -953,6 +1003,12 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
         P.nrows = (int) rows.size();
         P.area = P.rows + rows.size();
         P.narea = (int) area.size();
+        P.lam0 = P.area + area.size();
+        P.n0 = (int) lam0.size();
+        if (one) {
+            if ((st = ws->alt_fb0.ensure(rows.size() * lam0.size())) != CGS_OK) return st;
+            P.fb0 = (const uint8_t *) ws->alt_fb0.ptr;
+        }
     }
     {
         float *h = (float *) ws->sincos_host.ptr;
@@ -982,6 +1038,15 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
     P.key = key;
     P.tie_mode = o->tie_mode;
     P.hist_dump = (hist_dump_host && mi) ? (uint32_t *) ws->hist.ptr : nullptr;
+    if (P.fb0) {   // calcMIAlt's fixed bins along lam0 for this image and bin count
+        const size_t n = (size_t) P.nrows * (size_t) P.n0;
+        const unsigned blocks = (unsigned) std::min<size_t>((n + 255) / 256, 4096);
+        if (fixed->pixtype == CGS_PIX_U8)
+            hipLaunchKernelGGL(alt_fbin0_kernel<uint8_t>, dim3(blocks), dim3(256), 0, stream, P, (uint8_t *) ws->alt_fb0.ptr);
+        else
+            hipLaunchKernelGGL(alt_fbin0_kernel<float>, dim3(blocks), dim3(256), 0, stream, P, (uint8_t *) ws->alt_fb0.ptr);
+        HIP_TRY(hipGetLastError());
+    }
     const bool timed = (o->flags & CGS_FLAG_TIME_KERNEL) != 0;
     if (timed) HIP_TRY(ws->record_start(stream));
     HIP_TRY(alt_dispatch(P, moved->pixtype, fixed->pixtype, stream));

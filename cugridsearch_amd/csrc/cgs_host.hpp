@@ -39,7 +39,8 @@ struct Workspace {
     DevBuf prep, prep_host;     // proposal-network input descriptors
     DevBuf cov, cov_host;       // covariance partial sums
     DevBuf fixp, fixq;          // line-traversal costs: zero-padded fixed image, its tap table
-    DevBuf alt_tab, alt_tab_host;   // calcMIAlt's row values and fixed-area table
+    DevBuf alt_tab, alt_tab_host;   // calcMIAlt's row values, fixed-area table and lambda sequence from 0
+    DevBuf alt_fb0;                 // calcMIAlt: fixed-image bins of the rows that start at lambda = 0
     int alt_tab_fw = 0, alt_tab_fh = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     // HIP events around one launch on the caller's stream; elapsed_ms()
