@@ -207,10 +207,7 @@ __device__ __forceinline__ int alt_bin(float v, int bins, float kb) {
 // LDS address of the joint histogram's cell (fixed bin fb, moving bin mb):
 // base + 4 mb + fb rowb, rows of bins + 1 counters (rowb = 4 (bins + 1)
 // bytes), a shift-add and a 24-bit multiply-add (both factors below 2^24).
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
-__device__ __forceinline__ uint32_t alt_lds_base(uint32_t *hist) {
-    return (uint32_t) (uintptr_t) (lds_u32 *) hist;
-}
+__device__ __forceinline__ uint32_t alt_lds_base(uint32_t *hist) { return lds_address(hist); }
 __device__ __forceinline__ uint32_t alt_cell(int fb, int mb, uint32_t rowb, uint32_t base) {
     uint32_t t, r;
     asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(t) : "v"(mb), "s"(base));
@@ -218,9 +215,7 @@ __device__ __forceinline__ uint32_t alt_cell(int fb, int mb, uint32_t rowb, uint
     return r;
 }
 
-__device__ __forceinline__ void alt_count(uint32_t addr) {
-    atomicAdd((uint32_t *) (lds_u32 *) (uintptr_t) addr, 1u);
-}
+__device__ __forceinline__ void alt_count(uint32_t addr) { lds_count(addr); }
 
 // MI of a wavefront's joint histogram h (row = fixed bin) over n samples,
 // binary64: mi = sum pxy (log2 pxy - (log2 px + log2 py)), hx / hy the

@@ -744,20 +744,29 @@ __device__ __forceinline__ int clamp_int(int v, int lo, int hi) {
     return r;
 }
 
-// floor(bins * (v / 256)) clamped to [0, bins-1].  POW2: bins is a power of
-// two, so both scalings are exact and one multiply by bins/256 gives the same
-// value.  U8: v is a bilinear value of a uint8 image, in [0, 255] up to a
-// few ulps (rounded convex combinations of taps in [0, 255]), so t lies in
-// [0, bins) and the clamp does nothing (lanes outside the image, whose v
-// may be anything, have their counter replaced by the caller).
-template <bool POW2, bool U8 = false>
-__device__ __forceinline__ int mi_bin_fast(float v, int bins, float fbins, float scale) {
-    const float t = POW2 ? v * scale : fbins * (v / 256.0f);
+// floor(bins * (v / 256)) clamped to [0, bins-1], as v * scale with scale =
+// bins / 256 (exact for bins < 2^24): v / 256 is exact for a normal v, so
+// both forms round the same real number bins * v / 256 once; where v / 256
+// would round (|v| < 2^-118) both give a t in (-1, 1) whose clamped floor is
+// 0, and a NaN gives NaN either way.  U8: v is a bilinear value of a uint8
+// image, in [0, 255] up to a few ulps (rounded convex combinations of taps in
+// [0, 255]), so t lies in [0, bins) and the clamp does nothing (lanes outside
+// the image, whose v may be anything, have their counter replaced by the
+// caller).
+template <bool U8 = false>
+__device__ __forceinline__ int mi_bin_fast(float v, int bins, float scale) {
+    const float t = v * scale;
     if constexpr (U8) return floor_to_int(t);
     else return clamp_int(floor_to_int(t), 0, bins - 1);
 }
 
-template <typename MT, typename FT, int GPT, bool FASTDIV, bool POW2>
+// LDS byte address of a shared-memory pointer, and a 32-bit counter add at
+// an LDS byte address.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ __forceinline__ uint32_t lds_address(uint32_t *p) { return (uint32_t) (uintptr_t) (lds_u32 *) p; }
+__device__ __forceinline__ void lds_count(uint32_t addr) { atomicAdd((uint32_t *) (lds_u32 *) (uintptr_t) addr, 1u); }
+
+template <typename MT, typename FT, int GPT, bool FASTDIV>
 __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_coef)[12], uint32_t *hist,
                                          uint32_t *cnt) {
     const int tid = threadIdx.x, lane = tid & 63;
@@ -774,8 +783,12 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
         __builtin_amdgcn_make_buffer_rsrc((void *) fix, (short) 0, (int) ((size_t) fw * fh * sizeof(FT)), 0x00020000);
     const int bins = uniform(P.bins), RS = bins + 1, HS = bins * RS;   // rows padded to bins + 1 counters
     const bool ceil_taps = uniform(P.ceil_taps) != 0;
-    const float fbins = (float) bins, bscale = fbins / 256.0f;
-    const uint32_t dummy = (uint32_t) (GPT * HS + lane);
+    const float bscale = (float) bins / 256.0f;
+    // LDS byte addresses: the histograms, one row, one histogram, and this
+    // lane's dummy counter past them (for samples outside the image)
+    const uint32_t hbase = __builtin_amdgcn_readfirstlane(lds_address(hist));
+    const uint32_t row4 = 4u * (uint32_t) RS, hist4 = 4u * (uint32_t) HS;
+    const uint32_t dummy = hbase + 4u * (uint32_t) (GPT * HS + lane);
 
     float B[GPT][3];
 #pragma unroll
@@ -808,7 +821,10 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
         for (int it = 0; it < steps; ++it, foff += fstep, yf += (float) rpi) {
             const unsigned long long vmask = it < full ? valid_mask : tail_mask;
             const float f = load_fix<FT>(frsrc, foff);   // 0 on lanes off the image
-            const uint32_t rowbin = (uint32_t) (mi_bin_fast<POW2>(f, bins, fbins, bscale) * RS);
+            uint32_t raddr;   // the fixed bin's row of the first histogram
+            asm("v_mad_u32_u24 %0, %1, %2, %3"
+                : "=v"(raddr)
+                : "v"(mi_bin_fast(f, bins, bscale)), "s"(row4), "v"(hbase));   // one scalar operand per VOP3
             unsigned long long in[GPT];
             float ax[GPT], ay[GPT];
             float4 q[GPT];
@@ -842,8 +858,9 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 }
                 ax[g] = __builtin_amdgcn_fractf(X);
                 ay[g] = __builtin_amdgcn_fractf(Y);
-                uint32_t off;
-                asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(off) : "v"(iy), "s"(wE), "v"(ix * EB + baseE));
+                uint32_t t, off;
+                asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(t) : "v"(ix), "i"(EB == 4 ? 2 : 4), "s"(baseE));
+                asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(off) : "v"(iy), "s"(wE), "v"(t));
                 q[g] = TapTable<MT>::load(rsrc, mask_select(m, kOffsetOut, off));
             }
 #pragma unroll
@@ -861,9 +878,12 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 const float top = fmaf(ax[g], tr, (1.0f - ax[g]) * q[g].x);
                 const float bottom = fmaf(ax[g], br, (1.0f - ax[g]) * bl);
                 const float v = fmaf(ay[g], bottom, (1.0f - ay[g]) * top);
-                const uint32_t h = (uint32_t) (g * HS) + rowbin +
-                                   (uint32_t) mi_bin_fast<POW2, std::is_same<MT, uint8_t>::value>(v, bins, fbins, bscale);
-                atomicAdd(&hist[mask_select(in[g], dummy, h)], 1u);
+                uint32_t a;
+                asm("v_lshl_add_u32 %0, %1, 2, %2"
+                    : "=v"(a)
+                    : "v"(mi_bin_fast<std::is_same<MT, uint8_t>::value>(v, bins, bscale)), "v"(raddr));
+                lds_count(mask_select(in[g], dummy, a));
+                raddr += hist4;   // the next grid point's histogram
             }
         }
     }
@@ -929,10 +949,8 @@ __device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, Bl
     uint32_t cnt[GPT];
 #pragma unroll
     for (int g = 0; g < GPT; ++g) cnt[g] = 0u;
-    const bool pow2 = (P.bins & (P.bins - 1)) == 0;
-    if (fastdiv && pow2) mi_sweep<MT, FT, GPT, true, true>(P, S.coef, hist, cnt);
-    else if (fastdiv) mi_sweep<MT, FT, GPT, true, false>(P, S.coef, hist, cnt);
-    else mi_sweep<MT, FT, GPT, false, false>(P, S.coef, hist, cnt);
+    if (fastdiv) mi_sweep<MT, FT, GPT, true>(P, S.coef, hist, cnt);
+    else mi_sweep<MT, FT, GPT, false>(P, S.coef, hist, cnt);
     if (lane == 0)
 #pragma unroll
         for (int g = 0; g < GPT; ++g) S.red[wave][g][3] = (double) cnt[g];
