@@ -64,8 +64,8 @@ struct AltParams {
     int nrows;
     const float *area;      // calcMIAlt: fixed area after n steps
     int narea;
-    const float *lam0;      // calcMIAlt: the lambda sequence from 0 (below 1)
-    int n0;
+    const float *lam0;      // calcMIAlt: the lambda sequence from 0 (below 1), then 2.0f up to n0pad entries
+    int n0, n0pad;
     const uint8_t *fb0;     // calcMIAlt, one channel: fixed-image bin of row k at lam0[j], entry k * n0 + j
     float *result;
     unsigned long long *key;
@@ -204,14 +204,27 @@ __device__ __forceinline__ int alt_bin(float v, int bins, float kb) {
     else return mi_bin(v, bins);
 }
 
+
 // LDS address of the joint histogram's cell (fixed bin fb, moving bin mb):
 // base + 4 mb + fb rowb, rows of bins + 1 counters (rowb = 4 (bins + 1)
 // bytes), a shift-add and a 24-bit multiply-add (both factors below 2^24).
 __device__ __forceinline__ uint32_t alt_lds_base(uint32_t *hist) { return lds_address(hist); }
 __device__ __forceinline__ uint32_t alt_cell(int fb, int mb, uint32_t rowb, uint32_t base) {
-    uint32_t t, r;
-    asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(t) : "v"(mb), "s"(base));
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(fb), "s"(rowb), "v"(t));
+    uint32_t r;
+    asm("v_lshl_add_u32 %0, %1, 2, %2\n\tv_mad_u32_u24 %0, %3, %4, %0"
+        : "=&v"(r)
+        : "v"(mb), "s"(base), "v"(fb), "s"(rowb));
+    return r;
+}
+
+// alt_cell with the moving bin trunc(t) (t >= 0 or NaN, which
+// v_cvt_i32_f32 takes to 0): the conversion, the shift-add and the mad in
+// one block.
+__device__ __forceinline__ uint32_t alt_cell_trunc(int fb, float t, uint32_t rowb, uint32_t base) {
+    uint32_t r;
+    asm("v_cvt_i32_f32 %0, %1\n\tv_lshl_add_u32 %0, %0, 2, %2\n\tv_mad_u32_u24 %0, %3, %4, %0"
+        : "=&v"(r)
+        : "v"(t), "s"(base), "v"(fb), "s"(rowb));
     return r;
 }
 
@@ -355,6 +368,50 @@ struct TabSampler {
         const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
         const float v = fmaf(ay, bottom, (1.0f - ay) * top);
         return in ? v : 0.0f;
+    }
+    // at(X, Y) in two halves, so that a caller can issue several loads before
+    // it waits on any: load_at forms the taps' offset by integer floors
+    // (v_cvt_flr), the weights by v_fract and the offset as a shift-add and
+    // a 24-bit mad -- for an in-image point (X, Y >= 0) floorf, X - floorf(X)
+    // and the fma index exactly -- and loads the taps.
+    // A uint8 image skips at's final select: off the image the taps read 0,
+    // so v is 0 for finite X, Y and NaN otherwise, and both truncate to bin 0
+    // (v_cvt_i32_f32 takes NaN to 0), the bin of the 0 that at returns;
+    // cell_of gives the bin's histogram cell.
+    struct Taps {
+        float4 q;
+        float ax, ay;
+        bool in;
+    };
+    __device__ __forceinline__ Taps load_at(float X, float Y) const {
+        constexpr int EB = TapTable<T>::kBytes;
+        Taps t;
+        t.in = X >= 0.0f && Y >= 0.0f && X < W && Y < H;
+        const int ix = floor_to_int(X), iy = floor_to_int(Y);
+        t.ax = __builtin_amdgcn_fractf(X);
+        t.ay = __builtin_amdgcn_fractf(Y);
+        uint32_t off;
+        asm("v_lshl_add_u32 %0, %1, %2, %3\n\tv_mad_i32_i24 %0, %4, %5, %0"
+            : "=&v"(off)
+            : "v"(ix), "i"(EB == 4 ? 2 : 4), "s"(base), "v"(iy), "s"(w * EB));
+        t.q = TapTable<T>::load(rsrc, t.in ? off : kOffsetOut);
+        return t;
+    }
+    // the histogram cell (fixed bin fb) of the value the taps give
+    __device__ __forceinline__ uint32_t cell_of(const Taps &t, int fb, int bins, float kb, uint32_t rowb,
+                                                uint32_t hbase) const {
+        const float ax = t.ax, ay = t.ay;
+        float tr = t.q.y, bl = t.q.z, br = t.q.w;
+        if constexpr (!kFinite) {
+            tr = ax != 0.0f ? t.q.y : t.q.x;
+            bl = ay != 0.0f ? t.q.z : t.q.x;
+            br = ay != 0.0f ? (ax != 0.0f ? t.q.w : t.q.z) : tr;
+        }
+        const float top = fmaf(ax, tr, (1.0f - ax) * t.q.x);
+        const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
+        const float v = fmaf(ay, bottom, (1.0f - ay) * top);
+        if constexpr (kFinite) return alt_cell_trunc(fb, v * kb, rowb, hbase);
+        else return alt_cell(fb, alt_bin<T>(t.in ? v : 0.0f, bins, kb), rowb, hbase);
     }
     // at(X, Y) for a point the caller knows to be inside the image.
     __device__ __forceinline__ float at_inside(float X, float Y) const {
@@ -688,42 +745,209 @@ __device__ __forceinline__ LamSeg uniform_seg(LamSeg g) {
     return g;
 }
 
-// calcMIAlt: rows from the host's table, lambda += 1/(W_f - 1) taken 64
-// consecutive steps per pass from the sequence's binade segments
-// (lam_segment), so that the short segments near lambda = 0 share a pass; the
-// fixed area after n steps comes from the host's table.
-template <typename MT, typename FT, bool ONE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void alt_norm_kernel(const AltParams P) {
-    uint32_t *hist = alt_lds_hist();
-    AltTail *tail = alt_tail(hist, P.bins, P.fh);
-    const int B2 = P.bins * P.bins;
+// A pass of 64 consecutive steps k0 .. k0 + 63 of the calcMIAlt sequence
+// lambda += s (s = 1/(W_f - 1)) filled from its binade segments: each lane
+// takes the last segment that starts at or before its step and still holds
+// it; the next segment starts at the literal add after the last covered step
+// (carry: the step before the pass, when none is covered), unless that add
+// reaches lend.  Lanes past the row's last step hold 2.  g is the segment in
+// force at the pass's start and, on return, at its end.
+__device__ __forceinline__ float fill_pass(LamSeg &g, int32_t k0, float carry, float s, float lend) {
+    const int32_t kl = k0 + (int) (threadIdx.x & 63);
+    float lam = 2.0f;
+    unsigned long long covered = 0ull;
+    for (int fill = 0; fill <= 64; ++fill) {
+        // t * d + a < 2^32 on every lane that can hold the segment
+        // (t <= its length + 63, its length * d <= 2^24 + d)
+        const uint32_t w = __umul24((uint32_t) (kl - g.k0), g.d) + g.a;
+        const unsigned long long m =
+            __builtin_amdgcn_ballot_w64(kl >= g.k0) & __builtin_amdgcn_ballot_w64(w <= g.lim);
+        lam = __uint_as_float(mask_select(m, __float_as_uint(lam), __float_as_uint((float) w * g.U)));
+        covered |= m;
+        if (covered >> 63) break;
+        const int last = covered ? 63 - __builtin_clzll(covered) : -1;
+        const float x = (last < 0 ? carry : lane_value(lam, last)) + s;
+        if (!(x < lend)) break;   // the row ends before it
+        g = uniform_seg(lam_segment(x, s, k0 + last + 1));
+    }
+    return lam;
+}
+
+// calcMIAlt on one-channel images: the rows of wavefront `wave`'s blocks
+// (blocks wave, wave + 4, ...) into the LDS joint histogram at byte address
+// hbase; returns the steps taken.  Every walked step has lambda < lend =
+// min(le, 1), the reference's lambda < le && lambda < 1 (NaN kept), and the
+// steps of a row grow with the lane, so the walked lanes of a pass form a
+// prefix and a pass that is not full ends the row.
+//  * A row that starts at lambda = 0 (or -0, whose later steps are the same
+//    and whose samples are those of +0) reads its steps from the lam0 table,
+//    each pass loading the next pass's while it works, and its fixed-image
+//    bins from the fb0 table.
+//  * Any other row takes a pass in closed form, (lane d + b) U, when its
+//    current segment holds all 64 of the pass's steps (a row from lambda >=
+//    1/2 stays in one segment to its end), and otherwise fills the pass from
+//    segments (fill_pass).  Its fixed sample lies on one fixed-image row:
+//    ri = (row + 0.5) (H_f - 1) is the same at every step, so its floor row
+//    and weight are taken once per row.
+template <typename MT, typename FT, bool POW2>
+__device__ __forceinline__ int32_t norm_rows_one(const AltParams &P, const float *h, uint32_t hbase) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int32_t gi = (int32_t) blockIdx.x;
-    for (int i = threadIdx.x; i < alt_hist_words(P.bins); i += kThreads) hist[i] = 0u;
-    __syncthreads();
-    const auto mov = make_sampler<MT, ONE>(P.mq, P.mov, P.C, P.mw, P.mh);
-    const auto fix = make_sampler<FT, ONE>(P.fq, P.fix, P.C, P.fw, P.fh);
-    const int C = ONE ? 1 : P.C;
+    const TabSampler<MT> mov(P.mq, P.mw, P.mh);
+    constexpr int FEB = TapTable<FT>::kBytes;
+    const int fw = uniform(P.fw), fh = uniform(P.fh);
+    const __amdgpu_buffer_rsrc_t frsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) uniform(P.fq), (short) 0, (int) (((size_t) fw * fh + 2 * (size_t) fw + 2) * FEB), 0x00020000);
+    const float sc = 1 / ((float) (fw - 1));
+    const float wm1 = (float) (fw - 1);
+    const float fmw = (float) P.mw, fmh = (float) P.mh;
+    const double dmw = (double) P.mw, dmh = (double) P.mh;
+    // lambda < 1 in steps of 1/(W_f - 1): at most W_f + 1 steps per row
+    const int max_passes = fw / 64 + 3;
+    const int n0 = uniform(P.n0);
+    const __amdgpu_buffer_rsrc_t lam0_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *) uniform(P.lam0), (short) 0, uniform(P.n0pad) * (int) sizeof(float), 0x00020000);
+    const __amdgpu_buffer_rsrc_t fb0_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *) uniform(P.fb0), (short) 0, uniform(P.nrows) * n0, 0x00020000);
     const int bins = P.bins;
-    float p[kMaxDim], c, s, h[8];
-    alt_grid_point(P, gi, p, &c, &s);
-    homography_norm(p, c, s, P.mw, P.mh, P.fw, P.fh, h);
+    const float kb = (float) bins / 256.0f;
+    const uint32_t rowb = 4u * (uint32_t) (bins + 1);
+    // the moving-image taps at step lambda of the line (px, py) + lambda (vx,
+    // vy), scaled back to pixels as the reference does, through binary64
+    // (rescale_pow2: one binary32 fma for power-of-two sizes)
+    auto moving_taps = [&](float lam, float vx, float vy, float px, float py) {
+        const float nx = lam * vx + px, ny = lam * vy + py;
+        float X, Y;
+        if constexpr (POW2) {
+            X = rescale_pow2(nx, fmw);
+            Y = rescale_pow2(ny, fmh);
+        } else {
+            X = (float) (((double) nx + 0.5) * dmw);
+            Y = (float) (((double) ny + 0.5) * dmh);
+        }
+        return mov.load_at(X, Y);
+    };
+    int32_t steps = 0;
+    for (int blk = wave; blk * 64 < P.nrows; blk += 4) {
+        const int k = blk * 64 + lane;
+        AltLine Lr{};
+        float rir = 0.0f;
+        if (k < P.nrows) {
+            const float row = P.rows[k];
+            Lr = line_norm(h, row);
+            rir = (float) (((double) row + 0.5) * (double) (fh - 1));
+        }
+        unsigned long long rows_in = __ballot(Lr.in);
+        while (rows_in) {
+            const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
+            rows_in &= rows_in - 1;
+            const float le = lane_value(Lr.le, j);
+            const float lend = le >= 1.0f ? 1.0f : le;
+            const float vx = lane_value(Lr.vx, j), vy = lane_value(Lr.vy, j);
+            const float px = lane_value(Lr.px, j), py = lane_value(Lr.py, j);
+            const float ls = lane_value(Lr.ls, j);
+            if (ls == 0.0f) {
+                uint32_t koff = 4u * (uint32_t) lane;
+                uint32_t fboff = (uint32_t) (blk * 64 + j) * (uint32_t) n0 + (uint32_t) lane;
+                float lam = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(lam0_rsrc, koff, 0, 0));
+                for (int pass = 0; pass < max_passes; ++pass) {
+                    // the next pass's steps (a pass follows only a full one, so
+                    // it lies inside the table: n0 steps, then 2.0f padding)
+                    const float nxt = __builtin_bit_cast(
+                        float, __builtin_amdgcn_raw_buffer_load_b32(lam0_rsrc, koff + 256u, 0, 0));
+                    const unsigned long long m = __builtin_amdgcn_ballot_w64(lam < lend);
+                    if (m == 0ull) break;
+                    if (lam < lend) {   // the same compare
+                        const int fb = (int) __builtin_amdgcn_raw_buffer_load_b8(fb0_rsrc, fboff, 0, 0);
+                        const auto tm = moving_taps(lam, vx, vy, px, py);
+                        alt_count(mov.cell_of(tm, fb, bins, kb, rowb, hbase));
+                    }
+                    steps += __popcll(m);
+                    if (m != ~0ull) break;   // the row ended in this pass
+                    lam = nxt;
+                    koff += 256u;
+                    fboff += 64u;
+                }
+            } else {
+                // (li, ri) lies inside the fixed image: 0 <= lambda < 1 gives
+                // 0 <= li <= (1 - 2^-24) (W_f - 1) < W_f, and the table's rows,
+                // -0.5 <= row < 0.5, give 0 <= ri < H_f ((row + 0.5) (H_f - 1)
+                // < H_f - 1 rounds to at most H_f - 1); for such points
+                // v_cvt_flr and v_fract are floorf and X - floorf(X) exactly
+                const float ri = lane_value(rir, j);
+                const float ay = __builtin_amdgcn_fractf(ri), cay = 1.0f - ay;
+                const uint32_t frow =
+                    (uint32_t) __builtin_amdgcn_readfirstlane((floor_to_int(ri) * fw + fw + 1) * FEB);
+                // g: the segment in force at the pass's start (it starts at or
+                // before step k0: fill_pass only starts segments inside the
+                // pass it fills); b = (k0 - g.k0) d + a, its value at step k0.
+                // While g holds a pass, b + 63 d <= lim < 2^24, so the next b is
+                // below 2^25; after a filled pass the next b is at most 64 d +
+                // a (g then starts inside the pass); d < 2^24 (an add that stays
+                // in a binade moves less than the binade's 2^24 units): every
+                // b + 63 d stays below 2^31
+                LamSeg g = uniform_seg(lam_segment(ls, sc, 0));
+                uint32_t laned = __umul24((uint32_t) lane, g.d);
+                uint32_t b = g.a;
+                float carry = 0.0f;
+                int32_t k0 = 0;
+                for (int pass = 0; pass < max_passes; ++pass) {
+                    float lam;
+                    const bool whole = b + 63u * g.d <= g.lim;   // g holds all 64 steps
+                    if (whole) {
+                        lam = (float) (laned + b) * g.U;
+                    } else {
+                        lam = fill_pass(g, k0, carry, sc, lend);
+                        laned = __umul24((uint32_t) lane, g.d);
+                    }
+                    const unsigned long long m = __builtin_amdgcn_ballot_w64(lam < lend);
+                    if (m == 0ull) break;
+                    if (lam < lend) {
+                        // both loads issued before either is waited on
+                        const auto tm = moving_taps(lam, vx, vy, px, py);
+                        const float li = lam * wm1;
+                        const int ix = floor_to_int(li);
+                        const float ax = __builtin_amdgcn_fractf(li);
+                        const float4 q = TapTable<FT>::load(frsrc, (uint32_t) (ix * FEB) + frow);
+                        float tr = q.y, bl = q.z, br = q.w;
+                        if constexpr (!TabSampler<FT>::kFinite) {
+                            tr = ax != 0.0f ? q.y : q.x;
+                            bl = ay != 0.0f ? q.z : q.x;
+                            br = ay != 0.0f ? (ax != 0.0f ? q.w : q.z) : tr;
+                        }
+                        const float top = fmaf(ax, tr, (1.0f - ax) * q.x);
+                        const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
+                        const int fb = alt_bin<FT>(fmaf(ay, bottom, cay * top), bins, kb);
+                        alt_count(mov.cell_of(tm, fb, bins, kb, rowb, hbase));
+                    }
+                    steps += __popcll(m);
+                    if (m != ~0ull) break;
+                    carry = lane_value(lam, 63);
+                    k0 += 64;
+                    b = whole ? b + 64u * g.d : (uint32_t) (k0 - g.k0) * g.d + g.a;
+                }
+            }
+        }
+    }
+    return steps;
+}
+
+// calcMIAlt on images of several channels (zero-padded copies): the rows of
+// wavefront `wave`'s blocks as norm_rows_one walks them from segments, each
+// step sampling every channel.
+template <typename MT, typename FT>
+__device__ __forceinline__ int32_t norm_rows_raw(const AltParams &P, const float *h, uint32_t hbase) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const RawSampler<MT> mov(P.mov, P.C, P.mw, P.mh);
+    const RawSampler<FT> fix(P.fix, P.C, P.fw, P.fh);
+    const int C = P.C, bins = P.bins;
     const float sc = 1 / ((float) (P.fw - 1));
     const float wm1 = (float) (P.fw - 1);
     const double dmw = (double) P.mw, dmh = (double) P.mh;
     const float fmw = (float) P.mw, fmh = (float) P.mh;
-    // power-of-two moving sizes: the binary64 rescale has an exact binary32 form
     const bool pow2 = (P.mw & (P.mw - 1)) == 0 && (P.mh & (P.mh - 1)) == 0;
-    // lambda < 1 in steps of 1/(W_f - 1): at most W_f + 1 steps per row
     const int max_passes = P.fw / 64 + 3;
-    const int n0 = P.n0;
-    const __amdgpu_buffer_rsrc_t lam0_rsrc =
-        __builtin_amdgcn_make_buffer_rsrc((void *) P.lam0, (short) 0, n0 * (int) sizeof(float), 0x00020000);
-    const __amdgpu_buffer_rsrc_t fb0_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void *) P.fb0, (short) 0, ONE ? P.nrows * n0 : 0, 0x00020000);
     const float kb = (float) bins / 256.0f;
     const uint32_t rowb = 4u * (uint32_t) (bins + 1);
-    const uint32_t hbase = __builtin_amdgcn_readfirstlane(alt_lds_base(hist));
     int32_t steps = 0;
     for (int blk = wave; blk * 64 < P.nrows; blk += 4) {
         const int k = blk * 64 + lane;
@@ -739,53 +963,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
             const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
             rows_in &= rows_in - 1;
             const float le = lane_value(Lr.le, j);
+            const float lend = le >= 1.0f ? 1.0f : le;
             const float vx = lane_value(Lr.vx, j), vy = lane_value(Lr.vy, j);
             const float px = lane_value(Lr.px, j), py = lane_value(Lr.py, j);
             const float ri = lane_value(rir, j);
-            const float ls = lane_value(Lr.ls, j);
-            // a one-channel row from lambda = 0 (or -0, whose later steps are the
-            // same and whose samples are those of +0) takes its steps and its
-            // fixed bins from the search's tables
-            const bool from0 = ONE && ls == 0.0f;
-            const uint32_t fb0_row = (uint32_t) (blk * 64 + j) * (uint32_t) P.n0;
-            // passes of 64 consecutive steps (lane i: step k0 + i), filled from
-            // as many segments as reach into the pass: each lane takes the last
-            // segment that starts at or before its step and still holds it
-            LamSeg g = uniform_seg(lam_segment(ls, sc, 0));
-            float carry = 0.0f;   // the step before the pass
+            LamSeg g = uniform_seg(lam_segment(lane_value(Lr.ls, j), sc, 0));
+            float carry = 0.0f;
             int32_t k0 = 0;
             for (int pass = 0; pass < max_passes; ++pass) {
-                const int32_t kl = k0 + lane;
-                float lam = 2.0f;   // lanes past the row's last step
-                unsigned long long covered = 0ull;
-                if (from0) {
-                    // offsets past the table read 0, which the select replaces
-                    const float t = __builtin_bit_cast(
-                        float, __builtin_amdgcn_raw_buffer_load_b32(lam0_rsrc, (uint32_t) kl * 4u, 0, 0));
-                    lam = kl < n0 ? t : 2.0f;
-                }
-                for (int fill = 0; fill <= 64 && !from0; ++fill) {
-                    // t * d + a < 2^32 on every lane that can hold the segment
-                    // (t <= its length + 63, its length * d <= 2^24 + d)
-                    const uint32_t w = __umul24((uint32_t) (kl - g.k0), g.d) + g.a;
-                    const unsigned long long m =
-                        __builtin_amdgcn_ballot_w64(kl >= g.k0) & __builtin_amdgcn_ballot_w64(w <= g.lim);
-                    lam = __uint_as_float(mask_select(m, __float_as_uint(lam), __float_as_uint((float) w * g.U)));
-                    covered |= m;
-                    if (covered >> 63) break;
-                    // the next segment starts at the literal add after the last
-                    // covered step (the step before the pass when none is)
-                    const int last = covered ? 63 - __builtin_clzll(covered) : -1;
-                    const float x = (last < 0 ? carry : lane_value(lam, last)) + sc;
-                    if (!(x < le) || !(x < 1.0f)) break;   // the row ends before it
-                    g = uniform_seg(lam_segment(x, sc, k0 + last + 1));
-                }
-                // one ballot per compare (see walk_unit_row); the steps grow
-                // with the lane, so the valid lanes form a prefix
-                const unsigned long long m =
-                    __builtin_amdgcn_ballot_w64(lam < le) & __builtin_amdgcn_ballot_w64(lam < 1.0f);
+                const float lam = fill_pass(g, k0, carry, sc, lend);
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(lam < lend);
                 if (m == 0ull) break;
-                if (lam < le && lam < 1.0f) {   // the same compares
+                if (lam < lend) {
                     float nx = lam * vx + px, ny = lam * vy + py;
                     if (pow2) {
                         nx = rescale_pow2(nx, fmw);
@@ -794,39 +983,51 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
                         nx = (float) (((double) nx + 0.5) * dmw);
                         ny = (float) (((double) ny + 0.5) * dmh);
                     }
-                    // (li, ri) lies inside the fixed image: 0 <= lambda < 1 gives
-                    // 0 <= li <= (1 - 2^-24) (W_f - 1) < W_f, and the table's rows,
-                    // -0.5 <= row < 0.5, give 0 <= ri < H_f ((row + 0.5) (H_f - 1)
-                    // < H_f - 1 rounds to at most H_f - 1)
-                    const float li = lam * wm1;
+                    const float li = lam * wm1;   // (li, ri) inside the fixed image (see norm_rows_one)
                     for (int ch = 0; ch < C; ++ch) {
-                        int fb;
-                        if constexpr (ONE) {
-                            if (from0)
-                                fb = (int) __builtin_amdgcn_raw_buffer_load_b8(fb0_rsrc, fb0_row + (uint32_t) kl, 0, 0);
-                            else
-                                fb = alt_bin<FT>(fix.at_inside(li, ri), bins, kb);
-                        } else {
-                            fb = alt_bin<FT>(fix.at(li, ri, ch), bins, kb);
-                        }
-                        const float mv = mov.at(nx, ny, ch);
-                        alt_count(alt_cell(fb, alt_bin<MT>(mv, bins, kb), rowb, hbase));
+                        const int fb = alt_bin<FT>(fix.at(li, ri, ch), bins, kb);
+                        alt_count(alt_cell(fb, alt_bin<MT>(mov.at(nx, ny, ch), bins, kb), rowb, hbase));
                     }
                 }
                 steps += __popcll(m);
-                if (m != ~0ull) break;   // the row ended in this pass
+                if (m != ~0ull) break;
                 carry = lane_value(lam, 63);
                 k0 += 64;
             }
         }
     }
+    return steps;
+}
+
+// calcMIAlt: rows from the host's table, lambda += 1/(W_f - 1) taken 64
+// consecutive steps per pass from the sequence's binade segments
+// (lam_segment), so that the short segments near lambda = 0 share a pass; the
+// fixed area after n steps comes from the host's table.  One grid point per
+// workgroup, its four wavefronts taking blocks of 64 rows in turn.  POW2: the
+// moving image's sides are powers of two (one-channel images only).
+template <typename MT, typename FT, bool ONE, bool POW2>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void alt_norm_kernel(const AltParams P) {
+    uint32_t *hist = alt_lds_hist();
+    AltTail *tail = alt_tail(hist, P.bins, P.fh);
+    const int B2 = P.bins * P.bins;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t gi = (int32_t) blockIdx.x;
+    for (int i = threadIdx.x; i < alt_hist_words(P.bins); i += kThreads) hist[i] = 0u;
+    __syncthreads();
+    float p[kMaxDim], c, s, h[8];
+    alt_grid_point(P, gi, p, &c, &s);
+    homography_norm(p, c, s, P.mw, P.mh, P.fw, P.fh, h);
+    const uint32_t hbase = __builtin_amdgcn_readfirstlane(alt_lds_base(hist));
+    int32_t steps;
+    if constexpr (ONE) steps = norm_rows_one<MT, FT, POW2>(P, h, hbase);
+    else steps = norm_rows_raw<MT, FT>(P, h, hbase);
     if (lane == 0) tail->steps[wave] = steps;
     __syncthreads();
     if (wave == 0) {
         const int32_t total = tail->steps[0] + tail->steps[1] + tail->steps[2] + tail->steps[3];
         const float fa = P.area[total < P.narea ? total : P.narea - 1];
         double mi = 0.0, hx, hy;
-        if (total > 0) wave_mi(hist, P.bins, P.bins + 1, (double) total * (double) C, &mi, &hx, &hy);
+        if (total > 0) wave_mi(hist, P.bins, P.bins + 1, (double) total * (double) P.C, &mi, &hx, &hy);
         if (P.hist_dump && gi == 0)
             for (int i = lane; i < B2; i += 64) P.hist_dump[i] = hist[(i / P.bins) * (P.bins + 1) + i % P.bins];
         if (lane == 0) tail->cost = (float) (-mi * (double) fa);
@@ -890,9 +1091,14 @@ static hipError_t tap_table(const cgs_image *im, void *dst, hipStream_t s) {
 template <typename MT, typename FT, bool ONE>
 static hipError_t alt_launch(const AltParams &P, hipStream_t s) {
     const size_t lds = alt_lds_bytes(P.bins, P.fh);
-    if (P.cost == CGS_COST_MI_ALT)
-        hipLaunchKernelGGL((alt_norm_kernel<MT, FT, ONE>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
-    else if (P.cost == CGS_COST_MI_STREAM_ALT)
+    const bool pow2 = (P.mw & (P.mw - 1)) == 0 && (P.mh & (P.mh - 1)) == 0;
+    if (P.cost == CGS_COST_MI_ALT) {
+        // POW2 = ONE && pow2 (only the one-channel walk reads it)
+        if (ONE && pow2)
+            hipLaunchKernelGGL((alt_norm_kernel<MT, FT, ONE, ONE>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
+        else
+            hipLaunchKernelGGL((alt_norm_kernel<MT, FT, ONE, false>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
+    } else if (P.cost == CGS_COST_MI_STREAM_ALT)
         hipLaunchKernelGGL((alt_step_mi_kernel<MT, FT, ONE>), dim3(P.grid.n), dim3(kThreads), lds, s, P);
     else if (P.cost == CGS_COST_SQD_ALT)
         hipLaunchKernelGGL((alt_step_kernel<MT, FT, true, ONE>), dim3((P.grid.n + 3) / 4), dim3(kThreads), 0, s, P);
@@ -919,7 +1125,7 @@ static hipError_t alt_dispatch(const AltParams &P, int mpix, int fpix, hipStream
 // (fix_area += 1.0 / (W_f H_f) through binary64, rounded to binary32, :198)
 // for n up to rows * (W_f + 2).
 static void mi_alt_tables(int fw, int fh, std::vector<float> &rows, std::vector<float> &area,
-                          std::vector<float> &lam0) {
+                          std::vector<float> &lam0, int *n0) {
     rows.clear();
     const float step = 1 / ((float) (fh - 1));
     for (float row = -0.5f; row < 0.5; row += step) rows.push_back(row);
@@ -927,6 +1133,10 @@ static void mi_alt_tables(int fw, int fh, std::vector<float> &rows, std::vector<
     lam0.clear();
     const float sc = 1 / ((float) (fw - 1));
     for (float lam = 0.0f; lam < 1.0f && lam0.size() <= (size_t) fw + 1; lam += sc) lam0.push_back(lam);
+    // then 2.0f (no row reaches it) through the pass after the one that holds
+    // entry n0, which the kernel loads while it walks that pass
+    *n0 = (int) lam0.size();
+    lam0.resize(((size_t) *n0 / 64 + 2) * 64, 2.0f);
     const size_t n = rows.size() * ((size_t) fw + 2) + 1;
     area.resize(n);
     const double d = 1.0 / (double) (fw * fh);
@@ -979,7 +1189,8 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
             return set_error(CGS_ERR_INVALID_ARGUMENT, "calcMIAlt needs a fixed image of at least 2 x 2");
         std::vector<float> rows, area;
         std::vector<float> lam0;
-        mi_alt_tables(fixed->width, fixed->height, rows, area, lam0);
+        int n0 = 0;
+        mi_alt_tables(fixed->width, fixed->height, rows, area, lam0, &n0);
         if (area.size() > (size_t) INT32_MAX) return set_error(CGS_ERR_TOO_LARGE, "calcMIAlt: fixed image too large");
         const size_t bytes = sizeof(float) * (rows.size() + area.size() + lam0.size());
         if (ws->alt_tab_fw != fixed->width || ws->alt_tab_fh != fixed->height) {
@@ -999,9 +1210,10 @@ cgs_status run_alt_search(Workspace *ws, const cgs_grid *grid, const cgs_image *
         P.area = P.rows + rows.size();
         P.narea = (int) area.size();
         P.lam0 = P.area + area.size();
-        P.n0 = (int) lam0.size();
+        P.n0 = n0;
+        P.n0pad = (int) lam0.size();
         if (one) {
-            if ((st = ws->alt_fb0.ensure(rows.size() * lam0.size())) != CGS_OK) return st;
+            if ((st = ws->alt_fb0.ensure(rows.size() * (size_t) n0)) != CGS_OK) return st;
             P.fb0 = (const uint8_t *) ws->alt_fb0.ptr;
         }
     }

@@ -16,6 +16,7 @@
 // rounded add.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -108,13 +109,14 @@ CGS_HD inline float unit_pass_chain(float x0, int j) {
 }
 
 // calcMIAlt scales a normalised coordinate back to pixels through binary64,
-// (float) (((double) x + 0.5) * W).  For W = 2^k that is (x + 0.5f) * W in
-// binary32: the scaling by 2^k is exact and commutes with both roundings
-// (binary64 cannot overflow; binary32 overflows to the same infinity), and
-// RN32(RN64(x + 0.5)) = RN32(x + 0.5), because x + 0.5 is exact in binary64
-// unless |x| < 2^-31 (both give 0.5, far from a binary32 tie) or
-// |x| >= 2^52 (both give x).
-CGS_HD inline float rescale_pow2(float x, float W) { return (x + 0.5f) * W; }
+// (float) (((double) x + 0.5) * W).  For W = 2^k that is one binary32 fma,
+// x * W + W / 2 rounded once: the scaling by 2^k is exact and commutes with
+// the roundings (binary64 cannot overflow; binary32 overflows to the same
+// infinity), so the binary64 form is RN32(RN64(x + 0.5)) * W and the fma
+// RN32(x + 0.5) * W, and RN32(RN64(x + 0.5)) = RN32(x + 0.5) because x + 0.5
+// is exact in binary64 unless |x| < 2^-31 (both give 0.5, far from a
+// binary32 tie) or |x| >= 2^52 (both give x).
+CGS_HD inline float rescale_pow2(float x, float W) { return fmaf(x, W, 0.5f * W); }
 
 // A segment of the lambda = lambda + s sequence (s > 0 normal) from x: the
 // steps k >= k0 with a + (k - k0) d <= lim (unsigned) are (float) (a + (k -

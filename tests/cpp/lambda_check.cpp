@@ -1,7 +1,8 @@
 // Checks the closed forms of cgs_lambda.hpp against the sequential binary32
 // loops they replace.  Prints "<mismatches> <comparisons> <passes of the
 // 512-wide rows from 0> <starts unit_pass_direct accepts> <starts through
-// unit_pass_chain> <rescales compared> <areas compared>".
+// unit_pass_chain> <rescales compared> <areas compared> <calcMIAlt passes
+// taken in closed form>".
 #include <cstdio>
 #include <cmath>
 #include <cstring>
@@ -13,44 +14,58 @@
 static bool same(float a, float b) { return std::memcmp(&a, &b, 4) == 0; }
 
 // alt_norm_kernel's walk of one calcMIAlt row, lane by lane: passes of 64
-// consecutive steps; in each, segments (lam_segment) are laid over the lanes
-// in order, lane i of the pass taking the last segment that starts at or
-// before step k0 + i and holds it (a + t d <= lim, with the 24-bit multiply
-// of the kernel); the next segment starts at the literal add after the last
-// covered step.  Unfilled lanes hold 2; the valid lanes (lambda < le and
-// lambda < 1) must form a prefix.  Returns the row's steps.
-static std::vector<float> packed_walk(float ls, float le, float s, int w, int *passes, long *segs, long *bad) {
+// consecutive steps.  A pass whose current segment holds all 64 steps is
+// taken in closed form, (lane d + b) U with b the segment's value at the
+// pass's first step, tracked in 32 bits as the kernel does; any other pass is
+// filled from
+// segments (lam_segment) laid over the lanes in order, lane i of the pass
+// taking the last segment that starts at or before step k0 + i and holds it
+// (a + t d <= lim, with the 24-bit multiply of the kernel); the next segment
+// starts at the literal add after the last covered step.  Unfilled lanes
+// hold 2; the valid lanes (lambda < lend = min(le, 1), NaN kept) must form a
+// prefix.  Returns the row's steps; *fast counts the closed-form passes.
+static std::vector<float> packed_walk(float ls, float le, float s, int w, int *passes, long *segs, long *fast,
+                                      long *bad) {
     std::vector<float> out;
+    const float lend = le >= 1.0f ? 1.0f : le;
     cgs::LamSeg g = cgs::lam_segment(ls, s, 0);
     float carry = 0.0f;
     int k0 = 0;
+    uint32_t b = g.a;   // (k0 - g.k0) d + a, kept in 32 bits as the kernel keeps it
     *passes = 0;
     for (int pass = 0; pass < w / 64 + 3; ++pass) {
         float lam[64];
         for (float &v : lam) v = 2.0f;
-        uint64_t covered = 0;
-        for (int fill = 0; fill <= 64; ++fill) {
-            uint64_t m = 0;
-            for (int i = 0; i < 64; ++i) {
-                const int kl = k0 + i;
-                const uint32_t t = (uint32_t) (kl - g.k0);
-                const uint32_t wv = (uint32_t) ((uint64_t) (t & 0xFFFFFFu) * (uint64_t) (g.d & 0xFFFFFFu)) + g.a;
-                if (kl >= g.k0 && wv <= g.lim) {
-                    m |= 1ull << i;
-                    lam[i] = (float) wv * g.U;
+        if (g.k0 > k0 || (uint64_t) b + 63u * (uint64_t) g.d >= (1ull << 31)) ++*bad;   // the kernel's invariants
+        const bool whole = b + 63u * g.d <= g.lim;
+        if (whole) {
+            for (int i = 0; i < 64; ++i) lam[i] = (float) ((uint32_t) i * g.d + b) * g.U;
+            ++*fast;
+        } else {
+            uint64_t covered = 0;
+            for (int fill = 0; fill <= 64; ++fill) {
+                uint64_t m = 0;
+                for (int i = 0; i < 64; ++i) {
+                    const int kl = k0 + i;
+                    const uint32_t t = (uint32_t) (kl - g.k0);
+                    const uint32_t wv = (uint32_t) ((uint64_t) (t & 0xFFFFFFu) * (uint64_t) (g.d & 0xFFFFFFu)) + g.a;
+                    if (kl >= g.k0 && wv <= g.lim) {
+                        m |= 1ull << i;
+                        lam[i] = (float) wv * g.U;
+                    }
                 }
+                covered |= m;
+                if (covered >> 63) break;
+                const int last = covered ? 63 - __builtin_clzll(covered) : -1;
+                const float x = (last < 0 ? carry : lam[last]) + s;
+                if (!(x < lend)) break;
+                g = cgs::lam_segment(x, s, k0 + last + 1);
+                ++*segs;
             }
-            covered |= m;
-            if (covered >> 63) break;
-            const int last = covered ? 63 - __builtin_clzll(covered) : -1;
-            const float x = (last < 0 ? carry : lam[last]) + s;
-            if (!(x < le) || !(x < 1.0f)) break;
-            g = cgs::lam_segment(x, s, k0 + last + 1);
-            ++*segs;
         }
         uint64_t m = 0;
         for (int i = 0; i < 64; ++i)
-            if (lam[i] < le && lam[i] < 1.0f) m |= 1ull << i;
+            if (lam[i] < lend) m |= 1ull << i;
         if (m == 0) break;
         ++*passes;
         if (m & (m + 1)) ++*bad;   // not a prefix
@@ -58,6 +73,7 @@ static std::vector<float> packed_walk(float ls, float le, float s, int w, int *p
         if (m != ~0ull) break;
         carry = lam[63];
         k0 += 64;
+        b = whole ? b + 64u * g.d : (uint32_t) (k0 - g.k0) * g.d + g.a;
     }
     return out;
 }
@@ -148,7 +164,7 @@ int main() {
     for (int w = 2; w <= 300; ++w) widths.push_back(w);
     for (int i = 0; i < 300; ++i) widths.push_back(300 + (int) (rng() % 8000));
     for (int w : {256, 257, 384, 385, 511, 512, 513, 640, 1000, 1024, 1025, 4096, 4097, 6001}) widths.push_back(w);
-    long segs = 0;
+    long segs = 0, fastpass = 0;
     for (int w : widths) {
         const float s = 1 / ((float) (w - 1));
         std::vector<float> xs = {0.0f, -0.0f, s, 0.5f * s, 0.25f, 0.5f, 0.499999970f, 1e-40f, 0.999999940f};
@@ -162,7 +178,7 @@ int main() {
                 for (float seq = x0; seq < le && seq < 1.0f && (int) want.size() < w + 4; seq = seq + s)
                     want.push_back(seq);
                 int passes = 0;
-                const std::vector<float> got = packed_walk(x0, le, s, w, &passes, &segs, &bad);
+                const std::vector<float> got = packed_walk(x0, le, s, w, &passes, &segs, &fastpass, &bad);
                 if (got.size() != want.size()) ++bad;
                 for (size_t k = 0; k < want.size() && k < got.size(); ++k, ++n)
                     if (!same(got[k], want[k])) ++bad;
@@ -214,6 +230,6 @@ int main() {
         ++areas;
         if (!same(lit, fast)) ++bad;
     }
-    std::printf("%ld %ld %ld %ld %ld %ld %ld\n", bad, n, runs512, direct, chains, rescales, areas);
+    std::printf("%ld %ld %ld %ld %ld %ld %ld %ld\n", bad, n, runs512, direct, chains, rescales, areas, fastpass);
     return 0;
 }

@@ -15,7 +15,7 @@ def test_closed_form_lambda_matches_sequential_loops(tmp_path):
                     "-I", os.path.join(ROOT, "cugridsearch_amd", "csrc"), "-o", exe,
                     os.path.join(HERE, "cpp", "lambda_check.cpp")], check=True)
     out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
-    bad, n, runs512, direct, chains, rescales, areas = (int(v) for v in out[:7])
+    bad, n, runs512, direct, chains, rescales, areas, fastpass = (int(v) for v in out[:8])
     assert n > 2_000_000 and bad == 0, (bad, n)
     # the direct form saw the integer starts and the starts from 16 up; the
     # chain every start below 2^23
@@ -27,5 +27,7 @@ def test_closed_form_lambda_matches_sequential_loops(tmp_path):
     # +0 and -0) takes its 512 steps in 8 full passes of 64, its short
     # segments near 0 sharing the first
     assert runs512 == 2 * 8, runs512
+    # most passes of the walks are whole-pass closed forms
+    assert fastpass > 100_000, fastpass
     # the moved area's closed-form repeated add equals the literal loop
     assert areas > 100_000, areas
