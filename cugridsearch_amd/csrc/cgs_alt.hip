@@ -339,7 +339,7 @@ struct TabSampler {
     // NaN there, which 0 * t would turn into NaN: it keeps the selects.
     static constexpr bool kFinite = std::is_same<T, uint8_t>::value;
     __amdgpu_buffer_rsrc_t rsrc;
-    int w;
+    int w, h;
     uint32_t base;   // byte offset of entry 0 of the image: (w + 1) entries
     float W, H;
     __device__ TabSampler(const void *table, int width, int height) {
@@ -347,6 +347,7 @@ struct TabSampler {
             (void *) uniform(table), (short) 0,
             (int) (((size_t) width * height + 2 * (size_t) width + 2) * TapTable<T>::kBytes), 0x00020000);
         w = width;
+        h = height;
         base = (uint32_t) ((width + 1) * TapTable<T>::kBytes);
         W = (float) width;
         H = (float) height;
@@ -377,7 +378,11 @@ struct TabSampler {
     // A uint8 image skips at's final select: off the image the taps read 0,
     // so v is 0 for finite X, Y and NaN otherwise, and both truncate to bin 0
     // (v_cvt_i32_f32 takes NaN to 0), the bin of the 0 that at returns;
-    // cell_of gives the bin's histogram cell.
+    // cell_of gives the bin's histogram cell.  Its in-image test is two
+    // unsigned compares of the integer floors: v_cvt_flr clamps infinities and
+    // huge values to INT_MIN / INT_MAX (outside) and takes NaN to 0, whose
+    // lane then reads taps inside the table but has the weight v_fract(NaN) =
+    // NaN, so v is NaN and its bin 0 all the same.
     struct Taps {
         float4 q;
         float ax, ay;
@@ -386,8 +391,9 @@ struct TabSampler {
     __device__ __forceinline__ Taps load_at(float X, float Y) const {
         constexpr int EB = TapTable<T>::kBytes;
         Taps t;
-        t.in = X >= 0.0f && Y >= 0.0f && X < W && Y < H;
         const int ix = floor_to_int(X), iy = floor_to_int(Y);
+        if constexpr (kFinite) t.in = (uint32_t) ix < (uint32_t) w && (uint32_t) iy < (uint32_t) h;
+        else t.in = X >= 0.0f && Y >= 0.0f && X < W && Y < H;
         t.ax = __builtin_amdgcn_fractf(X);
         t.ay = __builtin_amdgcn_fractf(Y);
         uint32_t off;
@@ -849,13 +855,14 @@ __device__ __forceinline__ int32_t norm_rows_one(const AltParams &P, const float
                 uint32_t koff = 4u * (uint32_t) lane;
                 uint32_t fboff = (uint32_t) (blk * 64 + j) * (uint32_t) n0 + (uint32_t) lane;
                 float lam = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(lam0_rsrc, koff, 0, 0));
-                for (int pass = 0; pass < max_passes; ++pass) {
+                // every pass but the last is full; the table's 2.0f padding
+                // ends the row at the latest in the pass that holds entry n0
+                while (true) {
                     // the next pass's steps (a pass follows only a full one, so
-                    // it lies inside the table: n0 steps, then 2.0f padding)
+                    // it lies inside the table)
                     const float nxt = __builtin_bit_cast(
                         float, __builtin_amdgcn_raw_buffer_load_b32(lam0_rsrc, koff + 256u, 0, 0));
                     const unsigned long long m = __builtin_amdgcn_ballot_w64(lam < lend);
-                    if (m == 0ull) break;
                     if (lam < lend) {   // the same compare
                         const int fb = (int) __builtin_amdgcn_raw_buffer_load_b8(fb0_rsrc, fboff, 0, 0);
                         const auto tm = moving_taps(lam, vx, vy, px, py);
@@ -885,22 +892,28 @@ __device__ __forceinline__ int32_t norm_rows_one(const AltParams &P, const float
                 // a (g then starts inside the pass); d < 2^24 (an add that stays
                 // in a binade moves less than the binade's 2^24 units): every
                 // b + 63 d stays below 2^31
+                // g covers step k0 - 1 at every pass after the first (the
+                // segment in force at a pass's end holds its last lane), so the
+                // step before a filled pass is (b - d) U.  Passes run while they
+                // are full: lambda grows at every step (s > 2^-24, half an ulp
+                // or more below 1), so a row ends; `filled` bounds the filled
+                // passes as well
                 LamSeg g = uniform_seg(lam_segment(ls, sc, 0));
                 uint32_t laned = __umul24((uint32_t) lane, g.d);
                 uint32_t b = g.a;
-                float carry = 0.0f;
-                int32_t k0 = 0;
-                for (int pass = 0; pass < max_passes; ++pass) {
+                int32_t k0 = 0, filled = 0;
+                while (true) {
                     float lam;
-                    const bool whole = b + 63u * g.d <= g.lim;   // g holds all 64 steps
-                    if (whole) {
+                    if (b + 63u * g.d <= g.lim) {   // g holds all 64 steps
                         lam = (float) (laned + b) * g.U;
+                        b += 64u * g.d;
                     } else {
-                        lam = fill_pass(g, k0, carry, sc, lend);
+                        if (++filled > max_passes) break;
+                        lam = fill_pass(g, k0, (float) (b - g.d) * g.U, sc, lend);
                         laned = __umul24((uint32_t) lane, g.d);
+                        b = (uint32_t) (k0 + 64 - g.k0) * g.d + g.a;
                     }
                     const unsigned long long m = __builtin_amdgcn_ballot_w64(lam < lend);
-                    if (m == 0ull) break;
                     if (lam < lend) {
                         // both loads issued before either is waited on
                         const auto tm = moving_taps(lam, vx, vy, px, py);
@@ -921,9 +934,7 @@ __device__ __forceinline__ int32_t norm_rows_one(const AltParams &P, const float
                     }
                     steps += __popcll(m);
                     if (m != ~0ull) break;
-                    carry = lane_value(lam, 63);
                     k0 += 64;
-                    b = whole ? b + 64u * g.d : (uint32_t) (k0 - g.k0) * g.d + g.a;
                 }
             }
         }

@@ -17,7 +17,7 @@ static bool same(float a, float b) { return std::memcmp(&a, &b, 4) == 0; }
 // consecutive steps.  A pass whose current segment holds all 64 steps is
 // taken in closed form, (lane d + b) U with b the segment's value at the
 // pass's first step, tracked in 32 bits as the kernel does; any other pass is
-// filled from
+// filled (the step before it taken as (b - d) U, checked against the walk) from
 // segments (lam_segment) laid over the lanes in order, lane i of the pass
 // taking the last segment that starts at or before step k0 + i and holds it
 // (a + t d <= lim, with the 24-bit multiply of the kernel); the next segment
@@ -42,6 +42,8 @@ static std::vector<float> packed_walk(float ls, float le, float s, int w, int *p
             for (int i = 0; i < 64; ++i) lam[i] = (float) ((uint32_t) i * g.d + b) * g.U;
             ++*fast;
         } else {
+            // the kernel takes the step before the pass from g: (b - d) U
+            if (pass > 0 && !same((float) (b - g.d) * g.U, carry)) ++*bad;
             uint64_t covered = 0;
             for (int fill = 0; fill <= 64; ++fill) {
                 uint64_t m = 0;
