@@ -434,12 +434,17 @@ struct TabSampler {
         const float bottom = fmaf(ax, br, (1.0f - ax) * bl);
         return fmaf(ay, bottom, (1.0f - ay) * top);
     }
+    // at_row for 0 <= X (every walked step): v_cvt_flr and v_fract are floorf
+    // and X - floorf(X) there; a finite top (uint8 taps) is its own
+    // fmaf(0, top, top).
     __device__ __forceinline__ float at_row(float X, int row, int) const {
-        const float fx = floorf(X), ax = X - fx;
-        const float4 q = TapTable<T>::load(rsrc, (uint32_t) ((row * w + (int) fx + w + 1) * TapTable<T>::kBytes));
+        const int fx = floor_to_int(X);
+        const float ax = __builtin_amdgcn_fractf(X);
+        const float4 q = TapTable<T>::load(rsrc, (uint32_t) ((row * w + fx + w + 1) * TapTable<T>::kBytes));
         const float tr = kFinite || ax != 0.0f ? q.y : q.x;
         const float top = fmaf(ax, tr, (1.0f - ax) * q.x);
-        return fmaf(0.0f, top, top);
+        if constexpr (kFinite) return top;
+        else return fmaf(0.0f, top, top);
     }
 };
 
@@ -702,12 +707,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) v
             const int32_t nrow = walk_unit_row<true>(
                 lane_value(Lr.ls, j), lane_value(Lr.le, j), (float) P.fw, lane_value(Lr.vx, j), lane_value(Lr.vy, j),
                 lane_value(Lr.px, j), lane_value(Lr.py, j), [&](float lam, float nx, float ny, bool ipass) {
-                    for (int ch = 0; ch < C; ++ch) {
-                        const float mv = mov.at(nx, ny, ch);
+                    if constexpr (ONE) {   // the moving taps' load issued first (load_at / cell_of)
+                        const auto tm = mov.load_at(nx, ny);
                         float fv;
-                        if constexpr (IROW) fv = ipass ? irow.at(lam, row) : fix.at_row(lam, row, ch);
-                        else fv = fix.at_row(lam, row, ch);
-                        alt_count(alt_cell(alt_bin<FT>(fv, bins, kb), alt_bin<MT>(mv, bins, kb), rowb, hbase));
+                        if constexpr (IROW) fv = ipass ? irow.at(lam, row) : fix.at_row(lam, row, 0);
+                        else fv = fix.at_row(lam, row, 0);
+                        alt_count(mov.cell_of(tm, alt_bin<FT>(fv, bins, kb), bins, kb, rowb, hbase));
+                    } else {
+                        for (int ch = 0; ch < C; ++ch) {
+                            const float mv = mov.at(nx, ny, ch);
+                            const float fv = fix.at_row(lam, row, ch);
+                            alt_count(alt_cell(alt_bin<FT>(fv, bins, kb), alt_bin<MT>(mv, bins, kb), rowb, hbase));
+                        }
                     }
                 });
             if (lane == j && nrow > 0) term = (double) nrow * (double) magr;
