@@ -94,6 +94,7 @@ SIGNATURES = {
     "cgs_search_noalloc": (ctypes.c_int, [_P(Grid), _P(Image), _P(Image), _P(SearchOpts), ctypes.c_void_p,
                                           _P(Argmin), ctypes.c_void_p]),
     "cgs_fma_peak": (ctypes.c_int, [_P(ctypes.c_float), ctypes.c_void_p]),
+    "cgs_check_rcp": (ctypes.c_int, [_P(ctypes.c_uint32), ctypes.c_void_p]),
     "cgs_find_extrema": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _P(Argmin),
                                         ctypes.c_void_p]),
     "cgs_covariance": (ctypes.c_int, [_P(Grid), ctypes.c_void_p, _P(ctypes.c_float), _P(Argmin), ctypes.c_void_p]),

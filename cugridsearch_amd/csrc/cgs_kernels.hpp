@@ -679,12 +679,16 @@ struct Recip {
 };
 
 // Reciprocal of d refined by one Newton step (v_rcp_f32, then
-// r += r (1 - d r)), and the quotient n / d from it by two correction steps.
-// This is the arithmetic of the compiler's correctly rounded f32 division
-// (v_div_scale ... v_div_fmas, v_div_fixup) when v_div_scale leaves its
-// operands unscaled, which holds for d in [2^-6, 2^6] and finite n with
-// n == 0 or 2^-90 < |n| < 2^90; the kernel checks the W range per grid
-// point and uses '/' otherwise.
+// r += r (1 - d r)), and the quotient n / d from it by one correction step.
+// The refined r is RN(1/d), the correctly rounded reciprocal, for every
+// binary32 mantissa (cgs_check_rcp checks all 2^23 of [1, 2) on the device;
+// v_rcp_f32 alone misses about one in nine), and the steps scale exactly
+// with d's exponent while 1/d stays normal.  With r = RN(1/d), q = RN(n r) is
+// within an ulp of n/d, rem = n - d q is exact (one fma), and RN(q + rem r)
+// = RN(n/d) (Markstein's theorem), so div_by is IEEE division for d in
+// [2^-6, 2^6] and n == 0 or 2^-90 < |n| < 2^90 (no intermediate underflows
+// or overflows); the kernel checks the W range per grid point and uses '/'
+// otherwise.
 __device__ __forceinline__ Recip refine_rcp(float d) {
     float r = __builtin_amdgcn_rcpf(d);
     const float e = fmaf(-d, r, 1.0f);
@@ -693,10 +697,8 @@ __device__ __forceinline__ Recip refine_rcp(float d) {
 }
 
 __device__ __forceinline__ float div_by(float n, Recip R) {
-    float q = n * R.r;
-    float rem = fmaf(-R.d, q, n);
-    q = fmaf(rem, R.r, q);
-    rem = fmaf(-R.d, q, n);
+    const float q = n * R.r;
+    const float rem = fmaf(-R.d, q, n);
     return fmaf(rem, R.r, q);
 }
 

@@ -1,26 +1,36 @@
 // cgs_probe.hip -- the device's measured non-packed FP32 FMA throughput, the
 // VALU roofline the search kernels are held against beside the published
-// peak (which counts packed FP32, an instruction this build does not emit).
+// peak.  The chains are v_fmac_f32 on VGPR operands only: on gfx950 a VALU
+// instruction that reads an SGPR (or is a conversion, a 24-bit integer
+// multiply, a three-operand integer op or a compare) issues at about one
+// wave64 instruction per CU-cycle, while fp32 adds, multiplies and fmas on
+// VGPRs and inline constants issue at up to about 1.6 (the published 2.0 per
+// CU-cycle, 157.3 TFLOP/s, counts two).
 #include <hip/hip_runtime.h>
 
 #include <string>
 
 #include "cgs.h"
 #include "cgs_host.hpp"
+#include "cgs_kernels.hpp"
 
 namespace cgs {
 
 constexpr int kProbeChains = 16;
 
 // Independent fmaf chains per thread, one store per thread (out holds
-// gridDim.x * blockDim.x floats).
+// gridDim.x * blockDim.x floats).  a and b are copied into VGPRs, so each fma
+// is v_fmac_f32 acc, va, vb with no SGPR operand.
 __global__ __launch_bounds__(256) void fma_chain_kernel(float *out, int iters, float a, float b) {
     float acc[kProbeChains];
 #pragma unroll
     for (int c = 0; c < kProbeChains; ++c) acc[c] = (float) (threadIdx.x + c);
+    float va, vb;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(va) : "s"(a));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(vb) : "s"(b));
     for (int i = 0; i < iters; ++i) {
 #pragma unroll
-        for (int c = 0; c < kProbeChains; ++c) acc[c] = fmaf(acc[c], a, b);
+        for (int c = 0; c < kProbeChains; ++c) acc[c] = fmaf(va, vb, acc[c]);
     }
     float s = 0.0f;
 #pragma unroll
@@ -28,9 +38,39 @@ __global__ __launch_bounds__(256) void fma_chain_kernel(float *out, int iters, f
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// For every binary32 mantissa in [1, 2) (thread m: d = 1 + m 2^-23), counts
+// the d whose reciprocal refined as refine_rcp does differs from the
+// correctly rounded 1.0f / d.
+__global__ __launch_bounds__(256) void rcp_check_kernel(unsigned *mismatches) {
+    const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (1u << 23)) return;
+    const float d = __uint_as_float(0x3F800000u | m);
+    if (__float_as_uint(refine_rcp(d).r) != __float_as_uint(1.0f / d)) atomicAdd(mismatches, 1u);
+}
+
 }  // namespace cgs
 
 using namespace cgs;
+
+extern "C" cgs_status cgs_check_rcp(uint32_t *mismatches_out, void *stream) {
+    clear_error();
+    if (!mismatches_out) return set_error(CGS_ERR_INVALID_ARGUMENT, "check_rcp: null output");
+    unsigned *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(unsigned)));
+    hipStream_t s = (hipStream_t) stream;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned), s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(rcp_check_kernel, dim3((1u << 23) / 256), dim3(256), 0, s, d);
+        e = hipGetLastError();
+    }
+    unsigned h = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void) hipFree(d);
+    if (e != hipSuccess) return set_error(CGS_ERR_HIP, std::string("check_rcp: ") + hipGetErrorString(e));
+    *mismatches_out = h;
+    return CGS_OK;
+}
 
 extern "C" cgs_status cgs_fma_peak(float *tflops_out, void *stream) {
     clear_error();

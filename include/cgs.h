@@ -118,8 +118,9 @@ typedef struct {
     int32_t tie_mode;    /* cgs_tie_mode */
     int32_t mi_bins;     /* joint-histogram bins per axis for the MI costs: 8..64 (reference: 64) */
     int32_t flags;       /* CGS_FLAG_* bits */
-    int32_t tile_gpt;    /* grid points per workgroup: 4, 6 or 8 for SAD/NCC/SQD, 2 or 4 for MI;
-                            0 = library default (6 or 8 for SAD/NCC/SQD, 4 for MI) */
+    int32_t tile_gpt;    /* grid points per workgroup: 4, 6 or 8 for SAD/NCC/SQD, 2, 4, 6 or 8 for MI
+                            (2 above 32 bins); 0 = library default (6 or 8 for SAD/NCC/SQD, 8 for
+                            MI up to 32 bins, 2 above) */
 } cgs_search_opts;
 
 /* opts.flags: time the search kernel with HIP events on the caller's
@@ -482,12 +483,19 @@ cgs_status cgs_cnn_get_grads(const cgs_cnn *net, float *grads);
  * gradient, then the bf16 working copies are refreshed.  Asynchronous. */
 cgs_status cgs_cnn_adam_step(cgs_cnn *net, float lr, float beta1, float beta2, float eps, void *stream);
 
-/* The current device's measured non-packed FP32 FMA throughput in TFLOP/s
- * (2 FLOP per fmaf): a kernel of independent fmaf chains, timed with HIP
- * events, best of three.  The VALU roofline the search kernels are measured
- * against beside the published peak, which counts packed FP32.  No
+/* The current device's measured FP32 FMA throughput in TFLOP/s (2 FLOP per
+ * fmaf): a kernel of independent v_fmac_f32 chains on VGPR operands (no
+ * packed FP32), timed with HIP events, best of three.  The VALU roofline the
+ * search kernels are measured against beside the published peak.  No
  * reference counterpart.  Allocates a small output buffer; synchronous. */
 cgs_status cgs_fma_peak(float *tflops_out, void *stream);
+
+/* Counts, on the current device, the binary32 mantissas d in [1, 2) whose
+ * reciprocal refined by one Newton step (v_rcp_f32, then r += r (1 - d r))
+ * is not the correctly rounded 1/d.  The MI kernels' exact quotients take
+ * one correction step from that reciprocal, which is IEEE division when the
+ * count is 0.  No reference counterpart.  Synchronous. */
+cgs_status cgs_check_rcp(uint32_t *mismatches_out, void *stream);
 
 /* Duration in ms of the calling thread's last timed search kernel (flags &
  * CGS_FLAG_TIME_KERNEL), measured with HIP events around that launch; -1

@@ -197,6 +197,17 @@ def test_work_order_and_tiles_bit_identical(gpu, oracle, counts):
         assert ok, (fn.name, bad)
 
 
+def test_refined_reciprocal_is_correctly_rounded(gpu):
+    """The MI kernels' exact quotients take one correction step from the
+    reciprocal refined by one Newton step, which is IEEE division only if that
+    reciprocal is the correctly rounded 1/d: checked on this device for every
+    binary32 mantissa of [1, 2) (the steps scale exactly with the exponent)."""
+    import ctypes
+    n = ctypes.c_uint32(123)
+    L.check(L.lib().cgs_check_rcp(ctypes.byref(n), None))
+    assert n.value == 0, n.value
+
+
 def test_mi32_full_and_histograms(gpu, oracle):
     m, f, _ = synth.make_pair(31, 80, sigma=3.0, perturb=4.0, dtype=np.uint8, gamma=True, noise=4.0)
     s, e, c = synth.grid_ranges(3, 4.0)
