@@ -161,6 +161,19 @@ def c1_cpu_and_gpu():
     return out
 
 
+VALU_RATE_FORMS = ["fmac_vgpr", "fma_sgpr", "cvt_f32_ubyte", "mad_u32_u24_sgpr", "rcp", "fmac_cvt_alternating"]
+
+
+def valu_rates():
+    """The device's VALU issue rates (wave64 instructions per CU-cycle of the
+    nominal clock) for the instruction forms of cgs_valu_rates."""
+    import ctypes
+    from cugridsearch_amd import _lib as L
+    v = (ctypes.c_float * len(VALU_RATE_FORMS))()
+    L.check(L.lib().cgs_valu_rates(v, len(VALU_RATE_FORMS), None))
+    return {k: round(float(x), 3) for k, x in zip(VALU_RATE_FORMS, v)}
+
+
 def fma_peak():
     """The device's measured FP32 FMA throughput (TFLOP/s): independent
     v_fmac_f32 chains on VGPR operands over every CU (cgs_fma_peak)."""
@@ -559,7 +572,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_VALU_TFLOPS, "traffic": traffic,
                      "measured_fma_peak": measured_peak, "frac_of_measured_fma_peak": achieved / measured_peak,
-                     "flop_per_pixel_eval": flop_pe},
+                     "valu_issue_per_cu_cycle": valu_rates(), "flop_per_pixel_eval": flop_pe},
         "lib": L.lib().cgs_version().decode(),
     }
     if accuracy:

@@ -95,6 +95,7 @@ SIGNATURES = {
                                           _P(Argmin), ctypes.c_void_p]),
     "cgs_fma_peak": (ctypes.c_int, [_P(ctypes.c_float), ctypes.c_void_p]),
     "cgs_check_rcp": (ctypes.c_int, [_P(ctypes.c_uint32), ctypes.c_void_p]),
+    "cgs_valu_rates": (ctypes.c_int, [_P(ctypes.c_float), ctypes.c_int, ctypes.c_void_p]),
     "cgs_find_extrema": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _P(Argmin),
                                         ctypes.c_void_p]),
     "cgs_covariance": (ctypes.c_int, [_P(Grid), ctypes.c_void_p, _P(ctypes.c_float), _P(Argmin), ctypes.c_void_p]),

@@ -38,6 +38,38 @@ __global__ __launch_bounds__(256) void fma_chain_kernel(float *out, int iters, f
     out[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// VALU issue-rate probes: each iteration of a kernel issues 16 independent
+// instructions of one form (inline asm, so the count is exact), with 8 waves
+// per SIMD.  Forms: v_fmac_f32 on VGPRs; v_fma_f32 with an SGPR operand;
+// v_cvt_f32_ubyte1; v_mad_u32_u24 with an SGPR operand; v_rcp_f32; and
+// v_fmac_f32 alternating with v_cvt_f32_ubyte1 (32 per iteration).
+#define CGS_PROBE_OUTS                                                                                           \
+    "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]), "+v"(a[8]), \
+        "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15])
+#define CGS_PROBE_X16(T) T(0) T(1) T(2) T(3) T(4) T(5) T(6) T(7) T(8) T(9) T(10) T(11) T(12) T(13) T(14) T(15)
+#define CGS_T_FMAC(i) "v_fmac_f32 %" #i ", %16, %17\n\t"
+#define CGS_T_FMAS(i) "v_fma_f32 %" #i ", %18, %17, %" #i "\n\t"
+#define CGS_T_CVT(i) "v_cvt_f32_ubyte1 %" #i ", %16\n\t"
+#define CGS_T_MAD(i) "v_mad_u32_u24 %" #i ", %16, %18, %17\n\t"
+#define CGS_T_RCP(i) "v_rcp_f32 %" #i ", %16\n\t"
+#define CGS_T_MIX(i) "v_fmac_f32 %" #i ", %16, %17\n\tv_cvt_f32_ubyte1 %" #i ", %16\n\t"
+#define CGS_RATE_KERNEL(NAME, T)                                                                              \
+    __global__ __launch_bounds__(256) void NAME(float *out, int iters, float xs, float ss) {                   \
+        float a[16];                                                                                           \
+        for (int i = 0; i < 16; ++i) a[i] = (float) (threadIdx.x + i);                                       \
+        const float x = xs + (float) threadIdx.x, y = xs * (float) threadIdx.x;                              \
+        for (int it = 0; it < iters; ++it) asm volatile(CGS_PROBE_X16(T) : CGS_PROBE_OUTS : "v"(x), "v"(y), "s"(ss)); \
+        float sum = 0.0f;                                                                                      \
+        for (int i = 0; i < 16; ++i) sum += a[i];                                                             \
+        out[blockIdx.x * blockDim.x + threadIdx.x] = sum;                                                      \
+    }
+CGS_RATE_KERNEL(rate_fmac_kernel, CGS_T_FMAC)
+CGS_RATE_KERNEL(rate_fma_sgpr_kernel, CGS_T_FMAS)
+CGS_RATE_KERNEL(rate_cvt_kernel, CGS_T_CVT)
+CGS_RATE_KERNEL(rate_mad24_kernel, CGS_T_MAD)
+CGS_RATE_KERNEL(rate_rcp_kernel, CGS_T_RCP)
+CGS_RATE_KERNEL(rate_mix_kernel, CGS_T_MIX)
+
 // For every binary32 mantissa in [1, 2) (thread m: d = 1 + m 2^-23), counts
 // the d whose reciprocal refined as refine_rcp does differs from the
 // correctly rounded 1.0f / d.
@@ -51,6 +83,49 @@ __global__ __launch_bounds__(256) void rcp_check_kernel(unsigned *mismatches) {
 }  // namespace cgs
 
 using namespace cgs;
+
+extern "C" cgs_status cgs_valu_rates(float *rates_out, int n, void *stream) {
+    clear_error();
+    typedef void (*RateKernel)(float *, int, float, float);
+    static const RateKernel kernels[CGS_VALU_RATE_COUNT] = {rate_fmac_kernel, rate_fma_sgpr_kernel, rate_cvt_kernel,
+                                                            rate_mad24_kernel, rate_rcp_kernel, rate_mix_kernel};
+    static const int per_iter[CGS_VALU_RATE_COUNT] = {16, 16, 16, 16, 16, 32};
+    if (!rates_out || n < CGS_VALU_RATE_COUNT)
+        return set_error(CGS_ERR_INVALID_ARGUMENT, "valu_rates: the output holds fewer than CGS_VALU_RATE_COUNT floats");
+    int dev = 0, cus = 0, khz = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, dev));
+    const int block = 256, blocks = cus * 8, iters = 4096;
+    float *out = nullptr;
+    HIP_TRY(hipMalloc(&out, sizeof(float) * (size_t) blocks * block));
+    hipStream_t s = (hipStream_t) stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int k = 0; k < CGS_VALU_RATE_COUNT && e == hipSuccess; ++k) {
+        float best = 0.0f;
+        for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+            e = hipEventRecord(e0, s);
+            hipLaunchKernelGGL(kernels[k], dim3(blocks), dim3(block), 0, s, out, iters, 1.0000001f, 0.5f);
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e == hipSuccess) e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            // wave64 instructions per CU and per cycle of the nominal clock
+            const double winstr = (double) blocks * (block / 64) * iters * per_iter[k];
+            const double rate = winstr / (ms * 1e-3 * khz * 1e3 * cus);
+            if (e == hipSuccess && ms > 0.0f && rate > best) best = (float) rate;
+        }
+        rates_out[k] = best;
+    }
+    if (e0) (void) hipEventDestroy(e0);
+    if (e1) (void) hipEventDestroy(e1);
+    (void) hipFree(out);
+    if (e != hipSuccess) return set_error(CGS_ERR_HIP, std::string("valu_rates: ") + hipGetErrorString(e));
+    return CGS_OK;
+}
 
 extern "C" cgs_status cgs_check_rcp(uint32_t *mismatches_out, void *stream) {
     clear_error();

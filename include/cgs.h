@@ -490,6 +490,20 @@ cgs_status cgs_cnn_adam_step(cgs_cnn *net, float lr, float beta1, float beta2, f
  * reference counterpart.  Allocates a small output buffer; synchronous. */
 cgs_status cgs_fma_peak(float *tflops_out, void *stream);
 
+/* VALU issue rates of the current device, in wave64 instructions per CU and
+ * per cycle of the nominal (maximum) clock, one float per form:
+ *   0 v_fmac_f32 on VGPR operands, 1 v_fma_f32 with an SGPR operand,
+ *   2 v_cvt_f32_ubyte1, 3 v_mad_u32_u24 with an SGPR operand, 4 v_rcp_f32,
+ *   5 v_fmac_f32 alternating with v_cvt_f32_ubyte1.
+ * Each from a kernel of independent instructions of that form, 8 waves per
+ * SIMD, best of three.  Forms 0 and 5 run well above 1 per CU-cycle (two
+ * VALU instructions can issue per cycle when one of them is a plain fp32 or
+ * int32 operation on VGPRs), forms 1-3 near 1, form 4 near 1/2: the ceiling
+ * a kernel's VALU stream meets depends on that mix.  No reference
+ * counterpart.  rates_out holds n >= CGS_VALU_RATE_COUNT floats. */
+#define CGS_VALU_RATE_COUNT 6
+cgs_status cgs_valu_rates(float *rates_out, int n, void *stream);
+
 /* Counts, on the current device, the binary32 mantissas d in [1, 2) whose
  * reciprocal refined by one Newton step (v_rcp_f32, then r += r (1 - d r))
  * is not the correctly rounded 1/d.  The MI kernels' exact quotients take
