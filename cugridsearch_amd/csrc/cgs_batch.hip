@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
         BlockShared<GM> m;
     } S;
     __shared__ uint32_t s_claim;
-    extern __shared__ uint32_t s_hist[];   // MI: GM histograms of the batch's largest bin count + 64 dummies
+    extern __shared__ uint32_t s_hist[];   // MI: GM histograms of the batch's largest bin count
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) s_claim = atomicAdd(counter, (uint32_t) per_claim);

@@ -48,11 +48,8 @@ constexpr int kThreads = 256;
 
 // LDS words of the MI kernel's joint histograms: GPT histograms of bins rows,
 // each row padded to bins + 1 counters (a column of the histogram then spreads
-// over the LDS banks instead of hitting one), then 64 per-lane dummy counters
-// for samples outside the image.
-__host__ __device__ inline size_t mi_hist_words(int gpt, int bins) {
-    return (size_t) gpt * bins * (bins + 1) + 64;
-}
+// over the LDS banks instead of hitting one).
+__host__ __device__ inline size_t mi_hist_words(int gpt, int bins) { return (size_t) gpt * bins * (bins + 1); }
 
 // Elements of zero padding after the moving image: covers every tap the
 // samplers can address for a sample point in [0, W] x [0, H].
@@ -671,8 +668,10 @@ __global__ __launch_bounds__(kThreads) void search_fast_kernel(const SearchParam
 // bilinear value follows the declared lerps, so every joint-histogram count
 // equals the oracle's.  The taps of a one-byte image are packed four to a
 // 32-bit word (tl, tr, bl, br), those of a float image four to a float4; both
-// tables are indexed like the fast kernel's.  Lanes outside the image add to
-// per-lane dummy counters past the histograms, so the atomics need no branch.
+// tables are indexed like the fast kernel's.  Lanes outside the image load
+// whatever taps their offsets reach (zeros past the table) and their counter
+// adds are masked off: each grid point's atomic runs under EXEC = its in-image
+// lane mask.
 
 struct Recip {
     float d, r;
@@ -771,7 +770,7 @@ __device__ __forceinline__ void lds_count(uint32_t addr) { atomicAdd((uint32_t *
 template <typename MT, typename FT, int GPT, bool FASTDIV>
 __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_coef)[12], uint32_t *hist,
                                          uint32_t *cnt) {
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x;
     const int fw = uniform(P.fw), fh = uniform(P.fh);
     const ColChunks chunks(fw, fh);
     const int mw = uniform(P.mw), mh = uniform(P.mh);
@@ -786,11 +785,9 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
     const int bins = uniform(P.bins), RS = bins + 1, HS = bins * RS;   // rows padded to bins + 1 counters
     const bool ceil_taps = uniform(P.ceil_taps) != 0;
     const float bscale = (float) bins / 256.0f;
-    // LDS byte addresses: the histograms, one row, one histogram, and this
-    // lane's dummy counter past them (for samples outside the image)
+    // LDS byte addresses: the histograms, one row, one histogram
     const uint32_t hbase = __builtin_amdgcn_readfirstlane(lds_address(hist));
     const uint32_t row4 = 4u * (uint32_t) RS, hist4 = 4u * (uint32_t) HS;
-    const uint32_t dummy = hbase + 4u * (uint32_t) (GPT * HS + lane);
 
     float B[GPT][3];
 #pragma unroll
@@ -850,9 +847,10 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 m &= vmask;
                 in[g] = m;
                 cnt[g] += (uint32_t) __builtin_popcountll(m);
-                // in-image sample points lie in [0, W] x [0, H]; other lanes'
-                // offsets are replaced below, so only the general-W path clamps
-                // (its discarded quotients may be huge or NaN)
+                // in-image sample points lie in [0, W] x [0, H]; other lanes load
+                // from any offset (the buffer's range check returns zeros past
+                // the table) and their counts are masked; the general-W path
+                // clamps its floors, whose discarded quotients may be NaN
                 int ix = floor_to_int(X), iy = floor_to_int(Y);
                 if constexpr (!FASTDIV) {
                     ix = clamp_int(ix, -1, mw);
@@ -863,7 +861,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 uint32_t t, off;
                 asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(t) : "v"(ix), "i"(EB == 4 ? 2 : 4), "s"(baseE));
                 asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(off) : "v"(iy), "s"(wE), "v"(t));
-                q[g] = TapTable<MT>::load(rsrc, mask_select(m, kOffsetOut, off));
+                q[g] = TapTable<MT>::load(rsrc, off);
             }
 #pragma unroll
             for (int g = 0; g < GPT; ++g) {
@@ -884,7 +882,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 asm("v_lshl_add_u32 %0, %1, 2, %2"
                     : "=v"(a)
                     : "v"(mi_bin_fast<std::is_same<MT, uint8_t>::value>(v, bins, bscale)), "v"(raddr));
-                lds_count(mask_select(in[g], dummy, a));
+                if (__builtin_amdgcn_inverse_ballot_w64(in[g])) lds_count(a);   // EXEC = the in-image lanes
                 raddr += hist4;   // the next grid point's histogram
             }
         }
@@ -972,7 +970,7 @@ __device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, Bl
 template <typename MT, typename FT, int GPT>
 __global__ __launch_bounds__(kThreads) void search_mi_kernel(const SearchParams P) {
     __shared__ BlockShared<GPT> S;
-    extern __shared__ uint32_t s_hist[];   // GPT histograms + 64 dummy counters
+    extern __shared__ uint32_t s_hist[];   // GPT histograms
     mi_block<MT, FT, GPT>(P, xcd_tile((int32_t) blockIdx.x, (int32_t) gridDim.x) * GPT, S, s_hist);
 }
 
