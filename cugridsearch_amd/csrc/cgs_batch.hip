@@ -179,9 +179,10 @@ hipError_t launch_batch(cgs_batch *b, hipStream_t s) {
 
 hipError_t launch_batch_any(cgs_batch *b, hipStream_t s) {
     if (b->host_descs.empty() || b->total_items == 0) return hipSuccess;
-    const bool f6 = b->tiles.fast_gpt == 6, m4 = b->tiles.mi_gpt == 4;
-    if (f6) return m4 ? launch_batch<6, 4>(b, s) : launch_batch<6, 2>(b, s);
-    return m4 ? launch_batch<4, 4>(b, s) : launch_batch<4, 2>(b, s);
+    const int m = b->tiles.mi_gpt;
+    if (b->tiles.fast_gpt == 6)
+        return m == 8 ? launch_batch<6, 8>(b, s) : m == 4 ? launch_batch<6, 4>(b, s) : launch_batch<6, 2>(b, s);
+    return m == 8 ? launch_batch<4, 8>(b, s) : m == 4 ? launch_batch<4, 4>(b, s) : launch_batch<4, 2>(b, s);
 }
 
 cgs_status parse_tiles(const cgs_tile_config *tiles, cgs_tile_config *out) {
@@ -192,7 +193,7 @@ cgs_status parse_tiles(const cgs_tile_config *tiles, cgs_tile_config *out) {
         if (tiles->wg_per_cu) t.wg_per_cu = tiles->wg_per_cu;
         if (tiles->items_per_claim) t.items_per_claim = tiles->items_per_claim;
     }
-    if ((t.fast_gpt != 4 && t.fast_gpt != 6) || (t.mi_gpt != 2 && t.mi_gpt != 4) || t.wg_per_cu < 1 ||
+    if ((t.fast_gpt != 4 && t.fast_gpt != 6) || (t.mi_gpt != 2 && t.mi_gpt != 4 && t.mi_gpt != 8) || t.wg_per_cu < 1 ||
         t.wg_per_cu > 8 || t.items_per_claim < 1 || t.items_per_claim > 64)
         return set_error(CGS_ERR_INVALID_ARGUMENT, "batch: bad tile configuration");
     *out = t;

@@ -19,9 +19,9 @@ from . import _lib as L
 PLAIN_CANDIDATES = {"fast": (4, 6, 8), "mi": (2, 4, 6, 8)}
 BATCH_CANDIDATES = [dict(fast_gpt=4, mi_gpt=4, wg_per_cu=4, items_per_claim=8),
                     dict(fast_gpt=6, mi_gpt=4, wg_per_cu=4, items_per_claim=8),
-                    dict(fast_gpt=4, mi_gpt=4, wg_per_cu=2, items_per_claim=8),
+                    dict(fast_gpt=6, mi_gpt=8, wg_per_cu=4, items_per_claim=8),
                     dict(fast_gpt=4, mi_gpt=2, wg_per_cu=4, items_per_claim=16),
-                    dict(fast_gpt=6, mi_gpt=4, wg_per_cu=8, items_per_claim=4)]
+                    dict(fast_gpt=6, mi_gpt=8, wg_per_cu=8, items_per_claim=4)]
 
 
 def cache_path():

@@ -353,7 +353,7 @@ typedef struct {
 /* Tile shapes of the persistent kernel; 0 fields take the defaults. */
 typedef struct {
     int32_t fast_gpt;        /* grid points per workgroup for SAD/NCC/SQD: 4 or 6 (default 6) */
-    int32_t mi_gpt;          /* grid points per workgroup for MI: 2 or 4 */
+    int32_t mi_gpt;          /* grid points per workgroup for MI: 2, 4 or 8 (default 4; 2 above 32 bins) */
     int32_t wg_per_cu;       /* resident workgroups per compute unit: 1..8 */
     int32_t items_per_claim; /* work items taken per atomic claim: 1..64 */
 } cgs_tile_config;

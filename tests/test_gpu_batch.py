@@ -55,7 +55,8 @@ def test_batch_eager_equals_plain(gpu):
 
 
 @pytest.mark.parametrize("tiles", [dict(fast_gpt=4, mi_gpt=2, wg_per_cu=2, items_per_claim=1),
-                                   dict(fast_gpt=6, mi_gpt=4, wg_per_cu=8, items_per_claim=64)])
+                                   dict(fast_gpt=6, mi_gpt=4, wg_per_cu=8, items_per_claim=64),
+                                   dict(fast_gpt=4, mi_gpt=8, wg_per_cu=4, items_per_claim=8)])
 def test_batch_tiles_identical(gpu, tiles):
     specs, pairs = make_batch(seed=12)
     want = plain(pairs)
