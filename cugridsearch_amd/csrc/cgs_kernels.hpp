@@ -48,8 +48,15 @@ constexpr int kThreads = 256;
 
 // LDS words of the MI kernel's joint histograms: GPT histograms of bins rows,
 // each row padded to bins + 1 counters (a column of the histogram then spreads
-// over the LDS banks instead of hitting one).
-__host__ __device__ inline size_t mi_hist_words(int gpt, int bins) { return (size_t) gpt * bins * (bins + 1); }
+// over the LDS banks instead of hitting one).  Above 2 grid points per
+// workgroup (at most 32 bins) every histogram takes the 32-bin stride, a
+// compile-time constant: the sweep reaches point g's histogram through the LDS
+// atomic's immediate offset instead of one address add per sample.
+constexpr int kMiHistFixed = 32 * 33;
+__host__ __device__ inline size_t mi_hist_stride(int gpt, int bins) {
+    return gpt > 2 ? (size_t) kMiHistFixed : (size_t) bins * (bins + 1);
+}
+__host__ __device__ inline size_t mi_hist_words(int gpt, int bins) { return (size_t) gpt * mi_hist_stride(gpt, bins); }
 
 // Elements of zero padding after the moving image: covers every tap the
 // samplers can address for a sample point in [0, W] x [0, H].
@@ -169,9 +176,18 @@ __device__ __forceinline__ void block_coefficients(const SearchParams &P, int32_
         for (int k = 0; k < 6; ++k) bounded = bounded && fabsf(co[k]) < 1e30f;
         const bool posw = wmin > 1e-5 * scale && wmin > 1e-20 && scale < 1e30 && bounded;
         s_coef[tid][9] = posw ? 1.0f : 0.0f;
-        // W within [1/64, 64] over the image: the refined-reciprocal quotient
-        // (div_by) then equals IEEE division (see div_by)
-        s_coef[tid][10] = (posw && wmin >= 1.0 / 64 && wmax <= 64.0) ? 1.0f : 0.0f;
+        // W within [1/64, 64] over the image and X, Y numerators of 0 or of
+        // magnitude in (2^-90, 2^90): the refined-reciprocal quotient (div_by)
+        // then equals IEEE division on every pixel.  Every nonzero X/Y
+        // coefficient at least 2^-60 in magnitude keeps the forms' nonzero
+        // values (integer x, y) multiples of 2^-83, so at least 2^-83; the
+        // corner bound keeps them under 2^89.
+        bool divsafe = wmin >= 1.0 / 64 && wmax <= 64.0;
+        for (int k = 0; k < 6; k += 3)
+            divsafe = divsafe && fabs((double) co[k]) * P.fw + fabs((double) co[k + 1]) * P.fh +
+                                         fabs((double) co[k + 2]) < 0x1p89;
+        for (int k = 0; k < 6; ++k) divsafe = divsafe && (co[k] == 0.0f || fabsf(co[k]) >= 0x1p-60f);
+        s_coef[tid][10] = (posw && divsafe) ? 1.0f : 0.0f;
     }
 }
 
@@ -261,8 +277,12 @@ __global__ __launch_bounds__(kThreads) void search_exact_kernel(const SearchPara
             const float Wn = form(cf[g][6], cf[g][7], cf[g][8], fxp, fyp);
             const float Xn = form(cf[g][0], cf[g][1], cf[g][2], fxp, fyp);
             const float Yn = form(cf[g][3], cf[g][4], cf[g][5], fxp, fyp);
-            if (quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm)) {
-                const float X = Xn / Wn, Y = Yn / Wn;
+            const float X = Xn / Wn, Y = Yn / Wn;
+            // inImage: MI on the rounded quotients (the reference's own test),
+            // SAD/NCC/SQD on the exact quotient (the declared rule)
+            const bool in = FAM == kFamMI ? (X >= 0.0f && Y >= 0.0f && X < Wm && Y < Hm)
+                                          : (quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm));
+            if (in) {
                 for (int c = 0; c < C; ++c) {
                     const float m = SAMP == CGS_SAMPLER_NEAREST ? sample_nearest(mov, X, Y, c, C, P.mw)
                                                                 : sample_bilinear(mov, X, Y, c, C, P.mw);
@@ -782,7 +802,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
         (void *) uniform(P.quads), (short) 0, (int) (((size_t) mw * mh + 2 * (size_t) mw + 2) * EB), 0x00020000);
     const __amdgpu_buffer_rsrc_t frsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *) fix, (short) 0, (int) ((size_t) fw * fh * sizeof(FT)), 0x00020000);
-    const int bins = uniform(P.bins), RS = bins + 1, HS = bins * RS;   // rows padded to bins + 1 counters
+    const int bins = uniform(P.bins), RS = bins + 1, HS = (int) mi_hist_stride(GPT, bins);   // rows padded
     const bool ceil_taps = uniform(P.ceil_taps) != 0;
     const float bscale = (float) bins / 256.0f;
     // LDS byte addresses: the histograms, one row, one histogram
@@ -820,10 +840,6 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
         for (int it = 0; it < steps; ++it, foff += fstep, yf += (float) rpi) {
             const unsigned long long vmask = it < full ? valid_mask : tail_mask;
             const float f = load_fix<FT>(frsrc, foff);   // 0 on lanes off the image
-            uint32_t raddr;   // the fixed bin's row of the first histogram
-            asm("v_mad_u32_u24 %0, %1, %2, %3"
-                : "=v"(raddr)
-                : "v"(mi_bin_fast(f, bins, bscale)), "s"(row4), "v"(hbase));   // one scalar operand per VOP3
             unsigned long long in[GPT];
             float ax[GPT], ay[GPT];
             float4 q[GPT];
@@ -832,37 +848,53 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 const float Xn = fmaf(B[g][0], yf, col[g][0]);
                 const float Yn = fmaf(B[g][1], yf, col[g][1]);
                 const float Wn = fmaf(B[g][2], yf, col[g][2]);
-                unsigned long long m;
+                // the sample point: the correctly rounded quotients; in the
+                // image (CudaImage::inImage on them, the MI costs' rule) when
+                // 0 <= X < W and 0 <= Y < H
                 float X, Y;
                 if constexpr (FASTDIV) {
-                    m = __builtin_amdgcn_ballot_w64(inside_posw(Xn, Yn, Wn, Wm, Hm));
                     const Recip R = refine_rcp(Wn);
                     X = div_by(Xn, R);
                     Y = div_by(Yn, R);
                 } else {
-                    m = __builtin_amdgcn_ballot_w64(quotient_in(Xn, Wn, Wm) && quotient_in(Yn, Wn, Hm));
                     X = Xn / Wn;
                     Y = Yn / Wn;
+                }
+                int ix = floor_to_int(X), iy = floor_to_int(Y);
+                unsigned long long m;
+                if constexpr (FASTDIV) {
+                    // X and Y are the IEEE quotients here, below 2^96 in
+                    // magnitude: 0 <= X < W exactly when 0 <= floor(X) < W, and
+                    // v_cvt_flr saturates what lies beyond int32.  One ballot
+                    // per compare: each is a v_cmp into an SGPR pair, where a
+                    // ballot of the && costs a v_cndmask and a v_cmp more
+                    m = __builtin_amdgcn_ballot_w64((uint32_t) ix < (uint32_t) mw) &
+                        __builtin_amdgcn_ballot_w64((uint32_t) iy < (uint32_t) mh);
+                } else {
+                    // W may be tiny, zero or negative: the quotients may be huge,
+                    // infinite or NaN (no compare holds); the floors stay near the
+                    // table for the lanes whose taps are discarded
+                    m = __builtin_amdgcn_ballot_w64(X >= 0.0f) & __builtin_amdgcn_ballot_w64(Y >= 0.0f) &
+                        __builtin_amdgcn_ballot_w64(X < Wm) & __builtin_amdgcn_ballot_w64(Y < Hm);
+                    ix = clamp_int(ix, -1, mw);
+                    iy = clamp_int(iy, -1, mh);
                 }
                 m &= vmask;
                 in[g] = m;
                 cnt[g] += (uint32_t) __builtin_popcountll(m);
-                // in-image sample points lie in [0, W] x [0, H]; other lanes load
-                // from any offset (the buffer's range check returns zeros past
-                // the table) and their counts are masked; the general-W path
-                // clamps its floors, whose discarded quotients may be NaN
-                int ix = floor_to_int(X), iy = floor_to_int(Y);
-                if constexpr (!FASTDIV) {
-                    ix = clamp_int(ix, -1, mw);
-                    iy = clamp_int(iy, -1, mh);
-                }
                 ax[g] = __builtin_amdgcn_fractf(X);
                 ay[g] = __builtin_amdgcn_fractf(Y);
                 uint32_t t, off;
                 asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(t) : "v"(ix), "i"(EB == 4 ? 2 : 4), "s"(baseE));
                 asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(off) : "v"(iy), "s"(wE), "v"(t));
+                // lanes outside the image load from any offset (past the table
+                // the range check returns zeros); their counts are masked
                 q[g] = TapTable<MT>::load(rsrc, off);
             }
+            uint32_t raddr;   // the fixed bin's row of the first histogram
+            asm("v_mad_u32_u24 %0, %1, %2, %3"
+                : "=v"(raddr)
+                : "v"(mi_bin_fast(f, bins, bscale)), "s"(row4), "v"(hbase));   // one scalar operand per VOP3
 #pragma unroll
             for (int g = 0; g < GPT; ++g) {
                 float tr = q[g].y, bl = q[g].z, br = q[g].w;
@@ -882,8 +914,13 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
                 asm("v_lshl_add_u32 %0, %1, 2, %2"
                     : "=v"(a)
                     : "v"(mi_bin_fast<std::is_same<MT, uint8_t>::value>(v, bins, bscale)), "v"(raddr));
-                if (__builtin_amdgcn_inverse_ballot_w64(in[g])) lds_count(a);   // EXEC = the in-image lanes
-                raddr += hist4;   // the next grid point's histogram
+                if constexpr (GPT > 2) {
+                    // EXEC = the in-image lanes; point g's histogram at a constant offset
+                    if (__builtin_amdgcn_inverse_ballot_w64(in[g])) lds_count(a + 4u * kMiHistFixed * g);
+                } else {
+                    if (__builtin_amdgcn_inverse_ballot_w64(in[g])) lds_count(a);
+                    raddr += hist4;   // the next grid point's histogram
+                }
             }
         }
     }
@@ -939,7 +976,7 @@ __device__ inline double mi_entropy_cost(const uint32_t *h, int bins, int rs, do
 template <typename MT, typename FT, int GPT>
 __device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, BlockShared<GPT> &S, uint32_t *hist) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int B2 = P.bins * P.bins, RS = P.bins + 1, HS = P.bins * RS;
+    const int B2 = P.bins * P.bins, RS = P.bins + 1, HS = (int) mi_hist_stride(GPT, P.bins);
     block_coefficients<GPT>(P, base, S.coef);
     for (int i = tid; i < (int) mi_hist_words(GPT, P.bins); i += kThreads) hist[i] = 0u;
     __syncthreads();

@@ -159,15 +159,16 @@ static hipError_t mi_launch(const SearchParams &P, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// 8 grid points per workgroup up to 32 bins (34 KiB of histograms; each
-// fixed pixel's load and bin shared by 8 points: 2% faster than 4 on 512^2
-// uint8 at 32 bins), 2 above; tile_gpt 2, 4, 6 or 8 overrides up to 32 bins.
+// 4 grid points per workgroup up to 32 bins (17 KiB of histograms, 63 VGPRs:
+// 8 waves per SIMD; on 512^2 uint8 at 32 bins 71.0 ms against 72.4 at 8
+// points, 4 waves per SIMD, and 71.1 at 6), 2 above; tile_gpt 2, 4, 6 or 8
+// overrides up to 32 bins.
 template <typename MT, typename FT>
 static hipError_t mi_gpt(const SearchParams &P, int gpt, hipStream_t stream) {
     if (P.bins > 32 || gpt == 2) return mi_launch<MT, FT, 2>(P, stream);
-    if (gpt == 4) return mi_launch<MT, FT, 4>(P, stream);
+    if (gpt == 8) return mi_launch<MT, FT, 8>(P, stream);
     if (gpt == 6) return mi_launch<MT, FT, 6>(P, stream);
-    return mi_launch<MT, FT, 8>(P, stream);
+    return mi_launch<MT, FT, 4>(P, stream);
 }
 
 static hipError_t dispatch_mi(const SearchParams &P, int mpix, int fpix, int gpt, hipStream_t stream) {

@@ -119,7 +119,7 @@ typedef struct {
     int32_t mi_bins;     /* joint-histogram bins per axis for the MI costs: 8..64 (reference: 64) */
     int32_t flags;       /* CGS_FLAG_* bits */
     int32_t tile_gpt;    /* grid points per workgroup: 4, 6 or 8 for SAD/NCC/SQD, 2, 4, 6 or 8 for MI
-                            (2 above 32 bins); 0 = library default (6 or 8 for SAD/NCC/SQD, 8 for
+                            (2 above 32 bins); 0 = library default (6 or 8 for SAD/NCC/SQD, 4 for
                             MI up to 32 bins, 2 above) */
 } cgs_search_opts;
 

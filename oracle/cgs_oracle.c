@@ -14,8 +14,10 @@
  *     F(x, y) = fmaf(B, y, fmaf(A, x, C)) with C = P - Q rounded once (the
  *     reference writes (A*x + B*y + P) - Q and leaves contraction to nvcc);
  *   - inImage is decided on the exact rational quotient of those rounded
- *     forms (0 <= X/W < width, 0 <= Y/W < height, by sign and fmaf tests);
- *     the sample point itself is the correctly rounded quotient;
+ *     forms (0 <= X/W < width, 0 <= Y/W < height, by sign and fmaf tests)
+ *     for SAD/NCC/SQD, and on the correctly rounded quotients themselves
+ *     (the reference's own test) for the MI costs; the sample point is the
+ *     correctly rounded quotient;
  *   - sin/cos of the rotation angle: (float)sin((double)theta), likewise cos;
  *   - division and sqrt: IEEE correctly rounded;
  *   - NCC / SQD / bilinear-SAD sums: binary64 accumulation of binary32 terms;
@@ -337,10 +339,15 @@ float orc_cost_point(const orc_image *moved, const orc_image *fixed, int cost, i
     orc_acc a = {0, 0, 0, 0, 0, 0};
     const int exact_int = (sampler == ORC_NEAREST && moved->pixtype == ORC_PIX_U8 &&
                            fixed->pixtype == ORC_PIX_U8);
+    const float mw = (float) moved->width, mh = (float) moved->height;
     for (int y = 0; y < fixed->height; y++) {
         for (int x = 0; x < fixed->width; x++) {
             float xy[2];
-            if (!orc_warp_point(coef, x, y, xy, (float) moved->width, (float) moved->height)) continue;
+            const int in = orc_warp_point(coef, x, y, xy, mw, mh);
+            /* the MI costs test CudaImage::inImage on calcNewCoordH's binary32
+             * quotients, as calcMIstream does (cudaErrorFunction_miStreams.cuh:263-268);
+             * SAD/NCC/SQD the declared exact-quotient rule */
+            if (is_mi ? !(xy[0] >= 0 && xy[1] >= 0 && xy[0] < mw && xy[1] < mh) : !in) continue;
             for (int c = 0; c < C; c++) {
                 const float m = sample(moved, xy[1], xy[0], c, sampler);
                 const float f = pix(fixed, ((int64_t) y * fixed->width + x) * C + c);

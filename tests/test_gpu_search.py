@@ -148,6 +148,39 @@ def test_mi32_512_histograms_bit_exact(gpu, oracle):
         assert close([cost], [oc], MI_RTOL)[0], (k, cost, oc)
 
 
+def test_mi_in_image_on_rounded_quotient(gpu, oracle):
+    """The 2^-24 window case at the moved image's lower edge (Y/W just below
+    48 that rounds to 48.0): the MI costs leave it out (inImage on the
+    rounded quotient), SAD/NCC/SQD keep it (exact quotient); the tiled search
+    and the single-point path both agree with the oracle, with W in
+    [1/64, 64] over the fixed image (h32 = 1: the refined-reciprocal MI
+    sweep), with W up to 101 (h32 = 100: the IEEE-division sweep), and with a
+    nonzero X coefficient below 2^-60 (h12 = 1e-25: the IEEE-division sweep
+    although W is in range)."""
+    rng = np.random.default_rng(8)
+    moved = rng.integers(0, 256, (48, 64), dtype=np.uint8)
+    fixed = rng.integers(0, 256, (2, 3), dtype=np.uint8)
+    gm, gf = A.CudaImage.from_numpy(moved), A.CudaImage.from_numpy(fixed)
+    for h32, h12 in ((1.0, 0.0), (100.0, 0.0), (1.0, 1e-25)):
+        p = np.array([0, h12, 10, 0, 0, 48 + 2 ** -18, 2 ** -23, h32], np.float32)
+        for base, want_count in ((A.calcMIstream, 4), (A.calcMI, 4), (A.calcNCC, 5), (A.calcSQD, 5),
+                                 (A.averageAbsoluteDifference, 5)):
+            fn = base.with_(param_mode=L.PARAM_RAW_H, bins=32)
+            oc, aux = oracle.cost_point(moved, fixed, fn.cost, fn.sampler, p, mode=L.PARAM_RAW_H, bins=32)[:2]
+            assert aux[4] == want_count, (h32, fn.name, aux[4])
+            cost, hist = A.eval_point(fn, gm, gf, p)
+            if hist is not None:
+                assert hist.sum() == want_count, (h32, fn.name, hist.sum())
+            got, (v, i), _ = gpu_search(fn, moved, fixed, p, p, [1] * 8)
+            # MI: exact quotients; SAD/NCC/SQD (both calls run the tiled
+            # kernel): the sample point is Xn * v_rcp(W), an ulp or so off the
+            # rounded quotient, which on 5 samples moves the cost by up to
+            # ~1e-4 (a wrong count moves it by 20% or more)
+            rtol = MI_RTOL if hist is not None else 1e-3
+            for where, c in (("eval_point", cost), ("search", got[0])):
+                assert close([c], [oc], rtol)[0], (h32, fn.name, where, c, oc)
+
+
 def _search_with(functor, m, f, grid, gpt, order):
     """cgs_search with an explicit tile size and work order (CGS_WORK_ORDER
     is read by every search call); returns the costs and the argmin."""
@@ -228,6 +261,35 @@ def test_mi32_full_and_histograms(gpu, oracle):
             oc, _, oh = oracle.cost_point(m, f, fb.cost, fb.sampler, p, bins=bins, want_hist=True)
             assert np.array_equal(hist, oh), (k, bins)
             assert close([cost], [oc], MI_RTOL)[0]
+
+
+@pytest.mark.parametrize("fw,fh,fdtype,bins", [(300, 149, np.uint8, 32), (200, 63, np.float32, 16),
+                                              (256, 6, np.uint8, 8)])
+def test_mi_chunk_shapes_and_bin_counts(gpu, oracle, fw, fh, fdtype, bins):
+    """The MI sweep on a full 256-column chunk plus a 44-column one (several
+    rows per step), on one 200-column chunk, on heights that are not multiples of
+    the row step, on uint8 and float fixed images, at 8, 16 and 32 bins (every
+    histogram at the 32-bin stride), with 4 and 8 grid points per workgroup:
+    costs against the oracle, joint histograms bit-exact."""
+    m, f, _ = synth.make_pair(fw * 1000 + fh, fw, fh, sigma=3.0, perturb=4.0, dtype=np.uint8)
+    f = f.astype(fdtype)
+    s, e, c = synth.grid_ranges(2, 3.0)
+    fn = A.calcMIstream.with_(bins=bins)
+    grid = A.CudaGrid(8)
+    grid.setStartPoint(s)
+    grid.setEndPoint(e)
+    grid.setNumSamples(c)
+    gm, gf = A.CudaImage.from_numpy(m), A.CudaImage.from_numpy(f)
+    want = oracle_search(oracle, fn, m, f, s, e, c)
+    for gpt in (4, 8):
+        got, _ = _search_with(fn, gm, gf, grid, gpt, None)
+        ok, bad = close(got, want, MI_RTOL)
+        assert ok, (gpt, bad, got[bad], want[bad])
+    for k in (0, 77, 255):
+        p = grid.getGridPoint(k)
+        cost, hist = A.eval_point(fn, gm, gf, p)
+        oc, _, oh = oracle.cost_point(m, f, fn.cost, fn.sampler, p, bins=bins, want_hist=True)
+        assert np.array_equal(hist, oh), k
 
 
 CASES = [

@@ -241,6 +241,36 @@ def test_in_image_on_exact_quotient(oracle):
     assert not oracle.warp_point(coef, 0, 0, 64, 48)[1]
 
 
+def _window_case(h32):
+    """Raw-H parameters whose row 0 holds the 2^-24 window case: pixel 0 lies
+    outside (Y = 48 + 2^-18), pixel 1's exact quotient lies below 48 but
+    rounds to 48.0, pixel 2 lies inside; row 1 (W = 1 + h32) lies inside,
+    mid-image."""
+    rng = np.random.default_rng(8)
+    moved = rng.integers(0, 256, (48, 64), dtype=np.uint8)
+    fixed = rng.integers(0, 256, (2, 3), dtype=np.uint8)
+    params = np.array([0, 0, 10, 0, 0, 48 + 2 ** -18, 2 ** -23, h32], np.float32)
+    return moved, fixed, params
+
+
+def test_mi_in_image_on_rounded_quotient(oracle):
+    """The MI costs test inImage on the rounded quotients, as calcMIstream
+    does (cudaErrorFunction_miStreams.cuh:263-268); SAD/NCC/SQD keep the
+    exact-quotient rule.  On the window pixel they disagree."""
+    for h32 in (1.0, 100.0):
+        moved, fixed, p = _window_case(h32)
+        coef = oracle.warp_coeffs(p, 0.0, 0.0, mode=oracle.PARAM_RAW_H)
+        (X, Y), inside = oracle.warp_point(coef, 1, 0, 64, 48)
+        assert Y == np.float32(48.0) and inside          # the window pixel
+        for cost, want in ((oracle.COST_MI, 4), (oracle.COST_MI_PLAIN, 4),
+                           (oracle.COST_SAD, 5), (oracle.COST_NCC, 5), (oracle.COST_SQD, 5)):
+            _, aux = oracle.cost_point(moved, fixed, cost, oracle.BILINEAR, p, mode=oracle.PARAM_RAW_H)[:2]
+            assert aux[4] == want, (h32, cost, aux[4], want)
+        _, _, h = oracle.cost_point(moved, fixed, oracle.COST_MI, oracle.BILINEAR, p, mode=oracle.PARAM_RAW_H,
+                                    want_hist=True)
+        assert h.sum() == 4
+
+
 # -------------------------------------------------------------- samplers
 
 def test_nearest_known_answers(oracle):
