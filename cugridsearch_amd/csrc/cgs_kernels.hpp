@@ -448,6 +448,30 @@ __device__ __forceinline__ void masked_add(float &acc, float v, unsigned long lo
                  : "scc");
 }
 
+// The NCC sums of one sample on the lanes of wave mask m, under one narrowed
+// EXEC: s0 += f2, s1 = fmaf(v, v, s1), s2 = fmaf(f, v, s2).
+__device__ __forceinline__ void masked_ncc(float &s0, float &s1, float &s2, float f2, float v, float f,
+                                           unsigned long long m) {
+    unsigned long long saved;
+    asm volatile("s_and_saveexec_b64 %3, %7\n\tv_add_f32 %0, %0, %4\n\tv_fmac_f32 %1, %5, %5\n\t"
+                 "v_fmac_f32 %2, %6, %5\n\ts_mov_b64 exec, %3"
+                 : "+v"(s0), "+v"(s1), "+v"(s2), "=&s"(saved)
+                 : "v"(f2), "v"(v), "v"(f), "s"(m)
+                 : "scc");
+}
+
+// The SQD sums of one sample on the lanes of wave mask m: s0 += f2,
+// s1 = fmaf(v, v, s1), s2 += dd.
+__device__ __forceinline__ void masked_sqd(float &s0, float &s1, float &s2, float f2, float v, float dd,
+                                           unsigned long long m) {
+    unsigned long long saved;
+    asm volatile("s_and_saveexec_b64 %3, %7\n\tv_add_f32 %0, %0, %4\n\tv_fmac_f32 %1, %5, %5\n\t"
+                 "v_add_f32 %2, %2, %6\n\ts_mov_b64 exec, %3"
+                 : "+v"(s0), "+v"(s1), "+v"(s2), "=&s"(saved)
+                 : "v"(f2), "v"(v), "v"(dd), "s"(m)
+                 : "scc");
+}
+
 // Positive-W in-image test from sign bits.  With Wn > 0, Xn never -0 and no
 // NaN: Xn >= 0 and fmaf(-Wm, Wn, Xn) < 0 (the exact-quotient test) holds
 // exactly when their sign bits differ, since Xn < 0 forces the second
@@ -576,20 +600,20 @@ __device__ __forceinline__ void fast_sweep(const SearchParams &P, const float (*
                 }
                 ax[g] = __builtin_amdgcn_fractf(X);
                 ay[g] = __builtin_amdgcn_fractf(Y);
-                q[g] = load_tap(rsrc, mask_select(m, kOffsetOut, tap_offset(ix, iy, w16, base16)));
+                // lanes outside the image load from any offset (past the table
+                // the range check returns zeros); every sum below is masked, so
+                // their values, finite or not, reach no sum
+                q[g] = load_tap(rsrc, tap_offset(ix, iy, w16, base16));
             }
 #pragma unroll
             for (int g = 0; g < GPT; ++g) {
-                // lanes outside read a zero tap: v == 0 exactly when the weights are finite
-                float v = fmaf(ay[g], fmaf(ax[g], q[g].w, q[g].z), fmaf(ax[g], q[g].y, q[g].x));
-                if constexpr (!POSW) v = __uint_as_float(mask_select(in[g], 0u, __float_as_uint(v)));
+                const float v = fmaf(ay[g], fmaf(ax[g], q[g].w, q[g].z), fmaf(ax[g], q[g].y, q[g].x));
                 if constexpr (COSTK == 0) {
                     masked_add(part[g][0], fabsf(v - f), in[g]);
+                } else if constexpr (COSTK == 1) {
+                    masked_ncc(part[g][0], part[g][1], part[g][2], f2, v, f, in[g]);
                 } else {
-                    masked_add(part[g][0], f2, in[g]);
-                    part[g][1] = fmaf(v, v, part[g][1]);
-                    if constexpr (COSTK == 1) part[g][2] = fmaf(f, v, part[g][2]);
-                    else masked_add(part[g][2], (v - f) * (v - f), in[g]);
+                    masked_sqd(part[g][0], part[g][1], part[g][2], f2, v, (v - f) * (v - f), in[g]);
                 }
             }
             if (++since == kFlush) {
