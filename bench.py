@@ -43,6 +43,9 @@ PROPOSAL_WORKLOAD = "proposal_1024"
 CNN_WORKLOAD = "cnn_train_step"
 MATCHER_WORKLOADS = {"matcher_256": 256, "matcher_512": 512}
 PEAK_BF16_MFMA_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
+# the texture path's data rate: 64 bytes per clock per CU (a 16-byte tap quad
+# per lane of a wave64 load: 16 clocks), 256 CUs at the nominal 2.4 GHz
+PEAK_TAP_PATH_TBPS = 64 * 256 * 2.4e9 / 1e12
 PROPOSAL_WEIGHTS = os.path.join(ROOT, "cugridsearch_amd", "weights", "proposal_c5.npz")
 
 
@@ -579,6 +582,11 @@ def main():
         line["accuracy"] = accuracy
     if steps_note:
         line["roofline"]["steps"] = steps_note
+    if not args.functor and spec["sampler"] == "bilinear" and spec["cost"] in ("ncc", "sad", "sqd"):
+        # the fast kernel reads one 16-byte tap quad per pixel-eval
+        tb = 16.0 * pe / (kernel_ms / 1000.0) / 1e12
+        line["roofline"]["tap_path"] = {"achieved": tb, "peak": PEAK_TAP_PATH_TBPS, "unit": "TB/s",
+                                        "frac": tb / PEAK_TAP_PATH_TBPS, "bytes_per_pixel_eval": 16}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(m, f, fn, start, end, count, pe_per_point, args.cpu_seconds)
         if args.workload == "bilinear_ncc_256" and not args.functor:
