@@ -52,11 +52,18 @@ constexpr int kThreads = 256;
 // workgroup (at most 32 bins) every histogram takes the 32-bin stride, a
 // compile-time constant: the sweep reaches point g's histogram through the LDS
 // atomic's immediate offset instead of one address add per sample.
+// Up to 4 points per workgroup the histograms are held twice, one copy for
+// the even lanes and one for the odd (neighbouring pixels, whose bins often
+// coincide, then never add to one counter in the same atomic), and the two
+// are summed before the entropy step.
 constexpr int kMiHistFixed = 32 * 33;
+__host__ __device__ constexpr int mi_hist_copies(int gpt) { return gpt <= 4 ? 2 : 1; }
 __host__ __device__ inline size_t mi_hist_stride(int gpt, int bins) {
     return gpt > 2 ? (size_t) kMiHistFixed : (size_t) bins * (bins + 1);
 }
-__host__ __device__ inline size_t mi_hist_words(int gpt, int bins) { return (size_t) gpt * mi_hist_stride(gpt, bins); }
+__host__ __device__ inline size_t mi_hist_words(int gpt, int bins) {
+    return (size_t) mi_hist_copies(gpt) * gpt * mi_hist_stride(gpt, bins);
+}
 
 // Elements of zero padding after the moving image: covers every tap the
 // samplers can address for a sample point in [0, W] x [0, H].
@@ -829,8 +836,9 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
     const int bins = uniform(P.bins), RS = bins + 1, HS = (int) mi_hist_stride(GPT, bins);   // rows padded
     const bool ceil_taps = uniform(P.ceil_taps) != 0;
     const float bscale = (float) bins / 256.0f;
-    // LDS byte addresses: the histograms, one row, one histogram
-    const uint32_t hbase = __builtin_amdgcn_readfirstlane(lds_address(hist));
+    // LDS byte addresses: the lane's copy of the histograms, one row, one histogram
+    const uint32_t hbase = __builtin_amdgcn_readfirstlane(lds_address(hist)) +
+                           (mi_hist_copies(GPT) == 2 && (threadIdx.x & 1) ? 4u * (uint32_t) (GPT * HS) : 0u);
     const uint32_t row4 = 4u * (uint32_t) RS, hist4 = 4u * (uint32_t) HS;
 
     float B[GPT][3];
@@ -1016,6 +1024,10 @@ __device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, Bl
 #pragma unroll
         for (int g = 0; g < GPT; ++g) S.red[wave][g][3] = (double) cnt[g];
     __syncthreads();
+    if constexpr (mi_hist_copies(GPT) == 2) {   // the odd lanes' copy into the even lanes'
+        for (int i = tid; i < GPT * HS; i += kThreads) hist[i] += hist[GPT * HS + i];
+        __syncthreads();
+    }
     for (int g = 0; g < GPT; ++g) {
         const double n = S.red[0][g][3] + S.red[1][g][3] + S.red[2][g][3] + S.red[3][g][3];
         const double r = mi_entropy_cost(hist + g * HS, P.bins, RS, n, P.cost, S.px, S.py, S.lpx, S.lpy, S.scr);

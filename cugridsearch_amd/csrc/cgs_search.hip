@@ -159,10 +159,10 @@ static hipError_t mi_launch(const SearchParams &P, hipStream_t stream) {
     return hipGetLastError();
 }
 
-// 4 grid points per workgroup up to 32 bins (17 KiB of histograms, 63 VGPRs:
-// 8 waves per SIMD; on 512^2 uint8 at 32 bins 71.0 ms against 72.4 at 8
-// points, 4 waves per SIMD, and 71.1 at 6), 2 above; tile_gpt 2, 4, 6 or 8
-// overrides up to 32 bins.
+// 4 grid points per workgroup up to 32 bins (their histograms in two copies,
+// 34 KiB; on 512^2 uint8 at 32 bins 67.5 ms against 71.0 at 6 points and
+// 72.0 at 8, which hold one copy), 2 above; tile_gpt 2, 4, 6 or 8 overrides
+// up to 32 bins.
 template <typename MT, typename FT>
 static hipError_t mi_gpt(const SearchParams &P, int gpt, hipStream_t stream) {
     if (P.bins > 32 || gpt == 2) return mi_launch<MT, FT, 2>(P, stream);
