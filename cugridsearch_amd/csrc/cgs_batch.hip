@@ -120,7 +120,6 @@ struct cgs_batch_s {
     std::vector<cgs_batch_pair> pairs;
     cgs_tile_config tiles{};
     int cus = 0;
-    int max_bins = 8;        // largest MI bin count of the batch (sizes the LDS histograms)
     uint32_t total_items = 0;
     bool has_mi = false;     // some descriptor of the current launch runs the MI sweep
     int nslots = 0;          // descriptors (= keys) of the current launch
@@ -169,7 +168,12 @@ int batch_kind(const cgs_batch_pair &p) {
 
 template <int GF, int GM>
 hipError_t launch_batch(cgs_batch *b, hipStream_t s) {
-    const size_t lds = b->has_mi ? mi_hist_words(GM, b->max_bins) * sizeof(uint32_t) : 0;
+    // the largest MI pair's histograms (bin count and copies both vary by pair)
+    size_t words = 0;
+    if (b->has_mi)
+        for (const cgs_batch_pair &p : b->pairs)
+            if (is_mi_kind(batch_kind(p))) words = std::max(words, mi_hist_words(GM, p.opts.mi_bins));
+    const size_t lds = words * sizeof(uint32_t);
     const int blocks = std::max(1, b->cus * b->tiles.wg_per_cu);
     hipLaunchKernelGGL((batch_kernel<GF, GM>), dim3(blocks), dim3(kThreads), lds, s, (const BatchDesc *) b->descs.ptr,
                        (int) b->host_descs.size(), b->total_items, (uint32_t *) b->counter.ptr,
@@ -223,7 +227,6 @@ cgs_status stage_alloc(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, con
     b->drop_graph();
     b->pairs.assign(pairs, pairs + n);
     b->tiles = t;
-    b->max_bins = 8;
     b->cus = cus;
     b->taps.resize(n);
     b->fixf.resize(n);
@@ -236,7 +239,6 @@ cgs_status stage_alloc(cgs_batch *b, const cgs_batch_pair *pairs, int32_t n, con
     std::vector<float> sc;
     for (int32_t i = 0; i < n; ++i) {
         const cgs_batch_pair &p = pairs[i];
-        if (is_mi_kind(batch_kind(p))) b->max_bins = std::max(b->max_bins, (int) p.opts.mi_bins);
         const int W = p.moved.width, H = p.moved.height;
         const size_t ntaps = (size_t) W * H + 2 * (size_t) W + 2;
         const size_t tap_bytes = batch_kind(p) == kBatchMIU8 ? 4 : sizeof(float4);

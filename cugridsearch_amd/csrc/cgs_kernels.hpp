@@ -52,17 +52,19 @@ constexpr int kThreads = 256;
 // workgroup (at most 32 bins) every histogram takes the 32-bin stride, a
 // compile-time constant: the sweep reaches point g's histogram through the LDS
 // atomic's immediate offset instead of one address add per sample.
-// Up to 4 points per workgroup the histograms are held twice, one copy for
-// the even lanes and one for the odd (neighbouring pixels, whose bins often
-// coincide, then never add to one counter in the same atomic), and the two
-// are summed before the entropy step.
+// Up to 4 points per workgroup and 32 bins the histograms are held twice,
+// one copy for the even lanes and one for the odd (neighbouring pixels, whose
+// bins often coincide, then never add to one counter in the same atomic), and
+// the two are summed before the entropy step.  Larger histograms keep one
+// copy: two would halve the workgroups a CU holds (64 bins: 79 ms with one,
+// 115 with two, on 512^2 uint8).
 constexpr int kMiHistFixed = 32 * 33;
-__host__ __device__ constexpr int mi_hist_copies(int gpt) { return gpt <= 4 ? 2 : 1; }
+__host__ __device__ inline int mi_hist_copies(int gpt, int bins) { return gpt <= 4 && bins <= 32 ? 2 : 1; }
 __host__ __device__ inline size_t mi_hist_stride(int gpt, int bins) {
     return gpt > 2 ? (size_t) kMiHistFixed : (size_t) bins * (bins + 1);
 }
 __host__ __device__ inline size_t mi_hist_words(int gpt, int bins) {
-    return (size_t) mi_hist_copies(gpt) * gpt * mi_hist_stride(gpt, bins);
+    return (size_t) mi_hist_copies(gpt, bins) * gpt * mi_hist_stride(gpt, bins);
 }
 
 // Elements of zero padding after the moving image: covers every tap the
@@ -838,7 +840,7 @@ __device__ __forceinline__ void mi_sweep(const SearchParams &P, const float (*s_
     const float bscale = (float) bins / 256.0f;
     // LDS byte addresses: the lane's copy of the histograms, one row, one histogram
     const uint32_t hbase = __builtin_amdgcn_readfirstlane(lds_address(hist)) +
-                           (mi_hist_copies(GPT) == 2 && (threadIdx.x & 1) ? 4u * (uint32_t) (GPT * HS) : 0u);
+                           (mi_hist_copies(GPT, bins) == 2 && (threadIdx.x & 1) ? 4u * (uint32_t) (GPT * HS) : 0u);
     const uint32_t row4 = 4u * (uint32_t) RS, hist4 = 4u * (uint32_t) HS;
 
     float B[GPT][3];
@@ -1024,7 +1026,7 @@ __device__ __forceinline__ void mi_block(const SearchParams &P, int32_t base, Bl
 #pragma unroll
         for (int g = 0; g < GPT; ++g) S.red[wave][g][3] = (double) cnt[g];
     __syncthreads();
-    if constexpr (mi_hist_copies(GPT) == 2) {   // the odd lanes' copy into the even lanes'
+    if (mi_hist_copies(GPT, P.bins) == 2) {   // the odd lanes' copy into the even lanes'
         for (int i = tid; i < GPT * HS; i += kThreads) hist[i] += hist[GPT * HS + i];
         __syncthreads();
     }

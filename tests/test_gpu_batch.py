@@ -66,6 +66,28 @@ def test_batch_tiles_identical(gpu, tiles):
     check_equal(pairs, b.results(), want)
 
 
+def test_batch_mixed_bin_counts(gpu):
+    """MI pairs of 16, 32, 33 and 64 bins in one batch at 2 grid points per MI
+    item: up to 32 bins a pair holds two histogram copies, above it one, and
+    the launch's LDS holds the largest of them (here a 32-bin pair's two
+    copies outgrow a 33-bin pair's one)."""
+    specs, pairs = make_batch(n=12, seed=14)
+    bins_cycle = [16, 32, 33, 64]
+    k = 0
+    mixed = []
+    for (fn, g, m, f, res), sp in zip(pairs, specs):
+        if sp["cost"] == "mi":
+            fn = fn.with_(bins=bins_cycle[k % len(bins_cycle)])
+            k += 1
+        mixed.append((fn, g, m, f, res))
+    assert k >= 4
+    want = plain(mixed)
+    b = A.BatchSearcher()
+    b.plan(mixed, tiles=dict(fast_gpt=6, mi_gpt=2, wg_per_cu=4, items_per_claim=4))
+    b.run(use_graph=False)
+    check_equal(mixed, b.results(), want)
+
+
 def test_batch_graph_replays(gpu):
     specs, pairs = make_batch(seed=13)
     s = torch.cuda.Stream()
