@@ -161,11 +161,12 @@ static hipError_t mi_launch(const SearchParams &P, hipStream_t stream) {
 
 // 4 grid points per workgroup up to 32 bins (their histograms in two copies,
 // 34 KiB; on 512^2 uint8 at 32 bins 67.5 ms against 71.0 at 6 points and
-// 72.0 at 8, which hold one copy), 2 above; tile_gpt 2, 4, 6 or 8 overrides
-// up to 32 bins.
+// 72.0 at 8, which hold one copy), 2 above, and 2 at 8 bins or fewer (where
+// most lanes of an atomic share a counter: 69.9 ms against 73.6 at 4); tile_gpt
+// 2, 4, 6 or 8 overrides up to 32 bins.
 template <typename MT, typename FT>
 static hipError_t mi_gpt(const SearchParams &P, int gpt, hipStream_t stream) {
-    if (P.bins > 32 || gpt == 2) return mi_launch<MT, FT, 2>(P, stream);
+    if (P.bins > 32 || gpt == 2 || (gpt == 0 && P.bins <= 8)) return mi_launch<MT, FT, 2>(P, stream);
     if (gpt == 8) return mi_launch<MT, FT, 8>(P, stream);
     if (gpt == 6) return mi_launch<MT, FT, 6>(P, stream);
     return mi_launch<MT, FT, 4>(P, stream);

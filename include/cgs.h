@@ -120,7 +120,7 @@ typedef struct {
     int32_t flags;       /* CGS_FLAG_* bits */
     int32_t tile_gpt;    /* grid points per workgroup: 4, 6 or 8 for SAD/NCC/SQD, 2, 4, 6 or 8 for MI
                             (2 above 32 bins); 0 = library default (6 or 8 for SAD/NCC/SQD, 4 for
-                            MI up to 32 bins, 2 above) */
+                            MI from 9 to 32 bins, 2 otherwise) */
 } cgs_search_opts;
 
 /* opts.flags: time the search kernel with HIP events on the caller's
