@@ -854,6 +854,9 @@ __device__ __forceinline__ int32_t norm_rows_one(const AltParams &P, const float
             rir = (float) (((double) row + 0.5) * (double) (fh - 1));
         }
         unsigned long long rows_in = __ballot(Lr.in);
+        // every row's first segment, one lane per row, for the rows that do
+        // not start at lambda = 0 (those read the tables)
+        const LamSeg g0 = lam_segment(Lr.ls, sc, 0);
         while (rows_in) {
             const int j = __builtin_amdgcn_readfirstlane(__builtin_ctzll(rows_in));
             rows_in &= rows_in - 1;
@@ -909,7 +912,12 @@ __device__ __forceinline__ int32_t norm_rows_one(const AltParams &P, const float
                 // are full: lambda grows at every step (s > 2^-24, half an ulp
                 // or more below 1), so a row ends; `filled` bounds the filled
                 // passes as well
-                LamSeg g = uniform_seg(lam_segment(ls, sc, 0));
+                LamSeg g;
+                g.k0 = 0;
+                g.a = (uint32_t) __builtin_amdgcn_readlane((int) g0.a, j);
+                g.d = (uint32_t) __builtin_amdgcn_readlane((int) g0.d, j);
+                g.lim = (uint32_t) __builtin_amdgcn_readlane((int) g0.lim, j);
+                g.U = lane_value(g0.U, j);
                 uint32_t laned = __umul24((uint32_t) lane, g.d);
                 uint32_t b = g.a;
                 int32_t k0 = 0, filled = 0;
